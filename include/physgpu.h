@@ -1,0 +1,254 @@
+/*
+ * physgpu.h — C ABI of the MI355X (gfx950) collision pipeline for phys-engine.
+ *
+ * The library replaces the collision section of phys::PhysicsWorld::step
+ * (src/world/physics_world.cpp:539-739): AABB refresh, broad phase and narrow
+ * phase.  Integration and the sequential-impulse solver stay host code that
+ * consumes the manifolds this library returns.
+ *
+ * Conventions
+ *   - No exceptions cross this boundary.  Every call returns a physgpu_status;
+ *     physgpu_last_error() gives a thread-local message for the last failure.
+ *     Where the reference throws std::invalid_argument the call returns
+ *     PHYSGPU_INVALID_ARGUMENT.
+ *   - A context is bound to one device and one HIP stream; every call is
+ *     stream-ordered.  Calls that return host data synchronise that stream.
+ *   - Arrays are plain pointers.  "SoA planes" means plane k of an array with
+ *     n entries starts at element k*n.
+ *   - Pair indices are 32-bit (the reference uses size_t); ordering is the
+ *     reference's canonical lexicographic (first, second) order, first < second.
+ */
+#ifndef PHYSGPU_H
+#define PHYSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum physgpu_status {
+    PHYSGPU_OK = 0,
+    PHYSGPU_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument */
+    PHYSGPU_CAPACITY_EXCEEDED = 2, /* output or context capacity too small */
+    PHYSGPU_DEVICE_ERROR = 3,      /* HIP runtime failure */
+    PHYSGPU_UNSUPPORTED = 4
+} physgpu_status;
+
+/* phys::BroadPhaseAlgorithm, include/phys/collision/broadphase.hpp:8-13.
+ * PHYSGPU_DYNAMIC_TREE is served by a linear BVH rebuilt per call. */
+typedef enum physgpu_broadphase_algorithm {
+    PHYSGPU_SWEEP_AND_PRUNE = 0,
+    PHYSGPU_UNIFORM_GRID = 1,
+    PHYSGPU_DYNAMIC_TREE = 2
+} physgpu_broadphase_algorithm;
+
+/* Alternatives of phys::Collider::shape (collider.hpp:27) plus convex hulls. */
+typedef enum physgpu_shape_type {
+    PHYSGPU_SPHERE = 0,
+    PHYSGPU_BOX = 1,
+    PHYSGPU_HULL = 2
+} physgpu_shape_type;
+
+/* Where caller arrays live. */
+typedef enum physgpu_memory {
+    PHYSGPU_HOST = 0,
+    PHYSGPU_DEVICE = 1
+} physgpu_memory;
+
+/* Same layout as phys::Aabb (include/phys/collision/aabb.hpp:10-13). */
+typedef struct physgpu_aabb {
+    float min[3];
+    float max[3];
+} physgpu_aabb;
+
+/* phys::BroadPhasePair (broadphase.hpp:15-19) with 32-bit indices. */
+typedef struct physgpu_pair {
+    uint32_t first;
+    uint32_t second;
+} physgpu_pair;
+
+/* phys::BroadPhaseStats (broadphase.hpp:21-42), field for field.  Timings
+ * are device time in ms measured with HIP events when timing is enabled on
+ * the context, else 0. */
+typedef struct physgpu_broadphase_stats {
+    double record_build_ms;
+    double record_sort_ms;
+    double sweep_ms;
+    double pair_sort_ms;
+    uint64_t x_window_comparisons;
+    uint64_t aabb_pairs;
+    uint64_t grid_comparisons;
+    uint64_t grid_entries;
+    uint64_t grid_overflow_aabbs;
+    double grid_cell_size;
+    uint64_t tree_insertions;
+    uint64_t tree_removals;
+    uint64_t tree_reinsertions;
+    uint64_t tree_node_pair_visits;
+    uint64_t tree_leaf_checks;
+    uint64_t tree_height;
+    uint64_t tree_proxy_count;
+} physgpu_broadphase_stats;
+
+/* phys::NarrowPhaseStats (step_stats.hpp:7-15) plus the hull classes. */
+typedef struct physgpu_narrowphase_stats {
+    uint64_t sphere_sphere_candidates;
+    uint64_t sphere_box_candidates;
+    uint64_t box_box_candidates;
+    uint64_t sphere_sphere_contacts;
+    uint64_t sphere_box_contacts;
+    uint64_t box_box_contacts;
+    uint64_t hull_candidates;   /* pairs with at least one hull (GJK/EPA) */
+    uint64_t hull_contacts;
+    uint64_t gjk_epa_failures;  /* GJK hit but EPA did not converge */
+} physgpu_narrowphase_stats;
+
+/* Shape-pair classes.  The reference's shapePair = index(A) + index(B)
+ * (physics_world.cpp:670-671) gives 0, 1, 2 for SS, SB/BS, BB; hull classes
+ * follow. */
+typedef enum physgpu_shape_pair {
+    PHYSGPU_PAIR_SPHERE_SPHERE = 0,
+    PHYSGPU_PAIR_SPHERE_BOX = 1,
+    PHYSGPU_PAIR_BOX_BOX = 2,
+    PHYSGPU_PAIR_SPHERE_HULL = 3,
+    PHYSGPU_PAIR_BOX_HULL = 4,
+    PHYSGPU_PAIR_HULL_HULL = 5
+} physgpu_shape_pair;
+
+/* One phys::ContactManifold (manifold.hpp:20-32) without the solver's impulse
+ * fields.  Anchors are collider-local for physgpu_generate_contacts (the
+ * NarrowPhase::generateContact contract) and body-local for the world-level
+ * physgpu_collide_world (physics_world.cpp:679-686).  160 bytes. */
+typedef struct physgpu_manifold {
+    uint32_t first;           /* collider index of A (bodyA) */
+    uint32_t second;          /* collider index of B (bodyB) */
+    float normal[3];          /* from A toward B */
+    uint32_t point_count;     /* 1..4 */
+    float anchor_a[4][3];
+    float anchor_b[4][3];
+    float penetration[4];
+    float friction;
+    float restitution;
+    uint32_t pair_index;      /* position in the candidate list */
+    uint32_t shape_pair;      /* physgpu_shape_pair */
+    uint32_t reserved[2];
+} physgpu_manifold;
+
+/* Collider description.  All arrays have `count` entries unless stated.
+ *   pose        7 SoA planes: px, py, pz, qx, qy, qz, qw (world transform of
+ *               the collider: body pose composed with the local transform)
+ *   shape_type  physgpu_shape_type per collider
+ *   shape_data  3 SoA planes: sphere (radius, -, -); box half extents;
+ *               hull: unused
+ *   hull_range  2 SoA planes (first point, point count) into hull_points;
+ *               may be NULL when no collider is a hull
+ *   hull_points 3 SoA planes of hull_point_count collider-local points */
+typedef struct physgpu_colliders {
+    uint32_t count;
+    const float* pose;
+    const uint8_t* shape_type;
+    const float* shape_data;
+    const uint32_t* hull_range;
+    uint32_t hull_point_count;
+    const float* hull_points;
+} physgpu_colliders;
+
+/* World-level extras for physgpu_collide_world, mirroring what
+ * PhysicsWorld::step reads besides the collider pose (physics_world.cpp:
+ * 570-581, 630-633, 679-690).  Arrays have colliders.count entries; any may be
+ * NULL (then: body = collider index, local transform = identity,
+ * friction = restitution = 0). */
+typedef struct physgpu_world_extras {
+    const uint32_t* body;        /* owning body per collider */
+    const float* local_pose;     /* 7 SoA planes: collider local transform */
+    const float* friction;       /* per collider (its body's value) */
+    const float* restitution;    /* per collider (its body's value) */
+} physgpu_world_extras;
+
+/* Narrow-phase options.  Zero-initialised means the defaults below. */
+typedef struct physgpu_narrowphase_options {
+    uint32_t gjk_max_iterations;  /* default 64 */
+    uint32_t epa_max_iterations;  /* default 64 */
+    float epa_tolerance;          /* default 1e-6 */
+    uint32_t reserved;
+} physgpu_narrowphase_options;
+
+typedef struct physgpu_context physgpu_context;
+
+/* Thread-local text for the last non-OK status on this thread. */
+const char* physgpu_last_error(void);
+
+/* Number of HIP devices visible (0 when none); never fails. */
+int physgpu_device_count(void);
+
+/* Context on `device` able to hold `max_colliders` colliders,
+ * `max_pairs` candidate pairs and `max_hull_points` hull points.  Device
+ * buffers are allocated here, once. */
+physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_colliders,
+                              uint64_t max_pairs, uint32_t max_hull_points);
+physgpu_status physgpu_destroy(physgpu_context* ctx);
+
+/* Launch subsequent work on `hip_stream` (a hipStream_t; NULL = the null stream). */
+physgpu_status physgpu_set_stream(physgpu_context* ctx, void* hip_stream);
+
+/* Enable per-kernel HIP-event timing (stats timings, physgpu_kernel_times). */
+physgpu_status physgpu_set_timing(physgpu_context* ctx, int enabled);
+
+/* Copy colliders into the context (memory says where the arrays live). */
+physgpu_status physgpu_set_colliders(physgpu_context* ctx, const physgpu_colliders* colliders,
+                                     physgpu_memory memory);
+
+/* Aabb::fromCollider (src/collision/aabb.cpp:8-32) for every resident
+ * collider; hulls take min/max of their transformed points.  When `out` is not
+ * NULL the bounds are copied to it (count entries). */
+physgpu_status physgpu_compute_aabbs(physgpu_context* ctx, physgpu_aabb* out);
+
+/* BroadPhase::findCandidatePairs (include/phys/collision/broadphase.hpp:52-54,
+ * src/collision/broadphase.cpp:186-291).  `bounds` (host) replaces the
+ * resident AABBs when not NULL.  Writes up to `capacity` pairs to `out`
+ * (host, may be NULL) and the full count to `*count`; returns
+ * PHYSGPU_CAPACITY_EXCEEDED when the pairs do not fit the context.
+ * `stats` (may be NULL) is overwritten.  Unlike the reference, the stateless
+ * call accepts PHYSGPU_DYNAMIC_TREE (an LBVH is built for the call).
+ * `grid_cell_size` <= 0 selects the automatic cell (the largest AABB side,
+ * widened by 1/1024). */
+physgpu_status physgpu_find_candidate_pairs(physgpu_context* ctx, const physgpu_aabb* bounds,
+                                            uint32_t n, int algorithm, double grid_cell_size,
+                                            physgpu_pair* out, uint64_t capacity, uint64_t* count,
+                                            physgpu_broadphase_stats* stats);
+
+/* NarrowPhase::generateContact (src/collision/narrowphase.cpp:326-374) for
+ * every pair, over the resident colliders.  `pairs` (host) NULL means the
+ * candidate pairs of the last broad phase.  Manifolds come back compacted in
+ * pair order (collider-local anchors).  `out` may be NULL. */
+physgpu_status physgpu_generate_contacts(physgpu_context* ctx, const physgpu_pair* pairs,
+                                         uint64_t n_pairs,
+                                         const physgpu_narrowphase_options* options,
+                                         physgpu_manifold* out, uint64_t capacity,
+                                         uint64_t* count, physgpu_narrowphase_stats* stats);
+
+/* The collision section of PhysicsWorld::step, device resident: AABBs,
+ * broad phase, narrow phase.  Results stay on the device (read them with the
+ * download calls); the two counts are returned. */
+physgpu_status physgpu_collide(physgpu_context* ctx, int algorithm, double grid_cell_size,
+                               const physgpu_narrowphase_options* options,
+                               uint64_t* pair_count, uint64_t* manifold_count);
+
+physgpu_status physgpu_download_pairs(physgpu_context* ctx, physgpu_pair* out, uint64_t capacity);
+physgpu_status physgpu_download_manifolds(physgpu_context* ctx, physgpu_manifold* out,
+                                          uint64_t capacity);
+physgpu_status physgpu_narrowphase_stats_get(physgpu_context* ctx,
+                                             physgpu_narrowphase_stats* stats);
+
+/* Average device time (ms) per launch of each kernel since timing was
+ * enabled.  Writes up to `capacity` entries; names are static strings. */
+physgpu_status physgpu_kernel_times(physgpu_context* ctx, const char** names, double* avg_ms,
+                                    uint64_t* launches, uint32_t capacity, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYSGPU_H */

@@ -1,0 +1,175 @@
+"""Seeded synthetic collider scenes (numpy, float32, SoA planes).
+
+Each scene holds exactly the arrays the C ABI takes (physgpu_colliders):
+  pose        float32 [7, n]  px, py, pz, qx, qy, qz, qw (world pose)
+  shape_type  uint8   [n]     0 sphere, 1 box, 2 hull
+  shape_data  float32 [3, n]  sphere radius / box half extents
+  hull_range  uint32  [2, n]  (first point, count) into hull_points, or None
+  hull_points float32 [3, H]  collider-local hull points, or None
+and, for world-level runs, body_position [3, n], local_pose [7, n],
+friction [n], restitution [n].
+
+Generators use numpy's PCG64 (np.random.default_rng), which gives identical
+bytes for a given seed on every machine running the same numpy, so CPU and
+GPU runs read the same inputs.  Workloads:
+  spheres     4,096 spheres, r ~ U[0.25, 0.5], centres U[-10, 10]^3
+  mixed       50% spheres r ~ U[0.2, 0.5], 40% boxes h ~ U[0.2, 0.6]^3,
+              10% hulls of 16 points on an ellipsoid, semi-axes U[0.3, 0.6]^3,
+              uniform unit quaternions, centres uniform in a cube of side L
+  stack       64 x 64 columns of 64 unit boxes sunk 1e-3 m into each other,
+              yaw U[-0.05, 0.05], lateral jitter U[-0.01, 0.01], on a ground
+              box of half extents (100, 0.5, 100)
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class Scene:
+    pose: np.ndarray
+    shape_type: np.ndarray
+    shape_data: np.ndarray
+    hull_range: np.ndarray = None
+    hull_points: np.ndarray = None
+    body_position: np.ndarray = None
+    local_pose: np.ndarray = None
+    friction: np.ndarray = None
+    restitution: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def count(self):
+        return int(self.shape_type.shape[0])
+
+    def subset(self, index):
+        """Scene of the colliders `index` (hull points re-based)."""
+        index = np.asarray(index, dtype=np.int64)
+        hr = hp = None
+        if self.hull_range is not None:
+            hr = np.zeros((2, len(index)), np.uint32)
+            pts = []
+            cursor = 0
+            for k, i in enumerate(index):
+                if self.shape_type[i] == abi.HULL:
+                    f, c = self.hull_range[:, i]
+                    hr[:, k] = (cursor, c)
+                    pts.append(self.hull_points[:, f:f + c])
+                    cursor += int(c)
+            hp = (np.concatenate(pts, axis=1) if pts else np.zeros((3, 1), np.float32))
+            hp = np.ascontiguousarray(hp, dtype=np.float32)
+        take = (lambda a: None if a is None else np.ascontiguousarray(a[..., index]))
+        return Scene(pose=take(self.pose), shape_type=take(self.shape_type),
+                     shape_data=take(self.shape_data), hull_range=hr, hull_points=hp,
+                     body_position=take(self.body_position), local_pose=take(self.local_pose),
+                     friction=take(self.friction), restitution=take(self.restitution),
+                     meta=dict(self.meta, subset=len(index)))
+
+
+def random_quaternions(rng, n):
+    """Uniform unit quaternions (normalised 4-D Gaussians), float32 [4, n] xyzw."""
+    q = rng.standard_normal((4, n))
+    q /= np.linalg.norm(q, axis=0, keepdims=True)
+    return q.astype(np.float32)
+
+
+def _identity_quats(n):
+    q = np.zeros((4, n), np.float32)
+    q[3] = 1.0
+    return q
+
+
+def spheres(n=4096, seed=20261203, half_side=10.0, rmin=0.25, rmax=0.5):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-half_side, half_side, (3, n)).astype(np.float32)
+    r = rng.uniform(rmin, rmax, n).astype(np.float32)
+    pose = np.concatenate([pos, _identity_quats(n)]).astype(np.float32)
+    data = np.zeros((3, n), np.float32)
+    data[0] = r
+    return Scene(pose=np.ascontiguousarray(pose), shape_type=np.zeros(n, np.uint8),
+                 shape_data=data, meta=dict(workload="spheres", n=n, seed=seed))
+
+
+def mixed_side_for(n, pairs_per_body=1.0):
+    """Cube side giving about `pairs_per_body` candidate pairs per body for the
+    mixed shape mix (calibrated: 1,048,576 bodies in a 256 m cube give ~0.26
+    pairs per body; pair density scales with n / L^3)."""
+    base_pairs_per_body = 0.263
+    side = 256.0 * (n / 1048576.0) ** (1.0 / 3.0)
+    return side * (base_pairs_per_body / pairs_per_body) ** (1.0 / 3.0)
+
+
+def mixed(n=1048576, seed=20261203, side=256.0, hull_points=16, with_hulls=True):
+    rng = np.random.default_rng(seed)
+    n_sph = n // 2
+    n_box = int(round(0.4 * n)) if with_hulls else n - n_sph
+    n_hull = n - n_sph - n_box
+    types = np.concatenate([np.zeros(n_sph, np.uint8), np.ones(n_box, np.uint8),
+                            np.full(n_hull, abi.HULL, np.uint8)])
+    types = types[rng.permutation(n)]
+    pos = rng.uniform(-side / 2, side / 2, (3, n)).astype(np.float32)
+    quat = random_quaternions(rng, n)
+    data = np.zeros((3, n), np.float32)
+    sph = types == abi.SPHERE
+    box = types == abi.BOX
+    hull = types == abi.HULL
+    data[0, sph] = rng.uniform(0.2, 0.5, sph.sum())
+    data[:, box] = rng.uniform(0.2, 0.6, (3, box.sum()))
+    hr = hp = None
+    if n_hull:
+        idx = np.nonzero(hull)[0]
+        axes = rng.uniform(0.3, 0.6, (3, n_hull))
+        dirs = rng.standard_normal((n_hull, hull_points, 3))
+        dirs /= np.linalg.norm(dirs, axis=2, keepdims=True)
+        pts = dirs * axes.T[:, None, :]
+        hp = np.ascontiguousarray(pts.reshape(-1, 3).T.astype(np.float32))
+        hr = np.zeros((2, n), np.uint32)
+        hr[0, idx] = np.arange(n_hull, dtype=np.uint32) * hull_points
+        hr[1, idx] = hull_points
+    pose = np.ascontiguousarray(np.concatenate([pos, quat]).astype(np.float32))
+    return Scene(pose=pose, shape_type=types, shape_data=data, hull_range=hr, hull_points=hp,
+                 meta=dict(workload="mixed", n=n, seed=seed, side=side))
+
+
+def stack(columns=64, height=64, seed=20261203, spacing=1.1, sink=1e-3):
+    """Resting stack: box k of a column at y = 0.5 + k (1 - sink) over a ground box."""
+    rng = np.random.default_rng(seed)
+    n = columns * columns * height + 1
+    cx, cz, k = np.meshgrid(np.arange(columns), np.arange(columns), np.arange(height),
+                            indexing="ij")
+    cx, cz, k = cx.ravel(), cz.ravel(), k.ravel()
+    m = cx.size
+    off = (columns - 1) * spacing / 2.0
+    x = cx * spacing - off + rng.uniform(-0.01, 0.01, m)
+    z = cz * spacing - off + rng.uniform(-0.01, 0.01, m)
+    y = 0.5 + k * (1.0 - sink)
+    yaw = rng.uniform(-0.05, 0.05, m)
+    pose = np.zeros((7, n), np.float32)
+    pose[0, :m], pose[1, :m], pose[2, :m] = x, y, z
+    pose[4, :m] = np.sin(yaw / 2)
+    pose[6, :m] = np.cos(yaw / 2)
+    pose[:, m] = (0, -0.5, 0, 0, 0, 0, 1)
+    data = np.full((3, n), 0.5, np.float32)
+    data[:, m] = (100.0, 0.5, 100.0)
+    return Scene(pose=pose, shape_type=np.ones(n, np.uint8), shape_data=data,
+                 meta=dict(workload="stack", n=n, seed=seed, columns=columns, height=height))
+
+
+def with_world_extras(scene, seed=7, local_rotation=True):
+    """World-level form of a scene: each collider on its own body at the pose
+    position with identity body rotation, the collider's orientation carried by
+    its local transform (the reference has no body-rotation setter), and
+    per-body friction/restitution."""
+    rng = np.random.default_rng(seed)
+    n = scene.count
+    scene.body_position = np.ascontiguousarray(scene.pose[:3].copy())
+    lp = np.zeros((7, n), np.float32)
+    lp[6] = 1.0
+    if local_rotation:
+        lp[3:7] = scene.pose[3:7]
+    scene.local_pose = lp
+    scene.friction = rng.uniform(-0.1, 1.0, n).astype(np.float32)
+    scene.restitution = rng.uniform(0.0, 0.8, n).astype(np.float32)
+    return scene
