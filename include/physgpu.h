@@ -103,7 +103,18 @@ typedef struct physgpu_narrowphase_stats {
     uint64_t hull_candidates;   /* pairs with at least one hull (GJK/EPA) */
     uint64_t hull_contacts;
     uint64_t gjk_epa_failures;  /* GJK hit but EPA did not converge */
+    uint64_t epa_overflows;     /* EPA polytope exceeded the bounds below */
 } physgpu_narrowphase_stats;
+
+/* Bounded EPA polytope of the GPU narrow phase.  The reference's EPA
+ * (src/collision/epa.cpp:64-130) is replayed exactly while it holds at most
+ * PHYSGPU_EPA_MAX_FACES faces and its horizon at most PHYSGPU_EPA_MAX_HORIZON
+ * edges; a pair that would exceed them gets no contact and is counted in
+ * epa_overflows.  (In FP32 the reference's polytope can duplicate faces
+ * without bound on near-degenerate pairs.) */
+#define PHYSGPU_EPA_MAX_FACES 256
+#define PHYSGPU_EPA_MAX_HORIZON 128
+#define PHYSGPU_EPA_MAX_ITERATIONS 64
 
 /* Shape-pair classes.  The reference's shapePair = index(A) + index(B)
  * (physics_world.cpp:670-671) gives 0, 1, 2 for SS, SB/BS, BB; hull classes
@@ -119,8 +130,8 @@ typedef enum physgpu_shape_pair {
 
 /* One phys::ContactManifold (manifold.hpp:20-32) without the solver's impulse
  * fields.  Anchors are collider-local for physgpu_generate_contacts (the
- * NarrowPhase::generateContact contract) and body-local for the world-level
- * physgpu_collide_world (physics_world.cpp:679-686).  160 bytes. */
+ * NarrowPhase::generateContact contract) and body-local after
+ * physgpu_set_world (physics_world.cpp:679-686).  160 bytes. */
 typedef struct physgpu_manifold {
     uint32_t first;           /* collider index of A (bodyA) */
     uint32_t second;          /* collider index of B (bodyB) */
@@ -155,7 +166,7 @@ typedef struct physgpu_colliders {
     const float* hull_points;
 } physgpu_colliders;
 
-/* World-level extras for physgpu_collide_world, mirroring what
+/* World-level extras for physgpu_set_world, mirroring what
  * PhysicsWorld::step reads besides the collider pose (physics_world.cpp:
  * 570-581, 630-633, 679-690).  Arrays have colliders.count entries; any may be
  * NULL (then: body = collider index, local transform = identity,
@@ -236,9 +247,24 @@ physgpu_status physgpu_collide(physgpu_context* ctx, int algorithm, double grid_
                                const physgpu_narrowphase_options* options,
                                uint64_t* pair_count, uint64_t* manifold_count);
 
+/* World-level form of the collision section (physics_world.cpp:539-739).
+ * `body_pose` (7 SoA planes, one entry per resident collider) is the pose of
+ * each collider's body; with `extras` the collider's world pose becomes
+ * body pose composed with its local transform (physics_world.cpp:660-667),
+ * pairs of colliders on the same body are dropped before the narrow phase
+ * (:630-633), anchors come back body-local (:679-686) and friction /
+ * restitution are combined (:687-690).  Every later physgpu_collide uses
+ * these until physgpu_set_colliders is called again. */
+physgpu_status physgpu_set_world(physgpu_context* ctx, const float* body_pose,
+                                 const physgpu_world_extras* extras, physgpu_memory memory);
+
 physgpu_status physgpu_download_pairs(physgpu_context* ctx, physgpu_pair* out, uint64_t capacity);
 physgpu_status physgpu_download_manifolds(physgpu_context* ctx, physgpu_manifold* out,
                                           uint64_t capacity);
+/* Counts of the last step: candidate pairs found (may exceed the capacity)
+ * and manifolds produced.  Synchronises the context's stream. */
+physgpu_status physgpu_step_counts(physgpu_context* ctx, uint64_t* pair_count,
+                                   uint64_t* manifold_count);
 physgpu_status physgpu_narrowphase_stats_get(physgpu_context* ctx,
                                              physgpu_narrowphase_stats* stats);
 

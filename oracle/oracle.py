@@ -68,9 +68,18 @@ class CpuCollision:
         getattr(L, p + "gjk_epa").argtypes = [
             C.POINTER(abi.Colliders), vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
             vp, vp, vp, vp]
+        if which == "oracle":
+            L.orc_set_epa_bounds.argtypes = [C.c_uint32, C.c_uint32]
+            L.orc_set_epa_bounds.restype = None
         getattr(L, p + "world_collide").argtypes = [
             C.POINTER(abi.Colliders), vp, vp, vp, vp, C.c_int, C.c_uint32, C.c_uint32,
             C.c_uint32, C.c_uint32, vp, C.c_uint64, C.POINTER(C.c_uint64), vp, vp]
+
+    def set_epa_bounds(self, max_faces=0, max_edges=0):
+        """Bounded EPA polytope (the GPU's limits) or 0, 0 for the reference's
+        unbounded behaviour."""
+        assert self.which == "oracle"
+        self.lib.orc_set_epa_bounds(max_faces, max_edges)
 
     def _check(self, rc, what):
         if rc not in (abi.OK,):

@@ -748,15 +748,28 @@ Face<T> makeFace(const std::vector<V3<T>>& v, uint32_t a, uint32_t b, uint32_t c
     return {a, b, c, n, d};
 }
 
-// computeEpa, epa.cpp:64-130.
-template <typename T>
-bool epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, uint32_t iterations,
-         T tol, V3<T>& normal, T& depth)
+// Peak polytope sizes seen by epa() (faces, horizon edges, vertices), for
+// sizing bounded GPU storage.
+std::atomic<uint32_t> g_peak_faces{0}, g_peak_edges{0}, g_peak_vertices{0};
+inline void note_peak(std::atomic<uint32_t>& a, size_t v)
 {
+    uint32_t cur = a.load();
+    while (v > cur && !a.compare_exchange_weak(cur, uint32_t(v))) {
+    }
+}
+
+// computeEpa, epa.cpp:64-130.  Returns 1 (converged), 0 (the reference's
+// failure) or -1 (a bound of opt was exceeded; never with the defaults).
+template <typename T>
+int epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, const NarrowOptions& opt,
+        V3<T>& normal, T& depth)
+{
+    const uint32_t iterations = opt.epaIterations;
+    const T tol = T(opt.epaTolerance);
     normal = {T(0), T(0), T(0)};
     depth = T(0);
     if (s.size < 4 || iterations == 0)
-        return false;
+        return 0;
     std::vector<V3<T>> v(s.p, s.p + 4);
     std::vector<Face<T>> faces = {makeFace(v, 0, 1, 2), makeFace(v, 0, 3, 1),
                                   makeFace(v, 0, 2, 3), makeFace(v, 1, 3, 2)};
@@ -765,7 +778,7 @@ bool epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, uint32
     };
     for (uint32_t it = 0; it < iterations; ++it) {
         if (faces.empty())
-            return false;
+            return 0;
         size_t best = 0;
         for (size_t f = 1; f < faces.size(); ++f)
             if (faces[f].d < faces[best].d)
@@ -776,18 +789,21 @@ bool epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, uint32
         if (sd - cf.d <= tol) {
             normal = cf.n;
             depth = maxOf(cf.d, T(0));
-            return true;
+            return 1;
         }
         uint32_t si = uint32_t(v.size());
         v.push_back(sup);
         std::vector<Edge> horizon;
         std::vector<Face<T>> keep;
+        bool overflow = false;
         auto addEdge = [&](uint32_t a, uint32_t b) {
             for (size_t e = 0; e < horizon.size(); ++e)
                 if (horizon[e].a == b && horizon[e].b == a) {
                     horizon.erase(horizon.begin() + e);
                     return;
                 }
+            if (opt.maxEdges && horizon.size() >= opt.maxEdges)
+                overflow = true;
             horizon.push_back({a, b});
         };
         for (const Face<T>& f : faces) {
@@ -799,13 +815,20 @@ bool epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, uint32
                 keep.push_back(f);
             }
         }
+        note_peak(g_peak_edges, horizon.size());
+        note_peak(g_peak_vertices, v.size());
+        if (overflow)
+            return -1;
         if (horizon.empty())
-            return false;
+            return 0;
+        if (opt.maxFaces && keep.size() + horizon.size() > opt.maxFaces)
+            return -1;
         faces = std::move(keep);
         for (const Edge& e : horizon)
             faces.push_back(makeFace(v, e.a, e.b, si));
+        note_peak(g_peak_faces, faces.size());
     }
-    return false;
+    return 0;
 }
 
 // Hull-involving pairs: GJK then EPA; one contact point at the midpoint of
@@ -823,9 +846,10 @@ bool convexPair(const Shape<T>& a, const Pose<T>& ta, const Shape<T>& b, const P
     }
     V3<T> n;
     T depth;
-    if (!epa(sa, sb, s, opt.epaIterations, T(opt.epaTolerance), n, depth)) {
+    const int st = epa(sa, sb, s, opt, n, depth);
+    if (st != 1) {
         if (status)
-            *status = 2;
+            *status = st == 0 ? 2 : 3;
         return false;
     }
     if (status)
@@ -918,9 +942,13 @@ struct View {
     }
 };
 
+uint32_t g_max_faces = 0, g_max_edges = 0;
+
 NarrowOptions makeOptions(uint32_t gjkIt, uint32_t epaIt, float tol)
 {
     NarrowOptions o;
+    o.maxFaces = g_max_faces;
+    o.maxEdges = g_max_edges;
     if (gjkIt)
         o.gjkIterations = gjkIt;
     if (epaIt)
@@ -984,6 +1012,27 @@ void findPairs(const physgpu_aabb* bounds, uint32_t n, int algorithm, uint32_t w
 extern "C" {
 
 const char* orc_last_error() { return lastError.c_str(); }
+
+// EPA polytope bounds used by the following calls (0, 0 = unbounded, the
+// reference's behaviour).  Test infrastructure: not thread-safe.
+void orc_set_epa_bounds(uint32_t maxFaces, uint32_t maxEdges)
+{
+    g_max_faces = maxFaces;
+    g_max_edges = maxEdges;
+}
+
+// Peak EPA polytope sizes since the last reset (faces, edges, vertices).
+void orc_epa_peak(uint32_t* out, int reset)
+{
+    out[0] = np::g_peak_faces.load();
+    out[1] = np::g_peak_edges.load();
+    out[2] = np::g_peak_vertices.load();
+    if (reset) {
+        np::g_peak_faces = 0;
+        np::g_peak_edges = 0;
+        np::g_peak_vertices = 0;
+    }
+}
 
 int orc_compute_aabbs(const physgpu_colliders* c, physgpu_aabb* out)
 {
@@ -1121,8 +1170,10 @@ int orc_gjk_epa(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t 
             }
             V3<float> nn;
             float d;
-            bool ok = np::epa(fa, fb, s, epaIterations, tolerance, nn, d);
-            status[k] = ok ? 1 : 2;
+            NarrowOptions opt = makeOptions(gjkIterations, epaIterations, tolerance);
+            opt.epaIterations = epaIterations;
+            const int st = np::epa(fa, fb, s, opt, nn, d);
+            status[k] = st == 1 ? 1 : (st == 0 ? 2 : 3);
             normal[3 * k] = nn.x;
             normal[3 * k + 1] = nn.y;
             normal[3 * k + 2] = nn.z;

@@ -148,6 +148,11 @@ struct NarrowOptions {
     uint32_t gjkIterations = 64;
     uint32_t epaIterations = 64;
     float epaTolerance = 1e-6f;
+    // Bounded polytope (0 = unbounded, the reference's behaviour): a pair whose
+    // EPA would hold more faces, or whose horizon would grow past maxEdges,
+    // stops with an overflow status, exactly where the HIP kernel stops.
+    uint32_t maxFaces = 0;
+    uint32_t maxEdges = 0;
 };
 
 // Narrow phase entry (NarrowPhase::generateContact, narrowphase.cpp:326-374,

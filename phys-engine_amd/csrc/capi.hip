@@ -1,0 +1,686 @@
+// C ABI of the collision library (include/physgpu.h): context lifecycle,
+// uploads, the stage sequence of PhysicsWorld::step's collision section
+// (src/world/physics_world.cpp:539-739), statistics and kernel timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace pg {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+physgpu_status fail(physgpu_status st, const std::string& msg)
+{
+    g_error = msg;
+    return st;
+}
+
+physgpu_status check_hip(hipError_t e, const char* what)
+{
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return PHYSGPU_DEVICE_ERROR;
+}
+
+void time_begin(physgpu_context* ctx, const char* name, Timer& t)
+{
+    t.name = name;
+    t.start = t.stop = nullptr;
+    if (!ctx->timing)
+        return;
+    auto take = [&]() {
+        hipEvent_t e = nullptr;
+        if (!ctx->event_pool.empty()) {
+            e = ctx->event_pool.back();
+            ctx->event_pool.pop_back();
+        } else {
+            hipEventCreate(&e);
+        }
+        return e;
+    };
+    t.start = take();
+    t.stop = take();
+    hipEventRecord(t.start, ctx->stream);
+}
+
+void time_end(physgpu_context* ctx, Timer& t)
+{
+    if (!ctx->timing || !t.start)
+        return;
+    hipEventRecord(t.stop, ctx->stream);
+    ctx->pending.push_back(t);
+}
+
+void collect_times(physgpu_context* ctx)
+{
+    for (Timer& t : ctx->pending) {
+        float ms = 0.0f;
+        hipEventSynchronize(t.stop);
+        hipEventElapsedTime(&ms, t.start, t.stop);
+        auto it = std::find_if(ctx->kernel_times.begin(), ctx->kernel_times.end(),
+                               [&](const KernelTime& k) { return std::strcmp(k.name, t.name) == 0; });
+        if (it == ctx->kernel_times.end()) {
+            ctx->kernel_times.push_back({t.name, 0.0, 0});
+            it = ctx->kernel_times.end() - 1;
+        }
+        it->total_ms += ms;
+        it->launches += 1;
+        ctx->event_pool.push_back(t.start);
+        ctx->event_pool.push_back(t.stop);
+    }
+    ctx->pending.clear();
+}
+
+namespace {
+
+template <typename T>
+hipError_t alloc(T*& p, size_t count)
+{
+    return hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+void free_all(physgpu_context* c)
+{
+    void* ptrs[] = {c->pose, c->shape_type, c->shape_data, c->hull_range, c->hull_points,
+                    c->aabb, c->sorted_aabb, c->sorted_index, c->items_a, c->items_b,
+                    c->histogram, c->cell_table, c->overflow_list, c->grid_flag, c->node_left,
+                    c->node_right, c->node_parent, c->node_last, c->node_box, c->node_flag,
+                    c->pairs_raw, c->pairs_tmp, c->buckets, c->slots, c->hit, c->tile_sums,
+                    c->manifolds, c->body, c->local_pose, c->friction, c->restitution,
+                    c->counters, c->world_body_pose};
+    for (void* p : ptrs)
+        if (p)
+            hipFree(p);
+    if (c->host_counters)
+        hipHostFree(c->host_counters);
+    for (Timer& t : c->pending) {
+        hipEventDestroy(t.start);
+        hipEventDestroy(t.stop);
+    }
+    for (hipEvent_t e : c->event_pool)
+        hipEventDestroy(e);
+    if (c->stream && c->owns_stream)
+        hipStreamDestroy(c->stream);
+}
+
+physgpu_status read_counters(physgpu_context* ctx)
+{
+    PG_TRY_HIP(hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->timing)
+        collect_times(ctx);
+    return PHYSGPU_OK;
+}
+
+// Broad phase on the resident AABB planes (ctx->n bodies).
+void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool window_stats)
+{
+    launch_aabb_reductions(ctx);
+    if (ctx->n >= 2) {
+        if (algorithm == PHYSGPU_SWEEP_AND_PRUNE) {
+            launch_sap_keys(ctx);
+            uint64_t* sorted = nullptr;
+            radix_sort(ctx, ctx->items_a, ctx->items_b, ctx->n, nullptr, 0, 32, 0, 0, &sorted);
+            launch_gather_sorted(ctx, sorted);
+            launch_sap_sweep(ctx);
+            if (window_stats)
+                launch_sap_window_tests(ctx);
+        } else if (algorithm == PHYSGPU_UNIFORM_GRID) {
+            launch_grid(ctx, cell);
+        } else {
+            launch_lbvh(ctx);
+        }
+    }
+    launch_canonical_pairs(ctx);
+    ctx->last_algorithm = algorithm;
+}
+
+void fill_broad_stats(physgpu_context* ctx, physgpu_broadphase_stats* s)
+{
+    std::memset(s, 0, sizeof(*s));
+    const Counters& c = *ctx->host_counters;
+    s->aabb_pairs = c.pair_count;
+    if (ctx->last_algorithm == PHYSGPU_SWEEP_AND_PRUNE) {
+        s->x_window_comparisons = c.window_tests;
+    } else if (ctx->last_algorithm == PHYSGPU_UNIFORM_GRID) {
+        s->grid_comparisons = c.grid_tests;
+        s->grid_entries = c.grid_entries;
+        s->grid_overflow_aabbs = c.overflow_count;
+        float cell;
+        std::memcpy(&cell, &c.cell_bits, 4);
+        s->grid_cell_size = cell;
+    } else if (ctx->last_algorithm == PHYSGPU_DYNAMIC_TREE) {
+        s->tree_insertions = ctx->n;
+        s->tree_node_pair_visits = c.tree_visits;
+        s->tree_leaf_checks = c.tree_leaf_checks;
+        s->tree_height = c.tree_height;
+        s->tree_proxy_count = ctx->n;
+    }
+}
+
+void fill_np_stats(const Counters& c, physgpu_narrowphase_stats* s)
+{
+    s->sphere_sphere_candidates = c.class_count[0];
+    s->sphere_box_candidates = c.class_count[1];
+    s->box_box_candidates = c.class_count[2];
+    s->sphere_sphere_contacts = c.class_contacts[0];
+    s->sphere_box_contacts = c.class_contacts[1];
+    s->box_box_contacts = c.class_contacts[2];
+    s->hull_candidates = uint64_t(c.class_count[3]) + c.class_count[4] + c.class_count[5];
+    s->hull_contacts = uint64_t(c.class_contacts[3]) + c.class_contacts[4] + c.class_contacts[5];
+    s->gjk_epa_failures = c.gjk_failures;
+    s->epa_overflows = c.epa_overflow;
+}
+
+physgpu_status apply_options(physgpu_context* ctx, const physgpu_narrowphase_options* o)
+{
+    physgpu_narrowphase_options d{64, 64, 1e-6f, 0};
+    if (o) {
+        if (o->gjk_max_iterations)
+            d.gjk_max_iterations = o->gjk_max_iterations;
+        if (o->epa_max_iterations)
+            d.epa_max_iterations = o->epa_max_iterations;
+        if (o->epa_tolerance > 0.0f)
+            d.epa_tolerance = o->epa_tolerance;
+    }
+    if (d.epa_max_iterations > PHYSGPU_EPA_MAX_ITERATIONS)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "epa_max_iterations above 64 is not supported");
+    ctx->np_options = d;
+    return PHYSGPU_OK;
+}
+
+physgpu_status copy_pairs_to_host(physgpu_context* ctx, physgpu_pair* out, uint64_t n)
+{
+    if (!out || n == 0)
+        return PHYSGPU_OK;
+    std::vector<uint64_t> tmp(n);
+    PG_TRY_HIP(hipMemcpyAsync(tmp.data(), ctx->pairs, n * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t k = 0; k < n; ++k) {
+        out[k].first = uint32_t(tmp[k] >> 32);
+        out[k].second = uint32_t(tmp[k]);
+    }
+    return PHYSGPU_OK;
+}
+
+}  // namespace
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+
+const char* physgpu_last_error(void) { return g_error.c_str(); }
+
+int physgpu_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return 0;
+    return n;
+}
+
+physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_colliders,
+                              uint64_t max_pairs, uint32_t max_hull_points)
+{
+    if (!out)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (max_colliders == 0 || max_colliders > (1u << 30))
+        return fail(PHYSGPU_INVALID_ARGUMENT, "max_colliders must be in [1, 2^30]");
+    if (max_pairs == 0 || max_pairs > 0xFFFFFFFFull)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "max_pairs must be in [1, 2^32)");
+    int count = physgpu_device_count();
+    if (device < 0 || device >= count)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "no such HIP device");
+    PG_TRY_HIP(hipSetDevice(device));
+    physgpu_context* c = new (std::nothrow) physgpu_context();
+    if (!c)
+        return fail(PHYSGPU_DEVICE_ERROR, "out of host memory");
+    c->device = device;
+    c->cap_n = max_colliders;
+    c->cap_p = max_pairs;
+    c->cap_h = max_hull_points;
+    uint32_t k = 2;
+    while (k < 9 && (1ull << (3 * k)) < 2ull * max_colliders)
+        ++k;
+    c->table_bits = 3 * k;
+    const size_t N = max_colliders, P = max_pairs;
+    const size_t tiles = std::max<size_t>(div_up(N, kSortTile), div_up(P, kSortTile));
+    c->hist_capacity = uint32_t(256 * tiles);
+    hipError_t e = hipSuccess;
+    auto A = [&](hipError_t r) {
+        if (e == hipSuccess)
+            e = r;
+    };
+    A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->owns_stream = true;
+    A(alloc(c->pose, 7 * N));
+    A(alloc(c->world_body_pose, 7 * N));
+    A(alloc(c->shape_type, N));
+    A(alloc(c->shape_data, 3 * N));
+    A(alloc(c->hull_range, 2 * N));
+    A(alloc(c->hull_points, 3 * size_t(std::max<uint32_t>(max_hull_points, 1))));
+    A(alloc(c->aabb, 6 * N));
+    A(alloc(c->sorted_aabb, 6 * N));
+    A(alloc(c->sorted_index, N));
+    A(alloc(c->items_a, N));
+    A(alloc(c->items_b, N));
+    A(alloc(c->histogram, c->hist_capacity));
+    A(alloc(c->cell_table, 2 * (size_t(1) << c->table_bits)));
+    A(alloc(c->overflow_list, N));
+    A(alloc(c->grid_flag, N));
+    A(alloc(c->node_left, N));
+    A(alloc(c->node_right, N));
+    A(alloc(c->node_parent, 2 * N));
+    A(alloc(c->node_last, N));
+    A(alloc(c->node_box, 6 * N));
+    A(alloc(c->node_flag, N));
+    A(alloc(c->pairs_raw, P));
+    A(alloc(c->pairs_tmp, P));
+    A(alloc(c->buckets, kClasses * P));
+    A(alloc(c->slots, P));
+    A(alloc(c->hit, P));
+    A(alloc(c->tile_sums, div_up(P, kScanTile) + 1));
+    A(alloc(c->manifolds, P));
+    A(alloc(c->body, N));
+    A(alloc(c->local_pose, 7 * N));
+    A(alloc(c->friction, N));
+    A(alloc(c->restitution, N));
+    A(alloc(c->counters, 1));
+    A(hipHostMalloc(reinterpret_cast<void**>(&c->host_counters), sizeof(Counters)));
+    if (e != hipSuccess) {
+        free_all(c);
+        delete c;
+        return check_hip(e, "physgpu_create allocation");
+    }
+    std::memset(c->host_counters, 0, sizeof(Counters));
+    c->pairs = c->pairs_raw;
+    launch_step_begin(c);
+    PG_TRY_HIP(hipStreamSynchronize(c->stream));
+    c->np_options = {64, 64, 1e-6f, 0};
+    *out = c;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_destroy(physgpu_context* ctx)
+{
+    if (!ctx)
+        return PHYSGPU_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream)
+        hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_set_stream(physgpu_context* ctx, void* hip_stream)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->owns_stream)
+        hipStreamDestroy(ctx->stream);
+    ctx->stream = static_cast<hipStream_t>(hip_stream);
+    ctx->owns_stream = false;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_set_timing(physgpu_context* ctx, int enabled)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    collect_times(ctx);
+    ctx->timing = enabled != 0;
+    ctx->kernel_times.clear();
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_set_colliders(physgpu_context* ctx, const physgpu_colliders* col,
+                                     physgpu_memory memory)
+{
+    if (!ctx || !col)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or colliders is NULL");
+    const uint32_t n = col->count;
+    if (n > ctx->cap_n)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "more colliders than the context holds");
+    if (n && (!col->pose || !col->shape_type || !col->shape_data))
+        return fail(PHYSGPU_INVALID_ARGUMENT, "pose, shape_type and shape_data are required");
+    if (col->hull_point_count > ctx->cap_h)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "more hull points than the context holds");
+    bool hulls = col->hull_range != nullptr;
+    if (memory == PHYSGPU_HOST) {
+        hulls = false;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint8_t t = col->shape_type[i];
+            if (t > PHYSGPU_HULL)
+                return fail(PHYSGPU_INVALID_ARGUMENT, "unknown shape type");
+            if (t == PHYSGPU_HULL) {
+                hulls = true;
+                if (!col->hull_range || !col->hull_points)
+                    return fail(PHYSGPU_INVALID_ARGUMENT, "hull collider without hull data");
+                const uint64_t first = col->hull_range[i], cnt = col->hull_range[n + i];
+                if (cnt == 0 || first + cnt > col->hull_point_count)
+                    return fail(PHYSGPU_INVALID_ARGUMENT, "hull_range outside hull_points");
+            }
+        }
+    } else if (memory != PHYSGPU_DEVICE) {
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown memory kind");
+    }
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    const hipMemcpyKind kind =
+        memory == PHYSGPU_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    if (n) {
+        PG_TRY_HIP(hipMemcpyAsync(ctx->pose, col->pose, 7ull * n * 4, kind, ctx->stream));
+        PG_TRY_HIP(hipMemcpyAsync(ctx->shape_type, col->shape_type, n, kind, ctx->stream));
+        PG_TRY_HIP(hipMemcpyAsync(ctx->shape_data, col->shape_data, 3ull * n * 4, kind, ctx->stream));
+        if (col->hull_range)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->hull_range, col->hull_range, 2ull * n * 4, kind,
+                                      ctx->stream));
+        if (col->hull_points && col->hull_point_count)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->hull_points, col->hull_points,
+                                      3ull * col->hull_point_count * 4, kind, ctx->stream));
+    }
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->n = n;
+    ctx->n_hull_points = col->hull_point_count;
+    ctx->has_hulls = hulls;
+    ctx->colliders_valid = true;
+    ctx->has_world = false;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_compute_aabbs(physgpu_context* ctx, physgpu_aabb* out)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ctx->colliders_valid)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "no colliders set");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    launch_compute_aabbs(ctx);
+    if (out && ctx->n) {
+        physgpu_aabb* staging = reinterpret_cast<physgpu_aabb*>(ctx->sorted_aabb);
+        launch_convert_aabbs_out(ctx, staging);
+        PG_TRY_HIP(hipMemcpyAsync(out, staging, sizeof(physgpu_aabb) * ctx->n,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    }
+    PG_TRY_HIP(hipGetLastError());
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_find_candidate_pairs(physgpu_context* ctx, const physgpu_aabb* bounds,
+                                            uint32_t n, int algorithm, double grid_cell_size,
+                                            physgpu_pair* out, uint64_t capacity, uint64_t* count,
+                                            physgpu_broadphase_stats* stats)
+{
+    if (!ctx || !count)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or count is NULL");
+    if (algorithm < PHYSGPU_SWEEP_AND_PRUNE || algorithm > PHYSGPU_DYNAMIC_TREE)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown broadphase algorithm");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    if (bounds) {
+        if (n > ctx->cap_n)
+            return fail(PHYSGPU_CAPACITY_EXCEEDED, "more bounds than the context holds");
+        for (uint32_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a)
+                if (!(bounds[i].min[a] <= bounds[i].max[a]) ||
+                    !(bounds[i].min[a] >= -3.402823466e+38f) ||
+                    !(bounds[i].max[a] <= 3.402823466e+38f))
+                    return fail(PHYSGPU_INVALID_ARGUMENT, "bounds must be finite and ordered");
+        ctx->n = n;
+        ctx->colliders_valid = false;
+        ctx->has_world = false;
+        if (n) {
+            physgpu_aabb* staging = reinterpret_cast<physgpu_aabb*>(ctx->sorted_aabb);
+            PG_TRY_HIP(hipMemcpyAsync(staging, bounds, sizeof(physgpu_aabb) * n,
+                                      hipMemcpyHostToDevice, ctx->stream));
+            launch_convert_aabbs_in(ctx, staging);
+        }
+    } else if (n != ctx->n) {
+        return fail(PHYSGPU_INVALID_ARGUMENT, "n must equal the resident collider count");
+    }
+    launch_step_begin(ctx);
+    run_broadphase(ctx, algorithm, grid_cell_size, stats != nullptr);
+    PG_TRY_HIP(hipGetLastError());
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const Counters& c = *ctx->host_counters;
+    *count = c.pair_count;
+    if (stats)
+        fill_broad_stats(ctx, stats);
+    if (c.tree_overflow)
+        return fail(PHYSGPU_DEVICE_ERROR, "LBVH traversal stack overflow");
+    if (c.pair_count > ctx->cap_p)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "candidate pairs exceed the context capacity");
+    st = copy_pairs_to_host(ctx, out, std::min<uint64_t>(capacity, c.pair_count));
+    if (st != PHYSGPU_OK)
+        return st;
+    if (out && capacity < c.pair_count)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "output capacity below the pair count");
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_generate_contacts(physgpu_context* ctx, const physgpu_pair* pairs,
+                                         uint64_t n_pairs,
+                                         const physgpu_narrowphase_options* options,
+                                         physgpu_manifold* out, uint64_t capacity,
+                                         uint64_t* count, physgpu_narrowphase_stats* stats)
+{
+    if (!ctx || !count)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or count is NULL");
+    if (!ctx->colliders_valid)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "no colliders set");
+    physgpu_status st = apply_options(ctx, options);
+    if (st != PHYSGPU_OK)
+        return st;
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    if (pairs) {
+        if (n_pairs > ctx->cap_p)
+            return fail(PHYSGPU_CAPACITY_EXCEEDED, "more pairs than the context holds");
+        for (uint64_t k = 0; k < n_pairs; ++k)
+            if (pairs[k].first >= ctx->n || pairs[k].second >= ctx->n)
+                return fail(PHYSGPU_INVALID_ARGUMENT, "pair index out of range");
+        launch_step_begin(ctx);
+        if (n_pairs)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->pairs_raw, pairs, n_pairs * sizeof(physgpu_pair),
+                                      hipMemcpyHostToDevice, ctx->stream));
+        launch_pairs_in(ctx, nullptr, n_pairs);
+    } else {
+        launch_np_begin(ctx);
+    }
+    launch_narrowphase(ctx, false);
+    PG_TRY_HIP(hipGetLastError());
+    st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const Counters& c = *ctx->host_counters;
+    *count = c.manifold_count;
+    if (stats)
+        fill_np_stats(c, stats);
+    if (out && c.manifold_count) {
+        const uint64_t m = std::min<uint64_t>(capacity, c.manifold_count);
+        PG_TRY_HIP(hipMemcpyAsync(out, ctx->manifolds, m * sizeof(physgpu_manifold),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+        if (m < c.manifold_count)
+            return fail(PHYSGPU_CAPACITY_EXCEEDED, "output capacity below the manifold count");
+    }
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_collide(physgpu_context* ctx, int algorithm, double grid_cell_size,
+                               const physgpu_narrowphase_options* options, uint64_t* pair_count,
+                               uint64_t* manifold_count)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ctx->colliders_valid)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "no colliders set");
+    if (algorithm < PHYSGPU_SWEEP_AND_PRUNE || algorithm > PHYSGPU_DYNAMIC_TREE)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown broadphase algorithm");
+    physgpu_status st = apply_options(ctx, options);
+    if (st != PHYSGPU_OK)
+        return st;
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    launch_step_begin(ctx);
+    if (ctx->has_world)
+        launch_compose_world(ctx);
+    launch_compute_aabbs(ctx);
+    run_broadphase(ctx, algorithm, grid_cell_size, false);
+    launch_narrowphase(ctx, ctx->has_world);
+    PG_TRY_HIP(hipGetLastError());
+    if (!pair_count && !manifold_count)
+        return PHYSGPU_OK;
+    st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const Counters& c = *ctx->host_counters;
+    if (pair_count)
+        *pair_count = c.pair_count;
+    if (manifold_count)
+        *manifold_count = c.manifold_count;
+    if (c.tree_overflow)
+        return fail(PHYSGPU_DEVICE_ERROR, "LBVH traversal stack overflow");
+    if (c.pair_count > ctx->cap_p)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "candidate pairs exceed the context capacity");
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_set_world(physgpu_context* ctx, const float* body_pose,
+                                 const physgpu_world_extras* extras, physgpu_memory memory)
+{
+    if (!ctx || !body_pose)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or body_pose is NULL");
+    if (!ctx->colliders_valid)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "set colliders first");
+    const uint32_t n = ctx->n;
+    if (memory == PHYSGPU_HOST && extras && extras->body)
+        for (uint32_t i = 0; i < n; ++i)
+            if (extras->body[i] >= n)
+                return fail(PHYSGPU_INVALID_ARGUMENT, "body index out of range");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    const hipMemcpyKind kind =
+        memory == PHYSGPU_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    if (n) {
+        PG_TRY_HIP(hipMemcpyAsync(ctx->world_body_pose, body_pose, 7ull * n * 4, kind, ctx->stream));
+        const bool has_local = extras && extras->local_pose;
+        if (has_local)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->local_pose, extras->local_pose, 7ull * n * 4, kind,
+                                      ctx->stream));
+        if (extras && extras->body)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->body, extras->body, 4ull * n, kind, ctx->stream));
+        if (extras && extras->friction)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->friction, extras->friction, 4ull * n, kind, ctx->stream));
+        if (extras && extras->restitution)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->restitution, extras->restitution, 4ull * n, kind,
+                                      ctx->stream));
+        ctx->world_has_local = has_local;
+        ctx->world_has_body = extras && extras->body;
+        ctx->world_has_friction = extras && extras->friction;
+        ctx->world_has_restitution = extras && extras->restitution;
+    }
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->has_world = true;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_download_pairs(physgpu_context* ctx, physgpu_pair* out, uint64_t capacity)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const uint64_t held = ctx->host_counters->sort_count;
+    st = copy_pairs_to_host(ctx, out, std::min<uint64_t>(capacity, held));
+    if (st != PHYSGPU_OK)
+        return st;
+    if (capacity < ctx->host_counters->pair_count)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "capacity below the pair count");
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_download_manifolds(physgpu_context* ctx, physgpu_manifold* out,
+                                          uint64_t capacity)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const uint64_t m = std::min<uint64_t>(capacity, ctx->host_counters->manifold_count);
+    if (out && m) {
+        PG_TRY_HIP(hipMemcpyAsync(out, ctx->manifolds, m * sizeof(physgpu_manifold),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    if (capacity < ctx->host_counters->manifold_count)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "capacity below the manifold count");
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_step_counts(physgpu_context* ctx, uint64_t* pair_count,
+                                   uint64_t* manifold_count)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    if (pair_count)
+        *pair_count = ctx->host_counters->pair_count;
+    if (manifold_count)
+        *manifold_count = ctx->host_counters->manifold_count;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_narrowphase_stats_get(physgpu_context* ctx, physgpu_narrowphase_stats* s)
+{
+    if (!ctx || !s)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or stats is NULL");
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    fill_np_stats(*ctx->host_counters, s);
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_kernel_times(physgpu_context* ctx, const char** names, double* avg_ms,
+                                    uint64_t* launches, uint32_t capacity, uint32_t* count)
+{
+    if (!ctx || !count)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or count is NULL");
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    collect_times(ctx);
+    *count = uint32_t(ctx->kernel_times.size());
+    for (uint32_t i = 0; i < capacity && i < ctx->kernel_times.size(); ++i) {
+        const KernelTime& k = ctx->kernel_times[i];
+        if (names)
+            names[i] = k.name;
+        if (avg_ms)
+            avg_ms[i] = k.launches ? k.total_ms / double(k.launches) : 0.0;
+        if (launches)
+            launches[i] = k.launches;
+    }
+    return PHYSGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+// Uniform-grid broad phase (same pair set as findGridPairs,
+// src/collision/broadphase.cpp:90-182: every inclusive AABB overlap, once).
+//
+// The reference inserts each AABB into up to 64 cells and de-duplicates with
+// sort + unique.  The gfx950 form inserts each body once, into the cell of its
+// AABB's min corner, and queries the 27 neighbour cells:
+//   * cell side c >= every in-grid body's largest side (widened by 2^-9), so
+//     two overlapping in-grid bodies have min corners less than one cell
+//     apart on every axis and their home cells are neighbours;
+//   * bodies larger than the cell (or with cell coordinates out of range) go
+//     to an overflow list and are tested against every body, as the
+//     reference's overflow path does (broadphase.cpp:162-166);
+//   * home-cell keys are 3-D Morton codes of the cell coordinates wrapped to
+//     k bits per axis (table of 2^3k entries, 2^3k >= 2N): spatially close
+//     cells are close in memory; far cells that wrap onto the same key only
+//     add AABB tests, never change the result;
+//   * a pair is emitted by the query of its lower original index, so each
+//     unordered pair appears once (the 27 neighbour keys are distinct for
+//     k >= 2, and every body sits in exactly one cell).
+// Cell coordinates use floor(min / c) in double precision, a monotone map.
+// Automatic cell: the largest AABB side, capped at twice the mean side (the
+// rest overflow), widened by 2^-8 (in-grid bodies must be at most 1 - 2^-9
+// of the cell).
+#include "devmath.hpp"
+#include "internal.hpp"
+#include "wave.hpp"
+
+namespace pg {
+namespace {
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t cell_key(int32_t x, int32_t y, int32_t z, uint32_t mask)
+{
+    return spread3(uint32_t(x) & mask) | (spread3(uint32_t(y) & mask) << 1) |
+           (spread3(uint32_t(z) & mask) << 2);
+}
+
+// Cell side for this step: explicit (> 0) or automatic from the reductions.
+__device__ __forceinline__ float step_cell(const Counters* c, float requested, uint32_t n)
+{
+    if (requested > 0.0f)
+        return requested;
+    const float wmax = key_float(c->wmax_key);
+    const double mean = n ? double(c->width_sum_fx) / 65536.0 / double(n) : 1.0;
+    float cell = fminf(wmax, float(2.0 * mean));
+    if (!(cell > 0.0f))
+        cell = 1.0f;
+    return cell * (1.0f + 1.0f / 256.0f);
+}
+
+// Home cell of a body; false when it must use the overflow path.
+__device__ __forceinline__ bool home_cell(float x, float y, float z, double inv, int32_t cell[3])
+{
+    const double v[3] = {floor(double(x) * inv), floor(double(y) * inv), floor(double(z) * inv)};
+    for (int a = 0; a < 3; ++a) {
+        if (!(v[a] > -1073741824.0 && v[a] < 1073741824.0))
+            return false;
+        cell[a] = int32_t(v[a]);
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float* __restrict__ aabb,
+                                                      Counters* counters, float requested,
+                                                      uint32_t bits_per_axis,
+                                                      uint64_t* __restrict__ items,
+                                                      uint32_t* __restrict__ overflow,
+                                                      uint32_t* __restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float cell = step_cell(counters, requested, n);
+    if (i == 0)
+        counters->cell_bits = __float_as_uint(cell);
+    bool in_grid = false;
+    uint32_t key = 1u << (3 * bits_per_axis);  // sorts after every cell key
+    if (i < n) {
+        const float lo[3] = {aabb[i], aabb[n + i], aabb[2 * n + i]};
+        const float hi[3] = {aabb[3 * n + i], aabb[4 * n + i], aabb[5 * n + i]};
+        const float limit = cell * (1.0f - 1.0f / 512.0f);
+        const bool fits = (hi[0] - lo[0]) <= limit && (hi[1] - lo[1]) <= limit &&
+                          (hi[2] - lo[2]) <= limit;
+        int32_t c[3];
+        if (fits && home_cell(lo[0], lo[1], lo[2], 1.0 / double(cell), c)) {
+            in_grid = true;
+            key = cell_key(c[0], c[1], c[2], (1u << bits_per_axis) - 1u);
+        }
+        items[i] = (uint64_t)key | ((uint64_t)i << 32);
+        flag[i] = in_grid ? 1u : 0u;
+    }
+    // Overflow list and entry count, one atomic per wave each.
+    const bool over = i < n && !in_grid;
+    const uint64_t om = __ballot(over);
+    const uint64_t gm = __ballot(in_grid);
+    const uint32_t lane = lane_id();
+    if (om) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)om) - 1u;
+        uint32_t base = 0;
+        if (lane == leader)
+            base = atomicAdd(&counters->overflow_count, (uint32_t)__popcll(om));
+        base = __shfl(base, (int)leader);
+        if (over)
+            overflow[base + __popcll(om & lanes_below(lane))] = i;
+    }
+    if (gm && lane == (uint32_t)__ffsll((unsigned long long)gm) - 1u)
+        atomicAdd(&counters->grid_entries, (uint32_t)__popcll(gm));
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(uint32_t* __restrict__ p, uint64_t count,
+                                                 uint32_t value)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// Cell ranges over the sorted in-grid prefix [0, entries).
+__global__ __launch_bounds__(kBlock) void k_cell_ranges(uint32_t n, const uint64_t* __restrict__ items,
+                                                        const Counters* counters,
+                                                        uint32_t table_size,
+                                                        uint32_t* __restrict__ table)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t entries = counters->grid_entries;
+    if (p >= entries)
+        return;
+    const uint32_t key = uint32_t(items[p]);
+    if (p == 0 || uint32_t(items[p - 1]) != key)
+        table[key] = p;
+    if (p + 1 == entries || uint32_t(items[p + 1]) != key)
+        table[table_size + key] = p + 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float* __restrict__ s,
+                                                       const uint32_t* __restrict__ sidx,
+                                                       const uint32_t* __restrict__ table,
+                                                       uint32_t table_size, uint32_t bits_per_axis,
+                                                       Counters* counters,
+                                                       uint64_t* __restrict__ out, uint64_t cap)
+{
+    const uint32_t entries = counters->grid_entries;
+    const float cell = __uint_as_float(counters->cell_bits);
+    const double inv = 1.0 / double(cell);
+    const uint32_t mask = (1u << bits_per_axis) - 1u;
+    const uint32_t* start = table;
+    const uint32_t* end = table + table_size;
+    unsigned long long tests = 0;
+    // Grid-stride over sorted positions; all lanes stay in the loop together
+    // so that the wave-wide append is executed convergently.
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (entries + stride - 1) / stride;
+    for (uint32_t round = 0; round < rounds; ++round) {
+        const uint32_t p = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const bool active = p < entries;
+        float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        uint32_t a = 0;
+        int32_t c[3] = {0, 0, 0};
+        if (active) {
+            for (int ax = 0; ax < 3; ++ax) {
+                lo[ax] = s[ax * (size_t)n + p];
+                hi[ax] = s[(3 + ax) * (size_t)n + p];
+            }
+            a = sidx[p];
+            home_cell(lo[0], lo[1], lo[2], inv, c);
+        }
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    uint32_t q = 0, qe = 0;
+                    if (active) {
+                        const uint32_t key = cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask);
+                        q = start[key];
+                        qe = end[key];
+                        if (qe <= q)
+                            q = qe = 0;  // empty (table cleared to zero)
+                    }
+                    // Lanes walk their own cells; the wave iterates until all are done.
+                    for (;;) {
+                        const bool more = q < qe;
+                        if (__ballot(more) == 0)
+                            break;
+                        bool hit = false;
+                        uint32_t b = 0;
+                        if (more) {
+                            b = sidx[q];
+                            if (a < b) {
+                                ++tests;
+                                hit = lo[0] <= s[3 * (size_t)n + q] && hi[0] >= s[q] &&
+                                      lo[1] <= s[4 * (size_t)n + q] && hi[1] >= s[(size_t)n + q] &&
+                                      lo[2] <= s[5 * (size_t)n + q] && hi[2] >= s[2 * (size_t)n + q];
+                            }
+                            ++q;
+                        }
+                        wave_append(hit, pair_value(a, b), out, &counters->pair_count, cap);
+                    }
+                }
+    }
+    tests = wave_sum(tests);
+    if (lane_id() == 0)
+        atomicAdd(&counters->grid_tests, tests);
+}
+
+// Overflow bodies against every body (broadphase.cpp:163-166): pair (o, j)
+// is tested when j != o and (j is in the grid or o < j).
+__global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const float* __restrict__ aabb,
+                                                          const uint32_t* __restrict__ overflow,
+                                                          const uint32_t* __restrict__ flag,
+                                                          Counters* counters,
+                                                          uint64_t* __restrict__ out, uint64_t cap)
+{
+    const uint32_t count = counters->overflow_count;
+    unsigned long long tests = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    for (uint32_t k = blockIdx.y; k < count; k += gridDim.y) {
+        const uint32_t o = overflow[k];
+        const float olo[3] = {aabb[o], aabb[n + o], aabb[2 * n + o]};
+        const float ohi[3] = {aabb[3 * n + o], aabb[4 * n + o], aabb[5 * n + o]};
+        for (uint32_t round = 0; round < rounds; ++round) {
+            const uint32_t j = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
+            bool hit = false;
+            if (j < n && j != o && (flag[j] || o < j)) {
+                ++tests;
+                hit = olo[0] <= aabb[3 * n + j] && ohi[0] >= aabb[j] &&
+                      olo[1] <= aabb[4 * n + j] && ohi[1] >= aabb[n + j] &&
+                      olo[2] <= aabb[5 * n + j] && ohi[2] >= aabb[2 * n + j];
+            }
+            wave_append(hit, pair_value(o, j), out, &counters->pair_count, cap);
+        }
+    }
+    tests = wave_sum(tests);
+    if (lane_id() == 0 && tests)
+        atomicAdd(&counters->grid_tests, tests);
+}
+
+}  // namespace
+
+void launch_grid(physgpu_context* ctx, double cell_size)
+{
+    const uint32_t n = ctx->n;
+    const uint32_t bpa = ctx->table_bits / 3;
+    const uint32_t table_size = 1u << ctx->table_bits;
+    Timer t;
+    time_begin(ctx, "grid_keys", t);
+    hipLaunchKernelGGL(k_grid_keys, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
+                       ctx->aabb, ctx->counters, float(cell_size), bpa, ctx->items_a,
+                       ctx->overflow_list, ctx->grid_flag);
+    time_end(ctx, t);
+    uint64_t* sorted = nullptr;
+    radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 3 * bpa + 1, 0, 0, &sorted);
+    time_begin(ctx, "grid_table_clear", t);
+    hipLaunchKernelGGL(k_fill, dim3(stride_grid(2ull * table_size)), dim3(kBlock), 0, ctx->stream,
+                       ctx->cell_table, 2ull * table_size, 0u);
+    time_end(ctx, t);
+    time_begin(ctx, "grid_cell_ranges", t);
+    hipLaunchKernelGGL(k_cell_ranges, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
+                       sorted, ctx->counters, table_size, ctx->cell_table);
+    time_end(ctx, t);
+    launch_gather_sorted(ctx, sorted);
+    time_begin(ctx, "grid_query", t);
+    hipLaunchKernelGGL(k_grid_query, dim3(stride_grid(n)), dim3(kBlock), 0, ctx->stream, n,
+                       ctx->sorted_aabb, ctx->sorted_index, ctx->cell_table, table_size, bpa,
+                       ctx->counters, ctx->pairs_raw, ctx->cap_p);
+    time_end(ctx, t);
+    time_begin(ctx, "grid_overflow", t);
+    hipLaunchKernelGGL(k_grid_overflow, dim3(stride_grid(n), 16), dim3(kBlock), 0, ctx->stream, n,
+                       ctx->aabb, ctx->overflow_list, ctx->grid_flag, ctx->counters,
+                       ctx->pairs_raw, ctx->cap_p);
+    time_end(ctx, t);
+}
+
+}  // namespace pg

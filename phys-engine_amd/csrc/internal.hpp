@@ -1,0 +1,213 @@
+// Internal context of the collision library: device buffers (SoA planes in
+// HBM), device counters, launch and timing helpers.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/physgpu.h"
+
+namespace pg {
+
+constexpr int kClasses = 6;        // physgpu_shape_pair values
+constexpr uint32_t kBlock = 256;   // threads per workgroup for streaming kernels
+constexpr uint32_t kStrideGrid = 2048;  // workgroups of grid-stride kernels (8 per CU)
+
+// Radix sort tiling: 256 threads x 8 items per workgroup.
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortItems = 8;
+constexpr uint32_t kSortTile = kSortThreads * kSortItems;
+
+// Compaction tiling.
+constexpr uint32_t kScanTile = 2048;
+
+// Device-side counters of one collision step.  Zeroed by the step-begin
+// kernel; read back by the host only when it asks for counts.
+struct Counters {
+    unsigned long long pair_count;      // pairs emitted by the broad phase (may exceed capacity)
+    unsigned long long window_tests;    // SAP x-window comparisons
+    unsigned long long grid_tests;      // grid AABB tests (incl. overflow tests)
+    unsigned long long tree_visits;     // LBVH node visits
+    unsigned long long tree_leaf_checks;
+    unsigned long long width_sum_fx;    // sum of max AABB sides, 2^-16 fixed point
+    uint32_t grid_entries;              // bodies placed in a cell
+    uint32_t overflow_count;            // grid bodies tested against everything
+    uint32_t wmax_x_key;                // float_key of the largest x side
+    uint32_t wmax_key;                  // float_key of the largest side over all axes
+    uint32_t cmin_key[3];               // float_key bounds of AABB centres
+    uint32_t cmax_key[3];
+    uint32_t class_count[kClasses];     // candidates per shape-pair class
+    uint32_t class_contacts[kClasses];  // contacts per class
+    uint32_t gjk_failures;              // GJK hit, EPA did not converge
+    uint32_t epa_overflow;              // EPA exceeded its bounded polytope
+    uint32_t manifold_count;
+    uint32_t tree_height;
+    uint32_t sort_count;                // min(pair_count, capacity): pairs held
+    uint32_t cell_bits;                 // grid cell size used (float bits)
+    uint32_t tree_overflow;             // LBVH traversal stack overflows (must stay 0)
+    uint32_t pad;
+};
+
+struct Timer {
+    const char* name;
+    hipEvent_t start, stop;
+};
+
+struct KernelTime {
+    const char* name;
+    double total_ms;
+    uint64_t launches;
+};
+
+}  // namespace pg
+
+struct physgpu_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+    uint32_t cap_n = 0;
+    uint64_t cap_p = 0;
+    uint32_t cap_h = 0;
+
+    // Colliders (SoA planes).
+    uint32_t n = 0;
+    uint32_t n_hull_points = 0;
+    float* pose = nullptr;        // 7 x cap_n
+    uint8_t* shape_type = nullptr;
+    float* shape_data = nullptr;  // 3 x cap_n
+    uint32_t* hull_range = nullptr;  // 2 x cap_n
+    float* hull_points = nullptr;    // 3 x cap_h
+    bool has_hulls = false;
+    bool colliders_valid = false;
+
+    // AABBs (6 planes: min x y z, max x y z) and their sorted copy.
+    float* aabb = nullptr;
+    float* sorted_aabb = nullptr;
+    uint32_t* sorted_index = nullptr;
+
+    // Radix sort scratch (u64 items).
+    uint64_t* items_a = nullptr;
+    uint64_t* items_b = nullptr;
+    uint32_t* histogram = nullptr;  // 256 x tiles
+    uint32_t hist_capacity = 0;
+
+    // Grid.
+    uint32_t* cell_table = nullptr;  // 2 x table_size (start, end)
+    uint32_t table_bits = 0;
+    uint32_t* overflow_list = nullptr;
+    uint32_t* grid_flag = nullptr;   // 1 when the body sits in a cell
+
+    // LBVH: internal nodes [0, n-1), leaves by sorted position.
+    int32_t* node_left = nullptr;
+    int32_t* node_right = nullptr;
+    int32_t* node_parent = nullptr;   // internal parents (n-1) then leaf parents (n)
+    uint32_t* node_last = nullptr;    // last leaf position covered by an internal node
+    float* node_box = nullptr;        // 6 x (n-1) internal node bounds
+    uint32_t* node_flag = nullptr;    // refit arrival flags
+
+    // Pairs as first << 32 | second: raw emission buffer, sort scratch, and
+    // the canonical (sorted) list, which aliases one of the two.
+    uint64_t* pairs_raw = nullptr;
+    uint64_t* pairs_tmp = nullptr;
+    uint64_t* pairs = nullptr;
+
+    // Narrow phase.
+    uint32_t* buckets = nullptr;       // kClasses x cap_p candidate indices
+    physgpu_manifold* slots = nullptr; // one record per candidate
+    uint32_t* hit = nullptr;           // per candidate
+    uint32_t* tile_sums = nullptr;     // compaction
+    physgpu_manifold* manifolds = nullptr;  // compacted, candidate order
+    physgpu_narrowphase_options np_options{};
+
+    // World-level extras (optional): per collider, the pose of its body, its
+    // body index, its local transform and its body's materials.
+    float* world_body_pose = nullptr;  // 7 x cap_n
+    uint32_t* body = nullptr;
+    float* local_pose = nullptr;
+    float* friction = nullptr;
+    float* restitution = nullptr;
+    bool has_world = false;
+    bool world_has_local = false;
+    bool world_has_body = false;
+    bool world_has_friction = false;
+    bool world_has_restitution = false;
+
+    // Counters (device) and a pinned host mirror.
+    pg::Counters* counters = nullptr;
+    pg::Counters* host_counters = nullptr;
+
+    // Last step.
+    int last_algorithm = -1;
+    double last_cell_size = 0.0;
+
+    // Timing.
+    bool timing = false;
+    std::vector<pg::Timer> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<pg::KernelTime> kernel_times;
+};
+
+namespace pg {
+
+// Error helpers (thread-local message behind physgpu_last_error).
+void set_error(const std::string& msg);
+physgpu_status fail(physgpu_status st, const std::string& msg);
+physgpu_status check_hip(hipError_t e, const char* what);
+
+#define PG_TRY_HIP(expr)                                               \
+    do {                                                               \
+        hipError_t e_ = (expr);                                        \
+        if (e_ != hipSuccess)                                          \
+            return ::pg::check_hip(e_, #expr);                         \
+    } while (0)
+
+// Timing brackets around one launch (no-ops unless ctx->timing).
+void time_begin(physgpu_context* ctx, const char* name, Timer& t);
+void time_end(physgpu_context* ctx, Timer& t);
+void collect_times(physgpu_context* ctx);
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
+inline uint32_t stride_grid(uint64_t work)
+{
+    uint64_t g = (work + kBlock - 1) / kBlock;
+    if (g < 1)
+        g = 1;
+    return uint32_t(g < kStrideGrid ? g : kStrideGrid);
+}
+
+// ---- launchers implemented in the kernel translation units -------------
+// aabb.hip
+void launch_step_begin(physgpu_context* ctx);
+void launch_compute_aabbs(physgpu_context* ctx);
+void launch_aabb_reductions(physgpu_context* ctx);
+void launch_convert_aabbs_out(physgpu_context* ctx, physgpu_aabb* dev_out);
+void launch_convert_aabbs_in(physgpu_context* ctx, const physgpu_aabb* dev_in);
+void launch_compose_world(physgpu_context* ctx);
+// radix_sort.hip: stable sort of n u64 items by `bits` bits starting at bit
+// `shift0` (and, when bits1 > 0, then by `bits1` bits at `shift1`, which is the
+// more significant field).  Result ends in *result (one of a, b).
+void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
+                const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
+                uint32_t bits1, uint64_t** result);
+// sap.hip
+void launch_sap_keys(physgpu_context* ctx);
+void launch_gather_sorted(physgpu_context* ctx, const uint64_t* sorted_items);
+void launch_sap_sweep(physgpu_context* ctx);
+void launch_sap_window_tests(physgpu_context* ctx);
+// grid.hip
+void launch_grid(physgpu_context* ctx, double cell_size);
+// lbvh.hip
+void launch_lbvh(physgpu_context* ctx);
+// pairs.hip
+void launch_canonical_pairs(physgpu_context* ctx);
+void launch_pairs_out(physgpu_context* ctx, physgpu_pair* dev_out, uint64_t count);
+void launch_pairs_in(physgpu_context* ctx, const physgpu_pair* dev_in, uint64_t count);
+// narrowphase.hip
+void launch_np_begin(physgpu_context* ctx);
+void launch_narrowphase(physgpu_context* ctx, bool world);
+
+}  // namespace pg

@@ -1,0 +1,349 @@
+"""Host-side mirror of phys-engine's collision interfaces over libphysgpu.so.
+
+Names and argument meaning follow the reference:
+  BroadPhaseAlgorithm                 include/phys/collision/broadphase.hpp:8-13
+  BroadPhase.find_candidate_pairs     broadphase.hpp:52-54 (raises ValueError for
+                                      DynamicTree, as the reference throws
+                                      std::invalid_argument, broadphase.cpp:200-201)
+  NarrowPhase.generate_contacts       narrowphase.hpp:36-39, batched over pairs
+  CollisionPipeline.collide           the collision section of PhysicsWorld::step
+                                      (physics_world.cpp:539-739), device resident
+All work runs in the HIP library; when it cannot be loaded every call raises.
+"""
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import MANIFOLD_DTYPE
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphysgpu.so")
+_lib = None
+
+
+class PhysGpuError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"physgpu status {status}: {message}")
+        self.status = status
+
+
+class CapacityError(PhysGpuError):
+    pass
+
+
+class BroadPhaseAlgorithm(enum.IntEnum):
+    SweepAndPrune = abi.SWEEP_AND_PRUNE
+    UniformGrid = abi.UNIFORM_GRID
+    DynamicTree = abi.DYNAMIC_TREE
+
+
+def library():
+    """Load libphysgpu.so (built by phys-engine_amd/build.py); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with phys-engine_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, i32, dbl = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_double
+    L.physgpu_last_error.restype = C.c_char_p
+    L.physgpu_device_count.restype = C.c_int
+    sigs = {
+        "physgpu_create": [C.POINTER(vp), i32, u32, u64, u32],
+        "physgpu_destroy": [vp],
+        "physgpu_set_stream": [vp, vp],
+        "physgpu_set_timing": [vp, i32],
+        "physgpu_set_colliders": [vp, C.POINTER(abi.Colliders), i32],
+        "physgpu_compute_aabbs": [vp, vp],
+        "physgpu_find_candidate_pairs": [vp, vp, u32, i32, dbl, vp, u64, C.POINTER(u64),
+                                         C.POINTER(abi.BroadPhaseStats)],
+        "physgpu_generate_contacts": [vp, vp, u64, C.POINTER(abi.NarrowPhaseOptions), vp, u64,
+                                      C.POINTER(u64), C.POINTER(abi.NarrowPhaseStats)],
+        "physgpu_collide": [vp, i32, dbl, C.POINTER(abi.NarrowPhaseOptions), C.POINTER(u64),
+                            C.POINTER(u64)],
+        "physgpu_set_world": [vp, vp, C.POINTER(abi.WorldExtras), i32],
+        "physgpu_download_pairs": [vp, vp, u64],
+        "physgpu_download_manifolds": [vp, vp, u64],
+        "physgpu_narrowphase_stats_get": [vp, C.POINTER(abi.NarrowPhaseStats)],
+        "physgpu_step_counts": [vp, C.POINTER(u64), C.POINTER(u64)],
+        "physgpu_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(dbl), C.POINTER(u64), u32,
+                                 C.POINTER(u32)],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(status, allow=()):
+    if status == abi.OK or status in allow:
+        return status
+    msg = library().physgpu_last_error().decode(errors="replace")
+    if status == abi.CAPACITY_EXCEEDED:
+        raise CapacityError(status, msg)
+    if status == abi.INVALID_ARGUMENT:
+        raise ValueError(f"physgpu: {msg}")
+    raise PhysGpuError(status, msg)
+
+
+def device_count():
+    return library().physgpu_device_count()
+
+
+def _options(gjk_iterations=64, epa_iterations=64, epa_tolerance=1e-6):
+    o = abi.NarrowPhaseOptions()
+    o.gjk_max_iterations = gjk_iterations
+    o.epa_max_iterations = epa_iterations
+    o.epa_tolerance = epa_tolerance
+    return o
+
+
+def _pairs_array(raw):
+    return raw.view(np.uint32).reshape(-1, 2)
+
+
+class CollisionPipeline:
+    """One device context: resident colliders, AABBs, pairs and manifolds."""
+
+    def __init__(self, max_colliders, max_pairs=None, max_hull_points=0, device=0):
+        L = library()
+        self._ctx = C.c_void_p()
+        max_pairs = max_pairs or max(4 * max_colliders, 1024)
+        _check(L.physgpu_create(C.byref(self._ctx), device, max_colliders, max_pairs,
+                                max_hull_points))
+        self.max_colliders = max_colliders
+        self.max_pairs = max_pairs
+        self.count = 0
+        self._keep = None
+
+    def close(self):
+        if self._ctx:
+            library().physgpu_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def set_stream(self, hip_stream_handle):
+        _check(library().physgpu_set_stream(self._ctx, C.c_void_p(hip_stream_handle)))
+
+    def set_timing(self, enabled=True):
+        _check(library().physgpu_set_timing(self._ctx, int(bool(enabled))))
+
+    def set_colliders(self, scene):
+        c = abi.colliders_struct(scene)
+        self._keep = scene
+        _check(library().physgpu_set_colliders(self._ctx, C.byref(c), abi.HOST))
+        self.count = scene.count
+
+    def set_world(self, scene):
+        """World-level form: body poses, body indices, local poses, materials."""
+        n = scene.count
+        body_pose = np.zeros((7, n), np.float32)
+        body_pose[:3] = scene.body_position
+        body_pose[6] = 1.0
+        ex = abi.WorldExtras()
+        body = getattr(scene, "body", None)
+        if body is None:
+            body = np.arange(n, dtype=np.uint32)
+        keep = [body_pose, np.ascontiguousarray(body, np.uint32)]
+        ex.body = abi.u32ptr(keep[1])
+        ex.local_pose = abi.fptr(scene.local_pose)
+        ex.friction = abi.fptr(scene.friction)
+        ex.restitution = abi.fptr(scene.restitution)
+        _check(library().physgpu_set_world(self._ctx, body_pose.ctypes.data, C.byref(ex),
+                                           abi.HOST))
+        self._world_keep = keep
+
+    def compute_aabbs(self):
+        out = np.zeros((max(self.count, 1), 6), np.float32)
+        _check(library().physgpu_compute_aabbs(self._ctx, out.ctypes.data))
+        return out[:self.count]
+
+    def find_candidate_pairs(self, bounds=None, algorithm=BroadPhaseAlgorithm.SweepAndPrune,
+                             cell_size=0.0, with_stats=False):
+        L = library()
+        count = C.c_uint64(0)
+        stats = abi.BroadPhaseStats()
+        if bounds is not None:
+            bounds = np.ascontiguousarray(bounds, np.float32).reshape(-1, 6)
+            n = bounds.shape[0]
+            bptr = bounds.ctypes.data
+        else:
+            n, bptr = self.count, None
+        out = np.zeros(self.max_pairs, np.uint64)
+        _check(L.physgpu_find_candidate_pairs(self._ctx, bptr, n, int(algorithm), cell_size,
+                                              out.ctypes.data, self.max_pairs, C.byref(count),
+                                              C.byref(stats) if with_stats else None))
+        if bounds is not None:
+            self.count = n
+        pairs = _pairs_array(out[:count.value])
+        return (pairs, stats.as_dict()) if with_stats else pairs
+
+    def generate_contacts(self, pairs=None, gjk_iterations=64, epa_iterations=64,
+                          epa_tolerance=1e-6, with_stats=False):
+        L = library()
+        count = C.c_uint64(0)
+        stats = abi.NarrowPhaseStats()
+        opt = _options(gjk_iterations, epa_iterations, epa_tolerance)
+        if pairs is not None:
+            pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+            pptr, npairs = pairs.ctypes.data, pairs.shape[0]
+            cap = max(npairs, 1)
+        else:
+            pptr, npairs, cap = None, 0, self.max_pairs
+        out = np.zeros(cap, MANIFOLD_DTYPE)
+        _check(L.physgpu_generate_contacts(self._ctx, pptr, npairs, C.byref(opt), out.ctypes.data,
+                                           cap, C.byref(count), C.byref(stats)))
+        m = out[:count.value]
+        return (m, stats.as_dict()) if with_stats else m
+
+    def collide(self, algorithm=BroadPhaseAlgorithm.SweepAndPrune, cell_size=0.0,
+                gjk_iterations=64, epa_iterations=64, epa_tolerance=1e-6, sync=True):
+        """Collision step on the resident colliders; returns (pairs, manifolds)
+        counts when sync, else None (work stays queued on the stream)."""
+        opt = _options(gjk_iterations, epa_iterations, epa_tolerance)
+        if not sync:
+            _check(library().physgpu_collide(self._ctx, int(algorithm), cell_size, C.byref(opt),
+                                             None, None))
+            return None
+        p, m = C.c_uint64(0), C.c_uint64(0)
+        _check(library().physgpu_collide(self._ctx, int(algorithm), cell_size, C.byref(opt),
+                                         C.byref(p), C.byref(m)))
+        return p.value, m.value
+
+    def step_counts(self):
+        """(candidate pairs, manifolds) of the last step."""
+        p, m = C.c_uint64(0), C.c_uint64(0)
+        _check(library().physgpu_step_counts(self._ctx, C.byref(p), C.byref(m)))
+        return p.value, m.value
+
+    def pairs(self):
+        """Canonical candidate pairs of the last broad phase, uint32 [P, 2]."""
+        p, _ = self.step_counts()
+        out = np.zeros(max(p, 1), np.uint64)
+        _check(library().physgpu_download_pairs(self._ctx, out.ctypes.data, p))
+        return _pairs_array(out[:p])
+
+    def manifolds(self):
+        """Manifolds of the last narrow phase, in candidate order."""
+        _, m = self.step_counts()
+        out = np.zeros(max(m, 1), MANIFOLD_DTYPE)
+        _check(library().physgpu_download_manifolds(self._ctx, out.ctypes.data, m))
+        return out[:m]
+
+    def narrowphase_stats(self):
+        stats = abi.NarrowPhaseStats()
+        _check(library().physgpu_narrowphase_stats_get(self._ctx, C.byref(stats)))
+        return stats.as_dict()
+
+    def kernel_times(self):
+        L = library()
+        cap = 64
+        names = (C.c_char_p * cap)()
+        avg = (C.c_double * cap)()
+        launches = (C.c_uint64 * cap)()
+        count = C.c_uint32(0)
+        _check(L.physgpu_kernel_times(self._ctx, names, avg, launches, cap, C.byref(count)))
+        return {names[i].decode(): (avg[i], launches[i]) for i in range(min(count.value, cap))}
+
+
+class BroadPhase:
+    """BroadPhase::findCandidatePairs (broadphase.hpp:52-54) on the GPU."""
+
+    _pipe = None
+
+    @classmethod
+    def _pipeline(cls, n):
+        if cls._pipe is None or cls._pipe.max_colliders < n:
+            if cls._pipe is not None:
+                cls._pipe.close()
+            cls._pipe = CollisionPipeline(max(n, 1024), max(16 * n, 1 << 16))
+        return cls._pipe
+
+    @classmethod
+    def find_candidate_pairs(cls, bounds, stats=None,
+                             algorithm=BroadPhaseAlgorithm.SweepAndPrune):
+        """bounds: float32 [N, 6] (min xyz, max xyz).  Returns uint32 [P, 2]
+        lexicographic pairs; fills `stats` (a dict) when given."""
+        if algorithm == BroadPhaseAlgorithm.DynamicTree:
+            raise ValueError("DynamicTree requires a persistent DynamicAabbTree instance")
+        bounds = np.ascontiguousarray(bounds, np.float32).reshape(-1, 6)
+        pipe = cls._pipeline(bounds.shape[0])
+        pairs, st = pipe.find_candidate_pairs(bounds, algorithm, with_stats=True)
+        if stats is not None:
+            stats.clear()
+            stats.update(st)
+        return pairs
+
+
+class DynamicAabbTree:
+    """Pair query of DynamicAabbTree (dynamic_aabb_tree.hpp:25-31) over an LBVH
+    rebuilt per query from the proxies' tight bounds."""
+
+    def __init__(self):
+        self._bounds = {}
+        self._next = 0
+        self._free = []
+
+    def create_proxy(self, bounds, user_index):
+        b = np.asarray(bounds, np.float32).reshape(6)
+        if not (np.all(np.isfinite(b)) and np.all(b[:3] <= b[3:])):
+            raise ValueError("bounds must be finite and ordered")
+        pid = self._free.pop() if self._free else self._next
+        if pid == self._next:
+            self._next += 1
+        self._bounds[pid] = (b, int(user_index))
+        return pid
+
+    def destroy_proxy(self, proxy):
+        if proxy not in self._bounds:
+            raise ValueError("invalid proxy")
+        del self._bounds[proxy]
+        self._free.append(proxy)
+
+    def update_proxy(self, proxy, bounds):
+        if proxy not in self._bounds:
+            raise ValueError("invalid proxy")
+        b = np.asarray(bounds, np.float32).reshape(6)
+        if not (np.all(np.isfinite(b)) and np.all(b[:3] <= b[3:])):
+            raise ValueError("bounds must be finite and ordered")
+        self._bounds[proxy] = (b, self._bounds[proxy][1])
+        return False
+
+    def find_candidate_pairs(self, stats=None):
+        items = sorted(self._bounds.values(), key=lambda t: t[1])
+        if not items:
+            return np.zeros((0, 2), np.uint32)
+        bounds = np.stack([b for b, _ in items])
+        users = np.array([u for _, u in items], np.uint32)
+        pipe = BroadPhase._pipeline(len(items))
+        pairs, st = pipe.find_candidate_pairs(bounds, BroadPhaseAlgorithm.DynamicTree,
+                                              with_stats=True)
+        if stats is not None:
+            stats.clear()
+            stats.update(st)
+        mapped = np.sort(users[pairs], axis=1) if len(pairs) else pairs
+        order = np.lexsort((mapped[:, 1], mapped[:, 0])) if len(mapped) else []
+        return mapped[order]
+
+    def size(self):
+        return len(self._bounds)

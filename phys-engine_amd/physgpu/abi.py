@@ -31,6 +31,11 @@ PAIR_SPHERE_HULL = 3
 PAIR_BOX_HULL = 4
 PAIR_HULL_HULL = 5
 
+# Bounded EPA polytope of the GPU narrow phase (include/physgpu.h).
+EPA_MAX_FACES = 256
+EPA_MAX_HORIZON = 128
+EPA_MAX_ITERATIONS = 64
+
 
 class Aabb(C.Structure):
     _fields_ = [("min", C.c_float * 3), ("max", C.c_float * 3)]
@@ -76,6 +81,7 @@ class NarrowPhaseStats(C.Structure):
         ("hull_candidates", C.c_uint64),
         ("hull_contacts", C.c_uint64),
         ("gjk_epa_failures", C.c_uint64),
+        ("epa_overflows", C.c_uint64),
     ]
 
     def as_dict(self):

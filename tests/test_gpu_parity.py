@@ -1,0 +1,129 @@
+"""HIP path against the CPU oracle, bit for bit, on seeded scenes small enough
+for the oracle to finish in seconds.  Every call goes through the C ABI
+(libphysgpu.so) via the physgpu host module."""
+import numpy as np
+import pytest
+
+from conftest import assert_manifolds_equal, compact
+from physgpu import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+ALGOS = [abi.SWEEP_AND_PRUNE, abi.UNIFORM_GRID, abi.DYNAMIC_TREE]
+
+
+def _cases():
+    return {
+        "spheres4096": scenes.spheres(4096, seed=11),
+        "mixed4096": scenes.mixed(4096, seed=12, side=scenes.mixed_side_for(4096, 1.0),
+                                  with_hulls=False),
+        "hulls4096": scenes.mixed(4096, seed=13, side=scenes.mixed_side_for(4096, 1.0)),
+        "stack8": scenes.stack(columns=8, height=8, seed=14),
+    }
+
+
+CASES = _cases()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def bounded_oracle(oracle):
+    """Expected results use the GPU's EPA polytope bounds."""
+    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    yield
+    oracle.set_epa_bounds(0, 0)
+
+
+@pytest.fixture(scope="module")
+def pipe(physgpu):
+    p = physgpu.CollisionPipeline(70000, 1 << 20, max_hull_points=16 * 70000)
+    yield p
+    p.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_aabbs_bitwise(pipe, oracle, name):
+    sc = CASES[name]
+    print(f"case {name}", flush=True)
+    pipe.set_colliders(sc)
+    got = pipe.compute_aabbs()
+    want = oracle.compute_aabbs(sc)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("name", list(CASES))
+def test_pairs_exact(pipe, oracle, name, algo):
+    sc = CASES[name]
+    print(f"case {name} algo {algo}", flush=True)
+    pipe.set_colliders(sc)
+    aabbs = pipe.compute_aabbs()
+    got, stats = pipe.find_candidate_pairs(algorithm=algo, with_stats=True)
+    want, ostats = oracle.find_pairs(aabbs, algo)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+    assert stats["aabb_pairs"] == len(want)
+    if algo == abi.SWEEP_AND_PRUNE:
+        assert stats["x_window_comparisons"] == ostats["x_window_comparisons"]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_contacts_bitwise(pipe, oracle, name):
+    sc = CASES[name]
+    print(f"case {name}", flush=True)
+    pipe.set_colliders(sc)
+    aabbs = oracle.compute_aabbs(sc)
+    pairs, _ = oracle.find_pairs(aabbs)
+    rec, hit = oracle.generate_contacts(sc, pairs)
+    got, stats = pipe.generate_contacts(pairs, with_stats=True)
+    assert_manifolds_equal(got, compact(rec, hit), name)
+    assert np.array_equal(got["pair_index"], np.nonzero(hit)[0].astype(np.uint32))
+    st, _, _, _ = oracle.gjk_epa(sc, pairs)
+    hull = (sc.shape_type[pairs[:, 0]] == abi.HULL) | (sc.shape_type[pairs[:, 1]] == abi.HULL)
+    assert stats["epa_overflows"] == int(((st == 3) & hull).sum())
+    assert stats["gjk_epa_failures"] == int(((st == 2) & hull).sum())
+    for cls, key in ((0, "sphere_sphere"), (1, "sphere_box"), (2, "box_box")):
+        assert stats[key + "_contacts"] == int((hit & (rec["shape_pair"] == cls)).sum())
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("name", list(CASES))
+def test_collide_pipeline(pipe, oracle, name, algo):
+    sc = CASES[name]
+    print(f"case {name} algo {algo}", flush=True)
+    pipe.set_colliders(sc)
+    npairs, nman = pipe.collide(algorithm=algo)
+    pairs = pipe.pairs()
+    man = pipe.manifolds()
+    aabbs = oracle.compute_aabbs(sc)
+    want_pairs, _ = oracle.find_pairs(aabbs, algo)
+    assert np.array_equal(pairs, want_pairs)
+    rec, hit = oracle.generate_contacts(sc, want_pairs)
+    assert_manifolds_equal(man, compact(rec, hit), name)
+    assert npairs == len(want_pairs) and nman == int(hit.sum())
+
+
+def test_boundary_bounds(pipe, oracle):
+    """The reference's boundary AABB sets (tests/collision/broadphase_tests.cpp:89-105)."""
+    F = np.float32
+    sets = [
+        np.zeros((0, 6), F),
+        np.zeros((1, 6), F),
+        np.tile(np.array([-1, -1, -1, 1, 1, 1], F), (16, 1)),
+        np.array([[4, 0, 0, 5, 1, 1], [-2, -2, -2, 0, 0, 0], [0, 0, 0, 1, 1, 1],
+                  [1, 0, 0, 2, 1, 1], [1, 1, 0, 2, 2, 1], [1, 1, 1, 2, 2, 2], [0, 4, 0, 1, 5, 1],
+                  [0, 0, 4, 1, 1, 5], [1, 1, 1, 1, 1, 1], [-100, -0.5, -100, 100, 0, 100]], F),
+    ]
+    big = np.finfo(np.float32).max
+    sets.append(np.array([[-0.5, -0.5, -0.5, 0.5, 0.5, 0.5], [0.5, -0.5, -0.5, 1.5, 0.5, 0.5],
+                          [-1.5, -0.5, -0.5, -0.5, 0.5, 0.5],
+                          [-100000, -0.5, -100000, 100000, 0, 100000],
+                          [big, 0, 0, big, 0, 0], [big, 0, 0, big, 0, 0],
+                          [-2147483648.0, 0, 0, -2147483648.0, 0, 0],
+                          [-2147483648.0, 0, 0, -2147483648.0, 0, 0]], F))
+    sets.append(np.array([[-1, -1, -1, 1, 1, 1], [-big, -big, -big, big, big, big]], F))
+    for k, b in enumerate(sets):
+        for algo in ALGOS:
+            print(f"set {k} algo {algo}", flush=True)
+            got = pipe.find_candidate_pairs(bounds=b, algorithm=algo)
+            want, _ = oracle.find_pairs(b, algo)
+            assert np.array_equal(got, want), (k, algo)
