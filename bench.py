@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Collision-step benchmark on one MI355X: 1,048,576 mixed bodies.
+
+A step is one pass of the collision section of PhysicsWorld::step over
+device-resident colliders: AABB refresh, broad phase (SAP, uniform grid or
+LBVH), canonical pair order, narrow phase (SS, SB, BB SAT + manifold,
+GJK/EPA for hulls) and ordered manifold compaction.  Inputs are generated
+once (seeded) and uploaded before timing.
+
+Prints one JSON line (see the "contract" fields) to stdout; details go to
+stderr.  `value` is the ms/step of the fastest broad phase (all three return
+the same pair set); the per-algorithm numbers are in `per_algorithm`.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "phys-engine_amd"))
+
+import numpy as np  # noqa: E402
+
+import physgpu  # noqa: E402
+from physgpu import abi, scenes  # noqa: E402
+
+ALGOS = {"sap": abi.SWEEP_AND_PRUNE, "grid": abi.UNIFORM_GRID, "tree": abi.DYNAMIC_TREE}
+HBM_PEAK_GBS = 8000.0
+VALU_FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, n, pairs, hull_bodies, hull_points, table_entries):
+    """Algorithmic (compulsory) HBM bytes of one launch, for the roofline.
+
+    Per body: pose 28 B + type 1 B + shape 12 B read, AABB 24 B written;
+    sorted records are 24 B of bounds + 4 B of index; sort items are 8 B;
+    a pair record is 8 B."""
+    return {
+        "aabb_from_pose": n * 65 + hull_bodies * 8 + hull_points * 12,
+        "aabb_reduce": n * 24,
+        "sap_keys": n * (4 + 8),
+        "gather_sorted": n * (8 + 24 + 28),
+        "sap_sweep": n * 28 + pairs * 8,
+        "grid_keys": n * (24 + 8 + 4),
+        "grid_table_clear": table_entries * 8,
+        "grid_cell_ranges": n * 8,
+        "grid_query": n * 28 + pairs * 8,
+        "lbvh_morton": n * (24 + 8),
+        "lbvh_query": n * 28 + pairs * 8,
+    }.get(name)
+
+
+def run_algorithm(pipe, algo, steps, warmup):
+    for _ in range(warmup):
+        pipe.collide(algorithm=algo, sync=False)
+    pipe.step_counts()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.collide(algorithm=algo, sync=False)
+    npairs, nman = pipe.step_counts()  # synchronises the stream
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    # Second pass with per-kernel HIP events on the library's stream.
+    pipe.set_timing(True)
+    for _ in range(steps):
+        pipe.collide(algorithm=algo, sync=False)
+    pipe.step_counts()
+    kt = pipe.kernel_times()
+    pipe.set_timing(False)
+    return ms, npairs, nman, kt
+
+
+def cpu_baseline(scene, seconds_budget=30.0):
+    """The oracle's CPU restatement of the same collision step (port of the
+    reference: its grid broad phase single-threaded as in the reference, the
+    narrow phase on a worker pool), one step of the full workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    orc = O.CpuCollision("oracle")
+    # The GPU's EPA polytope bounds: the reference's unbounded FP32 EPA can
+    # duplicate faces without end on a few hull pairs (minutes per pair).
+    orc.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    cores = min(os.cpu_count() or 1, 16)
+    w = scenes.with_world_extras(scenes.Scene(pose=scene.pose, shape_type=scene.shape_type,
+                                              shape_data=scene.shape_data,
+                                              hull_range=scene.hull_range,
+                                              hull_points=scene.hull_points),
+                                 local_rotation=True)
+    t0 = time.perf_counter()
+    m, t, s = orc.world_collide(w, algorithm=abi.UNIFORM_GRID, bp_workers=1, np_workers=cores,
+                                warmup=0, steps=1, capacity=4 * scene.count)
+    wall = time.perf_counter() - t0
+    step_ms = t["broadphase_ms"] + t["narrowphase_ms"] + t["aabb_ms"]
+    return {"value": round(step_ms, 3), "unit": "ms/step", "cores": cores, "kind": "port",
+            "sample": f"1 step of the full {scene.count}-body workload (grid broad phase, "
+                      f"1 thread; narrow phase on {cores} threads, EPA bounded as on the GPU); broadphase "
+                      f"{t['broadphase_ms']:.1f} ms, narrowphase {t['narrowphase_ms']:.1f} ms, "
+                      f"AABB {t['aabb_ms']:.1f} ms; {len(m)} manifolds; wall {wall:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--bodies", type=int, default=1048576)
+    ap.add_argument("--algorithms", default="sap,grid,tree")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+    n = args.bodies
+    side = 256.0 * (n / 1048576.0) ** (1.0 / 3.0)
+    log(f"scene: {n} mixed bodies, side {side:.1f} m")
+    scene = scenes.mixed(n, seed=20261203, side=side)
+    hull_bodies = int((scene.shape_type == abi.HULL).sum())
+    hull_points = 0 if scene.hull_points is None else scene.hull_points.shape[1]
+    pipe = physgpu.CollisionPipeline(n, max_pairs=max(4 * n // 8, 1 << 16),
+                                     max_hull_points=max(hull_points, 1))
+    pipe.set_colliders(scene)
+    table_entries = 0
+    k = 2
+    while k < 9 and (1 << (3 * k)) < 2 * n:
+        k += 1
+    table_entries = 1 << (3 * k)
+
+    per = {}
+    for name in args.algorithms.split(","):
+        algo = ALGOS[name]
+        ms, npairs, nman, kt = run_algorithm(pipe, algo, args.steps, args.warmup)
+        per[name] = {"ms_per_step": round(ms, 4), "pairs": npairs, "manifolds": nman,
+                     "kernels": {k: round(v[0] * 1e3, 2) for k, v in kt.items()}}
+        log(f"{name}: {ms:.3f} ms/step, {npairs} pairs, {nman} manifolds")
+        for kn, (avg, launches) in sorted(kt.items(), key=lambda x: -x[1][0] * x[1][1]):
+            log(f"   {kn:24s} {avg * 1e3:9.2f} us x {launches // args.steps}")
+    best = min(per, key=lambda k: per[k]["ms_per_step"])
+    ms = per[best]["ms_per_step"]
+    npairs = per[best]["pairs"]
+
+    # Roofline of the dominant bandwidth kernel of the headline step.
+    kt = per[best]["kernels"]
+    ranked = sorted(kt.items(), key=lambda x: -x[1])
+    roof = None
+    for kn, us in ranked:
+        b = kernel_bytes(kn, n, npairs, hull_bodies, hull_points, table_entries)
+        if b is None:
+            continue
+        achieved = b / (us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "kernel": kn, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": b, "avg_us": us}
+        break
+
+    line = {
+        "metric": "collision ms/step at 1M bodies",
+        "value": ms,
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"mixed-{n}: 50% spheres r U[0.2,0.5], 40% boxes h U[0.2,0.6], "
+                               f"10% 16-point hulls, random orientations, {side:.0f} m cube",
+                   "bodies": n, "broadphase": best, "candidate_pairs": npairs,
+                   "manifolds": per[best]["manifolds"], "parallelism": "1 GPU"},
+        "pairs_per_s": round(npairs / (ms * 1e-3), 1),
+        "manifolds_per_s": round(per[best]["manifolds"] / (ms * 1e-3), 1),
+        "per_algorithm": {k: {kk: vv for kk, vv in v.items() if kk != "kernels"}
+                          for k, v in per.items()},
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline(scene)
+        except Exception as e:  # reported, never fatal for the GPU number
+            line["cpu_baseline"] = {"error": repr(e)}
+    log(json.dumps(per, indent=1))
+    print(json.dumps(line), flush=True)
+    pipe.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1215,7 +1215,7 @@ int orc_world_collide(const physgpu_colliders* c, const float* bodyPosition,
             else
                 local[i] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 1.0f}};
         }
-        NarrowOptions opt;
+        NarrowOptions opt = makeOptions(0, 0, 0.0f);
         std::vector<physgpu_aabb> bounds(n);
         std::vector<Pair> pairs;
         std::vector<physgpu_manifold> results;

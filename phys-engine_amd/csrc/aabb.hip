@@ -6,14 +6,17 @@
 //   hull    component-wise min/max of pos + rotate(q, p_k) over the points
 //           (no hull in the reference; this is the builder's definition)
 // One thread per collider; the seven pose planes, type and shape planes are
-// read coalesced and the six AABB planes written coalesced.
+// read coalesced and one 32-byte AABB record (wave.hpp Rec) written per body.
 //
-// k_reduce reads the AABB planes once and produces what the broad phases
+// k_reduce reads the AABB records once and produces what the broad phases
 // need: largest x side (SAP row window bound), largest side and mean side
 // (grid cell), and centre bounds (LBVH Morton quantisation).  Wave-level
 // reductions, one atomic per wave.
 #include "devmath.hpp"
 #include "internal.hpp"
+#include "wave.hpp"
+
+#include <algorithm>
 
 namespace pg {
 namespace {
@@ -33,7 +36,7 @@ __global__ void k_step_begin(Counters* c)
 __global__ __launch_bounds__(kBlock) void k_compute_aabbs(
     uint32_t n, const float* __restrict__ pose, const uint8_t* __restrict__ type,
     const float* __restrict__ sdata, const uint32_t* __restrict__ hrange,
-    const float* __restrict__ hpts, uint32_t nh, float* __restrict__ aabb)
+    const float* __restrict__ hpts, uint32_t nh, float4* __restrict__ aabb)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -67,12 +70,8 @@ __global__ __launch_bounds__(kBlock) void k_compute_aabbs(
             mx = {max_ref(mx.x, w.x), max_ref(mx.y, w.y), max_ref(mx.z, w.z)};
         }
     }
-    aabb[i] = mn.x;
-    aabb[n + i] = mn.y;
-    aabb[2 * n + i] = mn.z;
-    aabb[3 * n + i] = mx.x;
-    aabb[4 * n + i] = mx.y;
-    aabb[5 * n + i] = mx.z;
+    const float b[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
+    store_rec(aabb, i, b, i);
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
@@ -94,7 +93,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float* __restrict__ aabb,
+__global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __restrict__ aabb,
                                                    Counters* c)
 {
     uint32_t wx = 0, wall = 0;
@@ -103,8 +102,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float* __re
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += gridDim.x * blockDim.x) {
         float w[3];
+        const Rec r = load_rec(aabb, i);
         for (int a = 0; a < 3; ++a) {
-            const float lo = aabb[a * n + i], hi = aabb[(3 + a) * n + i];
+            const float lo = r.b[a], hi = r.b[3 + a];
             w[a] = hi - lo;
             // Centre as lo/2 + hi/2 cannot overflow for finite bounds.
             const uint32_t ck = float_key(lo * 0.5f + hi * 0.5f);
@@ -125,42 +125,64 @@ __global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float* __re
         cmn[a] = wave_min_u32(cmn[a]);
         cmx[a] = wave_max_u32(cmx[a]);
     }
+    // Workgroup reduction in LDS, then one atomic per value per workgroup.
+    __shared__ uint32_t s_u[kBlock / 64][8];
+    __shared__ unsigned long long s_sum[kBlock / 64];
+    const uint32_t wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0) {
-        atomicMax(&c->wmax_x_key, wx);
-        atomicMax(&c->wmax_key, wall);
-        atomicAdd(&c->width_sum_fx, wsum);
+        s_u[wave][0] = wx;
+        s_u[wave][1] = wall;
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&c->cmin_key[a], cmn[a]);
-            atomicMax(&c->cmax_key[a], cmx[a]);
+            s_u[wave][2 + a] = cmn[a];
+            s_u[wave][5 + a] = cmx[a];
+        }
+        s_sum[wave] = wsum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < kBlock / 64; ++w) {
+            s_u[0][0] = max(s_u[0][0], s_u[w][0]);
+            s_u[0][1] = max(s_u[0][1], s_u[w][1]);
+            for (int a = 0; a < 3; ++a) {
+                s_u[0][2 + a] = min(s_u[0][2 + a], s_u[w][2 + a]);
+                s_u[0][5 + a] = max(s_u[0][5 + a], s_u[w][5 + a]);
+            }
+            s_sum[0] += s_sum[w];
+        }
+        atomicMax(&c->wmax_x_key, s_u[0][0]);
+        atomicMax(&c->wmax_key, s_u[0][1]);
+        atomicAdd(&c->width_sum_fx, s_sum[0]);
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&c->cmin_key[a], s_u[0][2 + a]);
+            atomicMax(&c->cmax_key[a], s_u[0][5 + a]);
         }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_aabb_soa_to_aos(uint32_t n, const float* aabb,
-                                                            physgpu_aabb* out)
+__global__ __launch_bounds__(kBlock) void k_rec_to_aabb(uint32_t n, const float4* __restrict__ rec,
+                                                        physgpu_aabb* __restrict__ out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    const Rec r = load_rec(rec, i);
     physgpu_aabb b;
     for (int a = 0; a < 3; ++a) {
-        b.min[a] = aabb[a * n + i];
-        b.max[a] = aabb[(3 + a) * n + i];
+        b.min[a] = r.b[a];
+        b.max[a] = r.b[3 + a];
     }
     out[i] = b;
 }
 
-__global__ __launch_bounds__(kBlock) void k_aabb_aos_to_soa(uint32_t n, const physgpu_aabb* in,
-                                                            float* aabb)
+__global__ __launch_bounds__(kBlock) void k_aabb_to_rec(uint32_t n, const physgpu_aabb* __restrict__ in,
+                                                        float4* __restrict__ rec)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const physgpu_aabb b = in[i];
-    for (int a = 0; a < 3; ++a) {
-        aabb[a * n + i] = b.min[a];
-        aabb[(3 + a) * n + i] = b.max[a];
-    }
+    const float f[6] = {b.min[0], b.min[1], b.min[2], b.max[0], b.max[1], b.max[2]};
+    store_rec(rec, i, f, i);
 }
 
 // World pose of each collider: body pose composed with the collider's local
@@ -228,8 +250,10 @@ void launch_aabb_reductions(physgpu_context* ctx)
         return;
     Timer t;
     time_begin(ctx, "aabb_reduce", t);
-    hipLaunchKernelGGL(k_reduce, dim3(stride_grid(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
-                       ctx->aabb, ctx->counters);
+    // One workgroup per CU keeps the per-workgroup atomics few.
+    const uint32_t grid = std::min<uint32_t>(div_up(ctx->n, kBlock), 256);
+    hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->aabb,
+                       ctx->counters);
     time_end(ctx, t);
 }
 
@@ -237,16 +261,16 @@ void launch_convert_aabbs_out(physgpu_context* ctx, physgpu_aabb* dev_out)
 {
     if (ctx->n == 0)
         return;
-    hipLaunchKernelGGL(k_aabb_soa_to_aos, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0,
-                       ctx->stream, ctx->n, ctx->aabb, dev_out);
+    hipLaunchKernelGGL(k_rec_to_aabb, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0, ctx->stream,
+                       ctx->n, ctx->aabb, dev_out);
 }
 
 void launch_convert_aabbs_in(physgpu_context* ctx, const physgpu_aabb* dev_in)
 {
     if (ctx->n == 0)
         return;
-    hipLaunchKernelGGL(k_aabb_aos_to_soa, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0,
-                       ctx->stream, ctx->n, dev_in, ctx->aabb);
+    hipLaunchKernelGGL(k_aabb_to_rec, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0, ctx->stream,
+                       ctx->n, dev_in, ctx->aabb);
 }
 
 }  // namespace pg

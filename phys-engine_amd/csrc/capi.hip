@@ -91,11 +91,12 @@ hipError_t alloc(T*& p, size_t count)
 void free_all(physgpu_context* c)
 {
     void* ptrs[] = {c->pose, c->shape_type, c->shape_data, c->hull_range, c->hull_points,
-                    c->aabb, c->sorted_aabb, c->sorted_index, c->items_a, c->items_b,
-                    c->histogram, c->cell_table, c->overflow_list, c->grid_flag, c->node_left,
-                    c->node_right, c->node_parent, c->node_last, c->node_box, c->node_flag,
+                    c->aabb, c->sorted_aabb, c->items_a, c->items_b,
+                    c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_flag, c->node_left,
+                    c->node_right, c->node_parent, c->node_first, c->node_last, c->node_box, c->level_box,
                     c->pairs_raw, c->pairs_tmp, c->buckets, c->slots, c->hit, c->tile_sums,
                     c->manifolds, c->body, c->local_pose, c->friction, c->restitution,
+                    c->epa_list, c->epa_simplex,
                     c->counters, c->world_body_pose};
     for (void* p : ptrs)
         if (p)
@@ -272,12 +273,12 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->shape_data, 3 * N));
     A(alloc(c->hull_range, 2 * N));
     A(alloc(c->hull_points, 3 * size_t(std::max<uint32_t>(max_hull_points, 1))));
-    A(alloc(c->aabb, 6 * N));
-    A(alloc(c->sorted_aabb, 6 * N));
-    A(alloc(c->sorted_index, N));
+    A(alloc(c->aabb, 2 * N));
+    A(alloc(c->sorted_aabb, 2 * N));
     A(alloc(c->items_a, N));
     A(alloc(c->items_b, N));
     A(alloc(c->histogram, c->hist_capacity));
+    A(alloc(c->digit_total, 256));
     A(alloc(c->cell_table, 2 * (size_t(1) << c->table_bits)));
     A(alloc(c->overflow_list, N));
     A(alloc(c->grid_flag, N));
@@ -285,8 +286,9 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->node_right, N));
     A(alloc(c->node_parent, 2 * N));
     A(alloc(c->node_last, N));
-    A(alloc(c->node_box, 6 * N));
-    A(alloc(c->node_flag, N));
+    A(alloc(c->node_box, 2 * N));
+    A(alloc(c->node_first, N));
+    A(alloc(c->level_box, 2 * (N / 32 + N / 1024 + N / 32768 + 3)));
     A(alloc(c->pairs_raw, P));
     A(alloc(c->pairs_tmp, P));
     A(alloc(c->buckets, kClasses * P));
@@ -294,6 +296,8 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->hit, P));
     A(alloc(c->tile_sums, div_up(P, kScanTile) + 1));
     A(alloc(c->manifolds, P));
+    A(alloc(c->epa_list, P));
+    A(alloc(c->epa_simplex, 12 * P));
     A(alloc(c->body, N));
     A(alloc(c->local_pose, 7 * N));
     A(alloc(c->friction, N));

@@ -3,7 +3,7 @@
 //
 // The reference inserts each AABB into up to 64 cells and de-duplicates with
 // sort + unique.  The gfx950 form inserts each body once, into the cell of its
-// AABB's min corner, and queries the 27 neighbour cells:
+// AABB's min corner, and queries the neighbour cells (at most 27):
 //   * cell side c >= every in-grid body's largest side (widened by 2^-9), so
 //     two overlapping in-grid bodies have min corners less than one cell
 //     apart on every axis and their home cells are neighbours;
@@ -18,7 +18,7 @@
 //     unordered pair appears once (the 27 neighbour keys are distinct for
 //     k >= 2, and every body sits in exactly one cell).
 // Cell coordinates use floor(min / c) in double precision, a monotone map.
-// Automatic cell: the largest AABB side, capped at twice the mean side (the
+// Automatic cell: the largest AABB side, capped at 8x the mean side (the
 // rest overflow), widened by 2^-8 (in-grid bodies must be at most 1 - 2^-9
 // of the cell).
 #include "devmath.hpp"
@@ -51,7 +51,7 @@ __device__ __forceinline__ float step_cell(const Counters* c, float requested, u
         return requested;
     const float wmax = key_float(c->wmax_key);
     const double mean = n ? double(c->width_sum_fx) / 65536.0 / double(n) : 1.0;
-    float cell = fminf(wmax, float(2.0 * mean));
+    float cell = fminf(wmax, float(8.0 * mean));
     if (!(cell > 0.0f))
         cell = 1.0f;
     return cell * (1.0f + 1.0f / 256.0f);
@@ -69,7 +69,7 @@ __device__ __forceinline__ bool home_cell(float x, float y, float z, double inv,
     return true;
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float* __restrict__ aabb,
+__global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* __restrict__ aabb,
                                                       Counters* counters, float requested,
                                                       uint32_t bits_per_axis,
                                                       uint64_t* __restrict__ items,
@@ -83,23 +83,23 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float* _
     bool in_grid = false;
     uint32_t key = 1u << (3 * bits_per_axis);  // sorts after every cell key
     if (i < n) {
-        const float lo[3] = {aabb[i], aabb[n + i], aabb[2 * n + i]};
-        const float hi[3] = {aabb[3 * n + i], aabb[4 * n + i], aabb[5 * n + i]};
+        const Rec r = load_rec(aabb, i);
         const float limit = cell * (1.0f - 1.0f / 512.0f);
-        const bool fits = (hi[0] - lo[0]) <= limit && (hi[1] - lo[1]) <= limit &&
-                          (hi[2] - lo[2]) <= limit;
-        int32_t c[3];
-        if (fits && home_cell(lo[0], lo[1], lo[2], 1.0 / double(cell), c)) {
+        const bool fits = (r.b[3] - r.b[0]) <= limit && (r.b[4] - r.b[1]) <= limit &&
+                          (r.b[5] - r.b[2]) <= limit;
+        int32_t c[3], top[3];
+        const double inv = 1.0 / double(cell);
+        if (fits && home_cell(r.b[0], r.b[1], r.b[2], inv, c) &&
+            home_cell(r.b[3], r.b[4], r.b[5], inv, top)) {
             in_grid = true;
             key = cell_key(c[0], c[1], c[2], (1u << bits_per_axis) - 1u);
         }
         items[i] = (uint64_t)key | ((uint64_t)i << 32);
         flag[i] = in_grid ? 1u : 0u;
     }
-    // Overflow list and entry count, one atomic per wave each.
+    // Overflow list (rare: one atomic per wave that has an overflow body).
     const bool over = i < n && !in_grid;
     const uint64_t om = __ballot(over);
-    const uint64_t gm = __ballot(in_grid);
     const uint32_t lane = lane_id();
     if (om) {
         const uint32_t leader = (uint32_t)__ffsll((unsigned long long)om) - 1u;
@@ -110,8 +110,12 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float* _
         if (over)
             overflow[base + __popcll(om & lanes_below(lane))] = i;
     }
-    if (gm && lane == (uint32_t)__ffsll((unsigned long long)gm) - 1u)
-        atomicAdd(&counters->grid_entries, (uint32_t)__popcll(gm));
+}
+
+// Bodies in cells = all bodies minus the overflow ones.
+__global__ void k_grid_entries(uint32_t n, Counters* counters)
+{
+    counters->grid_entries = n - counters->overflow_count;
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(uint32_t* __restrict__ p, uint64_t count,
@@ -122,10 +126,10 @@ __global__ __launch_bounds__(kBlock) void k_fill(uint32_t* __restrict__ p, uint6
         p[i] = value;
 }
 
-// Cell ranges over the sorted in-grid prefix [0, entries).
+// Cell ranges over the sorted in-grid prefix [0, entries): table[2 key] =
+// first position, table[2 key + 1] = one past the last (0, 0 = empty).
 __global__ __launch_bounds__(kBlock) void k_cell_ranges(uint32_t n, const uint64_t* __restrict__ items,
                                                         const Counters* counters,
-                                                        uint32_t table_size,
                                                         uint32_t* __restrict__ table)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,53 +138,53 @@ __global__ __launch_bounds__(kBlock) void k_cell_ranges(uint32_t n, const uint64
         return;
     const uint32_t key = uint32_t(items[p]);
     if (p == 0 || uint32_t(items[p - 1]) != key)
-        table[key] = p;
+        table[2 * size_t(key)] = p;
     if (p + 1 == entries || uint32_t(items[p + 1]) != key)
-        table[table_size + key] = p + 1;
+        table[2 * size_t(key) + 1] = p + 1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float* __restrict__ s,
-                                                       const uint32_t* __restrict__ sidx,
+// Candidates of a body: bodies whose home cell lies, on every axis, in
+// [home - 1, floor(max / cell)] (a partner's min is below this body's max and
+// less than one cell below its min), i.e. 2 or 3 cells per axis.
+__global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4* __restrict__ s,
                                                        const uint32_t* __restrict__ table,
-                                                       uint32_t table_size, uint32_t bits_per_axis,
+                                                       uint32_t bits_per_axis,
                                                        Counters* counters,
                                                        uint64_t* __restrict__ out, uint64_t cap)
 {
+    __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
+    WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
     const uint32_t entries = counters->grid_entries;
     const float cell = __uint_as_float(counters->cell_bits);
     const double inv = 1.0 / double(cell);
     const uint32_t mask = (1u << bits_per_axis) - 1u;
-    const uint32_t* start = table;
-    const uint32_t* end = table + table_size;
+    const uint2* range = reinterpret_cast<const uint2*>(table);
     unsigned long long tests = 0;
-    // Grid-stride over sorted positions; all lanes stay in the loop together
+    // Grid-stride over sorted positions; all lanes stay in the loops together
     // so that the wave-wide append is executed convergently.
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (entries + stride - 1) / stride;
     for (uint32_t round = 0; round < rounds; ++round) {
         const uint32_t p = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
         const bool active = p < entries;
-        float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-        uint32_t a = 0;
-        int32_t c[3] = {0, 0, 0};
+        Rec me{{0, 0, 0, 0, 0, 0}, 0};
+        int32_t c[3] = {0, 0, 0}, top[3] = {0, 0, 0};  // both in range for in-grid bodies
         if (active) {
-            for (int ax = 0; ax < 3; ++ax) {
-                lo[ax] = s[ax * (size_t)n + p];
-                hi[ax] = s[(3 + ax) * (size_t)n + p];
-            }
-            a = sidx[p];
-            home_cell(lo[0], lo[1], lo[2], inv, c);
+            me = load_rec(s, p);
+            home_cell(me.b[0], me.b[1], me.b[2], inv, c);
+            home_cell(me.b[3], me.b[4], me.b[5], inv, top);
         }
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     uint32_t q = 0, qe = 0;
-                    if (active) {
-                        const uint32_t key = cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask);
-                        q = start[key];
-                        qe = end[key];
+                    if (active && c[0] + dx <= top[0] && c[1] + dy <= top[1] &&
+                        c[2] + dz <= top[2]) {
+                        const uint2 r = range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask)];
+                        q = r.x;
+                        qe = r.y;
                         if (qe <= q)
-                            q = qe = 0;  // empty (table cleared to zero)
+                            q = qe = 0;
                     }
                     // Lanes walk their own cells; the wave iterates until all are done.
                     for (;;) {
@@ -190,19 +194,20 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float* 
                         bool hit = false;
                         uint32_t b = 0;
                         if (more) {
-                            b = sidx[q];
-                            if (a < b) {
+                            const Rec o = load_rec(s, q);
+                            b = o.index;
+                            if (me.index < b) {
                                 ++tests;
-                                hit = lo[0] <= s[3 * (size_t)n + q] && hi[0] >= s[q] &&
-                                      lo[1] <= s[4 * (size_t)n + q] && hi[1] >= s[(size_t)n + q] &&
-                                      lo[2] <= s[5 * (size_t)n + q] && hi[2] >= s[2 * (size_t)n + q];
+                                hit = rec_overlap(me.b, o.b);
                             }
                             ++q;
                         }
-                        wave_append(hit, pair_value(a, b), out, &counters->pair_count, cap);
+                        wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count,
+                                cap);
                     }
                 }
     }
+    wq_flush(queue, out, &counters->pair_count, cap);
     tests = wave_sum(tests);
     if (lane_id() == 0)
         atomicAdd(&counters->grid_tests, tests);
@@ -210,32 +215,32 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float* 
 
 // Overflow bodies against every body (broadphase.cpp:163-166): pair (o, j)
 // is tested when j != o and (j is in the grid or o < j).
-__global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const float* __restrict__ aabb,
+__global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const float4* __restrict__ aabb,
                                                           const uint32_t* __restrict__ overflow,
                                                           const uint32_t* __restrict__ flag,
                                                           Counters* counters,
                                                           uint64_t* __restrict__ out, uint64_t cap)
 {
+    __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
+    WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
     const uint32_t count = counters->overflow_count;
     unsigned long long tests = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (n + stride - 1) / stride;
     for (uint32_t k = blockIdx.y; k < count; k += gridDim.y) {
         const uint32_t o = overflow[k];
-        const float olo[3] = {aabb[o], aabb[n + o], aabb[2 * n + o]};
-        const float ohi[3] = {aabb[3 * n + o], aabb[4 * n + o], aabb[5 * n + o]};
+        const Rec ro = load_rec(aabb, o);
         for (uint32_t round = 0; round < rounds; ++round) {
             const uint32_t j = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
             bool hit = false;
             if (j < n && j != o && (flag[j] || o < j)) {
                 ++tests;
-                hit = olo[0] <= aabb[3 * n + j] && ohi[0] >= aabb[j] &&
-                      olo[1] <= aabb[4 * n + j] && ohi[1] >= aabb[n + j] &&
-                      olo[2] <= aabb[5 * n + j] && ohi[2] >= aabb[2 * n + j];
+                hit = rec_overlap(ro.b, load_rec(aabb, j).b);
             }
-            wave_append(hit, pair_value(o, j), out, &counters->pair_count, cap);
+            wq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap);
         }
     }
+    wq_flush(queue, out, &counters->pair_count, cap);
     tests = wave_sum(tests);
     if (lane_id() == 0 && tests)
         atomicAdd(&counters->grid_tests, tests);
@@ -254,6 +259,7 @@ void launch_grid(physgpu_context* ctx, double cell_size)
                        ctx->aabb, ctx->counters, float(cell_size), bpa, ctx->items_a,
                        ctx->overflow_list, ctx->grid_flag);
     time_end(ctx, t);
+    hipLaunchKernelGGL(k_grid_entries, dim3(1), dim3(1), 0, ctx->stream, n, ctx->counters);
     uint64_t* sorted = nullptr;
     radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 3 * bpa + 1, 0, 0, &sorted);
     time_begin(ctx, "grid_table_clear", t);
@@ -262,13 +268,13 @@ void launch_grid(physgpu_context* ctx, double cell_size)
     time_end(ctx, t);
     time_begin(ctx, "grid_cell_ranges", t);
     hipLaunchKernelGGL(k_cell_ranges, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
-                       sorted, ctx->counters, table_size, ctx->cell_table);
+                       sorted, ctx->counters, ctx->cell_table);
     time_end(ctx, t);
     launch_gather_sorted(ctx, sorted);
     time_begin(ctx, "grid_query", t);
     hipLaunchKernelGGL(k_grid_query, dim3(stride_grid(n)), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->sorted_aabb, ctx->sorted_index, ctx->cell_table, table_size, bpa,
-                       ctx->counters, ctx->pairs_raw, ctx->cap_p);
+                       ctx->sorted_aabb, ctx->cell_table, bpa, ctx->counters, ctx->pairs_raw,
+                       ctx->cap_p);
     time_end(ctx, t);
     time_begin(ctx, "grid_overflow", t);
     hipLaunchKernelGGL(k_grid_overflow, dim3(stride_grid(n), 16), dim3(kBlock), 0, ctx->stream, n,

@@ -49,7 +49,7 @@ struct Counters {
     uint32_t sort_count;                // min(pair_count, capacity): pairs held
     uint32_t cell_bits;                 // grid cell size used (float bits)
     uint32_t tree_overflow;             // LBVH traversal stack overflows (must stay 0)
-    uint32_t pad;
+    uint32_t epa_count;                 // hull pairs GJK found intersecting (EPA work)
 };
 
 struct Timer {
@@ -84,15 +84,16 @@ struct physgpu_context {
     bool has_hulls = false;
     bool colliders_valid = false;
 
-    // AABBs (6 planes: min x y z, max x y z) and their sorted copy.
-    float* aabb = nullptr;
-    float* sorted_aabb = nullptr;
-    uint32_t* sorted_index = nullptr;
+    // AABB records (2 float4 per body, see wave.hpp Rec) and their sorted copy
+    // (the record's index field holds the original collider index).
+    float4* aabb = nullptr;
+    float4* sorted_aabb = nullptr;
 
     // Radix sort scratch (u64 items).
     uint64_t* items_a = nullptr;
     uint64_t* items_b = nullptr;
     uint32_t* histogram = nullptr;  // 256 x tiles
+    uint32_t* digit_total = nullptr;  // 256 per pass
     uint32_t hist_capacity = 0;
 
     // Grid.
@@ -105,9 +106,10 @@ struct physgpu_context {
     int32_t* node_left = nullptr;
     int32_t* node_right = nullptr;
     int32_t* node_parent = nullptr;   // internal parents (n-1) then leaf parents (n)
+    uint32_t* node_first = nullptr;   // first leaf position covered by an internal node
     uint32_t* node_last = nullptr;    // last leaf position covered by an internal node
-    float* node_box = nullptr;        // 6 x (n-1) internal node bounds
-    uint32_t* node_flag = nullptr;    // refit arrival flags
+    float4* node_box = nullptr;       // internal node bounds, 2 float4 per node
+    float4* level_box = nullptr;      // boxes of 32 / 1024 / 32768-leaf groups
 
     // Pairs as first << 32 | second: raw emission buffer, sort scratch, and
     // the canonical (sorted) list, which aliases one of the two.
@@ -120,6 +122,8 @@ struct physgpu_context {
     physgpu_manifold* slots = nullptr; // one record per candidate
     uint32_t* hit = nullptr;           // per candidate
     uint32_t* tile_sums = nullptr;     // compaction
+    uint32_t* epa_list = nullptr;      // EPA work: candidate | class << 29
+    float* epa_simplex = nullptr;      // 12 floats per EPA work item
     physgpu_manifold* manifolds = nullptr;  // compacted, candidate order
     physgpu_narrowphase_options np_options{};
 

@@ -9,8 +9,10 @@
 //   2. one thread per internal node finds its key range and split with
 //      binary searches over common-prefix lengths (index bits break ties of
 //      equal codes);
-//   3. bottom-up refit: each leaf climbs, the second arrival at a node merges
-//      its children (device-scope fences make the first child's box visible).
+//   3. node bounds: an internal node covers a contiguous range of sorted
+//      leaves, so its box is a range union built from precomputed boxes of
+//      aligned 32 / 1024 / 32768-leaf groups; no atomics or fences (a
+//      bottom-up refit would need device-scope fences between XCDs).
 // Query: one thread per leaf, stack traversal; a subtree is skipped when all
 // of its leaves sit at or before the querying leaf in sorted order, so each
 // pair is found once.  Children are encoded with bit 31 set for leaves; a
@@ -36,7 +38,7 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v)
     return v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_morton(uint32_t n, const float* __restrict__ aabb,
+__global__ __launch_bounds__(kBlock) void k_morton(uint32_t n, const float4* __restrict__ aabb,
                                                    const Counters* counters,
                                                    uint64_t* __restrict__ items)
 {
@@ -44,10 +46,11 @@ __global__ __launch_bounds__(kBlock) void k_morton(uint32_t n, const float* __re
     if (i >= n)
         return;
     uint32_t code = 0;
+    const Rec r = load_rec(aabb, i);
     for (int a = 0; a < 3; ++a) {
         const float lo = key_float(counters->cmin_key[a]);
         const float hi = key_float(counters->cmax_key[a]);
-        const float c = aabb[a * n + i] * 0.5f + aabb[(3 + a) * n + i] * 0.5f;
+        const float c = r.b[a] * 0.5f + r.b[3 + a] * 0.5f;
         const float ext = hi - lo;
         float u = ext > 0.0f ? (c - lo) / ext : 0.0f;
         u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
@@ -71,14 +74,13 @@ __global__ __launch_bounds__(kBlock) void k_build(uint32_t n, const uint64_t* __
                                                   int32_t* __restrict__ left,
                                                   int32_t* __restrict__ right,
                                                   int32_t* __restrict__ parent,
-                                                  uint32_t* __restrict__ last,
-                                                  uint32_t* __restrict__ flag)
+                                                  uint32_t* __restrict__ first_out,
+                                                  uint32_t* __restrict__ last)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int ni = int(n);
     if (i >= ni - 1)
         return;
-    flag[i] = 0;
     const int d = (delta(items, ni, i, i + 1) - delta(items, ni, i, i - 1)) >= 0 ? 1 : -1;
     const int dmin = delta(items, ni, i, i - d);
     int lmax = 2;
@@ -103,88 +105,109 @@ __global__ __launch_bounds__(kBlock) void k_build(uint32_t n, const uint64_t* __
     const int32_t rc = (lastpos == split + 1) ? int32_t(uint32_t(split + 1) | kLeaf) : split + 1;
     left[i] = lc;
     right[i] = rc;
+    first_out[i] = uint32_t(first);
     last[i] = uint32_t(lastpos);
     // Parents: internal nodes in [0, n-1), leaves in [n-1, 2n-1).
     parent[(lc & int32_t(kLeaf)) ? (ni - 1) + (lc & 0x7FFFFFFF) : lc] = i;
     parent[(rc & int32_t(kLeaf)) ? (ni - 1) + (rc & 0x7FFFFFFF) : rc] = i;
 }
 
-__device__ __forceinline__ void load_box(const float* __restrict__ s, size_t n,
-                                         const float* __restrict__ nodes, size_t m, int32_t child,
-                                         float b[6])
+// Node bounds without inter-thread synchronisation: an internal node covers
+// the sorted leaves [first, last], so its box is the union of that range,
+// assembled from aligned blocks of 32^3, 32^2, 32 leaves and single leaves.
+// (min/max are exact, so the order of the unions does not change the box.)
+constexpr uint32_t kSpan = 32;
+
+__device__ __forceinline__ void unite(float u[6], const float4* __restrict__ r, size_t i)
 {
-    if (child & int32_t(kLeaf)) {
-        const size_t p = uint32_t(child) & 0x7FFFFFFFu;
-        for (int a = 0; a < 6; ++a)
-            b[a] = s[a * n + p];
+    const Rec b = load_rec(r, i);
+    for (int k = 0; k < 3; ++k) {
+        u[k] = min_ref(u[k], b.b[k]);
+        u[3 + k] = max_ref(u[3 + k], b.b[3 + k]);
+    }
+}
+
+// Boxes of consecutive groups of 32 records (one per half wave).
+__global__ __launch_bounds__(kBlock) void k_group_boxes(uint32_t count, const float4* __restrict__ in,
+                                                        float4* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = i < count;
+    float b[6];
+    if (ok) {
+        const Rec r = load_rec(in, i);
+        for (int k = 0; k < 6; ++k)
+            b[k] = r.b[k];
     } else {
-        for (int a = 0; a < 6; ++a)
-            b[a] = nodes[a * m + child];
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, const float* __restrict__ s,
-                                                  const int32_t* __restrict__ left,
-                                                  const int32_t* __restrict__ right,
-                                                  const int32_t* __restrict__ parent,
-                                                  uint32_t* __restrict__ flag,
-                                                  float* __restrict__ nodes, Counters* counters)
-{
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || n < 2)
-        return;
-    const size_t m = n - 1;
-    int32_t node = parent[(n - 1) + p];
-    uint32_t depth = 1;
-    while (node >= 0) {
-        if (depth > 2 * kStack) {  // deeper than any 62-bit-key tree: malformed
-            atomicAdd(&counters->tree_overflow, 1u);
-            return;
-        }
-        __threadfence();
-        if (atomicAdd(&flag[node], 1u) == 0)
-            return;  // the sibling's thread merges this node
-        __threadfence();
-        float a[6], b[6];
-        load_box(s, n, nodes, m, left[node], a);
-        load_box(s, n, nodes, m, right[node], b);
         for (int k = 0; k < 3; ++k) {
-            nodes[k * m + node] = min_ref(a[k], b[k]);
-            nodes[(3 + k) * m + node] = max_ref(a[3 + k], b[3 + k]);
+            b[k] = __int_as_float(0x7f800000);
+            b[3 + k] = -__int_as_float(0x7f800000);
         }
-        ++depth;
-        node = node == 0 ? -1 : parent[node];
     }
-    atomicMax(&counters->tree_height, depth);
+    for (uint32_t off = 1; off < kSpan; off <<= 1)
+        for (int k = 0; k < 3; ++k) {
+            b[k] = fminf(b[k], __shfl_xor(b[k], off));
+            b[3 + k] = fmaxf(b[3 + k], __shfl_xor(b[3 + k], off));
+        }
+    if ((i % kSpan) == 0 && ok)
+        store_rec(out, i / kSpan, b, 0);
 }
 
-__device__ __forceinline__ bool overlap6(const float a[6], const float b[6])
+__global__ __launch_bounds__(kBlock) void k_node_boxes(uint32_t n, const float4* __restrict__ leaves,
+                                                       const float4* __restrict__ l1,
+                                                       const float4* __restrict__ l2,
+                                                       const float4* __restrict__ l3,
+                                                       const uint32_t* __restrict__ first,
+                                                       const uint32_t* __restrict__ last,
+                                                       float4* __restrict__ nodes)
 {
-    return a[0] <= b[3] && a[3] >= b[0] && a[1] <= b[4] && a[4] >= b[1] && a[2] <= b[5] &&
-           a[5] >= b[2];
+    const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node + 1 >= n)
+        return;
+    uint32_t i = first[node];
+    const uint32_t l = last[node];
+    float u[6];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = __int_as_float(0x7f800000);
+        u[3 + k] = -__int_as_float(0x7f800000);
+    }
+    constexpr uint32_t s1 = kSpan, s2 = kSpan * kSpan, s3 = kSpan * kSpan * kSpan;
+    while (i <= l) {
+        if (i % s3 == 0 && uint64_t(i) + s3 - 1 <= l) {
+            unite(u, l3, i / s3);
+            i += s3;
+        } else if (i % s2 == 0 && uint64_t(i) + s2 - 1 <= l) {
+            unite(u, l2, i / s2);
+            i += s2;
+        } else if (i % s1 == 0 && uint64_t(i) + s1 - 1 <= l) {
+            unite(u, l1, i / s1);
+            i += s1;
+        } else {
+            unite(u, leaves, i);
+            i += 1;
+        }
+    }
+    store_rec(nodes, node, u, 0);
 }
 
-__global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float* __restrict__ s,
-                                                  const uint32_t* __restrict__ sidx,
+__global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float4* __restrict__ s,
                                                   const int32_t* __restrict__ left,
                                                   const int32_t* __restrict__ right,
                                                   const uint32_t* __restrict__ last,
-                                                  const float* __restrict__ nodes,
+                                                  const float4* __restrict__ nodes,
                                                   Counters* counters, uint64_t* __restrict__ out,
                                                   uint64_t cap)
 {
+    __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
+    WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t m = n - 1;
     unsigned long long visits = 0, leaves = 0;
     int32_t stack[kStack];
     int sp = 0;
-    float box[6] = {0, 0, 0, 0, 0, 0};
-    uint32_t a = 0;
+    Rec me{{0, 0, 0, 0, 0, 0}, 0};
     const bool active = p < n && n >= 2;
     if (active) {
-        for (int k = 0; k < 6; ++k)
-            box[k] = s[k * (size_t)n + p];
-        a = sidx[p];
+        me = load_rec(s, p);
         stack[sp++] = 0;  // root
     }
     // Converged loop: every lane pops until all stacks are empty.  A lane
@@ -206,28 +229,23 @@ __global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float* __res
             if (entry & int32_t(kLeaf)) {
                 // A leaf pushed below already passed the order and overlap tests.
                 hit = true;
-                b = sidx[uint32_t(entry) & 0x7FFFFFFFu];
+                b = load_rec(s, uint32_t(entry) & 0x7FFFFFFFu).index;
             } else {
                 ++visits;
                 const int32_t kids[2] = {left[entry], right[entry]};
                 for (int c = 0; c < 2; ++c) {
                     const int32_t child = kids[c];
-                    float cb[6];
                     bool take = false;
                     if (child & int32_t(kLeaf)) {
                         const uint32_t q = uint32_t(child) & 0x7FFFFFFFu;
                         if (q <= p)
                             continue;
-                        for (int k = 0; k < 6; ++k)
-                            cb[k] = s[k * (size_t)n + q];
                         ++leaves;
-                        take = overlap6(box, cb);
+                        take = rec_overlap(me.b, load_rec(s, q).b);
                     } else {
                         if (last[child] <= p)
                             continue;
-                        for (int k = 0; k < 6; ++k)
-                            cb[k] = nodes[k * m + child];
-                        take = overlap6(box, cb);
+                        take = rec_overlap(me.b, load_rec(nodes, uint32_t(child)).b);
                     }
                     if (take) {
                         if (sp < kStack)
@@ -238,8 +256,9 @@ __global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float* __res
                 }
             }
         }
-        wave_append(hit, pair_value(a, b), out, &counters->pair_count, cap);
+        wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count, cap);
     }
+    wq_flush(queue, out, &counters->pair_count, cap);
     visits = wave_sum(visits);
     leaves = wave_sum(leaves);
     if (lane_id() == 0) {
@@ -265,18 +284,30 @@ void launch_lbvh(physgpu_context* ctx)
         return;
     time_begin(ctx, "lbvh_build", t);
     hipLaunchKernelGGL(k_build, dim3(div_up(n - 1, kBlock)), dim3(kBlock), 0, ctx->stream, n,
-                       sorted, ctx->node_left, ctx->node_right, ctx->node_parent, ctx->node_last,
-                       ctx->node_flag);
+                       sorted, ctx->node_left, ctx->node_right, ctx->node_parent, ctx->node_first,
+                       ctx->node_last);
     time_end(ctx, t);
-    time_begin(ctx, "lbvh_refit", t);
-    hipLaunchKernelGGL(k_refit, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->sorted_aabb, ctx->node_left, ctx->node_right, ctx->node_parent,
-                       ctx->node_flag, ctx->node_box, ctx->counters);
+    // Group boxes: 32, 1024 and 32768 leaves.
+    const uint32_t c1 = div_up(n, kSpan), c2 = div_up(c1, kSpan), c3 = div_up(c2, kSpan);
+    float4* l1 = ctx->level_box;
+    float4* l2 = l1 + 2 * size_t(c1);
+    float4* l3 = l2 + 2 * size_t(c2);
+    time_begin(ctx, "lbvh_node_boxes", t);
+    hipLaunchKernelGGL(k_group_boxes, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
+                       ctx->sorted_aabb, l1);
+    hipLaunchKernelGGL(k_group_boxes, dim3(div_up(c1, kBlock)), dim3(kBlock), 0, ctx->stream, c1,
+                       l1, l2);
+    hipLaunchKernelGGL(k_group_boxes, dim3(div_up(c2, kBlock)), dim3(kBlock), 0, ctx->stream, c2,
+                       l2, l3);
+    hipLaunchKernelGGL(k_node_boxes, dim3(div_up(n - 1, kBlock)), dim3(kBlock), 0, ctx->stream, n,
+                       ctx->sorted_aabb, l1, l2, l3, ctx->node_first, ctx->node_last,
+                       ctx->node_box);
     time_end(ctx, t);
+    (void)c3;
     time_begin(ctx, "lbvh_query", t);
     hipLaunchKernelGGL(k_query, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->sorted_aabb, ctx->sorted_index, ctx->node_left, ctx->node_right,
-                       ctx->node_last, ctx->node_box, ctx->counters, ctx->pairs_raw, ctx->cap_p);
+                       ctx->sorted_aabb, ctx->node_left, ctx->node_right, ctx->node_last,
+                       ctx->node_box, ctx->counters, ctx->pairs_raw, ctx->cap_p);
     time_end(ctx, t);
 }
 

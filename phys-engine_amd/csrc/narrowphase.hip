@@ -332,7 +332,7 @@ constexpr uint32_t kEpaMaxIterations = PHYSGPU_EPA_MAX_ITERATIONS;
 constexpr uint32_t kMaxVertices = 4 + kEpaMaxIterations;
 constexpr uint32_t kMaxFaces = PHYSGPU_EPA_MAX_FACES;
 constexpr uint32_t kMaxEdges = PHYSGPU_EPA_MAX_HORIZON;
-constexpr uint32_t kGjkGrid = 1024;   // bounds the private (scratch) polytope storage
+constexpr uint32_t kEpaGrid = 1024;   // workgroups of the EPA kernel (4 pairs in flight each)
 
 struct Shape {
     uint32_t type;
@@ -485,20 +485,51 @@ __device__ bool gjk(const Bodies& B, const Shape& sa, const pose& ta, const Shap
     return false;
 }
 
-struct Face {
-    uint8_t a, b, c, pad;
-    f3 n;
-    float d;
+// computeEpa (epa.cpp:64-130) run by one wavefront per pair: the polytope
+// lives in LDS, faces are spread over the 64 lanes (closest face = wave
+// arg-min with the reference's first-minimum rule, visibility = one ballot
+// per 64 faces, order-preserving in-place compaction of the kept faces), the
+// horizon is built by lane 0 in the reference's sequential order with its
+// erase-the-reversed-edge rule, and the new faces are made in parallel, in
+// horizon order.  Every float is computed by the same expression as the
+// reference, so results are bit-identical; only the storage is bounded.
+constexpr uint32_t kEpaWavesPerBlock = kBlock / 64;
+constexpr uint32_t kFaceChunks = (kMaxFaces + 63) / 64;
+
+constexpr uint32_t kEdgeKeys = kMaxVertices * (kMaxVertices - 1) / 2;  // unordered vertex pairs
+constexpr uint32_t kMaxVisibleEdges = 3 * kMaxFaces;
+
+struct EpaShared {
+    float vx[kMaxVertices], vy[kMaxVertices], vz[kMaxVertices];
+    uint32_t fabc[kMaxFaces];  // a | b << 8 | c << 16
+    float fnx[kMaxFaces], fny[kMaxFaces], fnz[kMaxFaces], fd[kMaxFaces];
+    uint8_t ea[kMaxEdges], eb[kMaxEdges];
+    int16_t key_sum[kEdgeKeys];                // horizon bookkeeping, all zero between uses
+    uint32_t push_bits[kMaxVisibleEdges / 32];
+    uint32_t ne, overflow;
 };
 
-// makeFace, epa.cpp:29-47.
-__device__ __forceinline__ Face make_face(const f3* v, uint32_t a, uint32_t b, uint32_t c)
+__device__ __forceinline__ uint32_t edge_key(uint32_t a, uint32_t b)
 {
-    f3 n = cross(sub(v[b], v[a]), sub(v[c], v[a]));
+    const uint32_t u = a < b ? a : b, v = a < b ? b : a;
+    return v * (v - 1) / 2 + u;
+}
+
+__device__ __forceinline__ f3 lds_vertex(const EpaShared& S, uint32_t i)
+{
+    return {S.vx[i], S.vy[i], S.vz[i]};
+}
+
+// makeFace, epa.cpp:29-47, over the LDS vertices; writes face slot f.
+__device__ __forceinline__ void make_face_lds(EpaShared& S, uint32_t f, uint32_t a, uint32_t b,
+                                              uint32_t c)
+{
+    const f3 va = lds_vertex(S, a);
+    f3 n = cross(sub(lds_vertex(S, b), va), sub(lds_vertex(S, c), va));
     const float len = length(n);
     if (len > 1e-8f)
         n = scale(n, 1.0f / len);
-    float d = dot(n, v[a]);
+    float d = dot(n, va);
     if (d < 0.0f) {
         const uint32_t t = b;
         b = c;
@@ -506,87 +537,351 @@ __device__ __forceinline__ Face make_face(const f3* v, uint32_t a, uint32_t b, u
         n = neg(n);
         d = -d;
     }
-    return {uint8_t(a), uint8_t(b), uint8_t(c), 0, n, d};
+    S.fabc[f] = a | (b << 8) | (c << 16);
+    S.fnx[f] = n.x;
+    S.fny[f] = n.y;
+    S.fnz[f] = n.z;
+    S.fd[f] = d;
 }
 
-// computeEpa, epa.cpp:64-130, with bounded polytope storage.
-// Returns 1 converged, 0 failed (reference failure), -1 storage exceeded.
-__device__ int epa(const Bodies& B, const Shape& sa, const pose& ta, const Shape& sb,
-                   const pose& tb, const Simplex& s, uint32_t iterations, float tol, f3& normal,
-                   float& depth)
+// Support map evaluated by the whole wave (same result in every lane).  A
+// hull's points are spread over the lanes; the arg-max keeps the lowest
+// index among equal dot products, as the sequential strict-> scan does.
+// A hull's points held one per lane for the whole EPA (hulls of at most 64
+// points), so support evaluations read no memory.
+struct WaveHull {
+    f3 p;
+    bool cached;
+};
+
+__device__ __forceinline__ WaveHull wave_hull(const Bodies& B, const Shape& s)
 {
+    WaveHull h{{0.0f, 0.0f, 0.0f}, s.type == PHYSGPU_HULL && s.count <= 64};
+    const uint32_t lane = lane_id();
+    if (h.cached && lane < s.count) {
+        const uint32_t p = s.first + lane, nh = B.nh;
+        h.p = {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+    }
+    return h;
+}
+
+__device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
+                           const WaveHull& h)
+{
+    if (s.type != PHYSGPU_HULL)
+        return support(B, s, t, d);
+    const f3 l = rotate(conj(t.q), d);
+    const uint32_t nh = B.nh, lane = lane_id();
+    float bv = 0.0f;
+    uint32_t bi = 0xFFFFFFFFu;
+    for (uint32_t base = 0; base < s.count; base += 64) {
+        const uint32_t k = base + lane;
+        if (k < s.count) {
+            const uint32_t p = s.first + k;
+            const f3 pt = h.cached ? h.p : f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+            const float v = dot(pt, l);
+            if (bi == 0xFFFFFFFFu || v > bv) {
+                bv = v;
+                bi = k;
+            }
+        }
+    }
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const float ov = __shfl_xor(bv, off);
+        const uint32_t oi = (uint32_t)__shfl_xor((int)bi, off);
+        const bool take = oi != 0xFFFFFFFFu &&
+                          (bi == 0xFFFFFFFFu || ov > bv || (ov == bv && oi < bi));
+        if (take) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    if (h.cached)
+        return to_world(f3{__shfl(h.p.x, (int)bi), __shfl(h.p.y, (int)bi), __shfl(h.p.z, (int)bi)},
+                        t);
+    const uint32_t p = s.first + bi;
+    return to_world(f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]}, t);
+}
+
+// Returns 1 converged, 0 failed (the reference's failure), -1 storage bound
+// exceeded.  `normal`/`depth` are uniform across the wave.
+__device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const pose& ta,
+                        const WaveHull& ha, const Shape& sb, const pose& tb, const WaveHull& hb,
+                        const f3 simplex[4], uint32_t iterations, float tol, f3& normal,
+                        float& depth)
+{
+    const uint32_t lane = lane_id();
     normal = {0.0f, 0.0f, 0.0f};
     depth = 0.0f;
-    if (s.size < 4 || iterations == 0)
+    if (iterations == 0)
         return 0;
-    f3 v[kMaxVertices];
-    Face faces[kMaxFaces];
-    uint8_t ea[kMaxEdges], eb[kMaxEdges];
-    uint32_t nv = 4, nf = 4;
-    for (int k = 0; k < 4; ++k)
-        v[k] = s.p[k];
-    faces[0] = make_face(v, 0, 1, 2);
-    faces[1] = make_face(v, 0, 3, 1);
-    faces[2] = make_face(v, 0, 2, 3);
-    faces[3] = make_face(v, 1, 3, 2);
+    if (lane < 4) {
+        S.vx[lane] = simplex[lane].x;
+        S.vy[lane] = simplex[lane].y;
+        S.vz[lane] = simplex[lane].z;
+    }
+    wave_sync();
+    if (lane < 4) {
+        // (0,1,2), (0,3,1), (0,2,3), (1,3,2) — epa.cpp:76-79.
+        const uint32_t fa = lane == 3 ? 1u : 0u;
+        const uint32_t fb = lane == 0 ? 1u : (lane == 1 ? 3u : (lane == 2 ? 2u : 3u));
+        const uint32_t fc = lane == 0 ? 2u : (lane == 1 ? 1u : (lane == 2 ? 3u : 2u));
+        make_face_lds(S, lane, fa, fb, fc);
+    }
+    wave_sync();
+    uint32_t nf = 4, nv = 4;
     for (uint32_t it = 0; it < iterations; ++it) {
         if (nf == 0)
             return 0;
-        uint32_t best = 0;
-        for (uint32_t f = 1; f < nf; ++f)
-            if (faces[f].d < faces[best].d)
-                best = f;
-        const Face cf = faces[best];
-        const f3 sup = sub(support(B, sa, ta, cf.n), support(B, sb, tb, neg(cf.n)));
-        if (dot(sup, cf.n) - cf.d <= tol) {
-            normal = cf.n;
-            depth = max_ref(cf.d, 0.0f);
+        // Closest face: first minimum of the face distances.
+        float bd = 0.0f;
+        uint32_t bi = 0xFFFFFFFFu;
+        for (uint32_t f = lane; f < nf; f += 64) {
+            const float d = S.fd[f];
+            if (bi == 0xFFFFFFFFu || d < bd) {
+                bd = d;
+                bi = f;
+            }
+        }
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const float od = __shfl_xor(bd, off);
+            const uint32_t oi = (uint32_t)__shfl_xor((int)bi, off);
+            const bool take = oi != 0xFFFFFFFFu &&
+                              (bi == 0xFFFFFFFFu || od < bd || (od == bd && oi < bi));
+            if (take) {
+                bd = od;
+                bi = oi;
+            }
+        }
+        const f3 cn{S.fnx[bi], S.fny[bi], S.fnz[bi]};
+        const float cd = S.fd[bi];
+        const f3 sup = sub(support_wave(B, sa, ta, cn, ha), support_wave(B, sb, tb, neg(cn), hb));
+        if (dot(sup, cn) - cd <= tol) {
+            normal = cn;
+            depth = max_ref(cd, 0.0f);
             return 1;
         }
         if (nv >= kMaxVertices)
             return -1;
         const uint32_t si = nv;
-        v[nv++] = sup;
-        uint32_t ne = 0;
-        uint32_t keep = 0;
-        bool overflow = false;
-        auto add_edge = [&](uint32_t a, uint32_t b) {
-            for (uint32_t e = 0; e < ne; ++e)
-                if (ea[e] == b && eb[e] == a) {
-                    for (uint32_t k = e + 1; k < ne; ++k) {
-                        ea[k - 1] = ea[k];
-                        eb[k - 1] = eb[k];
-                    }
-                    --ne;
-                    return;
-                }
-            if (ne < kMaxEdges) {
-                ea[ne] = uint8_t(a);
-                eb[ne] = uint8_t(b);
-                ++ne;
-            } else {
-                overflow = true;  // the unbounded horizon would grow past kMaxEdges
-            }
-        };
-        for (uint32_t f = 0; f < nf; ++f) {
-            const Face fc = faces[f];
-            if (dot(fc.n, sub(sup, v[fc.a])) > tol) {
-                add_edge(fc.a, fc.b);
-                add_edge(fc.b, fc.c);
-                add_edge(fc.c, fc.a);
-            } else {
-                faces[keep++] = fc;
-            }
+        if (lane == 0) {
+            S.vx[si] = sup.x;
+            S.vy[si] = sup.y;
+            S.vz[si] = sup.z;
         }
-        if (overflow)
+        ++nv;
+        wave_sync();
+        // Visibility of every face, one ballot per chunk of 64.
+        uint64_t vis[kFaceChunks];
+        uint32_t visible = 0;
+#pragma unroll
+        for (uint32_t ch = 0; ch < kFaceChunks; ++ch) {
+            const uint32_t f = ch * 64 + lane;
+            bool v = false;
+            if (f < nf) {
+                const uint32_t fa = S.fabc[f] & 0xFFu;
+                v = dot(f3{S.fnx[f], S.fny[f], S.fnz[f]}, sub(sup, lds_vertex(S, fa))) > tol;
+            }
+            vis[ch] = __ballot(v);
+            visible += (uint32_t)__popcll(vis[ch]);
+        }
+        // Horizon (addHorizonEdge, epa.cpp:49-60) in O(edges).  For one
+        // unordered vertex pair the reference's list only ever holds copies
+        // of one direction; with s = running sum of +1 (a < b) / -1 (a > b)
+        // over that pair's edges it holds |s| copies of direction sign(s).  An
+        // edge is appended exactly when it moves s away from zero (else it
+        // erases the oldest reversed copy), so the list size, and the point
+        // where it would pass kMaxEdges, follow from s alone; the survivors
+        // are the last |s_final| appended copies, kept in arrival order.
+        const uint32_t all_edges = 3 * visible;
+        if (all_edges <= 64) {
+            // Up to 64 edges: lane i holds edge i; per-pair sums, the list
+            // size and the survivor counts come from shuffles and a scan.
+            const bool mine = lane < all_edges;
+            uint32_t ea = 0, eb = 0, key = 0xFFFFFFFFu;
+            int delta = 0;
+            if (mine) {
+                uint32_t r = lane / 3, f = 0;
+                const uint32_t k = lane % 3;
+                for (uint32_t ch = 0; ch < kFaceChunks; ++ch) {
+                    const uint32_t c = (uint32_t)__popcll(vis[ch]);
+                    if (r < c) {
+                        uint64_t m = vis[ch];
+                        for (uint32_t t = 0; t < r; ++t)
+                            m &= m - 1;
+                        f = ch * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        break;
+                    }
+                    r -= c;
+                }
+                const uint32_t abc = S.fabc[f];
+                const uint32_t v3[3] = {abc & 0xFFu, (abc >> 8) & 0xFFu, (abc >> 16) & 0xFFu};
+                ea = v3[k];
+                eb = v3[(k + 1) % 3];
+                key = edge_key(ea, eb);
+                delta = ea < eb ? 1 : -1;
+            }
+            int prefix = 0, total = 0;
+            for (uint32_t j = 0; j < all_edges; ++j) {
+                const uint32_t kj = (uint32_t)__shfl((int)key, (int)j);
+                const int dj = __shfl(delta, (int)j);
+                if (kj == key) {
+                    total += dj;
+                    if (j <= lane)
+                        prefix += dj;
+                }
+            }
+            const int before = prefix - delta;
+            const bool append = mine && (prefix < 0 ? -prefix : prefix) > (before < 0 ? -before : before);
+            const int step = mine ? (append ? 1 : -1) : 0;
+            int size = step;
+            for (uint32_t off = 1; off < 64; off <<= 1) {
+                const int t = __shfl_up(size, off);
+                if (lane >= off)
+                    size += t;
+            }
+            const bool over = __ballot(append && size - 1 >= int(kMaxEdges)) != 0;
+            uint32_t later = 0;
+            for (uint32_t j = 0; j < all_edges; ++j) {
+                const uint32_t kj = (uint32_t)__shfl((int)key, (int)j);
+                const int aj = __shfl(int(append), (int)j);
+                if (j > lane && kj == key && aj)
+                    ++later;
+            }
+            const bool survive = append && int(later) < (total < 0 ? -total : total);
+            const uint64_t smask = __ballot(survive);
+            if (survive && !over) {
+                const uint32_t pos = (uint32_t)__popcll(smask & lanes_below(lane));
+                S.ea[pos] = uint8_t(ea);
+                S.eb[pos] = uint8_t(eb);
+            }
+            if (lane == 0) {
+                S.ne = over ? 0u : (uint32_t)__popcll(smask);
+                S.overflow = over ? 1u : 0u;
+            }
+        } else if (lane == 0) {
+            uint32_t total = 0, overflow = 0, nedges = 0;
+            // Pass 1, forward: running sums, append flags, list size; stops
+            // at the edge that would grow the list past kMaxEdges.
+            for (uint32_t ch = 0; ch < kFaceChunks && !overflow; ++ch) {
+                uint64_t m = vis[ch];
+                while (m && !overflow) {
+                    const uint32_t f = ch * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    m &= m - 1;
+                    const uint32_t abc = S.fabc[f];
+                    const uint32_t v3[3] = {abc & 0xFFu, (abc >> 8) & 0xFFu, (abc >> 16) & 0xFFu};
+                    for (int k = 0; k < 3; ++k) {
+                        const uint32_t ea = v3[k], eb = v3[(k + 1) % 3];
+                        const uint32_t key = edge_key(ea, eb);
+                        const int s0 = S.key_sum[key];
+                        const int s1 = s0 + (ea < eb ? 1 : -1);
+                        const bool append = (s1 < 0 ? -s1 : s1) > (s0 < 0 ? -s0 : s0);
+                        if (append && total >= kMaxEdges) {
+                            overflow = 1;
+                            break;
+                        }
+                        total = append ? total + 1 : total - 1;
+                        const uint32_t bit = 1u << (nedges & 31u);
+                        if (append)
+                            S.push_bits[nedges >> 5] |= bit;
+                        else
+                            S.push_bits[nedges >> 5] &= ~bit;
+                        S.key_sum[key] = int16_t(s1);
+                        ++nedges;
+                    }
+                }
+            }
+            // Pass 2, backward over those edges: the last |s| appended copies
+            // of each pair survive (their flag stays set); sums return to 0.
+            uint32_t rank = visible;
+            for (int ch = int(kFaceChunks) - 1; ch >= 0; --ch) {
+                uint64_t m = vis[ch];
+                while (m) {
+                    const uint32_t bit = 63u - (uint32_t)__clzll((unsigned long long)m);
+                    m &= ~(1ull << bit);
+                    --rank;
+                    const uint32_t abc = S.fabc[uint32_t(ch) * 64 + bit];
+                    const uint32_t v3[3] = {abc & 0xFFu, (abc >> 8) & 0xFFu, (abc >> 16) & 0xFFu};
+                    for (int k = 2; k >= 0; --k) {
+                        const uint32_t idx = 3 * rank + uint32_t(k);
+                        if (idx >= nedges)
+                            continue;
+                        const uint32_t key = edge_key(v3[k], v3[(k + 1) % 3]);
+                        const int sum = S.key_sum[key];
+                        const uint32_t flag = 1u << (idx & 31u);
+                        if ((S.push_bits[idx >> 5] & flag) && sum != 0)
+                            S.key_sum[key] = int16_t(sum > 0 ? sum - 1 : sum + 1);
+                        else
+                            S.push_bits[idx >> 5] &= ~flag;
+                    }
+                }
+            }
+            // Pass 3, forward: survivors in arrival order form the horizon.
+            uint32_t ne = 0;
+            if (!overflow) {
+                uint32_t idx = 0;
+                for (uint32_t ch = 0; ch < kFaceChunks; ++ch) {
+                    uint64_t m = vis[ch];
+                    while (m) {
+                        const uint32_t f = ch * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        m &= m - 1;
+                        const uint32_t abc = S.fabc[f];
+                        const uint32_t v3[3] = {abc & 0xFFu, (abc >> 8) & 0xFFu,
+                                                (abc >> 16) & 0xFFu};
+                        for (int k = 0; k < 3; ++k, ++idx)
+                            if (S.push_bits[idx >> 5] & (1u << (idx & 31u))) {
+                                S.ea[ne] = uint8_t(v3[k]);
+                                S.eb[ne] = uint8_t(v3[(k + 1) % 3]);
+                                ++ne;
+                            }
+                    }
+                }
+            }
+            S.ne = ne;
+            S.overflow = overflow;
+        }
+        wave_sync();
+        const uint32_t ne = S.ne;
+        if (S.overflow)
             return -1;
         if (ne == 0)
             return 0;
+        const uint32_t keep = nf - visible;
         if (keep + ne > kMaxFaces)
             return -1;
-        nf = keep;
-        for (uint32_t e = 0; e < ne; ++e)
-            faces[nf++] = make_face(v, ea[e], eb[e], si);
+        // Kept faces move down in order; every write lands at or below the
+        // slot the writing lane read, and after everything already placed.
+        uint32_t out = 0;
+#pragma unroll
+        for (uint32_t ch = 0; ch < kFaceChunks; ++ch) {
+            const uint32_t f = ch * 64 + lane;
+            const bool valid = f < nf;
+            uint32_t abc = 0;
+            float nx = 0.0f, ny = 0.0f, nz = 0.0f, d = 0.0f;
+            if (valid) {
+                abc = S.fabc[f];
+                nx = S.fnx[f];
+                ny = S.fny[f];
+                nz = S.fnz[f];
+                d = S.fd[f];
+            }
+            const uint64_t kept = __ballot(valid) & ~vis[ch];
+            wave_sync();
+            if (valid && !((vis[ch] >> lane) & 1ull)) {
+                const uint32_t pos = out + (uint32_t)__popcll(kept & lanes_below(lane));
+                S.fabc[pos] = abc;
+                S.fnx[pos] = nx;
+                S.fny[pos] = ny;
+                S.fnz[pos] = nz;
+                S.fd[pos] = d;
+            }
+            out += (uint32_t)__popcll(kept);
+            wave_sync();
+        }
+        for (uint32_t e = lane; e < ne; e += 64)
+            make_face_lds(S, out + e, S.ea[e], S.eb[e], si);
+        nf = out + ne;
+        wave_sync();
     }
     return 0;
 }
@@ -615,14 +910,26 @@ __device__ __forceinline__ void wave_count(bool pred, uint32_t* counter)
         atomicAdd(counter, (uint32_t)__popcll(mask));
 }
 
-__global__ __launch_bounds__(kBlock) void k_classify(Bodies B, World W,
-                                                     const uint64_t* __restrict__ pairs,
-                                                     Counters* c, uint32_t* __restrict__ buckets,
-                                                     uint64_t cap, uint32_t* __restrict__ hit)
+constexpr uint32_t kClassifyThreads = 1024;
+constexpr uint32_t kClassifyGrid = 64;
+
+// Shape-pair classification and bucketing.  Counts are aggregated per
+// workgroup of 1024 candidates in LDS, so each class costs one global atomic
+// per workgroup and round.
+__global__ __launch_bounds__(kClassifyThreads) void k_classify(Bodies B, World W,
+                                                               const uint64_t* __restrict__ pairs,
+                                                               Counters* c,
+                                                               uint32_t* __restrict__ buckets,
+                                                               uint64_t cap,
+                                                               uint32_t* __restrict__ hit)
 {
+    constexpr uint32_t kWavesPerBlock = kClassifyThreads / 64;
+    __shared__ uint32_t wave_count[kWavesPerBlock][kClasses];
+    __shared__ uint32_t block_base[kClasses];
     const uint32_t count = c->sort_count;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (count + stride - 1) / stride;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t k = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         uint32_t cls = kClasses;
@@ -635,9 +942,33 @@ __global__ __launch_bounds__(kBlock) void k_classify(Bodies B, World W,
             if (!(W.body && W.body[a] == W.body[b]))
                 cls = pair_class(B.type[a], B.type[b]);
         }
+        uint64_t mine = 0;
 #pragma unroll
-        for (uint32_t q = 0; q < kClasses; ++q)
-            wave_append_u32(cls == q, k, buckets + q * cap, &c->class_count[q]);
+        for (uint32_t q = 0; q < kClasses; ++q) {
+            const uint64_t m = __ballot(cls == q);
+            if (cls == q)
+                mine = m;
+            if (lane == 0)
+                wave_count[wave][q] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (threadIdx.x < kClasses) {
+            const uint32_t q = threadIdx.x;
+            uint32_t run = 0;
+            for (uint32_t w = 0; w < kWavesPerBlock; ++w) {
+                const uint32_t v = wave_count[w][q];
+                wave_count[w][q] = run;
+                run += v;
+            }
+            block_base[q] = run ? atomicAdd(&c->class_count[q], run) : 0u;
+        }
+        __syncthreads();
+        if (cls < kClasses) {
+            const uint32_t slot = block_base[cls] + wave_count[wave][cls] +
+                                  (uint32_t)__popcll(mine & lanes_below(lane));
+            buckets[cls * cap + slot] = k;
+        }
+        __syncthreads();
     }
 }
 
@@ -707,30 +1038,15 @@ __global__ __launch_bounds__(kBlock) void k_class(Bodies B, World W,
                                                   physgpu_manifold* __restrict__ slots,
                                                   uint32_t* __restrict__ hit)
 {
-    // Classes >= 3 (hull pairs) share one launch: buckets 3, 4, 5 in turn.
-    uint32_t total = c->class_count[kClass];
-    if (kClass == 3)
-        total += c->class_count[4] + c->class_count[5];
+    const uint32_t total = c->class_count[kClass];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (total + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool h = false;
-        bool failed = false, overflowed = false;
-        uint32_t cls = kClass;
+        const uint32_t cls = kClass;
         if (t < total) {
-            uint32_t q = t;
-            if (kClass == 3) {
-                const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
-                if (q >= c3 + c4) {
-                    cls = 5;
-                    q -= c3 + c4;
-                } else if (q >= c3) {
-                    cls = 4;
-                    q -= c3;
-                }
-            }
-            const uint32_t k = buckets[cls * cap + q];
+            const uint32_t k = buckets[cls * cap + t];
             const uint64_t pr = pairs[k];
             const uint32_t a = uint32_t(pr >> 32), b = uint32_t(pr);
             const pose ta = load_pose(B, a), tb = load_pose(B, b);
@@ -752,50 +1068,149 @@ __global__ __launch_bounds__(kBlock) void k_class(Bodies B, World W,
                 }
             } else if (kClass == PHYSGPU_PAIR_BOX_BOX) {
                 h = box_box(load_data(B, a), ta, load_data(B, b), tb, ct);
-            } else {
-                const uint32_t n = B.n;
-                Shape sa{B.type[a], load_data(B, a), 0, 0}, sb{B.type[b], load_data(B, b), 0, 0};
-                if (sa.type == PHYSGPU_HULL) {
-                    sa.first = B.hrange[a];
-                    sa.count = B.hrange[n + a];
-                }
-                if (sb.type == PHYSGPU_HULL) {
-                    sb.first = B.hrange[b];
-                    sb.count = B.hrange[n + b];
-                }
-                Simplex s;
-                if (gjk(B, sa, ta, sb, tb, opt.gjk_max_iterations, s)) {
-                    f3 nrm;
-                    float depth;
-                    const int st = epa(B, sa, ta, sb, tb, s, opt.epa_max_iterations,
-                                       opt.epa_tolerance, nrm, depth);
-                    if (st == 1) {
-                        h = true;
-                        ct.normal = nrm;
-                        ct.count = 1;
-                        ct.depth[0] = depth;
-                        const f3 w = scale(add(support(B, sa, ta, nrm), support(B, sb, tb, neg(nrm))),
-                                           0.5f);
-                        set_anchors(ct, 0, w, ta, tb);
-                    } else {
-                        failed = st == 0;
-                        overflowed = st < 0;
-                    }
-                }
             }
             if (h) {
                 store_contact(ct, k, a, b, cls, W, B.n, slots);
                 hit[k] = 1;
             }
         }
-        if (kClass == 3) {
-            for (uint32_t q = 3; q < kClasses; ++q)
-                wave_count(h && cls == q, &c->class_contacts[q]);
-            wave_count(failed, &c->gjk_failures);
-            wave_count(overflowed, &c->epa_overflow);
-        } else {
-            wave_count(h, &c->class_contacts[kClass]);
+        wave_count(h, &c->class_contacts[kClass]);
+    }
+}
+
+__device__ __forceinline__ Shape load_shape(const Bodies& B, uint32_t i)
+{
+    Shape s{B.type[i], load_data(B, i), 0, 0};
+    if (s.type == PHYSGPU_HULL) {
+        s.first = B.hrange[i];
+        s.count = B.hrange[B.n + i];
+    }
+    return s;
+}
+
+// Hull pairs, step 1: GJK one pair per lane (intersectGjk, gjk.cpp:128-158).
+// Hits go to the EPA work list (candidate | class << 29) with their final
+// simplex.
+__global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __restrict__ pairs,
+                                                Counters* c, const uint32_t* __restrict__ buckets,
+                                                uint64_t cap, physgpu_narrowphase_options opt,
+                                                uint32_t* __restrict__ list,
+                                                float* __restrict__ simplex_out)
+{
+    const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
+    const uint32_t total = c3 + c4 + c->class_count[5];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (total + stride - 1) / stride;
+    const uint32_t lane = lane_id();
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool intersect = false;
+        uint32_t entry = 0;
+        Simplex s;
+        s.size = 0;
+        if (t < total) {
+            uint32_t cls = 3, q = t;
+            if (q >= c3 + c4) {
+                cls = 5;
+                q -= c3 + c4;
+            } else if (q >= c3) {
+                cls = 4;
+                q -= c3;
+            }
+            const uint32_t k = buckets[cls * cap + q];
+            const uint64_t pr = pairs[k];
+            const uint32_t a = uint32_t(pr >> 32), b = uint32_t(pr);
+            intersect = gjk(B, load_shape(B, a), load_pose(B, a), load_shape(B, b), load_pose(B, b),
+                            opt.gjk_max_iterations, s);
+            entry = k | (cls << 29);
         }
+        const uint64_t mask = __ballot(intersect);
+        if (mask) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+            uint32_t base = 0;
+            if (lane == leader)
+                base = atomicAdd(&c->epa_count, (uint32_t)__popcll(mask));
+            base = __shfl(base, (int)leader);
+            if (intersect) {
+                const uint32_t slot = base + (uint32_t)__popcll(mask & lanes_below(lane));
+                list[slot] = entry;
+                float* o = simplex_out + size_t(slot) * 12;
+                for (int p = 0; p < 4; ++p) {
+                    o[3 * p] = s.p[p].x;
+                    o[3 * p + 1] = s.p[p].y;
+                    o[3 * p + 2] = s.p[p].z;
+                }
+            }
+        }
+    }
+}
+
+// Hull pairs, step 2: EPA, one wavefront per intersecting pair; the contact
+// point is the midpoint of the two deepest support points along the EPA
+// normal (A toward B).
+__global__ __launch_bounds__(kBlock) void k_epa(Bodies B, World W,
+                                                const uint64_t* __restrict__ pairs, Counters* c,
+                                                const uint32_t* __restrict__ list,
+                                                const float* __restrict__ simplex_in,
+                                                physgpu_narrowphase_options opt,
+                                                physgpu_manifold* __restrict__ slots,
+                                                uint32_t* __restrict__ hit)
+{
+    __shared__ EpaShared shared[kEpaWavesPerBlock];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    EpaShared& S = shared[wave];
+    for (uint32_t i = lane; i < kEdgeKeys; i += 64)
+        S.key_sum[i] = 0;
+    wave_sync();
+    const uint32_t total = c->epa_count;
+    uint32_t contacts[kClasses] = {0, 0, 0, 0, 0, 0};
+    uint32_t failures = 0, overflows = 0;
+    for (uint32_t e = blockIdx.x * kEpaWavesPerBlock + wave; e < total;
+         e += gridDim.x * kEpaWavesPerBlock) {
+        const uint32_t entry = list[e];
+        const uint32_t k = entry & 0x1FFFFFFFu, cls = entry >> 29;
+        const uint64_t pr = pairs[k];
+        const uint32_t a = uint32_t(pr >> 32), b = uint32_t(pr);
+        const Shape sa = load_shape(B, a), sb = load_shape(B, b);
+        const pose ta = load_pose(B, a), tb = load_pose(B, b);
+        f3 simplex[4];
+        const float* si = simplex_in + size_t(e) * 12;
+        for (int p = 0; p < 4; ++p)
+            simplex[p] = {si[3 * p], si[3 * p + 1], si[3 * p + 2]};
+        f3 n;
+        float depth;
+        const WaveHull ha = wave_hull(B, sa), hb = wave_hull(B, sb);
+        const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, simplex, opt.epa_max_iterations,
+                                opt.epa_tolerance, n, depth);
+        if (st == 1) {
+            const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
+                                   support_wave(B, sb, tb, neg(n), hb)),
+                               0.5f);
+            if (lane == 0) {
+                Contact ct;
+                ct.normal = n;
+                ct.count = 1;
+                ct.depth[0] = depth;
+                set_anchors(ct, 0, w, ta, tb);
+                store_contact(ct, k, a, b, cls, W, B.n, slots);
+                hit[k] = 1;
+            }
+            ++contacts[cls];
+        } else if (st == 0) {
+            ++failures;
+        } else {
+            ++overflows;
+        }
+        wave_sync();
+    }
+    if (lane == 0) {
+        for (uint32_t q = 3; q < kClasses; ++q)
+            if (contacts[q])
+                atomicAdd(&c->class_contacts[q], contacts[q]);
+        if (failures)
+            atomicAdd(&c->gjk_failures, failures);
+        if (overflows)
+            atomicAdd(&c->epa_overflow, overflows);
     }
 }
 
@@ -915,6 +1330,7 @@ __global__ void k_np_begin(Counters* c)
         c->gjk_failures = 0;
         c->epa_overflow = 0;
         c->manifold_count = 0;
+        c->epa_count = 0;
     }
 }
 
@@ -939,8 +1355,8 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
     const uint32_t grid = kStrideGrid;
     Timer t;
     time_begin(ctx, "np_classify", t);
-    hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
-                       ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit);
+    hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0, ctx->stream, B,
+                       W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit);
     time_end(ctx, t);
     time_begin(ctx, "np_sphere_sphere", t);
     hipLaunchKernelGGL(k_class<0>, dim3(grid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
@@ -955,9 +1371,14 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
                        ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots, ctx->hit);
     time_end(ctx, t);
     if (ctx->has_hulls) {
-        time_begin(ctx, "np_gjk_epa", t);
-        hipLaunchKernelGGL(k_class<3>, dim3(kGjkGrid), dim3(kBlock), 0, ctx->stream, B, W,
-                           ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
+        time_begin(ctx, "np_gjk", t);
+        hipLaunchKernelGGL(k_gjk, dim3(grid), dim3(kBlock), 0, ctx->stream, B, ctx->pairs,
+                           ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->epa_list,
+                           ctx->epa_simplex);
+        time_end(ctx, t);
+        time_begin(ctx, "np_epa", t);
+        hipLaunchKernelGGL(k_epa, dim3(kEpaGrid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
+                           ctx->counters, ctx->epa_list, ctx->epa_simplex, opt, ctx->slots,
                            ctx->hit);
         time_end(ctx, t);
     }

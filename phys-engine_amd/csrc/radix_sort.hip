@@ -8,7 +8,7 @@
 //
 // One pass = three kernels:
 //   upsweep    per-tile digit histogram (LDS atomics), digit-major layout
-//   scan       one workgroup turns the histogram into global offsets
+//   scan       one workgroup per digit scans that digit's tile counts
 //   downsweep  per-wave stable ranking with 8 ballots per item (64-wide
 //              wavefronts: peers = lanes whose digit matches on all 8 bits),
 //              running per-wave digit counters in LDS, tile reorder in LDS,
@@ -54,35 +54,53 @@ __global__ __launch_bounds__(kSortThreads) void k_upsweep(const uint64_t* __rest
     hist[threadIdx.x * tiles + tile] = counts[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (256 x tiles entries).
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ hist, uint32_t n,
-                                               const uint32_t* dev_count)
+// Exclusive scan of 256 values held one per thread of a 256-thread
+// workgroup (wave scans with shuffles, then the four wave totals).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
 {
-    __shared__ uint32_t sums[1024];
+    __shared__ uint32_t wave_sums[kSortThreads / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (lane >= off)
+            incl += t;
+    }
+    __syncthreads();
+    if (lane == 63)
+        wave_sums[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < kSortThreads / 64; ++w) {
+        if (w < wave)
+            before += wave_sums[w];
+        all += wave_sums[w];
+    }
+    *total = all;
+    return before + incl - v;
+}
+
+// One workgroup per digit: exclusive scan of that digit's row of tile counts
+// (hist[d * tiles + tile]) in place, and the digit's total.
+__global__ __launch_bounds__(kSortThreads) void k_scan_rows(uint32_t* __restrict__ hist,
+                                                            uint32_t n, const uint32_t* dev_count,
+                                                            uint32_t* __restrict__ digit_total)
+{
     const uint32_t count = item_count(n, dev_count);
     const uint32_t tiles = (count + kSortTile - 1) / kSortTile;
-    const uint32_t entries = tiles * 256;
-    const uint32_t per = (entries + 1023) / 1024;
-    const uint32_t begin = min(threadIdx.x * per, entries);
-    const uint32_t end = min(begin + per, entries);
-    uint32_t s = 0;
-    for (uint32_t i = begin; i < end; ++i)
-        s += hist[i];
-    sums[threadIdx.x] = s;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partial sums.
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? sums[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sums[threadIdx.x] += v;
-        __syncthreads();
+    uint32_t* row = hist + size_t(blockIdx.x) * tiles;
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < tiles; base += kSortThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < tiles ? row[i] : 0u;
+        uint32_t chunk;
+        const uint32_t ex = block_exclusive_scan(v, &chunk);
+        if (i < tiles)
+            row[i] = run + ex;
+        run += chunk;
     }
-    uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
-    for (uint32_t i = begin; i < end; ++i) {
-        uint32_t c = hist[i];
-        hist[i] = run;
-        run += c;
-    }
+    if (threadIdx.x == 0)
+        digit_total[blockIdx.x] = run;
 }
 
 constexpr uint32_t kWaves = kSortThreads / 64;
@@ -93,7 +111,8 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
                                                             uint32_t n,
                                                             const uint32_t* dev_count,
                                                             uint32_t shift,
-                                                            const uint32_t* __restrict__ hist)
+                                                            const uint32_t* __restrict__ hist,
+                                                            const uint32_t* __restrict__ digit_total)
 {
     __shared__ uint64_t tile_items[kSortTile];
     __shared__ uint32_t wave_count[kWaves][256];
@@ -112,7 +131,13 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
 
     for (uint32_t w = 0; w < kWaves; ++w)
         wave_count[w][threadIdx.x] = 0;
-    global_start[threadIdx.x] = hist[threadIdx.x * tiles + tile];
+    {
+        // Global start of this tile's digit run: all smaller digits, then
+        // this digit in earlier tiles.
+        uint32_t all;
+        const uint32_t digit_base = block_exclusive_scan(digit_total[threadIdx.x], &all);
+        global_start[threadIdx.x] = digit_base + hist[threadIdx.x * tiles + tile];
+    }
     __syncthreads();
 
     // Each wave ranks its contiguous sub-tile, 64 items at a time, in order.
@@ -218,12 +243,12 @@ void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                            device_count, shift, ctx->histogram);
         time_end(ctx, t);
         time_begin(ctx, "radix_scan", t);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->histogram, n,
-                           device_count);
+        hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(kSortThreads), 0, ctx->stream,
+                           ctx->histogram, n, device_count, ctx->digit_total);
         time_end(ctx, t);
         time_begin(ctx, "radix_downsweep", t);
         hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(kSortThreads), 0, ctx->stream, src,
-                           dst, n, device_count, shift, ctx->histogram);
+                           dst, n, device_count, shift, ctx->histogram, ctx->digit_total);
         time_end(ctx, t);
         uint64_t* tmp = src;
         src = dst;

@@ -1,7 +1,7 @@
 // Sweep and prune on the x axis (src/collision/broadphase.cpp:205-291).
 //
 // Records are sorted by (min.x, index) with the radix sort, then gathered into
-// sorted SoA planes.  The reference scans, for each row i, the records j > i
+// a sorted copy of the 32-byte AABB records.  The reference scans, for each row i, the records j > i
 // until min_j.x > max_i.x and emits the pair when Y and Z overlap inclusively.
 // On gfx950 the sweep is turned around: a workgroup owns 1024 consecutive
 // sorted records as *columns* (4 per lane, held in registers) and streams the
@@ -23,48 +23,40 @@ constexpr uint32_t kColsPerWave = 64 * kColsPerLane;
 constexpr uint32_t kColsPerBlock = kColsPerWave * (kBlock / 64);  // 1024
 constexpr uint32_t kRowChunk = kBlock;
 
-__global__ __launch_bounds__(kBlock) void k_sap_keys(uint32_t n, const float* __restrict__ aabb,
+__global__ __launch_bounds__(kBlock) void k_sap_keys(uint32_t n, const float4* __restrict__ aabb,
                                                      uint64_t* __restrict__ items)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    items[i] = (uint64_t)float_key(aabb[i]) | ((uint64_t)i << 32);
+    items[i] = (uint64_t)float_key(aabb[2 * size_t(i)].x) | ((uint64_t)i << 32);
 }
 
+// Sorted copy of the records (the index rides in the record).
 __global__ __launch_bounds__(kBlock) void k_gather_sorted(uint32_t n,
                                                           const uint64_t* __restrict__ items,
-                                                          const float* __restrict__ aabb,
-                                                          float* __restrict__ sorted,
-                                                          uint32_t* __restrict__ sorted_index)
+                                                          const float4* __restrict__ aabb,
+                                                          float4* __restrict__ sorted)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const uint32_t idx = uint32_t(items[i] >> 32);
-    sorted_index[i] = idx;
-#pragma unroll
-    for (int a = 0; a < 6; ++a)
-        sorted[a * n + i] = aabb[a * n + idx];
+    const size_t idx = size_t(items[i] >> 32);
+    sorted[2 * size_t(i)] = aabb[2 * idx];
+    sorted[2 * size_t(i) + 1] = aabb[2 * idx + 1];
 }
 
-__global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float* __restrict__ s,
-                                                      const uint32_t* __restrict__ sidx,
+__global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float4* __restrict__ s,
                                                       const Counters* counters_ro,
                                                       Counters* counters,
                                                       uint64_t* __restrict__ out, uint64_t cap)
 {
-    __shared__ float r_maxx[kRowChunk], r_miny[kRowChunk], r_maxy[kRowChunk];
-    __shared__ float r_minz[kRowChunk], r_maxz[kRowChunk];
-    __shared__ uint32_t r_idx[kRowChunk];
+    // Rows staged in LDS as {max.x, min.y, max.y, min.z} and {max.z, index}.
+    __shared__ float4 row_a[kRowChunk];
+    __shared__ float2 row_b[kRowChunk];
     __shared__ uint32_t s_lo;
-
-    const float* minx = s;
-    const float* miny = s + n;
-    const float* minz = s + 2 * (size_t)n;
-    const float* maxx = s + 3 * (size_t)n;
-    const float* maxy = s + 4 * (size_t)n;
-    const float* maxz = s + 5 * (size_t)n;
+    __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
+    WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
@@ -80,30 +72,32 @@ __global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float* _
         const uint32_t j = wave_first + k * 64u + lane;
         c_j[k] = j;
         if (j < n) {
-            c_minx[k] = minx[j];
-            c_miny[k] = miny[j];
-            c_maxy[k] = maxy[j];
-            c_minz[k] = minz[j];
-            c_maxz[k] = maxz[j];
-            c_idx[k] = sidx[j];
+            const Rec r = load_rec(s, j);
+            c_minx[k] = r.b[0];
+            c_miny[k] = r.b[1];
+            c_minz[k] = r.b[2];
+            c_maxy[k] = r.b[4];
+            c_maxz[k] = r.b[5];
+            c_idx[k] = r.index;
         } else {
             c_minx[k] = __int_as_float(0x7f800000);  // +inf: never <= max_i.x
             c_miny[k] = c_maxy[k] = c_minz[k] = c_maxz[k] = 0.0f;
             c_idx[k] = 0;
         }
     }
-    const float wave_min_x = wave_first < n ? minx[wave_first] : __int_as_float(0x7f800000);
+    const float wave_min_x =
+        wave_first < n ? s[2 * size_t(wave_first)].x : __int_as_float(0x7f800000);
     const uint32_t wave_last = min(wave_first + kColsPerWave, n);  // rows must be < this
 
     if (threadIdx.x == 0) {
         // First row that can reach column j0: min_i.x >= min_j0.x - max side.
         const float wmax = key_float(counters_ro->wmax_x_key);
-        const float x0 = minx[j0];
+        const float x0 = s[2 * size_t(j0)].x;
         const float bound = (x0 - wmax * 1.001f) - fabsf(x0) * 1e-6f;
         uint32_t lo = 0, hi = j0;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (minx[mid] < bound)
+            if (s[2 * size_t(mid)].x < bound)
                 lo = mid + 1;
             else
                 hi = mid;
@@ -119,50 +113,61 @@ __global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float* _
         {
             const uint32_t i = chunk + threadIdx.x;
             if (i < row_end) {
-                r_maxx[threadIdx.x] = maxx[i];
-                r_miny[threadIdx.x] = miny[i];
-                r_maxy[threadIdx.x] = maxy[i];
-                r_minz[threadIdx.x] = minz[i];
-                r_maxz[threadIdx.x] = maxz[i];
-                r_idx[threadIdx.x] = sidx[i];
+                const Rec r = load_rec(s, i);
+                row_a[threadIdx.x] = make_float4(r.b[3], r.b[1], r.b[4], r.b[2]);
+                row_b[threadIdx.x] = make_float2(r.b[5], __uint_as_float(r.index));
             }
         }
         __syncthreads();
-        const uint32_t rows = min(kRowChunk, row_end - chunk);
-        for (uint32_t r = 0; r < rows; ++r) {
-            const uint32_t i = chunk + r;
-            if (i + 1 >= wave_last)
-                break;  // no column of this wave lies after row i
-            const float rx = r_maxx[r];
-            if (rx < wave_min_x)
-                continue;  // row's window ends before this wave's columns
-            const float ry0 = r_miny[r], ry1 = r_maxy[r], rz0 = r_minz[r], rz1 = r_maxz[r];
-            const uint32_t ri = r_idx[r];
+        // Rows of this chunk that precede at least one column of this wave.
+        const uint32_t limit = wave_last > chunk + 1 ? wave_last - 1 - chunk : 0u;
+        const uint32_t rows = min(min(kRowChunk, row_end - chunk), limit);
+        // Four rows per step: their LDS reads are issued together.
+        for (uint32_t r0 = 0; r0 < rows; r0 += 4) {
+            float4 ra[4];
+            float2 rb[4];
 #pragma unroll
-            for (uint32_t k = 0; k < kColsPerLane; ++k) {
-                const bool hit = c_j[k] > i && c_minx[k] <= rx && ry0 <= c_maxy[k] &&
-                                 ry1 >= c_miny[k] && rz0 <= c_maxz[k] && rz1 >= c_minz[k];
-                wave_append(hit, pair_value(ri, c_idx[k]), out, &counters->pair_count, cap);
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t r = min(r0 + u, rows - 1);
+                ra[u] = row_a[r];
+                rb[u] = row_b[r];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t i = chunk + r0 + u;
+                // Past the chunk's rows, or the row's window ends before this
+                // wave's columns (wave-uniform).
+                if (r0 + u >= rows || ra[u].x < wave_min_x)
+                    continue;
+                const float rx = ra[u].x, ry0 = ra[u].y, ry1 = ra[u].z, rz0 = ra[u].w;
+                const float rz1 = rb[u].x;
+                const uint32_t ri = __float_as_uint(rb[u].y);
+#pragma unroll
+                for (uint32_t k = 0; k < kColsPerLane; ++k) {
+                    const bool hit = c_j[k] > i && c_minx[k] <= rx && ry0 <= c_maxy[k] &&
+                                     ry1 >= c_miny[k] && rz0 <= c_maxz[k] && rz1 >= c_minz[k];
+                    wq_push(queue, hit, pair_value(ri, c_idx[k]), out, &counters->pair_count,
+                            cap);
+                }
             }
         }
     }
+    wq_flush(queue, out, &counters->pair_count, cap);
 }
 
 // xWindowComparisons (broadphase.cpp:241): for row i, the reference tests
 // j = i+1 .. first j with min_j.x > max_i.x, counting j - i - 1.
-__global__ __launch_bounds__(kBlock) void k_window_tests(uint32_t n, const float* __restrict__ s,
+__global__ __launch_bounds__(kBlock) void k_window_tests(uint32_t n, const float4* __restrict__ s,
                                                          Counters* counters)
 {
-    const float* minx = s;
-    const float* maxx = s + 3 * (size_t)n;
     unsigned long long total = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += gridDim.x * blockDim.x) {
-        const float x = maxx[i];
+        const float x = s[2 * size_t(i)].w;
         uint32_t lo = i + 1, hi = n;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (minx[mid] > x)
+            if (s[2 * size_t(mid)].x > x)
                 hi = mid;
             else
                 lo = mid + 1;
@@ -190,8 +195,7 @@ void launch_gather_sorted(physgpu_context* ctx, const uint64_t* sorted_items)
     Timer t;
     time_begin(ctx, "gather_sorted", t);
     hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0,
-                       ctx->stream, ctx->n, sorted_items, ctx->aabb, ctx->sorted_aabb,
-                       ctx->sorted_index);
+                       ctx->stream, ctx->n, sorted_items, ctx->aabb, ctx->sorted_aabb);
     time_end(ctx, t);
 }
 
@@ -200,8 +204,8 @@ void launch_sap_sweep(physgpu_context* ctx)
     Timer t;
     time_begin(ctx, "sap_sweep", t);
     hipLaunchKernelGGL(k_sap_sweep, dim3(div_up(ctx->n, kColsPerBlock)), dim3(kBlock), 0,
-                       ctx->stream, ctx->n, ctx->sorted_aabb, ctx->sorted_index, ctx->counters,
-                       ctx->counters, ctx->pairs_raw, ctx->cap_p);
+                       ctx->stream, ctx->n, ctx->sorted_aabb, ctx->counters, ctx->counters,
+                       ctx->pairs_raw, ctx->cap_p);
     time_end(ctx, t);
 }
 
