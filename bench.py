@@ -103,6 +103,22 @@ def cpu_baseline(scene, seconds_budget=30.0):
                       f"AABB {t['aabb_ms']:.1f} ms; {len(m)} manifolds; wall {wall:.1f} s"}
 
 
+def copy_probe(nbytes=256 << 20):
+    """One device-to-device copy of known size (the HIP runtime's blit kernel),
+    used to scale rocprof FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {nbytes} bytes read + {nbytes} bytes written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,11 +127,15 @@ def main():
     ap.add_argument("--bodies", type=int, default=1048576)
     ap.add_argument("--algorithms", default="sap,grid,tree")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        copy_probe()
     n = args.bodies
     side = 256.0 * (n / 1048576.0) ** (1.0 / 3.0)
     log(f"scene: {n} mixed bodies, side {side:.1f} m")

@@ -751,6 +751,9 @@ Face<T> makeFace(const std::vector<V3<T>>& v, uint32_t a, uint32_t b, uint32_t c
 // Peak polytope sizes seen by epa() (faces, horizon edges, vertices), for
 // sizing bounded GPU storage.
 std::atomic<uint32_t> g_peak_faces{0}, g_peak_edges{0}, g_peak_vertices{0};
+// Histogram of EPA iterations per call (index = iterations run, 0..64) and of
+// visible-face counts per iteration (0..255, capped).
+std::atomic<uint64_t> g_epa_iters[66], g_epa_visible[257];
 inline void note_peak(std::atomic<uint32_t>& a, size_t v)
 {
     uint32_t cur = a.load();
@@ -776,7 +779,11 @@ int epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, const N
     struct Edge {
         uint32_t a, b;
     };
-    for (uint32_t it = 0; it < iterations; ++it) {
+    struct IterNote {
+        uint32_t it = 0;
+        ~IterNote() { g_epa_iters[it < 65 ? it : 65]++; }
+    } note;
+    for (uint32_t it = 0; it < iterations; ++it, ++note.it) {
         if (faces.empty())
             return 0;
         size_t best = 0;
@@ -817,6 +824,10 @@ int epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, const N
         }
         note_peak(g_peak_edges, horizon.size());
         note_peak(g_peak_vertices, v.size());
+        {
+            size_t vis = faces.size() - keep.size();
+            g_epa_visible[vis < 256 ? vis : 256]++;
+        }
         if (overflow)
             return -1;
         if (horizon.empty())
@@ -1012,6 +1023,21 @@ void findPairs(const physgpu_aabb* bounds, uint32_t n, int algorithm, uint32_t w
 extern "C" {
 
 const char* orc_last_error() { return lastError.c_str(); }
+
+// EPA iteration histogram (66 counters) and visible-face histogram (257).
+void orc_epa_histograms(uint64_t* iters, uint64_t* visible, int reset)
+{
+    for (int i = 0; i < 66; ++i) {
+        iters[i] = np::g_epa_iters[i].load();
+        if (reset)
+            np::g_epa_iters[i] = 0;
+    }
+    for (int i = 0; i < 257; ++i) {
+        visible[i] = np::g_epa_visible[i].load();
+        if (reset)
+            np::g_epa_visible[i] = 0;
+    }
+}
 
 // EPA polytope bounds used by the following calls (0, 0 = unbounded, the
 // reference's behaviour).  Test infrastructure: not thread-safe.

@@ -504,7 +504,6 @@ struct EpaShared {
     uint32_t fabc[kMaxFaces];  // a | b << 8 | c << 16
     float fnx[kMaxFaces], fny[kMaxFaces], fnz[kMaxFaces], fd[kMaxFaces];
     uint8_t ea[kMaxEdges], eb[kMaxEdges];
-    int16_t key_sum[kEdgeKeys];                // horizon bookkeeping, all zero between uses
     uint32_t push_bits[kMaxVisibleEdges / 32];
     uint32_t ne, overflow;
 };
@@ -605,10 +604,12 @@ __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
 
 // Returns 1 converged, 0 failed (the reference's failure), -1 storage bound
 // exceeded.  `normal`/`depth` are uniform across the wave.
-__device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const pose& ta,
-                        const WaveHull& ha, const Shape& sb, const pose& tb, const WaveHull& hb,
-                        const f3 simplex[4], uint32_t iterations, float tol, f3& normal,
-                        float& depth)
+// `key_sum` (kEdgeKeys entries, all zero between uses) is this wave's
+// bookkeeping table for horizons of more than 64 edges (rare, global memory).
+__device__ int epa_wave(EpaShared& S, int16_t* __restrict__ key_sum, const Bodies& B,
+                        const Shape& sa, const pose& ta, const WaveHull& ha, const Shape& sb,
+                        const pose& tb, const WaveHull& hb, const f3 simplex[4],
+                        uint32_t iterations, float tol, f3& normal, float& depth)
 {
     const uint32_t lane = lane_id();
     normal = {0.0f, 0.0f, 0.0f};
@@ -773,7 +774,7 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
                     for (int k = 0; k < 3; ++k) {
                         const uint32_t ea = v3[k], eb = v3[(k + 1) % 3];
                         const uint32_t key = edge_key(ea, eb);
-                        const int s0 = S.key_sum[key];
+                        const int s0 = key_sum[key];
                         const int s1 = s0 + (ea < eb ? 1 : -1);
                         const bool append = (s1 < 0 ? -s1 : s1) > (s0 < 0 ? -s0 : s0);
                         if (append && total >= kMaxEdges) {
@@ -786,7 +787,7 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
                             S.push_bits[nedges >> 5] |= bit;
                         else
                             S.push_bits[nedges >> 5] &= ~bit;
-                        S.key_sum[key] = int16_t(s1);
+                        key_sum[key] = int16_t(s1);
                         ++nedges;
                     }
                 }
@@ -807,10 +808,10 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
                         if (idx >= nedges)
                             continue;
                         const uint32_t key = edge_key(v3[k], v3[(k + 1) % 3]);
-                        const int sum = S.key_sum[key];
+                        const int sum = key_sum[key];
                         const uint32_t flag = 1u << (idx & 31u);
                         if ((S.push_bits[idx >> 5] & flag) && sum != 0)
-                            S.key_sum[key] = int16_t(sum > 0 ? sum - 1 : sum + 1);
+                            key_sum[key] = int16_t(sum > 0 ? sum - 1 : sum + 1);
                         else
                             S.push_bits[idx >> 5] &= ~flag;
                     }
@@ -1148,25 +1149,28 @@ __global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __rest
 // Hull pairs, step 2: EPA, one wavefront per intersecting pair; the contact
 // point is the midpoint of the two deepest support points along the EPA
 // normal (A toward B).
-__global__ __launch_bounds__(kBlock) void k_epa(Bodies B, World W,
-                                                const uint64_t* __restrict__ pairs, Counters* c,
-                                                const uint32_t* __restrict__ list,
-                                                const float* __restrict__ simplex_in,
-                                                physgpu_narrowphase_options opt,
-                                                physgpu_manifold* __restrict__ slots,
-                                                uint32_t* __restrict__ hit)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
+           const uint32_t* __restrict__ list, const float* __restrict__ simplex_in,
+           physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
+           uint32_t* __restrict__ hit, int16_t* __restrict__ key_tables)
 {
     __shared__ EpaShared shared[kEpaWavesPerBlock];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     EpaShared& S = shared[wave];
-    for (uint32_t i = lane; i < kEdgeKeys; i += 64)
-        S.key_sum[i] = 0;
-    wave_sync();
+    int16_t* key_sum = key_tables + size_t(blockIdx.x * kEpaWavesPerBlock + wave) * kEdgeKeys;
     const uint32_t total = c->epa_count;
     uint32_t contacts[kClasses] = {0, 0, 0, 0, 0, 0};
     uint32_t failures = 0, overflows = 0;
-    for (uint32_t e = blockIdx.x * kEpaWavesPerBlock + wave; e < total;
-         e += gridDim.x * kEpaWavesPerBlock) {
+    // Pairs are taken from a shared counter so long EPA runs do not pile up
+    // on one wave.
+    for (;;) {
+        uint32_t e = 0;
+        if (lane == 0)
+            e = atomicAdd(&c->epa_next, 1u);
+        e = __shfl(e, 0);
+        if (e >= total)
+            break;
         const uint32_t entry = list[e];
         const uint32_t k = entry & 0x1FFFFFFFu, cls = entry >> 29;
         const uint64_t pr = pairs[k];
@@ -1180,8 +1184,8 @@ __global__ __launch_bounds__(kBlock) void k_epa(Bodies B, World W,
         f3 n;
         float depth;
         const WaveHull ha = wave_hull(B, sa), hb = wave_hull(B, sb);
-        const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, simplex, opt.epa_max_iterations,
-                                opt.epa_tolerance, n, depth);
+        const int st = epa_wave(S, key_sum, B, sa, ta, ha, sb, tb, hb, simplex,
+                                opt.epa_max_iterations, opt.epa_tolerance, n, depth);
         if (st == 1) {
             const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
                                    support_wave(B, sb, tb, neg(n), hb)),
@@ -1331,10 +1335,16 @@ __global__ void k_np_begin(Counters* c)
         c->epa_overflow = 0;
         c->manifold_count = 0;
         c->epa_count = 0;
+        c->epa_next = 0;
     }
 }
 
 }  // namespace
+
+size_t epa_key_table_entries()
+{
+    return size_t(kEpaGrid) * kEpaWavesPerBlock * kEdgeKeys;
+}
 
 void launch_np_begin(physgpu_context* ctx)
 {
@@ -1379,7 +1389,7 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
         time_begin(ctx, "np_epa", t);
         hipLaunchKernelGGL(k_epa, dim3(kEpaGrid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
                            ctx->counters, ctx->epa_list, ctx->epa_simplex, opt, ctx->slots,
-                           ctx->hit);
+                           ctx->hit, ctx->epa_key_tables);
         time_end(ctx, t);
     }
     time_begin(ctx, "np_compact", t);
