@@ -96,7 +96,7 @@ void free_all(physgpu_context* c)
                     c->node_right, c->node_parent, c->node_first, c->node_last, c->node_box, c->level_box,
                     c->pairs_raw, c->pairs_tmp, c->buckets, c->slots, c->hit, c->tile_sums,
                     c->manifolds, c->body, c->local_pose, c->friction, c->restitution,
-                    c->epa_list, c->epa_simplex, c->epa_key_tables,
+                    c->epa_list, c->epa_simplex,
                     c->counters, c->world_body_pose};
     for (void* p : ptrs)
         if (p)
@@ -298,7 +298,6 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->manifolds, P));
     A(alloc(c->epa_list, P));
     A(alloc(c->epa_simplex, 12 * P));
-    A(alloc(c->epa_key_tables, epa_key_table_entries()));
     A(alloc(c->body, N));
     A(alloc(c->local_pose, 7 * N));
     A(alloc(c->friction, N));
@@ -312,9 +311,6 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     }
     std::memset(c->host_counters, 0, sizeof(Counters));
     c->pairs = c->pairs_raw;
-    // The EPA horizon tables must start at zero (kernels leave them zero).
-    PG_TRY_HIP(hipMemsetAsync(c->epa_key_tables, 0, epa_key_table_entries() * sizeof(int16_t),
-                              c->stream));
     launch_step_begin(c);
     PG_TRY_HIP(hipStreamSynchronize(c->stream));
     c->np_options = {64, 64, 1e-6f, 0};

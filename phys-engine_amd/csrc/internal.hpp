@@ -126,7 +126,6 @@ struct physgpu_context {
     uint32_t* tile_sums = nullptr;     // compaction
     uint32_t* epa_list = nullptr;      // EPA work: candidate | class << 29
     float* epa_simplex = nullptr;      // 12 floats per EPA work item
-    int16_t* epa_key_tables = nullptr; // per-wave horizon tables of the EPA kernel (kept zero)
     physgpu_manifold* manifolds = nullptr;  // compacted, candidate order
     physgpu_narrowphase_options np_options{};
 
@@ -215,7 +214,6 @@ void launch_pairs_out(physgpu_context* ctx, physgpu_pair* dev_out, uint64_t coun
 void launch_pairs_in(physgpu_context* ctx, const physgpu_pair* dev_in, uint64_t count);
 // narrowphase.hip
 void launch_np_begin(physgpu_context* ctx);
-size_t epa_key_table_entries();
 void launch_narrowphase(physgpu_context* ctx, bool world);
 
 }  // namespace pg

@@ -489,23 +489,28 @@ __device__ bool gjk(const Bodies& B, const Shape& sa, const pose& ta, const Shap
 // lives in LDS, faces are spread over the 64 lanes (closest face = wave
 // arg-min with the reference's first-minimum rule, visibility = one ballot
 // per 64 faces, order-preserving in-place compaction of the kept faces), the
-// horizon is built by lane 0 in the reference's sequential order with its
-// erase-the-reversed-edge rule, and the new faces are made in parallel, in
-// horizon order.  Every float is computed by the same expression as the
-// reference, so results are bit-identical; only the storage is bounded.
+// horizon is rebuilt from the visible faces' edges with the reference's
+// erase-the-reversed-edge rule (see horizon_wave), and the new faces are made
+// in parallel, in horizon order.  Every float is computed by the same
+// expression as the reference, so results are bit-identical; only the
+// storage is bounded.
 constexpr uint32_t kEpaWavesPerBlock = kBlock / 64;
 constexpr uint32_t kFaceChunks = (kMaxFaces + 63) / 64;
 
 constexpr uint32_t kEdgeKeys = kMaxVertices * (kMaxVertices - 1) / 2;  // unordered vertex pairs
+constexpr uint32_t kEdgeKeyBits = 12;
+static_assert(kEdgeKeys <= (1u << kEdgeKeyBits), "edge keys must fit kEdgeKeyBits");
 constexpr uint32_t kMaxVisibleEdges = 3 * kMaxFaces;
+static_assert(kMaxFaces <= 256 && kMaxVertices <= 256, "face / vertex indices are bytes");
 
 struct EpaShared {
     float vx[kMaxVertices], vy[kMaxVertices], vz[kMaxVertices];
     uint32_t fabc[kMaxFaces];  // a | b << 8 | c << 16
     float fnx[kMaxFaces], fny[kMaxFaces], fnz[kMaxFaces], fd[kMaxFaces];
     uint8_t ea[kMaxEdges], eb[kMaxEdges];
-    uint32_t push_bits[kMaxVisibleEdges / 32];
-    uint32_t ne, overflow;
+    uint8_t vis_list[kMaxFaces];                  // visible faces, in index order
+    uint32_t edge_bits[kMaxVisibleEdges / 32];    // per-edge flags of the horizon passes
+    int16_t key_sum[kEdgeKeys];                   // per vertex pair, all zero between uses
 };
 
 __device__ __forceinline__ uint32_t edge_key(uint32_t a, uint32_t b)
@@ -543,9 +548,6 @@ __device__ __forceinline__ void make_face_lds(EpaShared& S, uint32_t f, uint32_t
     S.fd[f] = d;
 }
 
-// Support map evaluated by the whole wave (same result in every lane).  A
-// hull's points are spread over the lanes; the arg-max keeps the lowest
-// index among equal dot products, as the sequential strict-> scan does.
 // A hull's points held one per lane for the whole EPA (hulls of at most 64
 // points), so support evaluations read no memory.
 struct WaveHull {
@@ -564,6 +566,9 @@ __device__ __forceinline__ WaveHull wave_hull(const Bodies& B, const Shape& s)
     return h;
 }
 
+// Support map evaluated by the whole wave (same result in every lane).  A
+// hull's points are spread over the lanes; the arg-max keeps the lowest
+// index among equal dot products, as the sequential strict-> scan does.
 __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
                            const WaveHull& h)
 {
@@ -572,44 +577,171 @@ __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
     const f3 l = rotate(conj(t.q), d);
     const uint32_t nh = B.nh, lane = lane_id();
     float bv = 0.0f;
-    uint32_t bi = 0xFFFFFFFFu;
+    uint32_t bi = kNoIndex;
     for (uint32_t base = 0; base < s.count; base += 64) {
         const uint32_t k = base + lane;
         if (k < s.count) {
             const uint32_t p = s.first + k;
             const f3 pt = h.cached ? h.p : f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
             const float v = dot(pt, l);
-            if (bi == 0xFFFFFFFFu || v > bv) {
+            if (bi == kNoIndex || v > bv) {
                 bv = v;
                 bi = k;
             }
         }
     }
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-        const float ov = __shfl_xor(bv, off);
-        const uint32_t oi = (uint32_t)__shfl_xor((int)bi, off);
-        const bool take = oi != 0xFFFFFFFFu &&
-                          (bi == 0xFFFFFFFFu || ov > bv || (ov == bv && oi < bi));
-        if (take) {
-            bv = ov;
-            bi = oi;
-        }
-    }
+    wave_arg_best<true>(bv, bi);
+    bi = uniform(bi);
     if (h.cached)
-        return to_world(f3{__shfl(h.p.x, (int)bi), __shfl(h.p.y, (int)bi), __shfl(h.p.z, (int)bi)},
+        return to_world(f3{lane_value(h.p.x, bi), lane_value(h.p.y, bi), lane_value(h.p.z, bi)},
                         t);
     const uint32_t p = s.first + bi;
     return to_world(f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]}, t);
 }
 
+// Edge i of the visible faces' edge list: face vis_list[i / 3], edge i % 3
+// of (a,b), (b,c), (c,a) — the order epa.cpp:106-113 feeds addHorizonEdge.
+struct EdgeLane {
+    uint32_t a, b, key;
+    int delta;       // +1 for a < b, -1 otherwise
+    bool valid;
+    uint64_t same;   // lanes of this chunk holding the same vertex pair
+};
+
+__device__ __forceinline__ EdgeLane edge_lane(const EpaShared& S, uint32_t i, uint32_t edges)
+{
+    EdgeLane e{0, 0, 0, 0, i < edges, 0};
+    if (e.valid) {
+        const uint32_t r = i / 3, k = i - 3 * r;
+        const uint32_t abc = S.fabc[S.vis_list[r]];
+        const uint32_t k1 = k == 2 ? 0u : k + 1;
+        e.a = (abc >> (8 * k)) & 0xFFu;
+        e.b = (abc >> (8 * k1)) & 0xFFu;
+        e.key = edge_key(e.a, e.b);
+        e.delta = e.a < e.b ? 1 : -1;
+    }
+    e.same = match_lanes<kEdgeKeyBits>(e.key, __ballot(e.valid));
+    return e;
+}
+
+__device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
+
+// Horizon (addHorizonEdge, epa.cpp:49-60) of `edges` visible-face edges.
+// For one unordered vertex pair the reference's list only ever holds copies
+// of one direction; with s = running sum of +1 (a < b) / -1 (a > b) over that
+// pair's edges it holds |s| copies of direction sign(s).  An edge is appended
+// exactly when it moves s away from zero (else it erases the oldest reversed
+// copy), so the list size, and the point where it would pass kMaxEdges,
+// follow from s alone; the survivors are the last |s_final| appended copies,
+// kept in arrival order.  Edges are taken 64 per step in arrival order; lanes
+// with the same pair find each other by ballots over the key bits, so sums
+// and counts inside a step are popcounts.  With more than one step, the
+// LDS table key_sum carries each pair's sum forward, then the count of
+// copies still to keep backward, ending all zero.  Writes the horizon to
+// S.ea/S.eb; returns its size, or -1 when the list would pass kMaxEdges.
+__device__ int horizon_wave(EpaShared& S, uint32_t edges)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t below = lanes_below(lane);
+    const uint64_t upto = below | (1ull << lane);
+    const uint32_t steps = (edges + 63) / 64;
+    const bool multi = steps > 1;
+    // Forward: append flags and list size.
+    int list_size = 0;
+    uint64_t single_append = 0;
+    for (uint32_t c = 0; c < steps; ++c) {
+        const EdgeLane e = edge_lane(S, c * 64 + lane, edges);
+        const uint64_t up = __ballot(e.valid && e.delta > 0);
+        const int carry = (multi && e.valid) ? int(S.key_sum[e.key]) : 0;
+        const int sum = carry + 2 * __popcll(e.same & up & upto) - __popcll(e.same & upto);
+        const bool append = e.valid && iabs(sum) > iabs(sum - e.delta);
+        const uint64_t A = __ballot(append), V = __ballot(e.valid);
+        const int size_before = list_size + 2 * __popcll(A & below) - __popcll(V & below);
+        const bool over = __ballot(append && size_before >= int(kMaxEdges)) != 0;
+        if (multi) {
+            if (e.valid && (e.same & ~upto) == 0)
+                S.key_sum[e.key] =
+                    int16_t(carry + 2 * __popcll(e.same & up) - __popcll(e.same));
+            if (lane == 0) {
+                S.edge_bits[2 * c] = uint32_t(A);
+                S.edge_bits[2 * c + 1] = uint32_t(A >> 32);
+            }
+        }
+        single_append = A;
+        list_size += 2 * __popcll(A) - __popcll(V);
+        wave_sync();
+        if (over) {
+            if (multi) {
+                for (uint32_t z = 0; z <= c; ++z) {
+                    const EdgeLane ez = edge_lane(S, z * 64 + lane, edges);
+                    if (ez.valid)
+                        S.key_sum[ez.key] = 0;
+                }
+                wave_sync();
+            }
+            return -1;
+        }
+    }
+    if (!multi) {
+        const EdgeLane e = edge_lane(S, lane, edges);
+        const uint64_t A = single_append;
+        const uint64_t up = __ballot(e.valid && e.delta > 0);
+        const int total = 2 * __popcll(e.same & up) - __popcll(e.same);
+        const bool survive = ((A >> lane) & 1ull) &&
+                             __popcll(e.same & A & ~upto) < iabs(total);
+        const uint64_t K = __ballot(survive);
+        if (survive) {
+            const uint32_t pos = (uint32_t)__popcll(K & below);
+            S.ea[pos] = uint8_t(e.a);
+            S.eb[pos] = uint8_t(e.b);
+        }
+        wave_sync();
+        return __popcll(K);
+    }
+    // Backward: the last |s_final| appended copies of each pair survive.
+    for (int c = int(steps) - 1; c >= 0; --c) {
+        const EdgeLane e = edge_lane(S, uint32_t(c) * 64 + lane, edges);
+        const uint64_t A =
+            uint64_t(S.edge_bits[2 * c]) | (uint64_t(S.edge_bits[2 * c + 1]) << 32);
+        const int keep = e.valid ? iabs(int(S.key_sum[e.key])) : 0;
+        const bool survive = ((A >> lane) & 1ull) && __popcll(e.same & A & ~upto) < keep;
+        const uint64_t K = __ballot(survive);
+        if (e.valid && (e.same & below) == 0) {
+            const int left = keep - __popcll(e.same & A);
+            S.key_sum[e.key] = int16_t(left > 0 ? left : 0);
+        }
+        if (lane == 0) {
+            S.edge_bits[2 * c] = uint32_t(K);
+            S.edge_bits[2 * c + 1] = uint32_t(K >> 32);
+        }
+        wave_sync();
+    }
+    // Survivors in arrival order.
+    uint32_t ne = 0;
+    for (uint32_t c = 0; c < steps; ++c) {
+        const uint32_t i = c * 64 + lane;
+        const uint64_t K =
+            uint64_t(S.edge_bits[2 * c]) | (uint64_t(S.edge_bits[2 * c + 1]) << 32);
+        if ((K >> lane) & 1ull) {
+            const uint32_t r = i / 3, k = i - 3 * r;
+            const uint32_t abc = S.fabc[S.vis_list[r]];
+            const uint32_t k1 = k == 2 ? 0u : k + 1;
+            const uint32_t pos = ne + (uint32_t)__popcll(K & below);
+            S.ea[pos] = uint8_t((abc >> (8 * k)) & 0xFFu);
+            S.eb[pos] = uint8_t((abc >> (8 * k1)) & 0xFFu);
+        }
+        ne += (uint32_t)__popcll(K);
+    }
+    wave_sync();
+    return int(ne);
+}
+
 // Returns 1 converged, 0 failed (the reference's failure), -1 storage bound
 // exceeded.  `normal`/`depth` are uniform across the wave.
-// `key_sum` (kEdgeKeys entries, all zero between uses) is this wave's
-// bookkeeping table for horizons of more than 64 edges (rare, global memory).
-__device__ int epa_wave(EpaShared& S, int16_t* __restrict__ key_sum, const Bodies& B,
-                        const Shape& sa, const pose& ta, const WaveHull& ha, const Shape& sb,
-                        const pose& tb, const WaveHull& hb, const f3 simplex[4],
-                        uint32_t iterations, float tol, f3& normal, float& depth)
+__device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const pose& ta,
+                        const WaveHull& ha, const Shape& sb, const pose& tb, const WaveHull& hb,
+                        const f3 simplex[4], uint32_t iterations, float tol, f3& normal,
+                        float& depth)
 {
     const uint32_t lane = lane_id();
     normal = {0.0f, 0.0f, 0.0f};
@@ -636,24 +768,16 @@ __device__ int epa_wave(EpaShared& S, int16_t* __restrict__ key_sum, const Bodie
             return 0;
         // Closest face: first minimum of the face distances.
         float bd = 0.0f;
-        uint32_t bi = 0xFFFFFFFFu;
+        uint32_t bi = kNoIndex;
         for (uint32_t f = lane; f < nf; f += 64) {
             const float d = S.fd[f];
-            if (bi == 0xFFFFFFFFu || d < bd) {
+            if (bi == kNoIndex || d < bd) {
                 bd = d;
                 bi = f;
             }
         }
-        for (uint32_t off = 1; off < 64; off <<= 1) {
-            const float od = __shfl_xor(bd, off);
-            const uint32_t oi = (uint32_t)__shfl_xor((int)bi, off);
-            const bool take = oi != 0xFFFFFFFFu &&
-                              (bi == 0xFFFFFFFFu || od < bd || (od == bd && oi < bi));
-            if (take) {
-                bd = od;
-                bi = oi;
-            }
-        }
+        wave_arg_best<false>(bd, bi);
+        bi = uniform(bi);
         const f3 cn{S.fnx[bi], S.fny[bi], S.fnz[bi]};
         const float cd = S.fd[bi];
         const f3 sup = sub(support_wave(B, sa, ta, cn, ha), support_wave(B, sb, tb, neg(cn), hb));
@@ -672,7 +796,8 @@ __device__ int epa_wave(EpaShared& S, int16_t* __restrict__ key_sum, const Bodie
         }
         ++nv;
         wave_sync();
-        // Visibility of every face, one ballot per chunk of 64.
+        // Visibility of every face, one ballot per chunk of 64; the visible
+        // faces are listed in index order.
         uint64_t vis[kFaceChunks];
         uint32_t visible = 0;
 #pragma unroll
@@ -684,167 +809,16 @@ __device__ int epa_wave(EpaShared& S, int16_t* __restrict__ key_sum, const Bodie
                 v = dot(f3{S.fnx[f], S.fny[f], S.fnz[f]}, sub(sup, lds_vertex(S, fa))) > tol;
             }
             vis[ch] = __ballot(v);
+            if (v)
+                S.vis_list[visible + (uint32_t)__popcll(vis[ch] & lanes_below(lane))] =
+                    uint8_t(f);
             visible += (uint32_t)__popcll(vis[ch]);
         }
-        // Horizon (addHorizonEdge, epa.cpp:49-60) in O(edges).  For one
-        // unordered vertex pair the reference's list only ever holds copies
-        // of one direction; with s = running sum of +1 (a < b) / -1 (a > b)
-        // over that pair's edges it holds |s| copies of direction sign(s).  An
-        // edge is appended exactly when it moves s away from zero (else it
-        // erases the oldest reversed copy), so the list size, and the point
-        // where it would pass kMaxEdges, follow from s alone; the survivors
-        // are the last |s_final| appended copies, kept in arrival order.
-        const uint32_t all_edges = 3 * visible;
-        if (all_edges <= 64) {
-            // Up to 64 edges: lane i holds edge i; per-pair sums, the list
-            // size and the survivor counts come from shuffles and a scan.
-            const bool mine = lane < all_edges;
-            uint32_t ea = 0, eb = 0, key = 0xFFFFFFFFu;
-            int delta = 0;
-            if (mine) {
-                uint32_t r = lane / 3, f = 0;
-                const uint32_t k = lane % 3;
-                for (uint32_t ch = 0; ch < kFaceChunks; ++ch) {
-                    const uint32_t c = (uint32_t)__popcll(vis[ch]);
-                    if (r < c) {
-                        uint64_t m = vis[ch];
-                        for (uint32_t t = 0; t < r; ++t)
-                            m &= m - 1;
-                        f = ch * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                        break;
-                    }
-                    r -= c;
-                }
-                const uint32_t abc = S.fabc[f];
-                const uint32_t v3[3] = {abc & 0xFFu, (abc >> 8) & 0xFFu, (abc >> 16) & 0xFFu};
-                ea = v3[k];
-                eb = v3[(k + 1) % 3];
-                key = edge_key(ea, eb);
-                delta = ea < eb ? 1 : -1;
-            }
-            int prefix = 0, total = 0;
-            for (uint32_t j = 0; j < all_edges; ++j) {
-                const uint32_t kj = (uint32_t)__shfl((int)key, (int)j);
-                const int dj = __shfl(delta, (int)j);
-                if (kj == key) {
-                    total += dj;
-                    if (j <= lane)
-                        prefix += dj;
-                }
-            }
-            const int before = prefix - delta;
-            const bool append = mine && (prefix < 0 ? -prefix : prefix) > (before < 0 ? -before : before);
-            const int step = mine ? (append ? 1 : -1) : 0;
-            int size = step;
-            for (uint32_t off = 1; off < 64; off <<= 1) {
-                const int t = __shfl_up(size, off);
-                if (lane >= off)
-                    size += t;
-            }
-            const bool over = __ballot(append && size - 1 >= int(kMaxEdges)) != 0;
-            uint32_t later = 0;
-            for (uint32_t j = 0; j < all_edges; ++j) {
-                const uint32_t kj = (uint32_t)__shfl((int)key, (int)j);
-                const int aj = __shfl(int(append), (int)j);
-                if (j > lane && kj == key && aj)
-                    ++later;
-            }
-            const bool survive = append && int(later) < (total < 0 ? -total : total);
-            const uint64_t smask = __ballot(survive);
-            if (survive && !over) {
-                const uint32_t pos = (uint32_t)__popcll(smask & lanes_below(lane));
-                S.ea[pos] = uint8_t(ea);
-                S.eb[pos] = uint8_t(eb);
-            }
-            if (lane == 0) {
-                S.ne = over ? 0u : (uint32_t)__popcll(smask);
-                S.overflow = over ? 1u : 0u;
-            }
-        } else if (lane == 0) {
-            uint32_t total = 0, overflow = 0, nedges = 0;
-            // Pass 1, forward: running sums, append flags, list size; stops
-            // at the edge that would grow the list past kMaxEdges.
-            for (uint32_t ch = 0; ch < kFaceChunks && !overflow; ++ch) {
-                uint64_t m = vis[ch];
-                while (m && !overflow) {
-                    const uint32_t f = ch * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                    m &= m - 1;
-                    const uint32_t abc = S.fabc[f];
-                    const uint32_t v3[3] = {abc & 0xFFu, (abc >> 8) & 0xFFu, (abc >> 16) & 0xFFu};
-                    for (int k = 0; k < 3; ++k) {
-                        const uint32_t ea = v3[k], eb = v3[(k + 1) % 3];
-                        const uint32_t key = edge_key(ea, eb);
-                        const int s0 = key_sum[key];
-                        const int s1 = s0 + (ea < eb ? 1 : -1);
-                        const bool append = (s1 < 0 ? -s1 : s1) > (s0 < 0 ? -s0 : s0);
-                        if (append && total >= kMaxEdges) {
-                            overflow = 1;
-                            break;
-                        }
-                        total = append ? total + 1 : total - 1;
-                        const uint32_t bit = 1u << (nedges & 31u);
-                        if (append)
-                            S.push_bits[nedges >> 5] |= bit;
-                        else
-                            S.push_bits[nedges >> 5] &= ~bit;
-                        key_sum[key] = int16_t(s1);
-                        ++nedges;
-                    }
-                }
-            }
-            // Pass 2, backward over those edges: the last |s| appended copies
-            // of each pair survive (their flag stays set); sums return to 0.
-            uint32_t rank = visible;
-            for (int ch = int(kFaceChunks) - 1; ch >= 0; --ch) {
-                uint64_t m = vis[ch];
-                while (m) {
-                    const uint32_t bit = 63u - (uint32_t)__clzll((unsigned long long)m);
-                    m &= ~(1ull << bit);
-                    --rank;
-                    const uint32_t abc = S.fabc[uint32_t(ch) * 64 + bit];
-                    const uint32_t v3[3] = {abc & 0xFFu, (abc >> 8) & 0xFFu, (abc >> 16) & 0xFFu};
-                    for (int k = 2; k >= 0; --k) {
-                        const uint32_t idx = 3 * rank + uint32_t(k);
-                        if (idx >= nedges)
-                            continue;
-                        const uint32_t key = edge_key(v3[k], v3[(k + 1) % 3]);
-                        const int sum = key_sum[key];
-                        const uint32_t flag = 1u << (idx & 31u);
-                        if ((S.push_bits[idx >> 5] & flag) && sum != 0)
-                            key_sum[key] = int16_t(sum > 0 ? sum - 1 : sum + 1);
-                        else
-                            S.push_bits[idx >> 5] &= ~flag;
-                    }
-                }
-            }
-            // Pass 3, forward: survivors in arrival order form the horizon.
-            uint32_t ne = 0;
-            if (!overflow) {
-                uint32_t idx = 0;
-                for (uint32_t ch = 0; ch < kFaceChunks; ++ch) {
-                    uint64_t m = vis[ch];
-                    while (m) {
-                        const uint32_t f = ch * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                        m &= m - 1;
-                        const uint32_t abc = S.fabc[f];
-                        const uint32_t v3[3] = {abc & 0xFFu, (abc >> 8) & 0xFFu,
-                                                (abc >> 16) & 0xFFu};
-                        for (int k = 0; k < 3; ++k, ++idx)
-                            if (S.push_bits[idx >> 5] & (1u << (idx & 31u))) {
-                                S.ea[ne] = uint8_t(v3[k]);
-                                S.eb[ne] = uint8_t(v3[(k + 1) % 3]);
-                                ++ne;
-                            }
-                    }
-                }
-            }
-            S.ne = ne;
-            S.overflow = overflow;
-        }
         wave_sync();
-        const uint32_t ne = S.ne;
-        if (S.overflow)
+        const int horizon = horizon_wave(S, 3 * visible);
+        if (horizon < 0)
             return -1;
+        const uint32_t ne = uint32_t(horizon);
         if (ne == 0)
             return 0;
         const uint32_t keep = nf - visible;
@@ -1153,12 +1127,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
            const uint32_t* __restrict__ list, const float* __restrict__ simplex_in,
            physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
-           uint32_t* __restrict__ hit, int16_t* __restrict__ key_tables)
+           uint32_t* __restrict__ hit)
 {
     __shared__ EpaShared shared[kEpaWavesPerBlock];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     EpaShared& S = shared[wave];
-    int16_t* key_sum = key_tables + size_t(blockIdx.x * kEpaWavesPerBlock + wave) * kEdgeKeys;
+    for (uint32_t i = lane; i < kEdgeKeys; i += 64)
+        S.key_sum[i] = 0;
+    wave_sync();
     const uint32_t total = c->epa_count;
     uint32_t contacts[kClasses] = {0, 0, 0, 0, 0, 0};
     uint32_t failures = 0, overflows = 0;
@@ -1168,7 +1144,7 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         uint32_t e = 0;
         if (lane == 0)
             e = atomicAdd(&c->epa_next, 1u);
-        e = __shfl(e, 0);
+        e = uniform(e);
         if (e >= total)
             break;
         const uint32_t entry = list[e];
@@ -1184,8 +1160,8 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         f3 n;
         float depth;
         const WaveHull ha = wave_hull(B, sa), hb = wave_hull(B, sb);
-        const int st = epa_wave(S, key_sum, B, sa, ta, ha, sb, tb, hb, simplex,
-                                opt.epa_max_iterations, opt.epa_tolerance, n, depth);
+        const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, simplex, opt.epa_max_iterations,
+                                opt.epa_tolerance, n, depth);
         if (st == 1) {
             const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
                                    support_wave(B, sb, tb, neg(n), hb)),
@@ -1341,11 +1317,6 @@ __global__ void k_np_begin(Counters* c)
 
 }  // namespace
 
-size_t epa_key_table_entries()
-{
-    return size_t(kEpaGrid) * kEpaWavesPerBlock * kEdgeKeys;
-}
-
 void launch_np_begin(physgpu_context* ctx)
 {
     hipLaunchKernelGGL(k_np_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->counters);
@@ -1389,7 +1360,7 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
         time_begin(ctx, "np_epa", t);
         hipLaunchKernelGGL(k_epa, dim3(kEpaGrid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
                            ctx->counters, ctx->epa_list, ctx->epa_simplex, opt, ctx->slots,
-                           ctx->hit, ctx->epa_key_tables);
+                           ctx->hit);
         time_end(ctx, t);
     }
     time_begin(ctx, "np_compact", t);

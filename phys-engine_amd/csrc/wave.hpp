@@ -149,3 +149,91 @@ __device__ __forceinline__ void wq_push(WaveQueue& q, bool pred, uint64_t value,
 }
 
 }  // namespace pg
+
+namespace pg {
+
+// Cross-lane moves through DPP (data-parallel primitives: a VALU operand
+// modifier, no LDS round trip).  Controls used: quad_perm [1,0,3,2] (0xB1),
+// quad_perm [2,3,0,1] (0x4E), row_half_mirror (0x141), row_mirror (0x140);
+// every lane of the row has a source, so every lane is written.
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, Ctrl, 0xF, 0xF, false);
+}
+
+// Value of lane `src` (wave-uniform index) in every lane.
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t src)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
+}
+
+__device__ __forceinline__ float lane_value(float v, uint32_t src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)src));
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
+
+// (value, index) candidates; index kNoIndex marks an empty lane.  Max=false
+// keeps the smallest value, Max=true the largest; equal values keep the lower
+// index.  This is a total order on the non-empty candidates (NaN aside), so
+// the reduction tree does not change the result: it is the first extreme
+// of a sequential scan in index order.
+template <bool Max>
+__device__ __forceinline__ void arg_take(float& v, uint32_t& i, float ov, uint32_t oi)
+{
+    const bool better = Max ? ov > v : ov < v;
+    if (oi != kNoIndex && (i == kNoIndex || better || (ov == v && oi < i))) {
+        v = ov;
+        i = oi;
+    }
+}
+
+template <bool Max, int Ctrl>
+__device__ __forceinline__ void arg_step(float& v, uint32_t& i)
+{
+    const float ov = __uint_as_float(dpp_u32<Ctrl>(__float_as_uint(v)));
+    const uint32_t oi = dpp_u32<Ctrl>(i);
+    arg_take<Max>(v, i, ov, oi);
+}
+
+// Wave-wide arg-min / arg-max; the result is returned uniform in every lane.
+template <bool Max>
+__device__ __forceinline__ void wave_arg_best(float& v, uint32_t& i)
+{
+    arg_step<Max, 0xB1>(v, i);
+    arg_step<Max, 0x4E>(v, i);
+    arg_step<Max, 0x141>(v, i);
+    arg_step<Max, 0x140>(v, i);
+    // Every lane of a 16-lane row now holds the row's result.
+    float bv = lane_value(v, 0);
+    uint32_t bi = lane_value(i, 0);
+    arg_take<Max>(bv, bi, lane_value(v, 16), lane_value(i, 16));
+    arg_take<Max>(bv, bi, lane_value(v, 32), lane_value(i, 32));
+    arg_take<Max>(bv, bi, lane_value(v, 48), lane_value(i, 48));
+    v = bv;
+    i = bi;
+}
+
+// Lanes whose `key` (below 2^Bits) equals this lane's, among the lanes of
+// `active`: one ballot per key bit.
+template <uint32_t Bits>
+__device__ __forceinline__ uint64_t match_lanes(uint32_t key, uint64_t active)
+{
+    uint64_t same = active;
+#pragma unroll
+    for (uint32_t b = 0; b < Bits; ++b) {
+        const bool set = (key >> b) & 1u;
+        const uint64_t m = __ballot(set);
+        same &= set ? m : ~m;
+    }
+    return same;
+}
+
+}  // namespace pg
