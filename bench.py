@@ -29,6 +29,17 @@ ALGOS = {"sap": abi.SWEEP_AND_PRUNE, "grid": abi.UNIFORM_GRID, "tree": abi.DYNAM
 HBM_PEAK_GBS = 8000.0
 VALU_FP32_PEAK_TFLOPS = 157.3
 
+# HBM bytes per dispatch measured with two rocprofv3 --pmc passes (FETCH_SIZE,
+# WRITE_SIZE) over `bench.py --algorithms grid` on MI355X, scaled by the
+# readings of a 256 MiB device copy (fetch x2.000, write x1.000); the summary
+# with its commands is profiles/grid_step_mixed1M_hbm_traffic.md.  Valid for
+# the 1,048,576-body mixed scene only.
+MEASURED_TRAFFIC_1M = {
+    "grid_query": 428.64e6,
+    "compute_aabbs": 105.73e6,
+    "gather_sorted": 168.23e6,
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -173,9 +184,10 @@ def main():
         if b is None:
             continue
         achieved = b / (us * 1e-6) / 1e9
+        traffic = MEASURED_TRAFFIC_1M.get(kn) if n == 1048576 else None
         roof = {"bound": "hbm", "kernel": kn, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": b, "avg_us": us}
+                "traffic": traffic, "bytes_per_launch": b, "avg_us": us}
         break
 
     line = {
