@@ -114,6 +114,106 @@ def cpu_baseline(scene, seconds_budget=30.0):
                       f"AABB {t['aabb_ms']:.1f} ms; {len(m)} manifolds; wall {wall:.1f} s"}
 
 
+def _hip():
+    import ctypes as C
+    return C.CDLL("libamdhip64.so")
+
+
+def batched_bench(worlds, steps, warmup, check_every=64):
+    """The batched-worlds form against per-world launches of the plain pipeline.
+
+    Batch: `worlds` mixed worlds of log-uniform size in [16, 4096] bodies in one
+    context, one launch per phase, step replayed from a HIP graph after an
+    autotuning pass.  Per-world: one plain context per world (colliders
+    resident), physgpu_collide called world after world on one stream.
+    Both run the uniform-grid broad phase.  Returns a dict for the JSON line."""
+    import ctypes as C
+    t0 = time.perf_counter()
+    sc, off, counts = scenes.batched_worlds(worlds)
+    log(f"batched: {worlds} worlds, {sc.count} bodies (generated in "
+        f"{time.perf_counter() - t0:.1f} s)")
+    grid = abi.UNIFORM_GRID
+    pipe = physgpu.CollisionPipeline(sc.count, max_pairs=4 * sc.count,
+                                     max_hull_points=sc.hull_points.shape[1])
+    pipe.set_colliders(sc)
+    pipe.set_worlds(off)
+    t0 = time.perf_counter()
+    tuning = pipe.autotune(algorithm=grid, repeats=2)
+    log(f"batched: autotune {time.perf_counter() - t0:.1f} s -> {tuning}")
+    for _ in range(warmup):
+        pipe.collide_graph(algorithm=grid, sync=False)
+    pipe.step_counts()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.collide_graph(algorithm=grid, sync=False)
+    npairs, nman = pipe.step_counts()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.collide(algorithm=grid, sync=False)
+    pipe.step_counts()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+    pairs_b, man_b = pipe.pairs(), pipe.manifolds()
+    po, mo = pipe.world_ranges()
+    log(f"batched: graph {graph_ms:.3f} ms/step, eager {eager_ms:.3f} ms/step, "
+        f"{npairs} pairs, {nman} manifolds")
+
+    # Per-world contexts on one shared stream.
+    hip = _hip()
+    stream = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(stream), C.c_uint(1)) == 0
+    t0 = time.perf_counter()
+    ctxs = []
+    for w in range(worlds):
+        sub = sc.subset(np.arange(off[w], off[w + 1]))
+        nw = sub.count
+        p = physgpu.CollisionPipeline(nw, max_pairs=max(4 * nw, 1024),
+                                      max_hull_points=max(sub.hull_points.shape[1], 1))
+        p.set_stream(stream.value)
+        p.set_colliders(sub)
+        ctxs.append(p)
+    log(f"per-world: {worlds} contexts set up in {time.perf_counter() - t0:.1f} s")
+    for p in ctxs:
+        p.collide(algorithm=grid, sync=False)
+    assert hip.hipStreamSynchronize(stream) == 0
+    loops = max(1, min(steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(loops):
+        for p in ctxs:
+            p.collide(algorithm=grid, sync=False)
+    assert hip.hipStreamSynchronize(stream) == 0
+    loop_ms = (time.perf_counter() - t0) * 1e3 / loops
+    # Identical results: counts of every world, all words of every check_every-th world.
+    same = True
+    for w, p in enumerate(ctxs):
+        pw, mw = p.step_counts()
+        if pw != po[w + 1] - po[w] or mw != mo[w + 1] - mo[w]:
+            same = False
+        if w % check_every == 0:
+            pp = p.pairs().astype(np.int64) + int(off[w])
+            same &= bool(np.array_equal(pp, pairs_b[po[w]:po[w + 1]].astype(np.int64)))
+            mm = p.manifolds().view(np.uint32).reshape(-1, 40)[:, 2:36]
+            mb = man_b[mo[w]:mo[w + 1]].view(np.uint32).reshape(-1, 40)[:, 2:36]
+            same &= bool(mm.shape == mb.shape and np.array_equal(mm, mb))
+    for p in ctxs:
+        p.close()
+    hip.hipStreamDestroy(stream)
+    pipe.close()
+    log(f"per-world loop: {loop_ms:.1f} ms per pass over {worlds} worlds; identical: {same}")
+    return {
+        "workload": f"{worlds} worlds, bodies log-uniform in [16, 4096] ({sc.count} total), "
+                    "mixed shapes, ~1 pair per body",
+        "bodies": sc.count, "pairs": int(npairs), "manifolds": int(nman),
+        "graph_ms_per_step": round(graph_ms, 4), "eager_ms_per_step": round(eager_ms, 4),
+        "worlds_per_s": round(worlds / (graph_ms * 1e-3), 1),
+        "per_world_loop_ms": round(loop_ms, 3),
+        "per_world_worlds_per_s": round(worlds / (loop_ms * 1e-3), 1),
+        "speedup_vs_per_world": round(loop_ms / graph_ms, 2),
+        "identical_to_per_world": same,
+        "tuning": tuning,
+    }
+
+
 def copy_probe(nbytes=256 << 20):
     """One device-to-device copy of known size (the HIP runtime's blit kernel),
     used to scale rocprof FETCH_SIZE / WRITE_SIZE readings."""
@@ -138,6 +238,8 @@ def main():
     ap.add_argument("--bodies", type=int, default=1048576)
     ap.add_argument("--algorithms", default="sap,grid,tree")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-worlds", type=int, default=4096,
+                    help="worlds of the batched-form comparison (0: skip it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -214,6 +316,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": None,
     }
+    if args.batched_worlds > 0:
+        line["batched"] = batched_bench(args.batched_worlds, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(scene)

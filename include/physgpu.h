@@ -186,6 +186,20 @@ typedef struct physgpu_narrowphase_options {
     uint32_t reserved;
 } physgpu_narrowphase_options;
 
+/* Launch shapes of the tunable kernels.  No counterpart in the reference
+ * (CPU code); physgpu_autotune picks them by timing, physgpu_set_tuning sets
+ * them.  Zero fields mean the default in brackets. */
+typedef struct physgpu_tuning {
+    uint32_t class_block[3];   /* workgroup size of the SS, SB/BS, BB kernels: 64, 128, 256 [256] */
+    uint32_t class_groups[3];  /* workgroups of those grid-stride kernels: 1..65536 [2048] */
+    uint32_t gjk_block;        /* GJK workgroup size: 64, 128, 256 [256] */
+    uint32_t gjk_groups;       /* [2048] */
+    uint32_t epa_groups;       /* EPA workgroups (4 wavefronts, one pair each in flight) [1024] */
+    uint32_t query_block;      /* grid query workgroup size: 64, 128, 256 [256] */
+    uint32_t query_groups;     /* [2048] */
+    uint32_t sort_items;       /* radix sort items per thread: 4, 8, 16 (tile 256 x items) [8] */
+} physgpu_tuning;
+
 typedef struct physgpu_context physgpu_context;
 
 /* Thread-local text for the last non-OK status on this thread. */
@@ -210,6 +224,14 @@ physgpu_status physgpu_set_timing(physgpu_context* ctx, int enabled);
 /* Copy colliders into the context (memory says where the arrays live). */
 physgpu_status physgpu_set_colliders(physgpu_context* ctx, const physgpu_colliders* colliders,
                                      physgpu_memory memory);
+
+/* Replace the poses of the resident colliders (7 SoA planes, one entry per
+ * resident collider) and keep their shapes: the per-step input of the AABB
+ * refresh (physics_world.cpp:539-551 reads the integrated poses).  A captured
+ * step graph stays valid.  Host memory is copied before the call returns;
+ * device memory is copied in stream order. */
+physgpu_status physgpu_update_poses(physgpu_context* ctx, const float* pose,
+                                    physgpu_memory memory);
 
 /* Aabb::fromCollider (src/collision/aabb.cpp:8-32) for every resident
  * collider; hulls take min/max of their transformed points.  When `out` is not
@@ -267,6 +289,54 @@ physgpu_status physgpu_step_counts(physgpu_context* ctx, uint64_t* pair_count,
                                    uint64_t* manifold_count);
 physgpu_status physgpu_narrowphase_stats_get(physgpu_context* ctx,
                                              physgpu_narrowphase_stats* stats);
+
+/* ---- Batches of worlds ---------------------------------------------------
+ * No counterpart in the reference, which steps one PhysicsWorld at a time;
+ * every world keeps the per-world semantics of PhysicsWorld::step
+ * (physics_world.cpp:518-782) and yields exactly the pairs and manifolds it
+ * would yield alone.  The resident colliders (physgpu_set_colliders) are the
+ * concatenation of `world_count` worlds: world w owns colliders
+ * [offsets[w], offsets[w + 1]), offsets[0] = 0, offsets[world_count] = the
+ * collider count.  Pairs only form inside a world.  Pair and manifold indices
+ * stay resident-collider indices; the canonical pair list is the
+ * concatenation of the per-world canonical lists (subtract offsets[w] for
+ * world-local indices).  world_count = 0 returns to a single world;
+ * physgpu_set_colliders also does.  AABBs computed before this call carry
+ * no world: run a step (or physgpu_compute_aabbs) before a stateless
+ * physgpu_find_candidate_pairs.  While worlds are set the broad phase is
+ * PHYSGPU_UNIFORM_GRID (one launch per phase for the whole batch); the other
+ * algorithms return PHYSGPU_UNSUPPORTED. */
+physgpu_status physgpu_set_worlds(physgpu_context* ctx, uint32_t world_count,
+                                  const uint32_t* offsets, physgpu_memory memory);
+
+/* Per-world ranges of the last step: world w's pairs are
+ * [pair_offsets[w], pair_offsets[w + 1]) of the canonical pair list and its
+ * manifolds [manifold_offsets[w], manifold_offsets[w + 1]) of the manifold
+ * list.  Each array has world_count + 1 entries (either may be NULL).
+ * Synchronises the context's stream. */
+physgpu_status physgpu_world_ranges(physgpu_context* ctx, uint64_t* pair_offsets,
+                                    uint64_t* manifold_offsets);
+
+/* physgpu_collide as a HIP graph: the first call captures the device-resident
+ * step (AABBs, broad phase, narrow phase, per-world ranges) on the context's
+ * stream and later calls replay it.  The graph is captured again after
+ * anything that changes the step: colliders, worlds, world extras, stream,
+ * tuning, algorithm, cell size or options.  New poses written into the
+ * resident buffers between replays (PHYSGPU_DEVICE uploads) are picked up.
+ * Needs a stream other than the legacy NULL stream.  Counts as in
+ * physgpu_collide (NULL pointers: no synchronisation). */
+physgpu_status physgpu_collide_graph(physgpu_context* ctx, int algorithm, double grid_cell_size,
+                                     const physgpu_narrowphase_options* options,
+                                     uint64_t* pair_count, uint64_t* manifold_count);
+
+/* Launch shapes.  physgpu_autotune times each tunable kernel of the step on
+ * the resident colliders (`repeats` steps per candidate, HIP events) for each
+ * candidate shape, one kernel at a time, keeps the fastest, and returns the
+ * result in `out` (may be NULL).  Results do not depend on the tuning. */
+physgpu_status physgpu_set_tuning(physgpu_context* ctx, const physgpu_tuning* tuning);
+physgpu_status physgpu_get_tuning(physgpu_context* ctx, physgpu_tuning* tuning);
+physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t repeats,
+                                physgpu_tuning* out);
 
 /* Average device time (ms) per launch of each kernel since timing was
  * enabled.  Writes up to `capacity` entries; names are static strings. */

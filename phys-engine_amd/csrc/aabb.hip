@@ -36,7 +36,8 @@ __global__ void k_step_begin(Counters* c)
 __global__ __launch_bounds__(kBlock) void k_compute_aabbs(
     uint32_t n, const float* __restrict__ pose, const uint8_t* __restrict__ type,
     const float* __restrict__ sdata, const uint32_t* __restrict__ hrange,
-    const float* __restrict__ hpts, uint32_t nh, float4* __restrict__ aabb)
+    const float* __restrict__ hpts, uint32_t nh, const uint32_t* __restrict__ world_of,
+    float4* __restrict__ aabb)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_compute_aabbs(
         }
     }
     const float b[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
-    store_rec(aabb, i, b, i);
+    store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
@@ -240,7 +241,8 @@ void launch_compute_aabbs(physgpu_context* ctx)
     time_begin(ctx, "aabb_from_pose", t);
     hipLaunchKernelGGL(k_compute_aabbs, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0,
                        ctx->stream, ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data,
-                       ctx->hull_range, ctx->hull_points, ctx->n_hull_points, ctx->aabb);
+                       ctx->hull_range, ctx->hull_points, ctx->n_hull_points,
+                       ctx->n_worlds ? ctx->world_of : nullptr, ctx->aabb);
     time_end(ctx, t);
 }
 

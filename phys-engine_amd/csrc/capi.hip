@@ -97,12 +97,17 @@ void free_all(physgpu_context* c)
                     c->pairs_raw, c->pairs_tmp, c->buckets, c->slots, c->hit, c->tile_sums,
                     c->manifolds, c->body, c->local_pose, c->friction, c->restitution,
                     c->epa_list, c->epa_simplex,
-                    c->counters, c->world_body_pose};
+                    c->counters, c->world_body_pose, c->world_offsets, c->world_of,
+                    c->world_pair_off, c->world_manifold_off};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
     if (c->host_counters)
         hipHostFree(c->host_counters);
+    if (c->graph_exec)
+        hipGraphExecDestroy(c->graph_exec);
+    if (c->graph)
+        hipGraphDestroy(c->graph);
     for (Timer& t : c->pending) {
         hipEventDestroy(t.start);
         hipEventDestroy(t.stop);
@@ -120,6 +125,70 @@ physgpu_status read_counters(physgpu_context* ctx)
     PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->timing)
         collect_times(ctx);
+    return PHYSGPU_OK;
+}
+
+void drop_graph(physgpu_context* ctx)
+{
+    if (ctx->graph_exec)
+        hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph)
+        hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_algorithm = -1;
+}
+
+physgpu_tuning default_tuning()
+{
+    physgpu_tuning t{};
+    for (int c = 0; c < 3; ++c) {
+        t.class_block[c] = kBlock;
+        t.class_groups[c] = kStrideGrid;
+    }
+    t.gjk_block = kBlock;
+    t.gjk_groups = kStrideGrid;
+    t.epa_groups = 1024;
+    t.query_block = kBlock;
+    t.query_groups = kStrideGrid;
+    t.sort_items = 8;
+    return t;
+}
+
+bool valid_block(uint32_t b) { return b == 64 || b == 128 || b == 256; }
+bool valid_groups(uint32_t g) { return g >= 1 && g <= 65536; }
+
+// Zero fields take the defaults; anything else must be a supported shape.
+physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
+{
+    const physgpu_tuning d = default_tuning();
+    out = in;
+    for (int c = 0; c < 3; ++c) {
+        if (!out.class_block[c])
+            out.class_block[c] = d.class_block[c];
+        if (!out.class_groups[c])
+            out.class_groups[c] = d.class_groups[c];
+        if (!valid_block(out.class_block[c]) || !valid_groups(out.class_groups[c]))
+            return fail(PHYSGPU_INVALID_ARGUMENT, "class kernel shape out of range");
+    }
+    if (!out.gjk_block)
+        out.gjk_block = d.gjk_block;
+    if (!out.gjk_groups)
+        out.gjk_groups = d.gjk_groups;
+    if (!out.epa_groups)
+        out.epa_groups = d.epa_groups;
+    if (!out.query_block)
+        out.query_block = d.query_block;
+    if (!out.query_groups)
+        out.query_groups = d.query_groups;
+    if (!out.sort_items)
+        out.sort_items = d.sort_items;
+    if (!valid_block(out.gjk_block) || !valid_groups(out.gjk_groups) ||
+        !valid_groups(out.epa_groups) || !valid_block(out.query_block) ||
+        !valid_groups(out.query_groups))
+        return fail(PHYSGPU_INVALID_ARGUMENT, "kernel shape out of range");
+    if (out.sort_items != 4 && out.sort_items != 8 && out.sort_items != 16)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "sort_items must be 4, 8 or 16");
     return PHYSGPU_OK;
 }
 
@@ -200,6 +269,51 @@ physgpu_status apply_options(physgpu_context* ctx, const physgpu_narrowphase_opt
     return PHYSGPU_OK;
 }
 
+// Checks shared by the device-resident step entry points.
+physgpu_status check_step(physgpu_context* ctx, int algorithm)
+{
+    if (!ctx->colliders_valid)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "no colliders set");
+    if (algorithm < PHYSGPU_SWEEP_AND_PRUNE || algorithm > PHYSGPU_DYNAMIC_TREE)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown broadphase algorithm");
+    if (ctx->n_worlds && algorithm != PHYSGPU_UNIFORM_GRID)
+        return fail(PHYSGPU_UNSUPPORTED, "a batch of worlds runs the uniform-grid broad phase");
+    return PHYSGPU_OK;
+}
+
+// The collision section of PhysicsWorld::step, enqueued on ctx->stream
+// without any host synchronisation (so it can be captured into a graph).
+void enqueue_step(physgpu_context* ctx, int algorithm, double cell)
+{
+    launch_step_begin(ctx);
+    if (ctx->has_world)
+        launch_compose_world(ctx);
+    launch_compute_aabbs(ctx);
+    run_broadphase(ctx, algorithm, cell, false);
+    launch_narrowphase(ctx, ctx->has_world);
+    launch_world_ranges(ctx);
+}
+
+physgpu_status step_counts_checked(physgpu_context* ctx, uint64_t* pair_count,
+                                   uint64_t* manifold_count)
+{
+    if (!pair_count && !manifold_count)
+        return PHYSGPU_OK;
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const Counters& c = *ctx->host_counters;
+    if (pair_count)
+        *pair_count = c.pair_count;
+    if (manifold_count)
+        *manifold_count = c.manifold_count;
+    if (c.tree_overflow)
+        return fail(PHYSGPU_DEVICE_ERROR, "LBVH traversal stack overflow");
+    if (c.pair_count > ctx->cap_p)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "candidate pairs exceed the context capacity");
+    return PHYSGPU_OK;
+}
+
 physgpu_status copy_pairs_to_host(physgpu_context* ctx, physgpu_pair* out, uint64_t n)
 {
     if (!out || n == 0)
@@ -250,6 +364,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     if (!c)
         return fail(PHYSGPU_DEVICE_ERROR, "out of host memory");
     c->device = device;
+    c->tuning = default_tuning();
     c->cap_n = max_colliders;
     c->cap_p = max_pairs;
     c->cap_h = max_hull_points;
@@ -258,7 +373,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
         ++k;
     c->table_bits = 3 * k;
     const size_t N = max_colliders, P = max_pairs;
-    const size_t tiles = std::max<size_t>(div_up(N, kSortTile), div_up(P, kSortTile));
+    const size_t tiles = std::max<size_t>(div_up(N, kSortMinTile), div_up(P, kSortMinTile));
     c->hist_capacity = uint32_t(256 * tiles);
     hipError_t e = hipSuccess;
     auto A = [&](hipError_t r) {
@@ -302,6 +417,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->local_pose, 7 * N));
     A(alloc(c->friction, N));
     A(alloc(c->restitution, N));
+    A(alloc(c->world_of, N));
     A(alloc(c->counters, 1));
     A(hipHostMalloc(reinterpret_cast<void**>(&c->host_counters), sizeof(Counters)));
     if (e != hipSuccess) {
@@ -335,6 +451,7 @@ physgpu_status physgpu_set_stream(physgpu_context* ctx, void* hip_stream)
     if (!ctx)
         return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
     PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
     if (ctx->owns_stream)
         hipStreamDestroy(ctx->stream);
     ctx->stream = static_cast<hipStream_t>(hip_stream);
@@ -404,6 +521,28 @@ physgpu_status physgpu_set_colliders(physgpu_context* ctx, const physgpu_collide
     ctx->has_hulls = hulls;
     ctx->colliders_valid = true;
     ctx->has_world = false;
+    ctx->n_worlds = 0;
+    drop_graph(ctx);
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_update_poses(physgpu_context* ctx, const float* pose,
+                                    physgpu_memory memory)
+{
+    if (!ctx || !pose)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or pose is NULL");
+    if (!ctx->colliders_valid)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "no colliders set");
+    if (memory != PHYSGPU_HOST && memory != PHYSGPU_DEVICE)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown memory kind");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    if (ctx->n)
+        PG_TRY_HIP(hipMemcpyAsync(ctx->pose, pose, 7ull * ctx->n * 4,
+                                  memory == PHYSGPU_HOST ? hipMemcpyHostToDevice
+                                                         : hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+    if (memory == PHYSGPU_HOST)
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
     return PHYSGPU_OK;
 }
 
@@ -448,6 +587,8 @@ physgpu_status physgpu_find_candidate_pairs(physgpu_context* ctx, const physgpu_
         ctx->n = n;
         ctx->colliders_valid = false;
         ctx->has_world = false;
+        ctx->n_worlds = 0;
+        drop_graph(ctx);
         if (n) {
             physgpu_aabb* staging = reinterpret_cast<physgpu_aabb*>(ctx->sorted_aabb);
             PG_TRY_HIP(hipMemcpyAsync(staging, bounds, sizeof(physgpu_aabb) * n,
@@ -456,6 +597,8 @@ physgpu_status physgpu_find_candidate_pairs(physgpu_context* ctx, const physgpu_
         }
     } else if (n != ctx->n) {
         return fail(PHYSGPU_INVALID_ARGUMENT, "n must equal the resident collider count");
+    } else if (ctx->n_worlds && algorithm != PHYSGPU_UNIFORM_GRID) {
+        return fail(PHYSGPU_UNSUPPORTED, "a batch of worlds runs the uniform-grid broad phase");
     }
     launch_step_begin(ctx);
     run_broadphase(ctx, algorithm, grid_cell_size, stats != nullptr);
@@ -533,35 +676,271 @@ physgpu_status physgpu_collide(physgpu_context* ctx, int algorithm, double grid_
 {
     if (!ctx)
         return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
-    if (!ctx->colliders_valid)
-        return fail(PHYSGPU_INVALID_ARGUMENT, "no colliders set");
-    if (algorithm < PHYSGPU_SWEEP_AND_PRUNE || algorithm > PHYSGPU_DYNAMIC_TREE)
-        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown broadphase algorithm");
-    physgpu_status st = apply_options(ctx, options);
+    physgpu_status st = check_step(ctx, algorithm);
+    if (st != PHYSGPU_OK)
+        return st;
+    st = apply_options(ctx, options);
     if (st != PHYSGPU_OK)
         return st;
     PG_TRY_HIP(hipSetDevice(ctx->device));
-    launch_step_begin(ctx);
-    if (ctx->has_world)
-        launch_compose_world(ctx);
-    launch_compute_aabbs(ctx);
-    run_broadphase(ctx, algorithm, grid_cell_size, false);
-    launch_narrowphase(ctx, ctx->has_world);
+    enqueue_step(ctx, algorithm, grid_cell_size);
     PG_TRY_HIP(hipGetLastError());
-    if (!pair_count && !manifold_count)
-        return PHYSGPU_OK;
-    st = read_counters(ctx);
+    return step_counts_checked(ctx, pair_count, manifold_count);
+}
+
+physgpu_status physgpu_collide_graph(physgpu_context* ctx, int algorithm, double grid_cell_size,
+                                     const physgpu_narrowphase_options* options,
+                                     uint64_t* pair_count, uint64_t* manifold_count)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    physgpu_status st = check_step(ctx, algorithm);
     if (st != PHYSGPU_OK)
         return st;
-    const Counters& c = *ctx->host_counters;
-    if (pair_count)
-        *pair_count = c.pair_count;
-    if (manifold_count)
-        *manifold_count = c.manifold_count;
-    if (c.tree_overflow)
-        return fail(PHYSGPU_DEVICE_ERROR, "LBVH traversal stack overflow");
-    if (c.pair_count > ctx->cap_p)
-        return fail(PHYSGPU_CAPACITY_EXCEEDED, "candidate pairs exceed the context capacity");
+    st = apply_options(ctx, options);
+    if (st != PHYSGPU_OK)
+        return st;
+    if (!ctx->stream)
+        return fail(PHYSGPU_UNSUPPORTED, "graph capture needs a stream other than the NULL stream");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    const bool same = ctx->graph_exec && ctx->graph_algorithm == algorithm &&
+                      ctx->graph_cell == grid_cell_size &&
+                      std::memcmp(&ctx->graph_options, &ctx->np_options,
+                                  sizeof(physgpu_narrowphase_options)) == 0;
+    if (!same) {
+        drop_graph(ctx);
+        // Per-kernel events are not recorded inside the graph.
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            enqueue_step(ctx, algorithm, grid_cell_size);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(ctx->stream, &g);
+            ctx->graph = g;
+        }
+        ctx->timing = timing;
+        if (e == hipSuccess)
+            e = hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            drop_graph(ctx);
+            return check_hip(e, "graph capture of the collision step");
+        }
+        ctx->graph_algorithm = algorithm;
+        ctx->graph_cell = grid_cell_size;
+        ctx->graph_options = ctx->np_options;
+    }
+    PG_TRY_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    ctx->last_algorithm = algorithm;
+    return step_counts_checked(ctx, pair_count, manifold_count);
+}
+
+physgpu_status physgpu_set_worlds(physgpu_context* ctx, uint32_t world_count,
+                                  const uint32_t* offsets, physgpu_memory memory)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ctx->colliders_valid)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "set colliders first");
+    drop_graph(ctx);
+    if (world_count == 0) {
+        ctx->n_worlds = 0;
+        return PHYSGPU_OK;
+    }
+    if (!offsets)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "offsets is NULL");
+    if (memory != PHYSGPU_HOST && memory != PHYSGPU_DEVICE)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown memory kind");
+    if (world_count > ctx->cap_n)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "more worlds than colliders the context holds");
+    std::vector<uint32_t> host;
+    if (memory == PHYSGPU_DEVICE) {
+        host.resize(size_t(world_count) + 1);
+        PG_TRY_HIP(hipMemcpy(host.data(), offsets, host.size() * 4, hipMemcpyDeviceToHost));
+    } else {
+        host.assign(offsets, offsets + size_t(world_count) + 1);
+    }
+    if (host[0] != 0 || host[world_count] != ctx->n)
+        return fail(PHYSGPU_INVALID_ARGUMENT,
+                    "offsets must start at 0 and end at the resident collider count");
+    for (uint32_t w = 0; w < world_count; ++w)
+        if (host[w + 1] < host[w])
+            return fail(PHYSGPU_INVALID_ARGUMENT, "offsets must be non-decreasing");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    if (world_count + 1 > ctx->world_capacity) {
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+        for (void* p : {(void*)ctx->world_offsets, (void*)ctx->world_pair_off,
+                        (void*)ctx->world_manifold_off})
+            if (p)
+                hipFree(p);
+        ctx->world_offsets = nullptr;
+        ctx->world_pair_off = ctx->world_manifold_off = nullptr;
+        ctx->world_capacity = 0;
+        const size_t cap = size_t(world_count) + 1;
+        PG_TRY_HIP(alloc(ctx->world_offsets, cap));
+        PG_TRY_HIP(alloc(ctx->world_pair_off, cap));
+        PG_TRY_HIP(alloc(ctx->world_manifold_off, cap));
+        ctx->world_capacity = uint32_t(cap);
+    }
+    PG_TRY_HIP(hipMemcpyAsync(ctx->world_offsets, host.data(), host.size() * 4,
+                              hipMemcpyHostToDevice, ctx->stream));
+    ctx->n_worlds = world_count;
+    launch_world_of(ctx);
+    PG_TRY_HIP(hipGetLastError());
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_world_ranges(physgpu_context* ctx, uint64_t* pair_offsets,
+                                    uint64_t* manifold_offsets)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ctx->n_worlds)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "no worlds set");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t(ctx->n_worlds) + 1) * sizeof(uint64_t);
+    if (pair_offsets)
+        PG_TRY_HIP(hipMemcpyAsync(pair_offsets, ctx->world_pair_off, bytes, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    if (manifold_offsets)
+        PG_TRY_HIP(hipMemcpyAsync(manifold_offsets, ctx->world_manifold_off, bytes,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_set_tuning(physgpu_context* ctx, const physgpu_tuning* tuning)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    physgpu_tuning t = default_tuning();
+    if (tuning) {
+        physgpu_status st = resolve_tuning(*tuning, t);
+        if (st != PHYSGPU_OK)
+            return st;
+    }
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->tuning = t;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_get_tuning(physgpu_context* ctx, physgpu_tuning* tuning)
+{
+    if (!ctx || !tuning)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or tuning is NULL");
+    *tuning = ctx->tuning;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t repeats,
+                                physgpu_tuning* out)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    physgpu_status st = check_step(ctx, algorithm);
+    if (st != PHYSGPU_OK)
+        return st;
+    if (repeats == 0)
+        repeats = 3;
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    const bool timing = ctx->timing;
+    collect_times(ctx);
+    const std::vector<KernelTime> saved = ctx->kernel_times;
+    ctx->timing = true;
+    // Average device time of the named kernels over `repeats` steps.
+    auto measure = [&](std::initializer_list<const char*> names, double& ms) -> physgpu_status {
+        enqueue_step(ctx, algorithm, 0.0);  // warm-up
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+        collect_times(ctx);
+        ctx->kernel_times.clear();
+        for (uint32_t r = 0; r < repeats; ++r)
+            enqueue_step(ctx, algorithm, 0.0);
+        PG_TRY_HIP(hipGetLastError());
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+        collect_times(ctx);
+        ms = 0.0;
+        for (const KernelTime& k : ctx->kernel_times)
+            for (const char* nm : names)
+                if (std::strcmp(k.name, nm) == 0)
+                    ms += k.total_ms / double(repeats);
+        return PHYSGPU_OK;
+    };
+    static const uint32_t blocks[] = {64, 128, 256};
+    static const uint32_t groups[] = {256, 512, 1024, 2048, 4096, 8192};
+    // One kernel at a time: try each (block, groups) shape, keep the fastest.
+    auto tune_shape = [&](uint32_t* block, uint32_t* group, const char* name) -> physgpu_status {
+        uint32_t best_b = *block, best_g = *group;
+        double best = 1e30;
+        for (uint32_t b : blocks)
+            for (uint32_t g : groups) {
+                if (block)
+                    *block = b;
+                *group = g;
+                double ms = 0.0;
+                physgpu_status s = measure({name}, ms);
+                if (s != PHYSGPU_OK)
+                    return s;
+                if (ms < best) {
+                    best = ms;
+                    best_b = b;
+                    best_g = g;
+                }
+            }
+        *block = best_b;
+        *group = best_g;
+        return PHYSGPU_OK;
+    };
+    static const char* class_names[3] = {"np_sphere_sphere", "np_sphere_box", "np_box_box"};
+    for (int c = 0; c < 3 && st == PHYSGPU_OK; ++c)
+        st = tune_shape(&ctx->tuning.class_block[c], &ctx->tuning.class_groups[c], class_names[c]);
+    if (st == PHYSGPU_OK && ctx->has_hulls)
+        st = tune_shape(&ctx->tuning.gjk_block, &ctx->tuning.gjk_groups, "np_gjk");
+    if (st == PHYSGPU_OK && ctx->has_hulls) {
+        static const uint32_t epa_groups[] = {256, 512, 768, 1024, 2048};
+        uint32_t best_g = ctx->tuning.epa_groups;
+        double best = 1e30;
+        for (uint32_t g : epa_groups) {
+            ctx->tuning.epa_groups = g;
+            double ms = 0.0;
+            st = measure({"np_epa"}, ms);
+            if (st != PHYSGPU_OK)
+                break;
+            if (ms < best) {
+                best = ms;
+                best_g = g;
+            }
+        }
+        ctx->tuning.epa_groups = best_g;
+    }
+    if (st == PHYSGPU_OK && algorithm == PHYSGPU_UNIFORM_GRID)
+        st = tune_shape(&ctx->tuning.query_block, &ctx->tuning.query_groups, "grid_query");
+    if (st == PHYSGPU_OK) {
+        static const uint32_t items[] = {4, 8, 16};
+        uint32_t best_i = ctx->tuning.sort_items;
+        double best = 1e30;
+        for (uint32_t it : items) {
+            ctx->tuning.sort_items = it;
+            double ms = 0.0;
+            st = measure({"radix_upsweep", "radix_scan", "radix_downsweep"}, ms);
+            if (st != PHYSGPU_OK)
+                break;
+            if (ms < best) {
+                best = ms;
+                best_i = it;
+            }
+        }
+        ctx->tuning.sort_items = best_i;
+    }
+    ctx->timing = timing;
+    ctx->kernel_times = saved;
+    if (st != PHYSGPU_OK)
+        return st;
+    if (out)
+        *out = ctx->tuning;
     return PHYSGPU_OK;
 }
 
@@ -600,6 +979,7 @@ physgpu_status physgpu_set_world(physgpu_context* ctx, const float* body_pose,
     }
     PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
     ctx->has_world = true;
+    drop_graph(ctx);
     return PHYSGPU_OK;
 }
 

@@ -18,6 +18,10 @@
 //     unordered pair appears once (the 27 neighbour keys are distinct for
 //     k >= 2, and every body sits in exactly one cell).
 // Cell coordinates use floor(min / c) in double precision, a monotone map.
+// Batches of worlds: every world has its own cells.  A world's keys are its
+// Morton keys XOR a per-world hash (a bijection, so one world's 27 neighbour
+// keys stay distinct); records of other worlds that land on the same key are
+// rejected by the world test.
 // Automatic cell: the largest AABB side, capped at 8x the mean side (the
 // rest overflow), widened by 2^-8 (in-grid bodies must be at most 1 - 2^-9
 // of the cell).
@@ -42,6 +46,12 @@ __device__ __forceinline__ uint32_t cell_key(int32_t x, int32_t y, int32_t z, ui
 {
     return spread3(uint32_t(x) & mask) | (spread3(uint32_t(y) & mask) << 1) |
            (spread3(uint32_t(z) & mask) << 2);
+}
+
+// Per-world key offset (0 for world 0): multiplicative hash to table bits.
+__device__ __forceinline__ uint32_t world_key(uint32_t world, uint32_t bits_per_axis)
+{
+    return world ? (world * 0x9E3779B1u) >> (32u - 3u * bits_per_axis) : 0u;
 }
 
 // Cell side for this step: explicit (> 0) or automatic from the reductions.
@@ -92,7 +102,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* 
         if (fits && home_cell(r.b[0], r.b[1], r.b[2], inv, c) &&
             home_cell(r.b[3], r.b[4], r.b[5], inv, top)) {
             in_grid = true;
-            key = cell_key(c[0], c[1], c[2], (1u << bits_per_axis) - 1u);
+            key = cell_key(c[0], c[1], c[2], (1u << bits_per_axis) - 1u) ^
+                  world_key(r.world, bits_per_axis);
         }
         items[i] = (uint64_t)key | ((uint64_t)i << 32);
         flag[i] = in_grid ? 1u : 0u;
@@ -167,12 +178,14 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
     for (uint32_t round = 0; round < rounds; ++round) {
         const uint32_t p = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
         const bool active = p < entries;
-        Rec me{{0, 0, 0, 0, 0, 0}, 0};
+        Rec me{{0, 0, 0, 0, 0, 0}, 0, 0};
         int32_t c[3] = {0, 0, 0}, top[3] = {0, 0, 0};  // both in range for in-grid bodies
+        uint32_t wk = 0;
         if (active) {
             me = load_rec(s, p);
             home_cell(me.b[0], me.b[1], me.b[2], inv, c);
             home_cell(me.b[3], me.b[4], me.b[5], inv, top);
+            wk = world_key(me.world, bits_per_axis);
         }
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
@@ -180,7 +193,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                     uint32_t q = 0, qe = 0;
                     if (active && c[0] + dx <= top[0] && c[1] + dy <= top[1] &&
                         c[2] + dz <= top[2]) {
-                        const uint2 r = range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask)];
+                        const uint2 r =
+                            range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask) ^ wk];
                         q = r.x;
                         qe = r.y;
                         if (qe <= q)
@@ -196,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                         if (more) {
                             const Rec o = load_rec(s, q);
                             b = o.index;
-                            if (me.index < b) {
+                            if (me.index < b && me.world == o.world) {
                                 ++tests;
                                 hit = rec_overlap(me.b, o.b);
                             }
@@ -234,8 +248,11 @@ __global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const floa
             const uint32_t j = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
             bool hit = false;
             if (j < n && j != o && (flag[j] || o < j)) {
-                ++tests;
-                hit = rec_overlap(ro.b, load_rec(aabb, j).b);
+                const Rec rj = load_rec(aabb, j);
+                if (rj.world == ro.world) {
+                    ++tests;
+                    hit = rec_overlap(ro.b, rj.b);
+                }
             }
             wq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap);
         }
@@ -272,7 +289,8 @@ void launch_grid(physgpu_context* ctx, double cell_size)
     time_end(ctx, t);
     launch_gather_sorted(ctx, sorted);
     time_begin(ctx, "grid_query", t);
-    hipLaunchKernelGGL(k_grid_query, dim3(stride_grid(n)), dim3(kBlock), 0, ctx->stream, n,
+    hipLaunchKernelGGL(k_grid_query, dim3(ctx->tuning.query_groups),
+                       dim3(ctx->tuning.query_block), 0, ctx->stream, n,
                        ctx->sorted_aabb, ctx->cell_table, bpa, ctx->counters, ctx->pairs_raw,
                        ctx->cap_p);
     time_end(ctx, t);

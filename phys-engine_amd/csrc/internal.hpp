@@ -17,10 +17,10 @@ constexpr int kClasses = 6;        // physgpu_shape_pair values
 constexpr uint32_t kBlock = 256;   // threads per workgroup for streaming kernels
 constexpr uint32_t kStrideGrid = 2048;  // workgroups of grid-stride kernels (8 per CU)
 
-// Radix sort tiling: 256 threads x 8 items per workgroup.
+// Radix sort tiling: 256 threads x (4, 8 or 16) items per workgroup (the
+// tuning's sort_items); buffers are sized for the smallest tile.
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortItems = 8;
-constexpr uint32_t kSortTile = kSortThreads * kSortItems;
+constexpr uint32_t kSortMinTile = kSortThreads * 4;
 
 // Compaction tiling.
 constexpr uint32_t kScanTile = 2048;
@@ -69,6 +69,7 @@ struct KernelTime {
 
 struct physgpu_context {
     int device = 0;
+    physgpu_tuning tuning{};
     hipStream_t stream = nullptr;
     bool owns_stream = false;
     uint32_t cap_n = 0;
@@ -141,6 +142,22 @@ struct physgpu_context {
     bool world_has_body = false;
     bool world_has_friction = false;
     bool world_has_restitution = false;
+
+    // Batch of worlds (n_worlds = 0: one world): collider offsets (n_worlds + 1),
+    // world of every collider, per-world pair and manifold ranges of the last step.
+    uint32_t n_worlds = 0;
+    uint32_t world_capacity = 0;
+    uint32_t* world_offsets = nullptr;
+    uint32_t* world_of = nullptr;
+    uint64_t* world_pair_off = nullptr;
+    uint64_t* world_manifold_off = nullptr;
+
+    // HIP graph of the device-resident step and what it was captured with.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_algorithm = -1;
+    double graph_cell = 0.0;
+    physgpu_narrowphase_options graph_options{};
 
     // Counters (device) and a pinned host mirror.
     pg::Counters* counters = nullptr;
@@ -215,5 +232,8 @@ void launch_pairs_in(physgpu_context* ctx, const physgpu_pair* dev_in, uint64_t 
 // narrowphase.hip
 void launch_np_begin(physgpu_context* ctx);
 void launch_narrowphase(physgpu_context* ctx, bool world);
+// batch.hip
+void launch_world_of(physgpu_context* ctx);
+void launch_world_ranges(physgpu_context* ctx);
 
 }  // namespace pg

@@ -332,7 +332,6 @@ constexpr uint32_t kEpaMaxIterations = PHYSGPU_EPA_MAX_ITERATIONS;
 constexpr uint32_t kMaxVertices = 4 + kEpaMaxIterations;
 constexpr uint32_t kMaxFaces = PHYSGPU_EPA_MAX_FACES;
 constexpr uint32_t kMaxEdges = PHYSGPU_EPA_MAX_HORIZON;
-constexpr uint32_t kEpaGrid = 1024;   // workgroups of the EPA kernel (4 pairs in flight each)
 
 struct Shape {
     uint32_t type;
@@ -1334,33 +1333,37 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
                   ctx->world_has_restitution ? ctx->restitution : nullptr};
     physgpu_narrowphase_options opt = ctx->np_options;
     const uint32_t grid = kStrideGrid;
+    const physgpu_tuning& tu = ctx->tuning;
     Timer t;
     time_begin(ctx, "np_classify", t);
     hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0, ctx->stream, B,
                        W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit);
     time_end(ctx, t);
     time_begin(ctx, "np_sphere_sphere", t);
-    hipLaunchKernelGGL(k_class<0>, dim3(grid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
-                       ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots, ctx->hit);
+    hipLaunchKernelGGL(k_class<0>, dim3(tu.class_groups[0]), dim3(tu.class_block[0]), 0,
+                       ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
+                       ctx->slots, ctx->hit);
     time_end(ctx, t);
     time_begin(ctx, "np_sphere_box", t);
-    hipLaunchKernelGGL(k_class<1>, dim3(grid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
-                       ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots, ctx->hit);
+    hipLaunchKernelGGL(k_class<1>, dim3(tu.class_groups[1]), dim3(tu.class_block[1]), 0,
+                       ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
+                       ctx->slots, ctx->hit);
     time_end(ctx, t);
     time_begin(ctx, "np_box_box", t);
-    hipLaunchKernelGGL(k_class<2>, dim3(grid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
-                       ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots, ctx->hit);
+    hipLaunchKernelGGL(k_class<2>, dim3(tu.class_groups[2]), dim3(tu.class_block[2]), 0,
+                       ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
+                       ctx->slots, ctx->hit);
     time_end(ctx, t);
     if (ctx->has_hulls) {
         time_begin(ctx, "np_gjk", t);
-        hipLaunchKernelGGL(k_gjk, dim3(grid), dim3(kBlock), 0, ctx->stream, B, ctx->pairs,
-                           ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->epa_list,
+        hipLaunchKernelGGL(k_gjk, dim3(tu.gjk_groups), dim3(tu.gjk_block), 0, ctx->stream, B,
+                           ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->epa_list,
                            ctx->epa_simplex);
         time_end(ctx, t);
         time_begin(ctx, "np_epa", t);
-        hipLaunchKernelGGL(k_epa, dim3(kEpaGrid), dim3(kBlock), 0, ctx->stream, B, W, ctx->pairs,
-                           ctx->counters, ctx->epa_list, ctx->epa_simplex, opt, ctx->slots,
-                           ctx->hit);
+        hipLaunchKernelGGL(k_epa, dim3(tu.epa_groups), dim3(kBlock), 0, ctx->stream, B, W,
+                           ctx->pairs, ctx->counters, ctx->epa_list, ctx->epa_simplex, opt,
+                           ctx->slots, ctx->hit);
         time_end(ctx, t);
     }
     time_begin(ctx, "np_compact", t);

@@ -30,11 +30,13 @@ __device__ __forceinline__ uint32_t item_count(uint32_t n, const uint32_t* dev_c
     return dev_count ? min(*dev_count, n) : n;
 }
 
+template <uint32_t kItems>
 __global__ __launch_bounds__(kSortThreads) void k_upsweep(const uint64_t* __restrict__ items,
                                                           uint32_t n, const uint32_t* dev_count,
                                                           uint32_t shift,
                                                           uint32_t* __restrict__ hist)
 {
+    constexpr uint32_t kSortTile = kSortThreads * kItems;
     __shared__ uint32_t counts[256];
     const uint32_t count = item_count(n, dev_count);
     const uint32_t tiles = (count + kSortTile - 1) / kSortTile;
@@ -45,7 +47,7 @@ __global__ __launch_bounds__(kSortThreads) void k_upsweep(const uint64_t* __rest
     __syncthreads();
     const uint32_t base = tile * kSortTile;
 #pragma unroll
-    for (uint32_t r = 0; r < kSortItems; ++r) {
+    for (uint32_t r = 0; r < kItems; ++r) {
         uint32_t i = base + r * kSortThreads + threadIdx.x;
         if (i < count)
             atomicAdd(&counts[digit_of(items[i], shift)], 1u);
@@ -82,10 +84,12 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
 
 // One workgroup per digit: exclusive scan of that digit's row of tile counts
 // (hist[d * tiles + tile]) in place, and the digit's total.
+template <uint32_t kItems>
 __global__ __launch_bounds__(kSortThreads) void k_scan_rows(uint32_t* __restrict__ hist,
                                                             uint32_t n, const uint32_t* dev_count,
                                                             uint32_t* __restrict__ digit_total)
 {
+    constexpr uint32_t kSortTile = kSortThreads * kItems;
     const uint32_t count = item_count(n, dev_count);
     const uint32_t tiles = (count + kSortTile - 1) / kSortTile;
     uint32_t* row = hist + size_t(blockIdx.x) * tiles;
@@ -104,8 +108,8 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_rows(uint32_t* __restrict
 }
 
 constexpr uint32_t kWaves = kSortThreads / 64;
-constexpr uint32_t kWaveItems = kSortTile / kWaves;  // contiguous sub-tile per wave
 
+template <uint32_t kItems>
 __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __restrict__ in,
                                                             uint64_t* __restrict__ out,
                                                             uint32_t n,
@@ -114,6 +118,8 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
                                                             const uint32_t* __restrict__ hist,
                                                             const uint32_t* __restrict__ digit_total)
 {
+    constexpr uint32_t kSortTile = kSortThreads * kItems;
+    constexpr uint32_t kWaveItems = kSortTile / kWaves;  // contiguous sub-tile per wave
     __shared__ uint64_t tile_items[kSortTile];
     __shared__ uint32_t wave_count[kWaves][256];
     __shared__ uint32_t tile_start[256];
@@ -141,12 +147,12 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
     __syncthreads();
 
     // Each wave ranks its contiguous sub-tile, 64 items at a time, in order.
-    uint64_t item[kSortItems];
-    uint32_t rank[kSortItems];
-    uint32_t dig[kSortItems];
+    uint64_t item[kItems];
+    uint32_t rank[kItems];
+    uint32_t dig[kItems];
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
 #pragma unroll
-    for (uint32_t r = 0; r < kSortItems; ++r) {
+    for (uint32_t r = 0; r < kItems; ++r) {
         const uint32_t local = wave * kWaveItems + r * 64u + lane;
         const bool ok = local < valid;
         item[r] = ok ? in[base + local] : 0ull;
@@ -205,7 +211,7 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
     __syncthreads();
 
 #pragma unroll
-    for (uint32_t r = 0; r < kSortItems; ++r) {
+    for (uint32_t r = 0; r < kItems; ++r) {
         const uint32_t local = wave * kWaveItems + r * 64u + lane;
         if (local < valid) {
             const uint32_t d = dig[r];
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
     // Coalesced scatter: consecutive tile slots of one digit go to
     // consecutive global slots.
 #pragma unroll
-    for (uint32_t r = 0; r < kSortItems; ++r) {
+    for (uint32_t r = 0; r < kItems; ++r) {
         const uint32_t s = r * kSortThreads + threadIdx.x;
         if (s < valid) {
             const uint64_t it = tile_items[s];
@@ -227,28 +233,27 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
     }
 }
 
-}  // namespace
-
-void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
-                const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
-                uint32_t bits1, uint64_t** result)
+template <uint32_t kItems>
+void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
+                      const uint32_t* device_count, uint32_t shift0, uint32_t bits0,
+                      uint32_t shift1, uint32_t bits1, uint64_t** result)
 {
-    const uint32_t tiles = div_up(n, kSortTile);
+    const uint32_t tiles = div_up(n, kSortThreads * kItems);
     uint64_t* src = a;
     uint64_t* dst = b;
     auto pass = [&](uint32_t shift) {
         Timer t;
         time_begin(ctx, "radix_upsweep", t);
-        hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(kSortThreads), 0, ctx->stream, src, n,
-                           device_count, shift, ctx->histogram);
+        hipLaunchKernelGGL(k_upsweep<kItems>, dim3(tiles), dim3(kSortThreads), 0, ctx->stream,
+                           src, n, device_count, shift, ctx->histogram);
         time_end(ctx, t);
         time_begin(ctx, "radix_scan", t);
-        hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(kSortThreads), 0, ctx->stream,
+        hipLaunchKernelGGL(k_scan_rows<kItems>, dim3(256), dim3(kSortThreads), 0, ctx->stream,
                            ctx->histogram, n, device_count, ctx->digit_total);
         time_end(ctx, t);
         time_begin(ctx, "radix_downsweep", t);
-        hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(kSortThreads), 0, ctx->stream, src,
-                           dst, n, device_count, shift, ctx->histogram, ctx->digit_total);
+        hipLaunchKernelGGL(k_downsweep<kItems>, dim3(tiles), dim3(kSortThreads), 0, ctx->stream,
+                           src, dst, n, device_count, shift, ctx->histogram, ctx->digit_total);
         time_end(ctx, t);
         uint64_t* tmp = src;
         src = dst;
@@ -261,6 +266,27 @@ void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
             pass(shift1 + s);
     }
     *result = src;
+}
+
+}  // namespace
+
+void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
+                const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
+                uint32_t bits1, uint64_t** result)
+{
+    // Tile = 256 threads x sort_items (4, 8 or 16); the histogram is sized for
+    // the smallest tile.
+    switch (ctx->tuning.sort_items) {
+    case 4:
+        radix_sort_tiled<4>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result);
+        break;
+    case 16:
+        radix_sort_tiled<16>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result);
+        break;
+    default:
+        radix_sort_tiled<8>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result);
+        break;
+    }
 }
 
 }  // namespace pg

@@ -62,24 +62,27 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 namespace pg {
 
 // AABB record in HBM: two float4, {min.x, min.y, min.z, max.x} and
-// {max.y, max.z, index bits, 0}; 32 bytes, one sector, so a scattered
-// candidate costs one 32-byte access instead of six 4-byte planes.
+// {max.y, max.z, index bits, world bits}; 32 bytes, one sector, so a
+// scattered candidate costs one 32-byte access instead of six 4-byte planes.
+// `world` is the collider's world in a batch of worlds (0 for one world):
+// only records of the same world can pair.
 struct Rec {
     float b[6];  // min x y z, max x y z
     uint32_t index;
+    uint32_t world;
 };
 
 __device__ __forceinline__ Rec load_rec(const float4* __restrict__ r, size_t i)
 {
     const float4 a = r[2 * i], c = r[2 * i + 1];
-    return {{a.x, a.y, a.z, a.w, c.x, c.y}, __float_as_uint(c.z)};
+    return {{a.x, a.y, a.z, a.w, c.x, c.y}, __float_as_uint(c.z), __float_as_uint(c.w)};
 }
 
 __device__ __forceinline__ void store_rec(float4* __restrict__ r, size_t i, const float b[6],
-                                          uint32_t index)
+                                          uint32_t index, uint32_t world = 0)
 {
     r[2 * i] = make_float4(b[0], b[1], b[2], b[3]);
-    r[2 * i + 1] = make_float4(b[4], b[5], __uint_as_float(index), 0.0f);
+    r[2 * i + 1] = make_float4(b[4], b[5], __uint_as_float(index), __uint_as_float(world));
 }
 
 // Inclusive AABB overlap (Aabb::overlaps, include/phys/collision/aabb.hpp:15-19).

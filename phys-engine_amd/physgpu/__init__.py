@@ -71,6 +71,14 @@ def library():
         "physgpu_step_counts": [vp, C.POINTER(u64), C.POINTER(u64)],
         "physgpu_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(dbl), C.POINTER(u64), u32,
                                  C.POINTER(u32)],
+        "physgpu_set_worlds": [vp, u32, vp, i32],
+        "physgpu_update_poses": [vp, vp, i32],
+        "physgpu_world_ranges": [vp, vp, vp],
+        "physgpu_collide_graph": [vp, i32, dbl, C.POINTER(abi.NarrowPhaseOptions),
+                                  C.POINTER(u64), C.POINTER(u64)],
+        "physgpu_set_tuning": [vp, C.POINTER(abi.Tuning)],
+        "physgpu_get_tuning": [vp, C.POINTER(abi.Tuning)],
+        "physgpu_autotune": [vp, i32, u32, C.POINTER(abi.Tuning)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -153,6 +161,15 @@ class CollisionPipeline:
         self._keep = scene
         _check(library().physgpu_set_colliders(self._ctx, C.byref(c), abi.HOST))
         self.count = scene.count
+        self.worlds = 0
+
+    def update_poses(self, pose):
+        """New poses (float32 [7, count]) for the resident colliders; shapes,
+        worlds and a captured step graph are kept."""
+        pose = np.ascontiguousarray(pose, np.float32)
+        if pose.shape != (7, self.count):
+            raise ValueError(f"pose must be [7, {self.count}]")
+        _check(library().physgpu_update_poses(self._ctx, pose.ctypes.data, abi.HOST))
 
     def set_world(self, scene):
         """World-level form: body poses, body indices, local poses, materials."""
@@ -229,6 +246,65 @@ class CollisionPipeline:
         _check(library().physgpu_collide(self._ctx, int(algorithm), cell_size, C.byref(opt),
                                          C.byref(p), C.byref(m)))
         return p.value, m.value
+
+    def collide_graph(self, algorithm=BroadPhaseAlgorithm.UniformGrid, cell_size=0.0,
+                      gjk_iterations=64, epa_iterations=64, epa_tolerance=1e-6, sync=True):
+        """collide() replayed from a HIP graph (captured on the first call and
+        again whenever the step's configuration changes)."""
+        opt = _options(gjk_iterations, epa_iterations, epa_tolerance)
+        L = library()
+        if not sync:
+            _check(L.physgpu_collide_graph(self._ctx, int(algorithm), cell_size, C.byref(opt),
+                                           None, None))
+            return None
+        p, m = C.c_uint64(0), C.c_uint64(0)
+        _check(L.physgpu_collide_graph(self._ctx, int(algorithm), cell_size, C.byref(opt),
+                                       C.byref(p), C.byref(m)))
+        return p.value, m.value
+
+    def set_worlds(self, offsets):
+        """Batch of worlds over the resident colliders: world w owns colliders
+        [offsets[w], offsets[w + 1]).  None or [] returns to one world."""
+        if offsets is None or len(offsets) == 0:
+            _check(library().physgpu_set_worlds(self._ctx, 0, None, abi.HOST))
+            self.worlds = 0
+            return
+        off = np.ascontiguousarray(offsets, np.uint32)
+        _check(library().physgpu_set_worlds(self._ctx, len(off) - 1, off.ctypes.data, abi.HOST))
+        self.worlds = len(off) - 1
+
+    def world_ranges(self):
+        """(pair_offsets, manifold_offsets) of the last step, uint64 [worlds + 1] each."""
+        w = getattr(self, "worlds", 0)
+        po = np.zeros(w + 1, np.uint64)
+        mo = np.zeros(w + 1, np.uint64)
+        _check(library().physgpu_world_ranges(self._ctx, po.ctypes.data, mo.ctypes.data))
+        return po, mo
+
+    def tuning(self):
+        t = abi.Tuning()
+        _check(library().physgpu_get_tuning(self._ctx, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, tuning=None):
+        """Set launch shapes from a dict (missing or zero fields: defaults)."""
+        if tuning is None:
+            _check(library().physgpu_set_tuning(self._ctx, None))
+            return
+        t = abi.Tuning()
+        for k, v in tuning.items():
+            if isinstance(v, (list, tuple)):
+                arr = getattr(t, k)
+                for i, x in enumerate(v):
+                    arr[i] = x
+            else:
+                setattr(t, k, v)
+        _check(library().physgpu_set_tuning(self._ctx, C.byref(t)))
+
+    def autotune(self, algorithm=BroadPhaseAlgorithm.UniformGrid, repeats=3):
+        t = abi.Tuning()
+        _check(library().physgpu_autotune(self._ctx, int(algorithm), repeats, C.byref(t)))
+        return t.as_dict()
 
     def step_counts(self):
         """(candidate pairs, manifolds) of the last step."""

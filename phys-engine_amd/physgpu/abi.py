@@ -140,6 +140,26 @@ class NarrowPhaseOptions(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("class_block", C.c_uint32 * 3),
+        ("class_groups", C.c_uint32 * 3),
+        ("gjk_block", C.c_uint32),
+        ("gjk_groups", C.c_uint32),
+        ("epa_groups", C.c_uint32),
+        ("query_block", C.c_uint32),
+        ("query_groups", C.c_uint32),
+        ("sort_items", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if not isinstance(v, int) else v
+        return out
+
+
 class WorldExtras(C.Structure):
     _fields_ = [
         ("body", C.POINTER(C.c_uint32)),

@@ -16,6 +16,9 @@ GPU runs read the same inputs.  Workloads:
   mixed       50% spheres r ~ U[0.2, 0.5], 40% boxes h ~ U[0.2, 0.6]^3,
               10% hulls of 16 points on an ellipsoid, semi-axes U[0.3, 0.6]^3,
               uniform unit quaternions, centres uniform in a cube of side L
+  batched_worlds  worlds with body counts floor(exp(U[ln 16, ln 4096])), each a
+              mixed scene in a cube of side L0 n^(1/3) (about one pair per
+              body), concatenated with per-world collider offsets
   stack       64 x 64 columns of 64 unit boxes sunk 1e-3 m into each other,
               yaw U[-0.05, 0.05], lateral jitter U[-0.01, 0.01], on a ground
               box of half extents (100, 0.5, 100)
@@ -131,6 +134,55 @@ def mixed(n=1048576, seed=20261203, side=256.0, hull_points=16, with_hulls=True)
     pose = np.ascontiguousarray(np.concatenate([pos, quat]).astype(np.float32))
     return Scene(pose=pose, shape_type=types, shape_data=data, hull_range=hr, hull_points=hp,
                  meta=dict(workload="mixed", n=n, seed=seed, side=side))
+
+
+def concat(parts):
+    """One scene holding `parts` back to back (hull ranges re-based) and the
+    collider offsets of each part (len(parts) + 1 entries)."""
+    counts = np.array([p.count for p in parts], np.int64)
+    offsets = np.zeros(len(parts) + 1, np.uint32)
+    offsets[1:] = np.cumsum(counts)
+    n = int(offsets[-1])
+    pose = np.concatenate([p.pose for p in parts], axis=1)
+    types = np.concatenate([p.shape_type for p in parts])
+    data = np.concatenate([p.shape_data for p in parts], axis=1)
+    hr = hp = None
+    if any(p.hull_range is not None for p in parts):
+        hr = np.zeros((2, n), np.uint32)
+        pts = []
+        cursor = 0
+        for p, o in zip(parts, offsets[:-1]):
+            if p.hull_range is None:
+                continue
+            hr[:, o:o + p.count] = p.hull_range
+            hr[0, o:o + p.count] += np.uint32(cursor)
+            pts.append(p.hull_points)
+            cursor += p.hull_points.shape[1]
+        hp = np.ascontiguousarray(np.concatenate(pts, axis=1), np.float32)
+    sc = Scene(pose=np.ascontiguousarray(pose), shape_type=np.ascontiguousarray(types),
+               shape_data=np.ascontiguousarray(data), hull_range=hr, hull_points=hp,
+               meta=dict(workload="concat", n=n, parts=len(parts)))
+    return sc, offsets
+
+
+def batched_worlds(worlds=4096, seed=20261203, min_bodies=16, max_bodies=4096,
+                   pairs_per_body=1.33):
+    """Independent mixed worlds of uneven size, concatenated.
+
+    World w has n_w = floor(exp(U[ln min, ln max])) bodies drawn like `mixed`
+    (seeded by (seed, w)) in a cube of side mixed_side_for(n_w, pairs_per_body)
+    = L0 n_w^(1/3).  `pairs_per_body` is the bulk density; the cube walls cut
+    the pairs of small worlds, and the default 1.33 gives about 1.0 candidate
+    pair (2 overlaps) per body over the batch.  Returns (scene, offsets, counts)."""
+    rng = np.random.default_rng(seed)
+    counts = np.floor(np.exp(rng.uniform(np.log(min_bodies), np.log(max_bodies),
+                                         worlds))).astype(np.int64)
+    parts = [mixed(int(c), seed=[seed, w], side=mixed_side_for(int(c), pairs_per_body))
+             for w, c in enumerate(counts)]
+    sc, offsets = concat(parts)
+    sc.meta = dict(workload="batched_worlds", worlds=worlds, n=sc.count, seed=seed,
+                   min_bodies=min_bodies, max_bodies=max_bodies, pairs_per_body=pairs_per_body)
+    return sc, offsets, counts
 
 
 def stack(columns=64, height=64, seed=20261203, spacing=1.1, sink=1e-3):
