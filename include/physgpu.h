@@ -338,6 +338,36 @@ physgpu_status physgpu_get_tuning(physgpu_context* ctx, physgpu_tuning* tuning);
 physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t repeats,
                                 physgpu_tuning* out);
 
+/* ---- Narrow-phase adjoint -------------------------------------------------
+ * No counterpart in the reference (which has no gradients); the forward is
+ * NarrowPhase::generateContact (narrowphase.cpp:132-324, sat.cpp:38-102).
+ * The differentiable outputs of manifold i of the last step (m = the step's
+ * manifold count) are its normal, and for k < point_count the depth
+ * penetration[k] and the world contact point whose collider-local forms are
+ * anchor_a[k] / anchor_b[k].  Supported classes: sphere-sphere, sphere-box /
+ * box-sphere and box-box (the SAT axis, clipping vertices or fallback point
+ * the forward chose); hull manifolds get zero gradients and are counted in
+ * *skipped.  Gradients are with respect to the colliders' world poses (the
+ * resident poses, or the composed ones after physgpu_set_world); quaternion
+ * gradients are tangent to the unit sphere (gradients with respect to the
+ * orientation q/|q|).
+ *
+ * Planes (SoA, m entries each): g_normal 3, g_depth 4 (plane k = point k),
+ * g_point 12 (plane 3k + axis); NULL means zero.  Outputs: grad_pose 7 x
+ * colliders (px py pz qx qy qz qw, summed over each collider's manifolds;
+ * overwritten) and/or grad_pair 14 x m (collider `first`'s 7, then
+ * `second`'s); either may be NULL.  `memory` says where all arrays live. */
+physgpu_status physgpu_contact_adjoint(physgpu_context* ctx, uint64_t m, const float* g_normal,
+                                       const float* g_depth, const float* g_point,
+                                       float* grad_pose, float* grad_pair, physgpu_memory memory,
+                                       uint64_t* skipped);
+
+/* The differentiable outputs themselves, recomputed by the adjoint's forward
+ * (bit-identical normals and depths to the manifolds): normal 3 x m, depth
+ * 4 x m, world points 12 x m (zero beyond point_count and for hulls). */
+physgpu_status physgpu_contact_geometry(physgpu_context* ctx, uint64_t m, float* normal,
+                                        float* depth, float* point, physgpu_memory memory);
+
 /* Average device time (ms) per launch of each kernel since timing was
  * enabled.  Writes up to `capacity` entries; names are static strings. */
 physgpu_status physgpu_kernel_times(physgpu_context* ctx, const char** names, double* avg_ms,

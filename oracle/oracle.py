@@ -71,6 +71,13 @@ class CpuCollision:
         if which == "oracle":
             L.orc_set_epa_bounds.argtypes = [C.c_uint32, C.c_uint32]
             L.orc_set_epa_bounds.restype = None
+            L.orc_contact_adjoint_f64.argtypes = [C.POINTER(abi.Colliders), vp, C.c_uint64, vp,
+                                                  vp, vp, C.c_uint32, vp, vp]
+            L.orc_contact_fd_f64.argtypes = [C.POINTER(abi.Colliders), vp, C.c_uint64, vp, vp, vp,
+                                             C.c_double, C.c_uint32, vp, vp]
+            L.orc_contact_geometry_f64.argtypes = [C.POINTER(abi.Colliders), vp, C.c_uint64, vp,
+                                                   vp, vp, vp]
+            L.orc_contact_choices.argtypes = [C.POINTER(abi.Colliders), vp, C.c_uint64, vp]
         getattr(L, p + "world_collide").argtypes = [
             C.POINTER(abi.Colliders), vp, vp, vp, vp, C.c_int, C.c_uint32, C.c_uint32,
             C.c_uint32, C.c_uint32, vp, C.c_uint64, C.POINTER(C.c_uint64), vp, vp]
@@ -174,3 +181,77 @@ class CpuCollision:
         t = dict(broadphase_ms=timing[0], narrowphase_ms=timing[1], aabb_ms=timing[2],
                  total_ms=timing[3], candidate_pairs=int(timing[4]), possible_pairs=timing[5])
         return out[:count.value], t, np_stats
+
+    # ---- narrow-phase adjoint (FP64): pairs are the manifolds' (first, second),
+    # upstream gradients SoA planes g_normal [3, m], g_depth [4, m], g_point [12, m].
+    @staticmethod
+    def _planes(a, rows, m):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.shape == (rows, m), (a.shape, rows, m)
+        return a
+
+    def contact_adjoint_f64(self, scene, pairs, g_normal=None, g_depth=None, g_point=None,
+                            workers=8):
+        """Dual-number (forward-mode) Jacobian-vector products: grad [14, m]
+        (A's px..qw then B's) and status [m] (1 done, 0 no contact, -1 hull)."""
+        c = abi.colliders_struct(scene)
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        m = pairs.shape[0]
+        gn, gd, gp = (self._planes(g_normal, 3, m), self._planes(g_depth, 4, m),
+                      self._planes(g_point, 12, m))
+        grad = np.zeros((14, max(m, 1)), np.float64)
+        status = np.zeros(max(m, 1), np.int32)
+        ptr = (lambda a: None if a is None else a.ctypes.data)
+        rc = self.lib.orc_contact_adjoint_f64(C.byref(c), pairs.ctypes.data, m, ptr(gn), ptr(gd),
+                                              ptr(gp), workers, grad.ctypes.data,
+                                              status.ctypes.data)
+        self._check(rc, "contact_adjoint_f64")
+        return grad[:, :m], status[:m]
+
+    def contact_fd_f64(self, scene, pairs, g_normal=None, g_depth=None, g_point=None, h=1e-6,
+                       workers=8):
+        """FP64 central differences of the same linearised loss: grad [14, m]
+        and stable [m] (1 when no discrete choice changed under the steps)."""
+        c = abi.colliders_struct(scene)
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        m = pairs.shape[0]
+        gn, gd, gp = (self._planes(g_normal, 3, m), self._planes(g_depth, 4, m),
+                      self._planes(g_point, 12, m))
+        grad = np.zeros((14, max(m, 1)), np.float64)
+        stable = np.zeros(max(m, 1), np.int32)
+        ptr = (lambda a: None if a is None else a.ctypes.data)
+        rc = self.lib.orc_contact_fd_f64(C.byref(c), pairs.ctypes.data, m, ptr(gn), ptr(gd),
+                                         ptr(gp), h, workers, grad.ctypes.data,
+                                         stable.ctypes.data)
+        self._check(rc, "contact_fd_f64")
+        return grad[:, :m], stable[:m].astype(bool)
+
+    def contact_geometry_f64(self, scene, pairs):
+        """FP64 normal [3, m], depth [4, m], world points [12, m], count [m]."""
+        c = abi.colliders_struct(scene)
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        m = pairs.shape[0]
+        normal = np.zeros((3, max(m, 1)), np.float64)
+        depth = np.zeros((4, max(m, 1)), np.float64)
+        point = np.zeros((12, max(m, 1)), np.float64)
+        count = np.zeros(max(m, 1), np.int32)
+        rc = self.lib.orc_contact_geometry_f64(C.byref(c), pairs.ctypes.data, m,
+                                               normal.ctypes.data, depth.ctypes.data,
+                                               point.ctypes.data, count.ctypes.data)
+        self._check(rc, "contact_geometry_f64")
+        return normal[:, :m], depth[:, :m], point[:, :m], count[:m]
+
+    CHOICE_FIELDS = ("hit", "sel", "flip", "fallback", "count", "branch", "face", "clamped",
+                     "support_a", "support_b", "zero_depths", "from_b")
+
+    def contact_choices(self, scene, pairs):
+        """Discrete choices of each pair's FP64 contact (dict of int arrays)."""
+        c = abi.colliders_struct(scene)
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        m = pairs.shape[0]
+        out = np.zeros((max(m, 1), 12), np.int32)
+        rc = self.lib.orc_contact_choices(C.byref(c), pairs.ctypes.data, m, out.ctypes.data)
+        self._check(rc, "contact_choices")
+        return {f: out[:m, i] for i, f in enumerate(self.CHOICE_FIELDS)}

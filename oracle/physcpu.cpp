@@ -48,9 +48,9 @@ Box6<T> aabbOf(const Shape<T>& s, const Pose<T>& t)
     if (s.type == kBox) {
         M3<T> m = toMat(t.q);
         V3<T> h = s.data;
-        V3<T> e{std::abs(m.m[0][0]) * h.x + std::abs(m.m[0][1]) * h.y + std::abs(m.m[0][2]) * h.z,
-                std::abs(m.m[1][0]) * h.x + std::abs(m.m[1][1]) * h.y + std::abs(m.m[1][2]) * h.z,
-                std::abs(m.m[2][0]) * h.x + std::abs(m.m[2][1]) * h.y + std::abs(m.m[2][2]) * h.z};
+        V3<T> e{absOf(m.m[0][0]) * h.x + absOf(m.m[0][1]) * h.y + absOf(m.m[0][2]) * h.z,
+                absOf(m.m[1][0]) * h.x + absOf(m.m[1][1]) * h.y + absOf(m.m[1][2]) * h.z,
+                absOf(m.m[2][0]) * h.x + absOf(m.m[2][1]) * h.y + absOf(m.m[2][2]) * h.z};
         return {sub(t.p, e), add(t.p, e)};
     }
     V3<T> w = toWorld(s.points[0], t);
@@ -322,8 +322,8 @@ template <typename T>
 T projectedRadius(V3<T> h, const Pose<T>& t, V3<T> axis)
 {
     M3<T> m = toMat(t.q);
-    return std::abs(dot(column(m, 0), axis)) * h.x + std::abs(dot(column(m, 1), axis)) * h.y +
-           std::abs(dot(column(m, 2), axis)) * h.z;
+    return absOf(dot(column(m, 0), axis)) * h.x + absOf(dot(column(m, 1), axis)) * h.y +
+           absOf(dot(column(m, 2), axis)) * h.z;
 }
 
 // overlapMidpoint, narrowphase.cpp:48-58.
@@ -345,12 +345,12 @@ void supportFace(V3<T> h, const Pose<T>& t, V3<T> dir, V3<T> out[4])
 {
     V3<T> l = rotate(conj(t.q), dir);
     int face = 0;
-    T largest = std::abs(l.x);
-    if (std::abs(l.y) > largest) {
+    T largest = absOf(l.x);
+    if (absOf(l.y) > largest) {
         face = 1;
-        largest = std::abs(l.y);
+        largest = absOf(l.y);
     }
-    if (std::abs(l.z) > largest)
+    if (absOf(l.z) > largest)
         face = 2;
     T comp[3] = {l.x, l.y, l.z};
     T sign = comp[face] >= T(0) ? T(1) : T(-1);
@@ -371,7 +371,7 @@ bool insideBox(V3<T> p, V3<T> h, const Pose<T>& t)
 {
     V3<T> l = rotate(conj(t.q), sub(p, t.p));
     const T tol = T(1e-4f);
-    return std::abs(l.x) <= h.x + tol && std::abs(l.y) <= h.y + tol && std::abs(l.z) <= h.z + tol;
+    return absOf(l.x) <= h.x + tol && absOf(l.y) <= h.y + tol && absOf(l.z) <= h.z + tol;
 }
 
 // duplicatePoint, narrowphase.cpp:109-120.
@@ -393,11 +393,11 @@ bool axisOverlap(V3<T> ha, const Pose<T>& ta, V3<T> hb, const Pose<T>& tb, V3<T>
 {
     T lsq = dot(axis, axis);
     if (lsq < T(1e-8f)) {
-        depth = std::numeric_limits<T>::max();
+        depth = bigOf<T>();
         return true;
     }
-    unit = scale(axis, T(1) / std::sqrt(lsq));
-    T dist = std::abs(dot(sub(tb.p, ta.p), unit));
+    unit = scale(axis, T(1) / sqrtOf(lsq));
+    T dist = absOf(dot(sub(tb.p, ta.p), unit));
     T d = (projectedRadius(ha, ta, unit) + projectedRadius(hb, tb, unit)) - dist;
     if (d < T(0))
         return false;
@@ -408,8 +408,10 @@ bool axisOverlap(V3<T> ha, const Pose<T>& ta, V3<T> hb, const Pose<T>& tb, V3<T>
 // testOBBOBB, sat.cpp:66-102.  The depth compared for the degenerate-axis
 // case is FLT_MAX in FP32; the FP64 twin keeps FLT_MAX too (never selected).
 template <typename T>
-bool sat(V3<T> ha, const Pose<T>& ta, V3<T> hb, const Pose<T>& tb, V3<T>& axisOut, T& depthOut)
+bool sat(V3<T> ha, const Pose<T>& ta, V3<T> hb, const Pose<T>& tb, V3<T>& axisOut, T& depthOut,
+         int* selOut = nullptr)
 {
+    int index = 0;
     depthOut = T(std::numeric_limits<float>::max());
     M3<T> ma = toMat(ta.q), mb = toMat(tb.q);
     V3<T> ea[3] = {column(ma, 0), column(ma, 1), column(ma, 2)};
@@ -422,7 +424,10 @@ bool sat(V3<T> ha, const Pose<T>& ta, V3<T> hb, const Pose<T>& tb, V3<T>& axisOu
         if (d < depthOut) {
             depthOut = d;
             axisOut = unit;
+            if (selOut)
+                *selOut = index;
         }
+        ++index;
         return true;
     };
     for (int i = 0; i < 3; ++i)
@@ -443,6 +448,7 @@ void setAnchors(Contact<T>& c, uint32_t k, V3<T> world, const Pose<T>& ta, const
 {
     c.anchorA[k] = toLocal(world, ta);
     c.anchorB[k] = toLocal(world, tb);
+    c.world[k] = world;
 }
 
 // intersectSphereSphere, narrowphase.cpp:132-160.
@@ -481,10 +487,14 @@ T clampTo(T v, T lo, T hi)
 
 // intersectSphereBox, narrowphase.cpp:162-248.
 template <typename T>
-bool sphereBox(T r, const Pose<T>& ts, V3<T> h, const Pose<T>& tb, Contact<T>& c)
+bool sphereBox(T r, const Pose<T>& ts, V3<T> h, const Pose<T>& tb, Contact<T>& c,
+               Choice* ch = nullptr)
 {
     V3<T> cl = rotate(conj(tb.q), sub(ts.p, tb.p));
     V3<T> closest{clampTo(cl.x, -h.x, h.x), clampTo(cl.y, -h.y, h.y), clampTo(cl.z, -h.z, h.z)};
+    if (ch)
+        ch->clamped = (closest.x != cl.x ? 1 : 0) | (closest.y != cl.y ? 2 : 0) |
+                      (closest.z != cl.z ? 4 : 0);
     V3<T> dl = sub(closest, cl);
     T dist = length(dl);
     V3<T> nl;
@@ -500,12 +510,18 @@ bool sphereBox(T r, const Pose<T>& ts, V3<T> h, const Pose<T>& tb, Contact<T>& c
                                   {T(0), T(1), T(0)},  {T(0), T(0), T(-1)}, {T(0), T(0), T(1)}};
         T nearest = face[0];
         nl = normals[0];
+        int chosen = 0;
         for (int k = 1; k < 6; ++k)
             if (face[k] < nearest) {
                 nearest = face[k];
                 nl = normals[k];
+                chosen = k;
             }
         pen = r + nearest;
+        if (ch) {
+            ch->branch = 1;
+            ch->face = chosen;
+        }
     }
     c.normal = rotate(tb.q, nl);
     c.count = 1;
@@ -519,17 +535,24 @@ bool sphereBox(T r, const Pose<T>& ts, V3<T> h, const Pose<T>& tb, Contact<T>& c
 
 // intersectBoxBox, narrowphase.cpp:250-324.
 template <typename T>
-bool boxBox(V3<T> ha, const Pose<T>& ta, V3<T> hb, const Pose<T>& tb, Contact<T>& c)
+bool boxBox(V3<T> ha, const Pose<T>& ta, V3<T> hb, const Pose<T>& tb, Contact<T>& c,
+            Choice* ch = nullptr)
 {
     V3<T> n;
     T pen;
-    if (!sat(ha, ta, hb, tb, n, pen))
+    int sel = 0;
+    if (!sat(ha, ta, hb, tb, n, pen, &sel))
         return false;
     V3<T> centerDelta = sub(tb.p, ta.p);
-    if (dot(centerDelta, n) < T(0))
+    const bool flip = dot(centerDelta, n) < T(0);
+    if (flip)
         n = neg(n);
+    if (ch) {
+        ch->sel = sel;
+        ch->flip = flip;
+    }
     c.normal = n;
-    V3<T> ref = std::abs(n.x) < T(0.9f) ? V3<T>{T(1), T(0), T(0)} : V3<T>{T(0), T(1), T(0)};
+    V3<T> ref = absOf(n.x) < T(0.9f) ? V3<T>{T(1), T(0), T(0)} : V3<T>{T(0), T(1), T(0)};
     V3<T> t1 = normalize(cross(n, ref));
     V3<T> t2 = cross(n, t1);
     T nc = (dot(ta.p, n) + projectedRadius(ha, ta, n) + dot(tb.p, n) - projectedRadius(hb, tb, n)) *
@@ -540,28 +563,42 @@ bool boxBox(V3<T> ha, const Pose<T>& ta, V3<T> hb, const Pose<T>& tb, Contact<T>
     T depths[4];
     uint32_t count = 0;
     auto faceCandidates = [&](V3<T> hs, const Pose<T>& ts, V3<T> ho, const Pose<T>& to,
-                              V3<T> dir) {
+                              V3<T> dir, int side) {
         T otherSurface = dot(boxSupport(ho, to, neg(dir)), dir);
+        if (ch) {
+            V3<T> l = rotate(conj(to.q), neg(dir));
+            ch->support[side] = (l.x >= T(0) ? 1 : 0) | (l.y >= T(0) ? 2 : 0) | (l.z >= T(0) ? 4 : 0);
+        }
         V3<T> verts[4];
         supportFace(hs, ts, dir, verts);
         for (int k = 0; k < 4; ++k) {
             if (!insideBox(verts[k], ho, to))
                 continue;
-            T d = maxOf(dot(verts[k], dir) - otherSurface, T(0));
+            T excess = dot(verts[k], dir) - otherSurface;
+            T d = maxOf(excess, T(0));
             V3<T> cand = sub(verts[k], scale(dir, d * T(0.5f)));
             if (count < 4 && !duplicate(pts, count, cand)) {
+                if (ch) {
+                    ch->side[count] = side;
+                    ch->corner[count] = k;
+                    ch->zero[count] = excess < T(0);
+                }
                 depths[count] = d;
                 pts[count++] = cand;
             }
         }
     };
-    faceCandidates(ha, ta, hb, tb, n);
-    faceCandidates(hb, tb, ha, ta, neg(n));
+    faceCandidates(ha, ta, hb, tb, n, 0);
+    faceCandidates(hb, tb, ha, ta, neg(n), 1);
     if (count == 0) {
         depths[0] = pen;
         pts[0] = fallback;
         count = 1;
+        if (ch)
+            ch->fallback = 1;
     }
+    if (ch)
+        ch->count = int(count);
     c.count = count;
     for (uint32_t k = 0; k < count; ++k) {
         c.depth[k] = depths[k];
@@ -903,6 +940,147 @@ template bool generateContact<float>(const Shape<float>&, const Pose<float>&, co
 template bool generateContact<double>(const Shape<double>&, const Pose<double>&,
                                       const Shape<double>&, const Pose<double>&,
                                       const NarrowOptions&, Contact<double>&, int*);
+
+} // namespace orc
+
+namespace orc {
+
+// ---------------------------------------------------------------- FP64 adjoint
+// Forward-mode dual numbers over the same templated contact code: 14 tangent
+// directions (position and quaternion of A, then of B), so one evaluation
+// gives the Jacobian of the normal, depths and world points with respect to
+// both poses.  The orientation enters as q/|q|, so quaternion derivatives are
+// those of the contact as a function of the normalised orientation.
+struct Dual {
+    static constexpr int N = 14;
+    double v;
+    double d[N];
+    Dual() : v(0.0) { std::fill(d, d + N, 0.0); }
+    Dual(double x) : v(x) { std::fill(d, d + N, 0.0); }
+};
+inline Dual operator+(const Dual& a, const Dual& b)
+{
+    Dual r(a.v + b.v);
+    for (int i = 0; i < Dual::N; ++i)
+        r.d[i] = a.d[i] + b.d[i];
+    return r;
+}
+inline Dual operator-(const Dual& a, const Dual& b)
+{
+    Dual r(a.v - b.v);
+    for (int i = 0; i < Dual::N; ++i)
+        r.d[i] = a.d[i] - b.d[i];
+    return r;
+}
+inline Dual operator-(const Dual& a)
+{
+    Dual r(-a.v);
+    for (int i = 0; i < Dual::N; ++i)
+        r.d[i] = -a.d[i];
+    return r;
+}
+inline Dual operator*(const Dual& a, const Dual& b)
+{
+    Dual r(a.v * b.v);
+    for (int i = 0; i < Dual::N; ++i)
+        r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    return r;
+}
+inline Dual operator/(const Dual& a, const Dual& b)
+{
+    Dual r(a.v / b.v);
+    for (int i = 0; i < Dual::N; ++i)
+        r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
+    return r;
+}
+inline bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+inline bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+inline bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+inline bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+inline bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+inline bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+template <>
+inline Dual absOf<Dual>(Dual x) { return x.v < 0.0 ? -x : x; }
+template <>
+inline Dual sqrtOf<Dual>(Dual x)
+{
+    Dual r(std::sqrt(x.v));
+    for (int i = 0; i < Dual::N; ++i)
+        r.d[i] = r.v > 0.0 ? x.d[i] / (2.0 * r.v) : 0.0;
+    return r;
+}
+template <>
+inline Dual bigOf<Dual>() { return Dual(std::numeric_limits<double>::max()); }
+
+// Sphere-sphere, sphere-box, box-sphere, box-box (NarrowPhase::generateContact,
+// narrowphase.cpp:326-374); hull pairs are not differentiated.
+template <typename T>
+bool contactNoHull(int typeA, V3<T> dataA, const Pose<T>& ta, int typeB, V3<T> dataB,
+                   const Pose<T>& tb, Contact<T>& c, Choice* ch)
+{
+    c.count = 0;
+    if (typeA == kSphere && typeB == kSphere)
+        return np::sphereSphere(dataA.x, ta, dataB.x, tb, c);
+    if (typeA == kSphere && typeB == kBox)
+        return np::sphereBox(dataA.x, ta, dataB, tb, c, ch);
+    if (typeA == kBox && typeB == kSphere) {
+        if (!np::sphereBox(dataB.x, tb, dataA, ta, c, ch))
+            return false;
+        c.normal = neg(c.normal);
+        return true;
+    }
+    if (typeA == kBox && typeB == kBox)
+        return np::boxBox(dataA, ta, dataB, tb, c, ch);
+    return false;
+}
+
+template <typename T>
+Pose<T> normalisedPose(const double p[7])
+{
+    Pose<T> t{{T(p[0]), T(p[1]), T(p[2])}, {T(p[3]), T(p[4]), T(p[5]), T(p[6])}};
+    T len = sqrtOf(t.q.x * t.q.x + t.q.y * t.q.y + t.q.z * t.q.z + t.q.w * t.q.w);
+    t.q = {t.q.x / len, t.q.y / len, t.q.z / len, t.q.w / len};
+    return t;
+}
+
+// Pose with tangent directions base..base+6 on its 7 components.
+inline Pose<Dual> normalisedPoseSeeded(const double p[7], int base)
+{
+    Dual c[7];
+    for (int q = 0; q < 7; ++q) {
+        c[q] = Dual(p[q]);
+        c[q].d[base + q] = 1.0;
+    }
+    Pose<Dual> t{{c[0], c[1], c[2]}, {c[3], c[4], c[5], c[6]}};
+    Dual len = sqrtOf(t.q.x * t.q.x + t.q.y * t.q.y + t.q.z * t.q.z + t.q.w * t.q.w);
+    t.q = {t.q.x / len, t.q.y / len, t.q.z / len, t.q.w / len};
+    return t;
+}
+
+// Upstream gradients of one manifold (SoA planes of m entries at index k).
+struct Upstream {
+    double n[3], d[4], p[4][3];
+    Upstream(const float* gn, const float* gd, const float* gp, uint64_t m, uint64_t k)
+    {
+        for (int a = 0; a < 3; ++a)
+            n[a] = gn ? gn[a * m + k] : 0.0;
+        for (int j = 0; j < 4; ++j) {
+            d[j] = gd ? gd[j * m + k] : 0.0;
+            for (int a = 0; a < 3; ++a)
+                p[j][a] = gp ? gp[(3 * j + a) * m + k] : 0.0;
+        }
+    }
+    // Linearised loss: g . (normal, depths, points) of a contact.
+    template <typename T>
+    T phi(const Contact<T>& c) const
+    {
+        T s = T(n[0]) * c.normal.x + T(n[1]) * c.normal.y + T(n[2]) * c.normal.z;
+        for (uint32_t j = 0; j < c.count && j < 4; ++j)
+            s = s + T(d[j]) * c.depth[j] + T(p[j][0]) * c.world[j].x +
+                T(p[j][1]) * c.world[j].y + T(p[j][2]) * c.world[j].z;
+        return s;
+    }
+};
 
 } // namespace orc
 
@@ -1330,6 +1508,175 @@ int orc_world_collide(const physgpu_colliders* c, const float* bodyPosition,
             for (size_t k = 0; k < contacts.size() && k < capacity; ++k)
                 out[k] = contacts[k];
         return contacts.size() > capacity ? PHYSGPU_CAPACITY_EXCEEDED : PHYSGPU_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// FP64 contact of pair k at poses pa, pb (7 doubles each, q normalised).
+static bool contactF64(View<double>& v, uint32_t a, uint32_t b, const double pa[7],
+                       const double pb[7], Contact<double>& c, Choice* ch)
+{
+    Shape<double> sa = v.shape(a), sb = v.shape(b);
+    return contactNoHull(sa.type, sa.data, normalisedPose<double>(pa), sb.type, sb.data,
+                         normalisedPose<double>(pb), c, ch);
+}
+
+static void posesOf(const physgpu_colliders* c, uint32_t i, double out[7])
+{
+    const uint32_t n = c->count;
+    for (int q = 0; q < 7; ++q)
+        out[q] = c->pose[q * n + i];
+}
+
+// Pairs are the manifolds' (first, second).  status[k]: 1 differentiated,
+// 0 no contact in FP64, -1 hull pair (not differentiated).  grad: 14 x n
+// planes (A's px py pz qx qy qz qw, then B's).
+int orc_contact_adjoint_f64(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t n,
+                            const float* gn, const float* gd, const float* gp, uint32_t workers,
+                            double* grad, int32_t* status)
+{
+    try {
+        View<double> v(c);
+        auto one = [&](size_t k) {
+            const uint32_t a = pairs[k].first, b = pairs[k].second;
+            for (int q = 0; q < 14; ++q)
+                grad[q * n + k] = 0.0;
+            Shape<double> sa = v.shape(a), sb = v.shape(b);
+            if (sa.type == kHull || sb.type == kHull) {
+                status[k] = -1;
+                return;
+            }
+            double pa[7], pb[7];
+            posesOf(c, a, pa);
+            posesOf(c, b, pb);
+            Pose<Dual> ta = normalisedPoseSeeded(pa, 0), tb = normalisedPoseSeeded(pb, 7);
+            V3<Dual> da{sa.data.x, sa.data.y, sa.data.z}, db{sb.data.x, sb.data.y, sb.data.z};
+            Contact<Dual> ct;
+            if (!contactNoHull(sa.type, da, ta, sb.type, db, tb, ct, nullptr)) {
+                status[k] = 0;
+                return;
+            }
+            const Dual phi = Upstream(gn, gd, gp, n, k).phi(ct);
+            for (int q = 0; q < 14; ++q)
+                grad[q * n + k] = phi.d[q];
+            status[k] = 1;
+        };
+        Pool::run(workers, n, one);
+        return PHYSGPU_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// Central differences (step h) of the linearised loss in FP64, each pose
+// component perturbed in turn (quaternions perturbed, then normalised).
+// stable[k] = 1 when every perturbed evaluation keeps the contact and all its
+// discrete choices.
+int orc_contact_fd_f64(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t n,
+                       const float* gn, const float* gd, const float* gp, double h,
+                       uint32_t workers, double* grad, int32_t* stable)
+{
+    try {
+        View<double> v(c);
+        auto one = [&](size_t k) {
+            const uint32_t a = pairs[k].first, b = pairs[k].second;
+            stable[k] = 0;
+            for (int q = 0; q < 14; ++q)
+                grad[q * n + k] = 0.0;
+            Shape<double> sa = v.shape(a), sb = v.shape(b);
+            if (sa.type == kHull || sb.type == kHull)
+                return;
+            double x[14];
+            posesOf(c, a, x);
+            posesOf(c, b, x + 7);
+            const Upstream up(gn, gd, gp, n, k);
+            Contact<double> c0;
+            Choice ch0;
+            if (!contactF64(v, a, b, x, x + 7, c0, &ch0))
+                return;
+            bool same = true;
+            for (int q = 0; q < 14; ++q) {
+                double f[2] = {0.0, 0.0};
+                for (int s = 0; s < 2; ++s) {
+                    double y[14];
+                    std::copy(x, x + 14, y);
+                    y[q] += s ? -h : h;
+                    Contact<double> cs;
+                    Choice ch;
+                    if (!contactF64(v, a, b, y, y + 7, cs, &ch) || !(ch == ch0) ||
+                        cs.count != c0.count)
+                        same = false;
+                    f[s] = up.phi(cs);
+                }
+                grad[q * n + k] = (f[0] - f[1]) / (2.0 * h);
+            }
+            stable[k] = same ? 1 : 0;
+        };
+        Pool::run(workers, n, one);
+        return PHYSGPU_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// Discrete choices of each pair's FP64 contact, 12 ints per pair: hit, sel,
+// flip, fallback, count, branch, face, clamped, support[0], support[1],
+// and the number of zero-clamped clipping depths, kept points from B's face.
+int orc_contact_choices(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t n,
+                        int32_t* out)
+{
+    try {
+        View<double> v(c);
+        for (uint64_t k = 0; k < n; ++k) {
+            double pa[7], pb[7];
+            posesOf(c, pairs[k].first, pa);
+            posesOf(c, pairs[k].second, pb);
+            Contact<double> ct{};
+            Choice ch;
+            const bool hit = contactF64(v, pairs[k].first, pairs[k].second, pa, pb, ct, &ch);
+            int zeros = 0, fromB = 0;
+            for (int j = 0; j < ch.count; ++j) {
+                zeros += ch.zero[j];
+                fromB += ch.side[j];
+            }
+            const int32_t row[12] = {hit, ch.sel, ch.flip, ch.fallback, ch.count, ch.branch,
+                                     ch.face, ch.clamped, ch.support[0], ch.support[1], zeros,
+                                     fromB};
+            std::copy(row, row + 12, out + 12 * k);
+        }
+        return PHYSGPU_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// FP64 normal (3 x n), depths (4 x n), world points (12 x n) and point count
+// of each pair (0 = no contact or hull pair), orientation q/|q|.
+int orc_contact_geometry_f64(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t n,
+                             double* normal, double* depth, double* point, int32_t* count)
+{
+    try {
+        View<double> v(c);
+        for (uint64_t k = 0; k < n; ++k) {
+            double pa[7], pb[7];
+            posesOf(c, pairs[k].first, pa);
+            posesOf(c, pairs[k].second, pb);
+            Contact<double> ct{};
+            const bool hit = contactF64(v, pairs[k].first, pairs[k].second, pa, pb, ct, nullptr);
+            count[k] = hit ? int32_t(ct.count) : 0;
+            const double nv[3] = {ct.normal.x, ct.normal.y, ct.normal.z};
+            for (int a = 0; a < 3; ++a)
+                normal[a * n + k] = hit ? nv[a] : 0.0;
+            for (int j = 0; j < 4; ++j) {
+                const bool used = hit && uint32_t(j) < ct.count;
+                depth[j * n + k] = used ? ct.depth[j] : 0.0;
+                point[(3 * j) * n + k] = used ? ct.world[j].x : 0.0;
+                point[(3 * j + 1) * n + k] = used ? ct.world[j].y : 0.0;
+                point[(3 * j + 2) * n + k] = used ? ct.world[j].z : 0.0;
+            }
+        }
+        return PHYSGPU_OK;
     } catch (const std::exception& e) {
         return fail(e);
     }

@@ -45,9 +45,19 @@ inline V3<T> cross(V3<T> a, V3<T> b)
 {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+// |x| and sqrt as the reference calls them; overloaded for the dual numbers
+// of the FP64 adjoint.
+template <typename T>
+inline T absOf(T x) { return std::abs(x); }
+template <typename T>
+inline T sqrtOf(T x) { return std::sqrt(x); }
+// Largest finite value (std::numeric_limits<T>::max()).
+template <typename T>
+inline T bigOf() { return std::numeric_limits<T>::max(); }
+
 // Math3d::length, math_utils.cpp:7-12.
 template <typename T>
-inline T length(V3<T> a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+inline T length(V3<T> a) { return sqrtOf(a.x * a.x + a.y * a.y + a.z * a.z); }
 // Math3d::normalize, math_utils.cpp:19-23: zero below 1e-8, else v * (1/len).
 template <typename T>
 inline V3<T> normalize(V3<T> a)
@@ -142,6 +152,30 @@ struct Contact {
     V3<T> anchorA[4];
     V3<T> anchorB[4];
     T depth[4];
+    V3<T> world[4];  // world contact points (anchors are their collider-local forms)
+};
+
+// Discrete choices of one sphere-box or box-box contact: equal choices at two
+// poses mean the contact is one smooth function between them.
+struct Choice {
+    int sel = -1, flip = 0, fallback = 0, count = 0;  // box-box SAT axis, sign, fallback
+    int side[4] = {0, 0, 0, 0}, corner[4] = {0, 0, 0, 0}, zero[4] = {0, 0, 0, 0};
+    // Box-box: sign bits of the other box's support corner per side.  Not part
+    // of the comparison: when the support is a face or an edge of the other
+    // box, every tied corner has the same projection on the normal for all
+    // nearby poses (the normal is that face's or edge's own axis).
+    int support[2] = {0, 0};
+    int branch = 0, face = 0, clamped = 0;             // sphere-box
+    bool operator==(const Choice& o) const
+    {
+        if (sel != o.sel || flip != o.flip || fallback != o.fallback || count != o.count ||
+            branch != o.branch || face != o.face || clamped != o.clamped)
+            return false;
+        for (int k = 0; k < 4; ++k)
+            if (side[k] != o.side[k] || corner[k] != o.corner[k] || zero[k] != o.zero[k])
+                return false;
+        return true;
+    }
 };
 
 struct NarrowOptions {

@@ -98,7 +98,8 @@ void free_all(physgpu_context* c)
                     c->manifolds, c->body, c->local_pose, c->friction, c->restitution,
                     c->epa_list, c->epa_simplex,
                     c->counters, c->world_body_pose, c->world_offsets, c->world_of,
-                    c->world_pair_off, c->world_manifold_off};
+                    c->world_pair_off, c->world_manifold_off, c->adjoint_scratch,
+                    c->adjoint_skipped};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -418,6 +419,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->friction, N));
     A(alloc(c->restitution, N));
     A(alloc(c->world_of, N));
+    A(alloc(c->adjoint_skipped, 1));
     A(alloc(c->counters, 1));
     A(hipHostMalloc(reinterpret_cast<void**>(&c->host_counters), sizeof(Counters)));
     if (e != hipSuccess) {
@@ -1045,6 +1047,112 @@ physgpu_status physgpu_narrowphase_stats_get(physgpu_context* ctx, physgpu_narro
         return st;
     fill_np_stats(*ctx->host_counters, s);
     return PHYSGPU_OK;
+}
+
+}  // extern "C"
+
+namespace pg {
+namespace {
+
+// Shared body of physgpu_contact_adjoint / physgpu_contact_geometry.
+physgpu_status contact_pass(physgpu_context* ctx, uint64_t m, const float* g_normal,
+                            const float* g_depth, const float* g_point, float* grad_pose,
+                            float* grad_pair, float* geom_normal, float* geom_depth,
+                            float* geom_point, physgpu_memory memory, uint64_t* skipped)
+{
+    if (memory != PHYSGPU_HOST && memory != PHYSGPU_DEVICE)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown memory kind");
+    if (!ctx->colliders_valid)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "no colliders set");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    if (m != ctx->host_counters->manifold_count)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "m must equal the manifold count of the last step");
+    const size_t M = size_t(m), N = size_t(ctx->n);
+    const float* dn = g_normal;
+    const float* dd = g_depth;
+    const float* dp = g_point;
+    float* gpose = grad_pose;
+    float* gpair = grad_pair;
+    float* on = geom_normal;
+    float* od = geom_depth;
+    float* op = geom_point;
+    if (memory == PHYSGPU_HOST) {
+        // Staging: inputs 19 m, pair gradients 14 m, pose gradients 7 n, geometry 19 m.
+        const size_t need = 52 * std::max<size_t>(M, 1) + 7 * N;
+        if (need > ctx->adjoint_capacity) {
+            PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+            if (ctx->adjoint_scratch)
+                hipFree(ctx->adjoint_scratch);
+            ctx->adjoint_scratch = nullptr;
+            ctx->adjoint_capacity = 0;
+            PG_TRY_HIP(alloc(ctx->adjoint_scratch, need));
+            ctx->adjoint_capacity = need;
+        }
+        float* s = ctx->adjoint_scratch;
+        auto stage_in = [&](const float* src, size_t planes, float* dst) -> const float* {
+            if (!src || !M)
+                return nullptr;
+            hipMemcpyAsync(dst, src, planes * M * 4, hipMemcpyHostToDevice, ctx->stream);
+            return dst;
+        };
+        dn = stage_in(g_normal, 3, s);
+        dd = stage_in(g_depth, 4, s + 3 * M);
+        dp = stage_in(g_point, 12, s + 7 * M);
+        gpair = grad_pair ? s + 19 * M : nullptr;
+        gpose = grad_pose ? s + 33 * M : nullptr;
+        on = geom_normal ? s + 33 * M + 7 * N : nullptr;
+        od = geom_depth ? s + 36 * M + 7 * N : nullptr;
+        op = geom_point ? s + 40 * M + 7 * N : nullptr;
+    }
+    launch_contact_adjoint(ctx, uint32_t(M), dn, dd, dp, on, od, op, gpair, gpose,
+                           ctx->adjoint_skipped);
+    PG_TRY_HIP(hipGetLastError());
+    if (memory == PHYSGPU_HOST) {
+        auto out = [&](float* dst, const float* src, size_t count) -> hipError_t {
+            if (!dst || !count)
+                return hipSuccess;
+            return hipMemcpyAsync(dst, src, count * 4, hipMemcpyDeviceToHost, ctx->stream);
+        };
+        PG_TRY_HIP(out(grad_pair, gpair, 14 * M));
+        PG_TRY_HIP(out(grad_pose, gpose, 7 * N));
+        PG_TRY_HIP(out(geom_normal, on, 3 * M));
+        PG_TRY_HIP(out(geom_depth, od, 4 * M));
+        PG_TRY_HIP(out(geom_point, op, 12 * M));
+    }
+    uint32_t sk = 0;
+    PG_TRY_HIP(hipMemcpyAsync(&sk, ctx->adjoint_skipped, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    if (skipped)
+        *skipped = sk;
+    return PHYSGPU_OK;
+}
+
+}  // namespace
+}  // namespace pg
+
+extern "C" {
+
+physgpu_status physgpu_contact_adjoint(physgpu_context* ctx, uint64_t m, const float* g_normal,
+                                       const float* g_depth, const float* g_point,
+                                       float* grad_pose, float* grad_pair, physgpu_memory memory,
+                                       uint64_t* skipped)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    return contact_pass(ctx, m, g_normal, g_depth, g_point, grad_pose, grad_pair, nullptr,
+                        nullptr, nullptr, memory, skipped);
+}
+
+physgpu_status physgpu_contact_geometry(physgpu_context* ctx, uint64_t m, float* normal,
+                                        float* depth, float* point, physgpu_memory memory)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    return contact_pass(ctx, m, nullptr, nullptr, nullptr, nullptr, nullptr, normal, depth, point,
+                        memory, nullptr);
 }
 
 physgpu_status physgpu_kernel_times(physgpu_context* ctx, const char** names, double* avg_ms,

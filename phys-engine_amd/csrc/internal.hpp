@@ -152,6 +152,11 @@ struct physgpu_context {
     uint64_t* world_pair_off = nullptr;
     uint64_t* world_manifold_off = nullptr;
 
+    // Adjoint staging for host-memory calls (floats) and its skip counter.
+    float* adjoint_scratch = nullptr;
+    size_t adjoint_capacity = 0;
+    uint32_t* adjoint_skipped = nullptr;
+
     // HIP graph of the device-resident step and what it was captured with.
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -232,6 +237,11 @@ void launch_pairs_in(physgpu_context* ctx, const physgpu_pair* dev_in, uint64_t 
 // narrowphase.hip
 void launch_np_begin(physgpu_context* ctx);
 void launch_narrowphase(physgpu_context* ctx, bool world);
+// adjoint.hip
+void launch_contact_adjoint(physgpu_context* ctx, uint32_t m, const float* g_normal,
+                            const float* g_depth, const float* g_point, float* geom_normal,
+                            float* geom_depth, float* geom_point, float* grad_pair,
+                            float* grad_pose, uint32_t* skipped);
 // batch.hip
 void launch_world_of(physgpu_context* ctx);
 void launch_world_ranges(physgpu_context* ctx);

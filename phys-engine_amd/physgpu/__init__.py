@@ -73,6 +73,8 @@ def library():
                                  C.POINTER(u32)],
         "physgpu_set_worlds": [vp, u32, vp, i32],
         "physgpu_update_poses": [vp, vp, i32],
+        "physgpu_contact_adjoint": [vp, u64, vp, vp, vp, vp, vp, i32, C.POINTER(u64)],
+        "physgpu_contact_geometry": [vp, u64, vp, vp, vp, i32],
         "physgpu_world_ranges": [vp, vp, vp],
         "physgpu_collide_graph": [vp, i32, dbl, C.POINTER(abi.NarrowPhaseOptions),
                                   C.POINTER(u64), C.POINTER(u64)],
@@ -305,6 +307,64 @@ class CollisionPipeline:
         t = abi.Tuning()
         _check(library().physgpu_autotune(self._ctx, int(algorithm), repeats, C.byref(t)))
         return t.as_dict()
+
+    def contact_geometry(self):
+        """Differentiable outputs of the last step's manifolds, recomputed by
+        the adjoint's forward: normal [3, m], depth [4, m], world points [12, m]."""
+        _, m = self.step_counts()
+        normal = np.zeros((3, max(m, 1)), np.float32)
+        depth = np.zeros((4, max(m, 1)), np.float32)
+        point = np.zeros((12, max(m, 1)), np.float32)
+        _check(library().physgpu_contact_geometry(self._ctx, m, normal.ctypes.data,
+                                                  depth.ctypes.data, point.ctypes.data, abi.HOST))
+        return normal[:, :m], depth[:, :m], point[:, :m]
+
+    def contact_adjoint(self, g_normal=None, g_depth=None, g_point=None, per_pair=False):
+        """Vector-Jacobian product of the last step's manifolds: upstream
+        gradients (planes [3, m], [4, m], [12, m]; None = 0) to pose gradients
+        [7, colliders] (px py pz qx qy qz qw), or per manifold [14, m] when
+        per_pair.  Returns (gradient, skipped hull manifolds)."""
+        _, m = self.step_counts()
+
+        def plane(a, rows):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float32)
+            if a.shape != (rows, m):
+                raise ValueError(f"upstream gradient must be [{rows}, {m}]")
+            return a
+        gn, gd, gp = plane(g_normal, 3), plane(g_depth, 4), plane(g_point, 12)
+        ptr = (lambda a: None if a is None else a.ctypes.data)
+        skipped = C.c_uint64(0)
+        if per_pair:
+            out = np.zeros((14, max(m, 1)), np.float32)
+            _check(library().physgpu_contact_adjoint(self._ctx, m, ptr(gn), ptr(gd), ptr(gp), None,
+                                                     out.ctypes.data, abi.HOST,
+                                                     C.byref(skipped)))
+            return out[:, :m], skipped.value
+        out = np.zeros((7, max(self.count, 1)), np.float32)
+        _check(library().physgpu_contact_adjoint(self._ctx, m, ptr(gn), ptr(gd), ptr(gp),
+                                                 out.ctypes.data, None, abi.HOST,
+                                                 C.byref(skipped)))
+        return out[:, :self.count], skipped.value
+
+    # Device-pointer forms (torch tensors' data_ptr()); the caller orders its
+    # own stream against these calls, which synchronise the context's stream.
+    def update_poses_device(self, pose_ptr):
+        _check(library().physgpu_update_poses(self._ctx, C.c_void_p(pose_ptr), abi.DEVICE))
+
+    def contact_geometry_device(self, m, normal_ptr, depth_ptr, point_ptr):
+        _check(library().physgpu_contact_geometry(self._ctx, m, C.c_void_p(normal_ptr),
+                                                  C.c_void_p(depth_ptr), C.c_void_p(point_ptr),
+                                                  abi.DEVICE))
+
+    def contact_adjoint_device(self, m, g_normal_ptr, g_depth_ptr, g_point_ptr, grad_pose_ptr):
+        skipped = C.c_uint64(0)
+        _check(library().physgpu_contact_adjoint(
+            self._ctx, m, C.c_void_p(g_normal_ptr), C.c_void_p(g_depth_ptr),
+            C.c_void_p(g_point_ptr), C.c_void_p(grad_pose_ptr), None, abi.DEVICE,
+            C.byref(skipped)))
+        return skipped.value
 
     def step_counts(self):
         """(candidate pairs, manifolds) of the last step."""

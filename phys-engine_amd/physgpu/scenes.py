@@ -19,6 +19,9 @@ GPU runs read the same inputs.  Workloads:
   batched_worlds  worlds with body counts floor(exp(U[ln 16, ln 4096])), each a
               mixed scene in a cube of side L0 n^(1/3) (about one pair per
               body), concatenated with per-world collider offsets
+  contact_pairs  isolated touching pairs, half box-box and half sphere-box /
+              box-sphere, penetration depth U[1e-3, 5e-2] m, away from SAT
+              axis ties and sphere-box region boundaries (adjoint checks)
   stack       64 x 64 columns of 64 unit boxes sunk 1e-3 m into each other,
               yaw U[-0.05, 0.05], lateral jitter U[-0.01, 0.01], on a ground
               box of half extents (100, 0.5, 100)
@@ -183,6 +186,127 @@ def batched_worlds(worlds=4096, seed=20261203, min_bodies=16, max_bodies=4096,
     sc.meta = dict(workload="batched_worlds", worlds=worlds, n=sc.count, seed=seed,
                    min_bodies=min_bodies, max_bodies=max_bodies, pairs_per_body=pairs_per_body)
     return sc, offsets, counts
+
+
+def _rotate(q, v):
+    """Rotate v [3, n] by unit quaternions q [4, n] (xyzw), float64."""
+    u, w = q[:3], q[3]
+    return ((w * w - (u * u).sum(0)) * v + 2.0 * (u * v).sum(0) * u +
+            2.0 * w * np.cross(u, v, axis=0))
+
+
+def _box_axes(q):
+    """Columns of the rotation matrix of q [4, n]: [3 (column), 3, n]."""
+    eye = np.eye(3)
+    return np.stack([_rotate(q, np.repeat(eye[:, k:k + 1], q.shape[1], 1)) for k in range(3)])
+
+
+def contact_pairs(pairs=65536, seed=20261203, spacing=8.0, tie_margin=1e-3, inside_fraction=0.0):
+    """Isolated touching pairs for adjoint checks: colliders (2i, 2i+1).
+
+    Even pairs are box-box: random orientations, B offset from A along a random
+    direction by the distance that gives SAT depth U[1e-3, 5e-2]; pairs whose
+    best and second-best SAT axes differ in depth by less than
+    tie_margin x centre distance (a 1e-3 rad turn cannot swap them) are drawn
+    again.  Odd pairs are sphere-box (sphere first on pairs 1 mod 4, box first
+    on 3 mod 4): the sphere centre sits outside the box at distance r - depth
+    from its closest point, at least 1e-3 m from the boundaries of the box's
+    face / edge / corner regions; with inside_fraction > 0 that share of the
+    sphere-box pairs instead has the sphere centre inside the box (depth
+    r + distance to the nearest face, nearest and second-nearest faces at
+    least 1e-3 m apart).  Pairs sit on a lattice `spacing` apart."""
+    rng = np.random.default_rng(seed)
+    n = 2 * pairs
+    side = int(np.ceil(pairs ** (1.0 / 3.0)))
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side), np.arange(side),
+                             indexing="ij")).reshape(3, -1)[:, :pairs].astype(np.float64)
+    centre = (g - (side - 1) / 2.0) * spacing
+    bb = np.arange(pairs) % 2 == 0
+    pose = np.zeros((7, n), np.float64)
+    types = np.zeros(n, np.uint8)
+    data = np.zeros((3, n), np.float64)
+
+    # Box-box.
+    idx = np.nonzero(bb)[0]
+    todo = idx.copy()
+    while len(todo):
+        k = len(todo)
+        qa = random_quaternions(rng, k).astype(np.float64)
+        qb = random_quaternions(rng, k).astype(np.float64)
+        ha = rng.uniform(0.2, 0.6, (3, k))
+        hb = rng.uniform(0.2, 0.6, (3, k))
+        u = rng.standard_normal((3, k))
+        u /= np.linalg.norm(u, axis=0)
+        depth = rng.uniform(1e-3, 5e-2, k)
+        ea, eb = _box_axes(qa), _box_axes(qb)
+        axes = [ea[i] for i in range(3)] + [eb[i] for i in range(3)]
+        axes += [np.cross(ea[i], eb[j], axis=0) for i in range(3) for j in range(3)]
+        s_k, d_fn = [], []
+        for a in axes:
+            ln = np.linalg.norm(a, axis=0)
+            ok = ln ** 2 >= 1e-8
+            a = a / np.where(ok, ln, 1.0)
+            ra = sum(np.abs((ea[c] * a).sum(0)) * ha[c] for c in range(3))
+            rb = sum(np.abs((eb[c] * a).sum(0)) * hb[c] for c in range(3))
+            ua = np.abs((u * a).sum(0))
+            s = np.where(ok & (ua > 1e-12), (ra + rb - depth) / np.maximum(ua, 1e-12), np.inf)
+            s_k.append(s)
+            d_fn.append((np.where(ok, ra + rb, np.inf), ua))
+        S = np.stack(s_k)
+        s_star = S.min(0)
+        depths = np.stack([r - s_star * ua for r, ua in d_fn])
+        srt = np.sort(depths, axis=0)
+        good = (np.isfinite(s_star) & (s_star > 0) &
+                (srt[1] - srt[0] >= tie_margin * s_star) & (np.abs(srt[0] - depth) < 1e-9))
+        ids = todo[good]
+        a_i, b_i = 2 * ids, 2 * ids + 1
+        pose[:3, a_i] = centre[:, ids]
+        pose[3:, a_i] = qa[:, good]
+        pose[:3, b_i] = centre[:, ids] + s_star[good] * u[:, good]
+        pose[3:, b_i] = qb[:, good]
+        types[a_i] = types[b_i] = 1
+        data[:, a_i] = ha[:, good]
+        data[:, b_i] = hb[:, good]
+        todo = todo[~good]
+
+    # Sphere-box.
+    idx = np.nonzero(~bb)[0]
+    todo = idx.copy()
+    while len(todo):
+        k = len(todo)
+        q = random_quaternions(rng, k).astype(np.float64)
+        h = rng.uniform(0.2, 0.6, (3, k))
+        r = rng.uniform(0.2, 0.5, k)
+        depth = rng.uniform(1e-3, 5e-2, k)
+        x = rng.uniform(-1.0, 1.0, (3, k)) * (h + r)
+        closest = np.clip(x, -h, h)
+        d = x - closest
+        dl = np.linalg.norm(d, axis=0)
+        local = closest + d / np.where(dl > 0, dl, 1.0) * (r - depth)
+        margin = np.abs(np.abs(local) - h).min(0)
+        good = (dl > 1e-3) & (margin >= 1e-3)
+        inside = rng.uniform(0.0, 1.0, k) < inside_fraction
+        if inside.any():
+            xin = rng.uniform(-0.9, 0.9, (3, k)) * h
+            faces = np.sort(np.concatenate([h - xin, h + xin]), axis=0)
+            local = np.where(inside, xin, local)
+            good = np.where(inside, faces[1] - faces[0] >= 1e-3, good)
+        ids = todo[good]
+        sphere_first = (ids % 4) == 1
+        box_i = np.where(sphere_first, 2 * ids + 1, 2 * ids)
+        sph_i = np.where(sphere_first, 2 * ids, 2 * ids + 1)
+        pose[:3, box_i] = centre[:, ids]
+        pose[3:, box_i] = q[:, good]
+        types[box_i] = 1
+        data[:, box_i] = h[:, good]
+        pose[:3, sph_i] = centre[:, ids] + _rotate(q[:, good], local[:, good])
+        pose[6, sph_i] = 1.0
+        types[sph_i] = 0
+        data[0, sph_i] = r[good]
+        todo = todo[~good]
+    return Scene(pose=np.ascontiguousarray(pose.astype(np.float32)), shape_type=types,
+                 shape_data=np.ascontiguousarray(data.astype(np.float32)),
+                 meta=dict(workload="contact_pairs", pairs=pairs, n=n, seed=seed))
 
 
 def stack(columns=64, height=64, seed=20261203, spacing=1.1, sink=1e-3):

@@ -32,6 +32,15 @@ def reference():
 
 @pytest.fixture(scope="session")
 def physgpu():
+    # torch ships its own HIP runtime; when torch's CUDA is initialised first,
+    # libphysgpu.so binds to that runtime and both work in one process (the
+    # other order leaves torch without a device).
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.zeros(1, device="cuda")
+    except ImportError:
+        pass
     import physgpu as P
     P.library()  # raises when the HIP library is missing: never a silent fallback
     if P.device_count() < 1:
