@@ -151,7 +151,7 @@ physgpu_tuning default_tuning()
     t.gjk_groups = kStrideGrid;
     t.epa_groups = 1024;
     t.query_block = kBlock;
-    t.query_groups = kStrideGrid;
+    t.query_groups = 1024;  // measured best at 1M bodies (256 x 1024: 120 us; 256 x 2048: 126 us)
     t.sort_items = 8;
     return t;
 }

@@ -175,8 +175,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
     // so that the wave-wide append is executed convergently.
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (entries + stride - 1) / stride;
+    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
     for (uint32_t round = 0; round < rounds; ++round) {
-        const uint32_t p = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t p = round * stride + block * blockDim.x + threadIdx.x;
         const bool active = p < entries;
         Rec me{{0, 0, 0, 0, 0, 0}, 0, 0};
         int32_t c[3] = {0, 0, 0}, top[3] = {0, 0, 0};  // both in range for in-grid bodies
@@ -187,44 +188,56 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
             home_cell(me.b[3], me.b[4], me.b[5], inv, top);
             wk = world_key(me.world, bits_per_axis);
         }
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    uint32_t q = 0, qe = 0;
-                    if (active && c[0] + dx <= top[0] && c[1] + dy <= top[1] &&
-                        c[2] + dz <= top[2]) {
-                        const uint2 r =
-                            range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask) ^ wk];
-                        q = r.x;
-                        qe = r.y;
-                        if (qe <= q)
-                            q = qe = 0;
-                    }
-                    // Lanes walk their own cells; the wave iterates until all are done.
-                    for (;;) {
-                        const bool more = q < qe;
-                        if (__ballot(more) == 0)
-                            break;
-                        bool hit = false;
-                        uint32_t b = 0;
-                        if (more) {
-                            const Rec o = load_rec(s, q);
-                            b = o.index;
-                            if (me.index < b && me.world == o.world) {
-                                ++tests;
-                                hit = rec_overlap(me.b, o.b);
-                            }
-                            ++q;
+        // One z-slab of 3 x 3 neighbour cells at a time: the 9 range loads are
+        // issued together, then each lane walks the candidates of all 9 cells
+        // in one loop, so the wave waits for one table round trip and
+        // max(candidates) record loads per slab instead of one round trip and
+        // one loop per cell.
+        for (int dz = -1; dz <= 1; ++dz) {
+            uint2 r[9];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const int dx = j % 3 - 1, dy = j / 3 - 1;
+                r[j] = make_uint2(0u, 0u);
+                if (active && c[0] + dx <= top[0] && c[1] + dy <= top[1] && c[2] + dz <= top[2])
+                    r[j] = range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask) ^ wk];
+            }
+            uint32_t q = 0, qe = 0, next = 0;
+            for (;;) {
+                if (q >= qe && next < 9) {
+                    // Next non-empty cell of the slab (static indices keep r in registers).
+                    uint32_t nq = 0, nqe = 0, nn = 9;
+#pragma unroll
+                    for (int j = 8; j >= 0; --j)
+                        if (uint32_t(j) >= next && r[j].y > r[j].x) {
+                            nq = r[j].x;
+                            nqe = r[j].y;
+                            nn = j + 1;
                         }
-                        wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count,
-                                cap);
-                    }
+                    q = nq;
+                    qe = nqe;
+                    next = nn;
                 }
+                const bool more = q < qe;
+                if (__ballot(more) == 0)
+                    break;
+                bool hit = false;
+                uint32_t b = 0;
+                if (more) {
+                    const Rec o = load_rec(s, q);
+                    b = o.index;
+                    if (me.index < b && me.world == o.world) {
+                        ++tests;
+                        hit = rec_overlap(me.b, o.b);
+                    }
+                    ++q;
+                }
+                wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count, cap);
+            }
+        }
     }
-    wq_flush(queue, out, &counters->pair_count, cap);
-    tests = wave_sum(tests);
-    if (lane_id() == 0)
-        atomicAdd(&counters->grid_tests, tests);
+    wq_flush_block(queue, out, &counters->pair_count, cap);
+    block_add(tests, &counters->grid_tests);
 }
 
 // Overflow bodies against every body (broadphase.cpp:163-166): pair (o, j)
@@ -257,10 +270,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const floa
             wq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap);
         }
     }
-    wq_flush(queue, out, &counters->pair_count, cap);
-    tests = wave_sum(tests);
-    if (lane_id() == 0 && tests)
-        atomicAdd(&counters->grid_tests, tests);
+    wq_flush_block(queue, out, &counters->pair_count, cap);
+    block_add(tests, &counters->grid_tests);
 }
 
 }  // namespace

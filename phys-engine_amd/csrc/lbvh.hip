@@ -258,13 +258,9 @@ __global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float4* __re
         }
         wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count, cap);
     }
-    wq_flush(queue, out, &counters->pair_count, cap);
-    visits = wave_sum(visits);
-    leaves = wave_sum(leaves);
-    if (lane_id() == 0) {
-        atomicAdd(&counters->tree_visits, visits);
-        atomicAdd(&counters->tree_leaf_checks, leaves);
-    }
+    wq_flush_block(queue, out, &counters->pair_count, cap);
+    block_add(visits, &counters->tree_visits);
+    block_add(leaves, &counters->tree_leaf_checks);
 }
 
 }  // namespace

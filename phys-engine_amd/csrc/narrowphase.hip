@@ -571,13 +571,6 @@ __device__ __forceinline__ void wave_append_u32(bool pred, uint32_t value, uint3
         out[base + __popcll(mask & lanes_below(lane))] = value;
 }
 
-__device__ __forceinline__ void wave_count(bool pred, uint32_t* counter)
-{
-    const uint64_t mask = __ballot(pred);
-    if (mask && lane_id() == (uint32_t)__ffsll((unsigned long long)mask) - 1u)
-        atomicAdd(counter, (uint32_t)__popcll(mask));
-}
-
 constexpr uint32_t kClassifyThreads = 1024;
 constexpr uint32_t kClassifyGrid = 64;
 
@@ -709,6 +702,7 @@ __global__ __launch_bounds__(kBlock) void k_class(Bodies B, World W,
     const uint32_t total = c->class_count[kClass];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (total + stride - 1) / stride;
+    uint32_t hits = 0;
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool h = false;
@@ -740,10 +734,11 @@ __global__ __launch_bounds__(kBlock) void k_class(Bodies B, World W,
             if (h) {
                 store_contact(ct, k, a, b, cls, W, B.n, slots);
                 hit[k] = 1;
+                ++hits;
             }
         }
-        wave_count(h, &c->class_contacts[kClass]);
     }
+    block_add(hits, &c->class_contacts[kClass]);
 }
 
 __device__ __forceinline__ Shape load_shape(const Bodies& B, uint32_t i)
@@ -876,15 +871,11 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         }
         wave_sync();
     }
-    if (lane == 0) {
-        for (uint32_t q = 3; q < kClasses; ++q)
-            if (contacts[q])
-                atomicAdd(&c->class_contacts[q], contacts[q]);
-        if (failures)
-            atomicAdd(&c->gjk_failures, failures);
-        if (overflows)
-            atomicAdd(&c->epa_overflow, overflows);
-    }
+    // Counts are wave-uniform: lane 0 carries the wavefront's share.
+    for (uint32_t q = 3; q < kClasses; ++q)
+        block_add(lane == 0 ? contacts[q] : 0u, &c->class_contacts[q]);
+    block_add(lane == 0 ? failures : 0u, &c->gjk_failures);
+    block_add(lane == 0 ? overflows : 0u, &c->epa_overflow);
 }
 
 // Compaction: hits per tile of kScanTile candidates.

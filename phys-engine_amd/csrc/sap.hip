@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float4* 
             }
         }
     }
-    wq_flush(queue, out, &counters->pair_count, cap);
+    wq_flush_block(queue, out, &counters->pair_count, cap);
 }
 
 // xWindowComparisons (broadphase.cpp:241): for row i, the reference tests
@@ -174,9 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_window_tests(uint32_t n, const float
         }
         total += lo - i - 1;
     }
-    total = wave_sum(total);
-    if ((threadIdx.x & 63u) == 0)
-        atomicAdd(&counters->window_tests, total);
+    block_add(total, &counters->window_tests);
 }
 
 }  // namespace

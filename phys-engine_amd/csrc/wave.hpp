@@ -43,6 +43,18 @@ __device__ __forceinline__ void wave_append(bool pred, uint64_t value, uint64_t*
     }
 }
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own
+// L2.  Renumbering workgroup b of `groups` so that each XCD runs one
+// contiguous range of logical workgroups keeps neighbouring (spatially
+// sorted) work in the same L2.  Identity when groups is not a multiple of 8.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t groups)
+{
+    constexpr uint32_t kXcds = 8;
+    if (groups % kXcds)
+        return b;
+    return (b % kXcds) * (groups / kXcds) + b / kXcds;
+}
+
 // Pair record: first << 32 | second with first < second.
 __device__ __forceinline__ uint64_t pair_value(uint32_t a, uint32_t b)
 {
@@ -149,6 +161,64 @@ __device__ __forceinline__ void wq_push(WaveQueue& q, bool pred, uint64_t value,
     if (pred)
         q.buf[q.count + (uint32_t)__popcll(mask & lanes_below(lane_id()))] = value;
     q.count += add;
+}
+
+// Device-scope atomics on one address are serialised by the memory system
+// (about 10 ns each on MI355X), so per-wavefront atomics at the end of a
+// kernel with thousands of wavefronts form a tail of their own.  These
+// helpers combine a workgroup first: one atomic per workgroup.  Every thread
+// of the workgroup must call them (they synchronise the workgroup).
+constexpr uint32_t kMaxBlockWaves = 16;
+
+template <typename T>
+__device__ __forceinline__ void block_add(T v, T* counter)
+{
+    __shared__ T part[kMaxBlockWaves];
+    for (uint32_t off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off);
+    const uint32_t wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane_id() == 0)
+        part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T sum = 0;
+        for (uint32_t w = 0; w < waves; ++w)
+            sum += part[w];
+        if (sum)
+            atomicAdd(counter, sum);
+    }
+}
+
+// Final flush of every wavefront's queue of the workgroup: one atomic for the
+// workgroup's total, then each wavefront writes its entries at its offset.
+__device__ __forceinline__ void wq_flush_block(WaveQueue& q, uint64_t* __restrict__ out,
+                                               unsigned long long* counter, uint64_t cap)
+{
+    __shared__ uint32_t offset[kMaxBlockWaves];
+    __shared__ unsigned long long base_all;
+    const uint32_t wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6, lane = lane_id();
+    __syncthreads();
+    if (lane == 0)
+        offset[wave] = q.count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            const uint32_t c = offset[w];
+            offset[w] = run;
+            run += c;
+        }
+        base_all = run ? atomicAdd(counter, (unsigned long long)run) : 0ull;
+    }
+    __syncthreads();
+    const unsigned long long base = base_all + offset[wave];
+    for (uint32_t i = lane; i < q.count; i += 64) {
+        const unsigned long long slot = base + i;
+        if (slot < cap)
+            out[slot] = q.buf[i];
+    }
+    q.count = 0;
 }
 
 }  // namespace pg
