@@ -432,38 +432,64 @@ class BroadPhase:
 
 
 class DynamicAabbTree:
-    """Pair query of DynamicAabbTree (dynamic_aabb_tree.hpp:25-31) over an LBVH
-    rebuilt per query from the proxies' tight bounds."""
+    """DynamicAabbTree (dynamic_aabb_tree.hpp:25-32): proxies with the
+    reference's fat-bounds bookkeeping (dynamic_aabb_tree.cpp:44-57, 69-81);
+    the pair query runs an LBVH built per query over the tight bounds."""
 
     def __init__(self):
         self._bounds = {}
+        self._fat = {}
         self._next = 0
         self._free = []
 
-    def create_proxy(self, bounds, user_index):
+    @staticmethod
+    def _checked(bounds):
         b = np.asarray(bounds, np.float32).reshape(6)
         if not (np.all(np.isfinite(b)) and np.all(b[:3] <= b[3:])):
             raise ValueError("bounds must be finite and ordered")
+        return b
+
+    @staticmethod
+    def _expanded(b, factor):
+        """Padding max(0.01, 10% of the side) x factor per axis, in double."""
+        lo, hi = b[:3].astype(np.float64), b[3:].astype(np.float64)
+        pad = np.maximum(0.01, (hi - lo) * 0.1) * factor
+        big = float(np.finfo(np.float32).max)
+        return np.concatenate([np.maximum(-big, lo - pad), np.minimum(big, hi + pad)]).astype(
+            np.float32)
+
+    @staticmethod
+    def _contains(outer, inner):
+        return bool(np.all(outer[:3] <= inner[:3]) and np.all(outer[3:] >= inner[3:]))
+
+    def create_proxy(self, bounds, user_index):
+        b = self._checked(bounds)
         pid = self._free.pop() if self._free else self._next
         if pid == self._next:
             self._next += 1
         self._bounds[pid] = (b, int(user_index))
+        self._fat[pid] = self._expanded(b, 1.0)
         return pid
 
     def destroy_proxy(self, proxy):
         if proxy not in self._bounds:
             raise ValueError("invalid proxy")
         del self._bounds[proxy]
+        del self._fat[proxy]
         self._free.append(proxy)
 
     def update_proxy(self, proxy, bounds):
+        """True when the proxy is reinserted (its bounds left the fat bounds,
+        or the fat bounds are more than 4x padding too large)."""
         if proxy not in self._bounds:
             raise ValueError("invalid proxy")
-        b = np.asarray(bounds, np.float32).reshape(6)
-        if not (np.all(np.isfinite(b)) and np.all(b[:3] <= b[3:])):
-            raise ValueError("bounds must be finite and ordered")
+        b = self._checked(bounds)
         self._bounds[proxy] = (b, self._bounds[proxy][1])
-        return False
+        fat = self._fat[proxy]
+        if self._contains(fat, b) and self._contains(self._expanded(b, 4.0), fat):
+            return False
+        self._fat[proxy] = self._expanded(b, 1.0)
+        return True
 
     def find_candidate_pairs(self, stats=None):
         items = sorted(self._bounds.values(), key=lambda t: t[1])
