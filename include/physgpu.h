@@ -191,10 +191,10 @@ typedef struct physgpu_narrowphase_options {
  * them.  Zero fields mean the default in brackets. */
 typedef struct physgpu_tuning {
     uint32_t class_block[3];   /* workgroup size of the SS, SB/BS, BB kernels: 64, 128, 256 [256] */
-    uint32_t class_groups[3];  /* workgroups of those grid-stride kernels: 1..65536 [2048] */
+    uint32_t class_groups[3];  /* workgroups of those grid-stride kernels: 1..8192 [2048] */
     uint32_t gjk_block;        /* GJK workgroup size: 64, 128, 256 [256] */
     uint32_t gjk_groups;       /* [2048] */
-    uint32_t epa_groups;       /* EPA workgroups (4 wavefronts, one pair each in flight) [1024] */
+    uint32_t epa_groups;       /* EPA workgroups (4 wavefronts, one pair each in flight) [2048] */
     uint32_t query_block;      /* grid query workgroup size: 64, 128, 256 [256] */
     uint32_t query_groups;     /* [1024] */
     uint32_t sort_items;       /* radix sort items per thread: 4, 8, 16 (tile 256 x items) [8] */

@@ -12,6 +12,8 @@
 // need: largest x side (SAP row window bound), largest side and mean side
 // (grid cell), and centre bounds (LBVH Morton quantisation).  Wave-level
 // reductions, one atomic per wave.
+#include <stddef.h>
+
 #include "devmath.hpp"
 #include "internal.hpp"
 #include "wave.hpp"
@@ -95,7 +97,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __restrict__ aabb,
-                                                   Counters* c)
+                                                   uint64_t* __restrict__ partials)
 {
     uint32_t wx = 0, wall = 0;
     uint32_t cmn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, cmx[3] = {0u, 0u, 0u};
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __r
         cmn[a] = wave_min_u32(cmn[a]);
         cmx[a] = wave_max_u32(cmx[a]);
     }
-    // Workgroup reduction in LDS, then one atomic per value per workgroup.
+    // Workgroup reduction in LDS, then one partial per value and workgroup.
     __shared__ uint32_t s_u[kBlock / 64][8];
     __shared__ unsigned long long s_sum[kBlock / 64];
     const uint32_t wave = threadIdx.x >> 6;
@@ -150,12 +152,14 @@ __global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __r
             }
             s_sum[0] += s_sum[w];
         }
-        atomicMax(&c->wmax_x_key, s_u[0][0]);
-        atomicMax(&c->wmax_key, s_u[0][1]);
-        atomicAdd(&c->width_sum_fx, s_sum[0]);
+        // One partial per value and workgroup (k_fold combines them).
+        uint64_t* P = partials + blockIdx.x;
+        P[size_t(kSlotWidthMaxX) * kMaxPartialBlocks] = s_u[0][0];
+        P[size_t(kSlotWidthMax) * kMaxPartialBlocks] = s_u[0][1];
+        P[size_t(kSlotWidthSum) * kMaxPartialBlocks] = s_sum[0];
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&c->cmin_key[a], s_u[0][2 + a]);
-            atomicMax(&c->cmax_key[a], s_u[0][5 + a]);
+            P[size_t(kSlotCentreMin + a) * kMaxPartialBlocks] = s_u[0][2 + a];
+            P[size_t(kSlotCentreMax + a) * kMaxPartialBlocks] = s_u[0][5 + a];
         }
     }
 }
@@ -252,10 +256,21 @@ void launch_aabb_reductions(physgpu_context* ctx)
         return;
     Timer t;
     time_begin(ctx, "aabb_reduce", t);
-    // One workgroup per CU keeps the per-workgroup atomics few.
-    const uint32_t grid = std::min<uint32_t>(div_up(ctx->n, kBlock), 256);
+    const uint32_t grid = std::min<uint32_t>(div_up(ctx->n, kBlock), 1024);
     hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->aabb,
-                       ctx->counters);
+                       ctx->partials);
+    FoldList fold{};
+    fold.e[0] = {kSlotWidthMaxX, grid, kFoldMax32, uint32_t(offsetof(Counters, wmax_x_key))};
+    fold.e[1] = {kSlotWidthMax, grid, kFoldMax32, uint32_t(offsetof(Counters, wmax_key))};
+    fold.e[2] = {kSlotWidthSum, grid, kFoldSum64, uint32_t(offsetof(Counters, width_sum_fx))};
+    for (uint32_t a = 0; a < 3; ++a) {
+        fold.e[3 + a] = {kSlotCentreMin + a, grid, kFoldMin32,
+                         uint32_t(offsetof(Counters, cmin_key) + 4 * a)};
+        fold.e[6 + a] = {kSlotCentreMax + a, grid, kFoldMax32,
+                         uint32_t(offsetof(Counters, cmax_key) + 4 * a)};
+    }
+    fold.count = 9;
+    launch_fold(ctx, fold);
     time_end(ctx, t);
 }
 

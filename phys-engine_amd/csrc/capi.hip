@@ -99,7 +99,7 @@ void free_all(physgpu_context* c)
                     c->epa_list, c->epa_simplex,
                     c->counters, c->world_body_pose, c->world_offsets, c->world_of,
                     c->world_pair_off, c->world_manifold_off, c->adjoint_scratch,
-                    c->adjoint_skipped};
+                    c->adjoint_skipped, c->partials};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -149,7 +149,7 @@ physgpu_tuning default_tuning()
     }
     t.gjk_block = kBlock;
     t.gjk_groups = kStrideGrid;
-    t.epa_groups = 1024;
+    t.epa_groups = 2048;  // measured at 1M bodies: 1024 -> 546 us, 2048 -> 452 us, 8192 -> 603 us
     t.query_block = kBlock;
     t.query_groups = 1024;  // measured best at 1M bodies (256 x 1024: 120 us; 256 x 2048: 126 us)
     t.sort_items = 8;
@@ -157,7 +157,7 @@ physgpu_tuning default_tuning()
 }
 
 bool valid_block(uint32_t b) { return b == 64 || b == 128 || b == 256; }
-bool valid_groups(uint32_t g) { return g >= 1 && g <= 65536; }
+bool valid_groups(uint32_t g) { return g >= 1 && g <= kMaxPartialBlocks; }
 
 // Zero fields take the defaults; anything else must be a supported shape.
 physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
@@ -224,7 +224,7 @@ void fill_broad_stats(physgpu_context* ctx, physgpu_broadphase_stats* s)
     if (ctx->last_algorithm == PHYSGPU_SWEEP_AND_PRUNE) {
         s->x_window_comparisons = c.window_tests;
     } else if (ctx->last_algorithm == PHYSGPU_UNIFORM_GRID) {
-        s->grid_comparisons = c.grid_tests;
+        s->grid_comparisons = c.grid_tests + c.grid_overflow_tests;
         s->grid_entries = c.grid_entries;
         s->grid_overflow_aabbs = c.overflow_count;
         float cell;
@@ -420,6 +420,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->restitution, N));
     A(alloc(c->world_of, N));
     A(alloc(c->adjoint_skipped, 1));
+    A(alloc(c->partials, size_t(kPartialSlots) * kMaxPartialBlocks));
     A(alloc(c->counters, 1));
     A(hipHostMalloc(reinterpret_cast<void**>(&c->host_counters), sizeof(Counters)));
     if (e != hipSuccess) {

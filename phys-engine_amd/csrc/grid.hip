@@ -25,6 +25,10 @@
 // Automatic cell: the largest AABB side, capped at 8x the mean side (the
 // rest overflow), widened by 2^-8 (in-grid bodies must be at most 1 - 2^-9
 // of the cell).
+#include <stddef.h>
+
+#include <algorithm>
+
 #include "devmath.hpp"
 #include "internal.hpp"
 #include "wave.hpp"
@@ -161,7 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                                                        const uint32_t* __restrict__ table,
                                                        uint32_t bits_per_axis,
                                                        Counters* counters,
-                                                       uint64_t* __restrict__ out, uint64_t cap)
+                                                       uint64_t* __restrict__ out, uint64_t cap,
+                                                       uint64_t* __restrict__ tests_part)
 {
     __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
     WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
         }
     }
     wq_flush_block(queue, out, &counters->pair_count, cap);
-    block_add(tests, &counters->grid_tests);
+    block_partial(tests, tests_part);
 }
 
 // Overflow bodies against every body (broadphase.cpp:163-166): pair (o, j)
@@ -246,7 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const floa
                                                           const uint32_t* __restrict__ overflow,
                                                           const uint32_t* __restrict__ flag,
                                                           Counters* counters,
-                                                          uint64_t* __restrict__ out, uint64_t cap)
+                                                          uint64_t* __restrict__ out, uint64_t cap,
+                                                          uint64_t* __restrict__ tests_part)
 {
     __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
     WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
@@ -271,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const floa
         }
     }
     wq_flush_block(queue, out, &counters->pair_count, cap);
-    block_add(tests, &counters->grid_tests);
+    block_partial(tests, tests_part + blockIdx.y * gridDim.x);
 }
 
 }  // namespace
@@ -303,13 +309,22 @@ void launch_grid(physgpu_context* ctx, double cell_size)
     hipLaunchKernelGGL(k_grid_query, dim3(ctx->tuning.query_groups),
                        dim3(ctx->tuning.query_block), 0, ctx->stream, n,
                        ctx->sorted_aabb, ctx->cell_table, bpa, ctx->counters, ctx->pairs_raw,
-                       ctx->cap_p);
+                       ctx->cap_p, ctx->partials + size_t(kSlotGridTests) * kMaxPartialBlocks);
     time_end(ctx, t);
     time_begin(ctx, "grid_overflow", t);
-    hipLaunchKernelGGL(k_grid_overflow, dim3(stride_grid(n), 16), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->aabb, ctx->overflow_list, ctx->grid_flag, ctx->counters,
-                       ctx->pairs_raw, ctx->cap_p);
+    // Overflow bodies are rare; a small grid keeps the idle launch cheap.
+    const dim3 og(std::min<uint32_t>(stride_grid(n), 512), 8);
+    hipLaunchKernelGGL(k_grid_overflow, og, dim3(kBlock), 0, ctx->stream, n, ctx->aabb,
+                       ctx->overflow_list, ctx->grid_flag, ctx->counters, ctx->pairs_raw,
+                       ctx->cap_p, ctx->partials + size_t(kSlotGridOverflowTests) * kMaxPartialBlocks);
     time_end(ctx, t);
+    FoldList fold{};
+    fold.e[0] = {kSlotGridTests, ctx->tuning.query_groups, kFoldSum64,
+                 uint32_t(offsetof(Counters, grid_tests))};
+    fold.e[1] = {kSlotGridOverflowTests, og.x * og.y, kFoldSum64,
+                 uint32_t(offsetof(Counters, grid_overflow_tests))};
+    fold.count = 2;
+    launch_fold(ctx, fold);
 }
 
 }  // namespace pg

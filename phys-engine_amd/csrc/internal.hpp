@@ -25,6 +25,41 @@ constexpr uint32_t kSortMinTile = kSortThreads * 4;
 // Compaction tiling.
 constexpr uint32_t kScanTile = 2048;
 
+// Per-workgroup partial results.  Device-scope atomics on one address are
+// serialised (about 11 ns each on MI355X), so statistics and reductions are
+// written as one partial per workgroup into a slot of `partials` and folded
+// by k_fold (one small launch) instead.  Kernels that write partials are
+// launched with at most kMaxPartialBlocks workgroups.
+constexpr uint32_t kMaxPartialBlocks = 8192;
+enum PartialSlot : uint32_t {
+    kSlotGridTests = 0,
+    kSlotGridOverflowTests,
+    kSlotWindowTests,
+    kSlotTreeVisits,
+    kSlotTreeLeaves,
+    kSlotWidthMaxX,   // AABB reductions: max / max / sum / min x3 / max x3
+    kSlotWidthMax,
+    kSlotWidthSum,
+    kSlotCentreMin,   // 3 slots
+    kSlotCentreMax = kSlotCentreMin + 3,  // 3 slots
+    kSlotContacts = kSlotCentreMax + 3,   // one per shape-pair class (6 slots)
+    kSlotGjkFailures = kSlotContacts + 6,
+    kSlotEpaOverflows,
+    kPartialSlots
+};
+enum FoldOp : uint32_t { kFoldSum64 = 0, kFoldSum32, kFoldMax32, kFoldMin32 };
+struct FoldEntry {
+    uint32_t slot;    // PartialSlot
+    uint32_t blocks;  // workgroups that wrote the slot
+    uint32_t op;      // FoldOp
+    uint32_t offset;  // byte offset of the destination in Counters
+};
+constexpr uint32_t kMaxFold = 12;
+struct FoldList {
+    FoldEntry e[kMaxFold];
+    uint32_t count;
+};
+
 // Device-side counters of one collision step.  Zeroed by the step-begin
 // kernel; read back by the host only when it asks for counts.
 struct Counters {
@@ -33,6 +68,7 @@ struct Counters {
     unsigned long long grid_tests;      // grid AABB tests (incl. overflow tests)
     unsigned long long tree_visits;     // LBVH node visits
     unsigned long long tree_leaf_checks;
+    unsigned long long grid_overflow_tests;  // overflow-path AABB tests (added to grid_tests)
     unsigned long long width_sum_fx;    // sum of max AABB sides, 2^-16 fixed point
     uint32_t grid_entries;              // bodies placed in a cell
     uint32_t overflow_count;            // grid bodies tested against everything
@@ -49,9 +85,7 @@ struct Counters {
     uint32_t sort_count;                // min(pair_count, capacity): pairs held
     uint32_t cell_bits;                 // grid cell size used (float bits)
     uint32_t tree_overflow;             // LBVH traversal stack overflows (must stay 0)
-    uint32_t epa_count;                 // hull pairs GJK found intersecting (EPA work)
-    uint32_t epa_next;                  // EPA work-queue cursor
-    uint32_t pad;
+    uint32_t pad[3];
 };
 
 struct Timer {
@@ -164,6 +198,9 @@ struct physgpu_context {
     double graph_cell = 0.0;
     physgpu_narrowphase_options graph_options{};
 
+    // Per-workgroup partials (kPartialSlots x kMaxPartialBlocks).
+    uint64_t* partials = nullptr;
+
     // Counters (device) and a pinned host mirror.
     pg::Counters* counters = nullptr;
     pg::Counters* host_counters = nullptr;
@@ -242,6 +279,9 @@ void launch_contact_adjoint(physgpu_context* ctx, uint32_t m, const float* g_nor
                             const float* g_depth, const float* g_point, float* geom_normal,
                             float* geom_depth, float* geom_point, float* grad_pair,
                             float* grad_pose, uint32_t* skipped);
+// counters.hip: folds partial slots into the counters (entries with distinct
+// destinations).
+void launch_fold(physgpu_context* ctx, const FoldList& list);
 // batch.hip
 void launch_world_of(physgpu_context* ctx);
 void launch_world_ranges(physgpu_context* ctx);

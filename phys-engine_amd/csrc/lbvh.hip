@@ -18,6 +18,10 @@
 // pair is found once.  Children are encoded with bit 31 set for leaves; a
 // stack entry with bit 31 is an overlapping leaf, emitted when popped (it is
 // never used to index the internal-node arrays).
+#include <stddef.h>
+
+#include <algorithm>
+
 #include "devmath.hpp"
 #include "internal.hpp"
 #include "wave.hpp"
@@ -196,71 +200,81 @@ __global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float4* __re
                                                   const uint32_t* __restrict__ last,
                                                   const float4* __restrict__ nodes,
                                                   Counters* counters, uint64_t* __restrict__ out,
-                                                  uint64_t cap)
+                                                  uint64_t cap, uint64_t* __restrict__ visits_part,
+                                                  uint64_t* __restrict__ leaves_part)
 {
     __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
     WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long visits = 0, leaves = 0;
-    int32_t stack[kStack];
-    int sp = 0;
-    Rec me{{0, 0, 0, 0, 0, 0}, 0};
-    const bool active = p < n && n >= 2;
-    if (active) {
-        me = load_rec(s, p);
-        stack[sp++] = 0;  // root
-    }
-    // Converged loop: every lane pops until all stacks are empty.  A lane
-    // never visits more nodes than the tree has; the cap only guards against
-    // a malformed tree.
     const unsigned long long max_visits = 2ull * n + 64;
-    for (;;) {
-        if (visits > max_visits) {
-            atomicAdd(&counters->tree_overflow, 1u);
-            sp = 0;
+    // Grid-stride over leaves; every lane stays in the loops so the
+    // wave-wide appends run converged.
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+    for (uint32_t round = 0; round < rounds; ++round) {
+        const uint32_t p = round * stride + block * blockDim.x + threadIdx.x;
+        unsigned long long my_visits = 0;
+        int32_t stack[kStack];
+        int sp = 0;
+        Rec me{{0, 0, 0, 0, 0, 0}, 0, 0};
+        const bool active = p < n && n >= 2;
+        if (active) {
+            me = load_rec(s, p);
+            stack[sp++] = 0;  // root
         }
-        const bool more = sp > 0;
-        if (__ballot(more) == 0)
-            break;
-        bool hit = false;
-        uint32_t b = 0;
-        if (more) {
-            const int32_t entry = stack[--sp];
-            if (entry & int32_t(kLeaf)) {
-                // A leaf pushed below already passed the order and overlap tests.
-                hit = true;
-                b = load_rec(s, uint32_t(entry) & 0x7FFFFFFFu).index;
-            } else {
-                ++visits;
-                const int32_t kids[2] = {left[entry], right[entry]};
-                for (int c = 0; c < 2; ++c) {
-                    const int32_t child = kids[c];
-                    bool take = false;
-                    if (child & int32_t(kLeaf)) {
-                        const uint32_t q = uint32_t(child) & 0x7FFFFFFFu;
-                        if (q <= p)
-                            continue;
-                        ++leaves;
-                        take = rec_overlap(me.b, load_rec(s, q).b);
-                    } else {
-                        if (last[child] <= p)
-                            continue;
-                        take = rec_overlap(me.b, load_rec(nodes, uint32_t(child)).b);
-                    }
-                    if (take) {
-                        if (sp < kStack)
-                            stack[sp++] = child;
-                        else
-                            atomicAdd(&counters->tree_overflow, 1u);
+        // Converged loop: every lane pops until all stacks are empty.  A lane
+        // never visits more nodes than the tree has; the cap only guards
+        // against a malformed tree.
+        for (;;) {
+            if (my_visits > max_visits) {
+                atomicAdd(&counters->tree_overflow, 1u);
+                sp = 0;
+            }
+            const bool more = sp > 0;
+            if (__ballot(more) == 0)
+                break;
+            bool hit = false;
+            uint32_t b = 0;
+            if (more) {
+                const int32_t entry = stack[--sp];
+                if (entry & int32_t(kLeaf)) {
+                    // A leaf pushed below already passed the order and overlap tests.
+                    hit = true;
+                    b = load_rec(s, uint32_t(entry) & 0x7FFFFFFFu).index;
+                } else {
+                    ++my_visits;
+                    const int32_t kids[2] = {left[entry], right[entry]};
+                    for (int c = 0; c < 2; ++c) {
+                        const int32_t child = kids[c];
+                        bool take = false;
+                        if (child & int32_t(kLeaf)) {
+                            const uint32_t q = uint32_t(child) & 0x7FFFFFFFu;
+                            if (q <= p)
+                                continue;
+                            ++leaves;
+                            take = rec_overlap(me.b, load_rec(s, q).b);
+                        } else {
+                            if (last[child] <= p)
+                                continue;
+                            take = rec_overlap(me.b, load_rec(nodes, uint32_t(child)).b);
+                        }
+                        if (take) {
+                            if (sp < kStack)
+                                stack[sp++] = child;
+                            else
+                                atomicAdd(&counters->tree_overflow, 1u);
+                        }
                     }
                 }
             }
+            wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count, cap);
         }
-        wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count, cap);
+        visits += my_visits;
     }
     wq_flush_block(queue, out, &counters->pair_count, cap);
-    block_add(visits, &counters->tree_visits);
-    block_add(leaves, &counters->tree_leaf_checks);
+    block_partial(visits, visits_part);
+    block_partial(leaves, leaves_part);
 }
 
 }  // namespace
@@ -301,10 +315,19 @@ void launch_lbvh(physgpu_context* ctx)
     time_end(ctx, t);
     (void)c3;
     time_begin(ctx, "lbvh_query", t);
-    hipLaunchKernelGGL(k_query, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
+    const uint32_t qgrid = std::min<uint32_t>(div_up(n, kBlock), 4096);
+    hipLaunchKernelGGL(k_query, dim3(qgrid), dim3(kBlock), 0, ctx->stream, n,
                        ctx->sorted_aabb, ctx->node_left, ctx->node_right, ctx->node_last,
-                       ctx->node_box, ctx->counters, ctx->pairs_raw, ctx->cap_p);
+                       ctx->node_box, ctx->counters, ctx->pairs_raw, ctx->cap_p,
+                       ctx->partials + size_t(kSlotTreeVisits) * kMaxPartialBlocks,
+                       ctx->partials + size_t(kSlotTreeLeaves) * kMaxPartialBlocks);
     time_end(ctx, t);
+    FoldList fold{};
+    fold.e[0] = {kSlotTreeVisits, qgrid, kFoldSum64, uint32_t(offsetof(Counters, tree_visits))};
+    fold.e[1] = {kSlotTreeLeaves, qgrid, kFoldSum64,
+                 uint32_t(offsetof(Counters, tree_leaf_checks))};
+    fold.count = 2;
+    launch_fold(ctx, fold);
 }
 
 }  // namespace pg

@@ -13,6 +13,8 @@
 //   compaction tile hit counts, one-workgroup scan, ordered gather: manifolds
 //              come out in candidate order like appendResult (:692-712).
 // All arithmetic replays the reference's FP32 operation order (devmath.hpp).
+#include <stddef.h>
+
 #include "contact.hpp"
 #include "devmath.hpp"
 #include "internal.hpp"
@@ -697,7 +699,8 @@ __global__ __launch_bounds__(kBlock) void k_class(Bodies B, World W,
                                                   const uint32_t* __restrict__ buckets,
                                                   uint64_t cap, physgpu_narrowphase_options opt,
                                                   physgpu_manifold* __restrict__ slots,
-                                                  uint32_t* __restrict__ hit)
+                                                  uint32_t* __restrict__ hit,
+                                                  uint64_t* __restrict__ contacts_part)
 {
     const uint32_t total = c->class_count[kClass];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -738,7 +741,7 @@ __global__ __launch_bounds__(kBlock) void k_class(Bodies B, World W,
             }
         }
     }
-    block_add(hits, &c->class_contacts[kClass]);
+    block_partial(hits, contacts_part);
 }
 
 __device__ __forceinline__ Shape load_shape(const Bodies& B, uint32_t i)
@@ -751,9 +754,71 @@ __device__ __forceinline__ Shape load_shape(const Bodies& B, uint32_t i)
     return s;
 }
 
-// Hull pairs, step 1: GJK one pair per lane (intersectGjk, gjk.cpp:128-158).
-// Hits go to the EPA work list (candidate | class << 29) with their final
-// simplex.
+// Hull pairs, step 1: GJK (intersectGjk, gjk.cpp:128-158) with 16 lanes per
+// pair (one 16-lane DPP row).  Each lane holds one point of each hull, so a
+// hull support map is one dot product and a 4-step row arg-max (first maximum
+// in index order, as the sequential strict-> scan) instead of a loop of
+// dependent loads; the simplex logic runs replicated in the row's lanes.
+// Hull pair t's result goes to slot t of the EPA work list: candidate |
+// class << 29 for a hit (with its final simplex), kNoEpa otherwise.
+constexpr uint32_t kGjkLanes = 16;
+constexpr uint32_t kNoEpa = 0xFFFFFFFFu;
+
+struct RowHull {
+    f3 p;         // point `rl` (lane within the row) of a hull with at most 16 points
+    bool cached;
+};
+
+__device__ __forceinline__ RowHull row_hull(const Bodies& B, const Shape& s, uint32_t rl)
+{
+    RowHull h{{0.0f, 0.0f, 0.0f}, s.type == PHYSGPU_HULL && s.count <= kGjkLanes};
+    if (h.cached && rl < s.count) {
+        const uint32_t p = s.first + rl, nh = B.nh;
+        h.p = {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+    }
+    return h;
+}
+
+// Support map evaluated by a row (same result in its 16 lanes).  All lanes of
+// the wave must call it together.
+__device__ f3 support_row(const Bodies& B, const Shape& s, const pose& t, f3 d, const RowHull& h,
+                          uint32_t rl, uint32_t row_base)
+{
+    if (s.type != PHYSGPU_HULL)
+        return support(B, s, t, d);
+    const f3 l = rotate(conj(t.q), d);
+    const uint32_t nh = B.nh;
+    float bv = 0.0f;
+    uint32_t bi = kNoIndex;
+    if (h.cached) {
+        if (rl < s.count) {
+            bv = dot(h.p, l);
+            bi = rl;
+        }
+    } else {
+        for (uint32_t k = rl; k < s.count; k += kGjkLanes) {
+            const uint32_t p = s.first + k;
+            const float v = dot(f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]}, l);
+            if (bi == kNoIndex || v > bv) {
+                bv = v;
+                bi = k;
+            }
+        }
+    }
+    row_arg_best<true>(bv, bi);
+    if (bi == kNoIndex)
+        bi = 0;  // no points: the first slot, as the sequential scan starts there
+    f3 pt;
+    if (h.cached) {
+        const int src = int(row_base + bi);
+        pt = {__shfl(h.p.x, src), __shfl(h.p.y, src), __shfl(h.p.z, src)};
+    } else {
+        const uint32_t p = s.first + bi;
+        pt = {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+    }
+    return to_world(pt, t);
+}
+
 __global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __restrict__ pairs,
                                                 Counters* c, const uint32_t* __restrict__ buckets,
                                                 uint64_t cap, physgpu_narrowphase_options opt,
@@ -762,16 +827,16 @@ __global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __rest
 {
     const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
     const uint32_t total = c3 + c4 + c->class_count[5];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t rounds = (total + stride - 1) / stride;
-    const uint32_t lane = lane_id();
+    const uint32_t rows = gridDim.x * blockDim.x / kGjkLanes;
+    const uint32_t rounds = (total + rows - 1) / rows;
+    const uint32_t lane = lane_id(), rl = lane % kGjkLanes, row_base = lane - rl;
     for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
-        bool intersect = false;
-        uint32_t entry = 0;
-        Simplex s;
-        s.size = 0;
-        if (t < total) {
+        const uint32_t t = r * rows + (blockIdx.x * blockDim.x + threadIdx.x) / kGjkLanes;
+        // Rows past the end run along with dummy data so the row reductions
+        // stay converged; they never report a hit.
+        bool done = t >= total;
+        uint32_t entry = 0, a = 0, b = 0;
+        if (!done) {
             uint32_t cls = 3, q = t;
             if (q >= c3 + c4) {
                 cls = 5;
@@ -782,22 +847,45 @@ __global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __rest
             }
             const uint32_t k = buckets[cls * cap + q];
             const uint64_t pr = pairs[k];
-            const uint32_t a = uint32_t(pr >> 32), b = uint32_t(pr);
-            intersect = gjk(B, load_shape(B, a), load_pose(B, a), load_shape(B, b), load_pose(B, b),
-                            opt.gjk_max_iterations, s);
+            a = uint32_t(pr >> 32);
+            b = uint32_t(pr);
             entry = k | (cls << 29);
         }
-        const uint64_t mask = __ballot(intersect);
-        if (mask) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-            uint32_t base = 0;
-            if (lane == leader)
-                base = atomicAdd(&c->epa_count, (uint32_t)__popcll(mask));
-            base = __shfl(base, (int)leader);
+        const Shape sa = load_shape(B, a), sb = load_shape(B, b);
+        const pose ta = load_pose(B, a), tb = load_pose(B, b);
+        const RowHull ha = row_hull(B, sa, rl), hb = row_hull(B, sb, rl);
+        Simplex s;
+        s.size = 0;
+        f3 dir{1.0f, 0.0f, 0.0f};
+        bool intersect = false;
+        for (uint32_t it = 0; it < opt.gjk_max_iterations; ++it) {
+            if (__ballot(!done) == 0)
+                break;
+            done = done || dot(dir, dir) < 1e-12f;
+            const f3 pa = support_row(B, sa, ta, dir, ha, rl, row_base);
+            const f3 pb = support_row(B, sb, tb, neg(dir), hb, rl, row_base);
+            if (!done) {
+                const f3 p = sub(pa, pb);
+                if (dot(p, dir) <= 0.0f) {
+                    done = true;
+                } else {
+                    for (uint32_t k = min(s.size, 3u); k > 0; --k)
+                        s.p[k] = s.p[k - 1];
+                    s.p[0] = p;
+                    s.size = min(s.size + 1, 4u);
+                    if (gjk_contains(s, dir)) {
+                        intersect = true;
+                        done = true;
+                    }
+                }
+            }
+        }
+        // The row's first lane records the result in the pair's own slot:
+        // the EPA entry (or kNoEpa) and, for a hit, the final simplex.
+        if (rl == 0 && t < total) {
+            list[t] = intersect ? entry : kNoEpa;
             if (intersect) {
-                const uint32_t slot = base + (uint32_t)__popcll(mask & lanes_below(lane));
-                list[slot] = entry;
-                float* o = simplex_out + size_t(slot) * 12;
+                float* o = simplex_out + size_t(t) * 12;
                 for (int p = 0; p < 4; ++p) {
                     o[3 * p] = s.p[p].x;
                     o[3 * p + 1] = s.p[p].y;
@@ -815,27 +903,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
            const uint32_t* __restrict__ list, const float* __restrict__ simplex_in,
            physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
-           uint32_t* __restrict__ hit)
+           uint32_t* __restrict__ hit, uint64_t* __restrict__ partials)
 {
     __shared__ EpaShared shared[kEpaWavesPerBlock];
+    __shared__ uint32_t next;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     EpaShared& S = shared[wave];
     for (uint32_t i = lane; i < kEdgeKeys; i += 64)
         S.key_sum[i] = 0;
-    wave_sync();
-    const uint32_t total = c->epa_count;
+    // Hull pairs are split evenly over the workgroups; inside a workgroup the
+    // wavefronts take pairs from an LDS counter, so long EPA runs do not pile
+    // up on one wavefront and no device atomic is needed.
+    const uint32_t total = c->class_count[3] + c->class_count[4] + c->class_count[5];
+    const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t begin = min(blockIdx.x * per, total), end = min(begin + per, total);
+    if (threadIdx.x == 0)
+        next = begin;
+    __syncthreads();
     uint32_t contacts[kClasses] = {0, 0, 0, 0, 0, 0};
     uint32_t failures = 0, overflows = 0;
-    // Pairs are taken from a shared counter so long EPA runs do not pile up
-    // on one wave.
     for (;;) {
         uint32_t e = 0;
         if (lane == 0)
-            e = atomicAdd(&c->epa_next, 1u);
+            e = atomicAdd(&next, 1u);
         e = uniform(e);
-        if (e >= total)
+        if (e >= end)
             break;
         const uint32_t entry = list[e];
+        if (entry == kNoEpa)
+            continue;
         const uint32_t k = entry & 0x1FFFFFFFu, cls = entry >> 29;
         const uint64_t pr = pairs[k];
         const uint32_t a = uint32_t(pr >> 32), b = uint32_t(pr);
@@ -873,9 +969,11 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
     }
     // Counts are wave-uniform: lane 0 carries the wavefront's share.
     for (uint32_t q = 3; q < kClasses; ++q)
-        block_add(lane == 0 ? contacts[q] : 0u, &c->class_contacts[q]);
-    block_add(lane == 0 ? failures : 0u, &c->gjk_failures);
-    block_add(lane == 0 ? overflows : 0u, &c->epa_overflow);
+        block_partial(lane == 0 ? contacts[q] : 0u,
+                      partials + size_t(kSlotContacts + q) * kMaxPartialBlocks);
+    block_partial(lane == 0 ? failures : 0u, partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks);
+    block_partial(lane == 0 ? overflows : 0u,
+                  partials + size_t(kSlotEpaOverflows) * kMaxPartialBlocks);
 }
 
 // Compaction: hits per tile of kScanTile candidates.
@@ -994,8 +1092,6 @@ __global__ void k_np_begin(Counters* c)
         c->gjk_failures = 0;
         c->epa_overflow = 0;
         c->manifold_count = 0;
-        c->epa_count = 0;
-        c->epa_next = 0;
     }
 }
 
@@ -1027,17 +1123,20 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
     time_begin(ctx, "np_sphere_sphere", t);
     hipLaunchKernelGGL(k_class<0>, dim3(tu.class_groups[0]), dim3(tu.class_block[0]), 0,
                        ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
-                       ctx->slots, ctx->hit);
+                       ctx->slots, ctx->hit,
+                       ctx->partials + size_t(kSlotContacts + 0) * kMaxPartialBlocks);
     time_end(ctx, t);
     time_begin(ctx, "np_sphere_box", t);
     hipLaunchKernelGGL(k_class<1>, dim3(tu.class_groups[1]), dim3(tu.class_block[1]), 0,
                        ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
-                       ctx->slots, ctx->hit);
+                       ctx->slots, ctx->hit,
+                       ctx->partials + size_t(kSlotContacts + 1) * kMaxPartialBlocks);
     time_end(ctx, t);
     time_begin(ctx, "np_box_box", t);
     hipLaunchKernelGGL(k_class<2>, dim3(tu.class_groups[2]), dim3(tu.class_block[2]), 0,
                        ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
-                       ctx->slots, ctx->hit);
+                       ctx->slots, ctx->hit,
+                       ctx->partials + size_t(kSlotContacts + 2) * kMaxPartialBlocks);
     time_end(ctx, t);
     if (ctx->has_hulls) {
         time_begin(ctx, "np_gjk", t);
@@ -1048,9 +1147,26 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
         time_begin(ctx, "np_epa", t);
         hipLaunchKernelGGL(k_epa, dim3(tu.epa_groups), dim3(kBlock), 0, ctx->stream, B, W,
                            ctx->pairs, ctx->counters, ctx->epa_list, ctx->epa_simplex, opt,
-                           ctx->slots, ctx->hit);
+                           ctx->slots, ctx->hit, ctx->partials);
         time_end(ctx, t);
     }
+    // Contact and failure counts from the per-workgroup partials.
+    FoldList fold{};
+    for (uint32_t q = 0; q < 3; ++q)
+        fold.e[q] = {kSlotContacts + q, tu.class_groups[q], kFoldSum32,
+                     uint32_t(offsetof(Counters, class_contacts) + 4 * q)};
+    fold.count = 3;
+    if (ctx->has_hulls) {
+        for (uint32_t q = 3; q < kClasses; ++q)
+            fold.e[q] = {kSlotContacts + q, tu.epa_groups, kFoldSum32,
+                         uint32_t(offsetof(Counters, class_contacts) + 4 * q)};
+        fold.e[6] = {kSlotGjkFailures, tu.epa_groups, kFoldSum32,
+                     uint32_t(offsetof(Counters, gjk_failures))};
+        fold.e[7] = {kSlotEpaOverflows, tu.epa_groups, kFoldSum32,
+                     uint32_t(offsetof(Counters, epa_overflow))};
+        fold.count = 8;
+    }
+    launch_fold(ctx, fold);
     time_begin(ctx, "np_compact", t);
     hipLaunchKernelGGL(k_tile_hits, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->hit,
                        ctx->counters, ctx->tile_sums);

@@ -11,6 +11,8 @@
 // read; a row that ends before the wave's first column is skipped with a
 // wave-uniform branch; otherwise all 64 lanes test their 4 columns.  Hits are
 // compacted with a wavefront ballot and one atomic per wave.
+#include <stddef.h>
+
 #include "devmath.hpp"
 #include "internal.hpp"
 #include "wave.hpp"
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float4* 
 // xWindowComparisons (broadphase.cpp:241): for row i, the reference tests
 // j = i+1 .. first j with min_j.x > max_i.x, counting j - i - 1.
 __global__ __launch_bounds__(kBlock) void k_window_tests(uint32_t n, const float4* __restrict__ s,
-                                                         Counters* counters)
+                                                         uint64_t* __restrict__ tests_part)
 {
     unsigned long long total = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_window_tests(uint32_t n, const float
         }
         total += lo - i - 1;
     }
-    block_add(total, &counters->window_tests);
+    block_partial(total, tests_part);
 }
 
 }  // namespace
@@ -211,9 +213,14 @@ void launch_sap_window_tests(physgpu_context* ctx)
 {
     Timer t;
     time_begin(ctx, "sap_window_tests", t);
-    hipLaunchKernelGGL(k_window_tests, dim3(stride_grid(ctx->n)), dim3(kBlock), 0, ctx->stream,
-                       ctx->n, ctx->sorted_aabb, ctx->counters);
+    const uint32_t grid = stride_grid(ctx->n);
+    hipLaunchKernelGGL(k_window_tests, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->n,
+                       ctx->sorted_aabb, ctx->partials + size_t(kSlotWindowTests) * kMaxPartialBlocks);
     time_end(ctx, t);
+    FoldList fold{};
+    fold.e[0] = {kSlotWindowTests, grid, kFoldSum64, uint32_t(offsetof(Counters, window_tests))};
+    fold.count = 1;
+    launch_fold(ctx, fold);
 }
 
 }  // namespace pg

@@ -190,6 +190,28 @@ __device__ __forceinline__ void block_add(T v, T* counter)
     }
 }
 
+// Workgroup sum of v written (by thread 0, no atomic) to slot[blockIdx.x].
+// Every thread of the workgroup must call it.
+template <typename T>
+__device__ __forceinline__ void block_partial(T v, uint64_t* slot)
+{
+    __shared__ uint64_t part[kMaxBlockWaves];
+    uint64_t x = uint64_t(v);
+    for (uint32_t off = 32; off > 0; off >>= 1)
+        x += __shfl_xor(x, off);
+    const uint32_t wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane_id() == 0)
+        part[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t sum = 0;
+        for (uint32_t w = 0; w < waves; ++w)
+            sum += part[w];
+        slot[blockIdx.x] = sum;
+    }
+}
+
 // Final flush of every wavefront's queue of the workgroup: one atomic for the
 // workgroup's total, then each wavefront writes its entries at its offset.
 __device__ __forceinline__ void wq_flush_block(WaveQueue& q, uint64_t* __restrict__ out,
@@ -274,6 +296,17 @@ __device__ __forceinline__ void arg_step(float& v, uint32_t& i)
     const float ov = __uint_as_float(dpp_u32<Ctrl>(__float_as_uint(v)));
     const uint32_t oi = dpp_u32<Ctrl>(i);
     arg_take<Max>(v, i, ov, oi);
+}
+
+// Arg-min / arg-max within each 16-lane row (the DPP row): every lane of a
+// row ends with its row's result.
+template <bool Max>
+__device__ __forceinline__ void row_arg_best(float& v, uint32_t& i)
+{
+    arg_step<Max, 0xB1>(v, i);
+    arg_step<Max, 0x4E>(v, i);
+    arg_step<Max, 0x141>(v, i);
+    arg_step<Max, 0x140>(v, i);
 }
 
 // Wave-wide arg-min / arg-max; the result is returned uniform in every lane.
