@@ -35,9 +35,9 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 # with its commands is profiles/grid_step_mixed1M_hbm_traffic.md.  Valid for
 # the 1,048,576-body mixed scene only.
 MEASURED_TRAFFIC_1M = {
-    "grid_query": 428.64e6,
+    "grid_query": 68.72e6,
     "compute_aabbs": 105.73e6,
-    "gather_sorted": 168.23e6,
+    "gather_sorted": 167.91e6,
 }
 
 
@@ -214,6 +214,39 @@ def batched_bench(worlds, steps, warmup, check_every=64):
     }
 
 
+def adjoint_bench(pairs, steps):
+    """Narrow-phase adjoint on `pairs` isolated touching pairs (half box-box,
+    half sphere-box, depth U[1e-3, 5e-2] m): loss = sum depth^2 + sum normal . w.
+    Device time of the adjoint kernel from HIP events (upstream gradients are
+    staged to the device before it, outside the kernel time)."""
+    sc = scenes.contact_pairs(pairs)
+    pipe = physgpu.CollisionPipeline(sc.count, 4 * sc.count)
+    pipe.set_colliders(sc)
+    _, m = pipe.collide(algorithm=abi.UNIFORM_GRID)
+    man = pipe.manifolds()
+    rng = np.random.default_rng(1)
+    w = rng.standard_normal((3, m))
+    w = (w / np.linalg.norm(w, axis=0)).astype(np.float32)
+    used = np.arange(4)[:, None] < man["point_count"][None, :]
+    gd = np.where(used, 2.0 * man["penetration"].T, 0.0).astype(np.float32)
+    pipe.contact_adjoint(w, gd)
+    pipe.set_timing(True)
+    for _ in range(steps):
+        pipe.contact_adjoint(w, gd)
+    kt = pipe.kernel_times()
+    pipe.collide(algorithm=abi.UNIFORM_GRID)
+    kf = pipe.kernel_times()
+    pipe.set_timing(False)
+    pipe.close()
+    adj_us = kt["np_adjoint"][0] * 1e3
+    fwd_us = sum(kf[k][0] for k in ("np_sphere_box", "np_box_box") if k in kf) * 1e3
+    log(f"adjoint: {m} manifolds, kernel {adj_us:.1f} us (forward SB+BB kernels {fwd_us:.1f} us)")
+    return {"workload": f"{m} touching pairs, half box-box, half sphere-box",
+            "manifolds": int(m), "adjoint_kernel_us": round(adj_us, 2),
+            "forward_kernels_us": round(fwd_us, 2),
+            "manifolds_per_s": round(m / (adj_us * 1e-6), 1)}
+
+
 def copy_probe(nbytes=256 << 20):
     """One device-to-device copy of known size (the HIP runtime's blit kernel),
     used to scale rocprof FETCH_SIZE / WRITE_SIZE readings."""
@@ -240,6 +273,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-worlds", type=int, default=4096,
                     help="worlds of the batched-form comparison (0: skip it)")
+    ap.add_argument("--adjoint-pairs", type=int, default=65536,
+                    help="touching pairs of the adjoint timing (0: skip it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -318,6 +353,8 @@ def main():
     }
     if args.batched_worlds > 0:
         line["batched"] = batched_bench(args.batched_worlds, args.steps, args.warmup)
+    if args.adjoint_pairs > 0:
+        line["adjoint"] = adjoint_bench(args.adjoint_pairs, args.steps)
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(scene)
