@@ -190,6 +190,7 @@ __device__ bool gjk(const Bodies& B, const Shape& sa, const pose& ta, const Shap
 // expression as the reference, so results are bit-identical; only the
 // storage is bounded.
 constexpr uint32_t kEpaWavesPerBlock = kBlock / 64;
+constexpr uint32_t kLongEpa = 12;
 constexpr uint32_t kFaceChunks = (kMaxFaces + 63) / 64;
 
 constexpr uint32_t kEdgeKeys = kMaxVertices * (kMaxVertices - 1) / 2;  // unordered vertex pairs
@@ -461,6 +462,12 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
     for (uint32_t it = 0; it < iterations; ++it) {
         if (nf == 0)
             return 0;
+        // Long runs (mostly sphere-hull pairs, whose polytope approaches a
+        // curved surface) bound the kernel's duration: past kLongEpa
+        // iterations the wavefront takes issue priority over the short runs
+        // sharing its SIMD (the caller resets it after the run).
+        if (it == kLongEpa)
+            __builtin_amdgcn_s_setprio(2);
         // Closest face: first minimum of the face distances.
         float bd = 0.0f;
         uint32_t bi = kNoIndex;
@@ -490,13 +497,16 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
             S.vz[si] = sup.z;
         }
         ++nv;
-        wave_sync();
+        // (The new vertex is read back only by make_face_lds, after the
+        // synchronisations below.)
         // Visibility of every face, one ballot per chunk of 64; the visible
-        // faces are listed in index order.
-        uint64_t vis[kFaceChunks];
+        // faces are listed in index order.  Chunks past nf are skipped.
+        uint64_t vis[kFaceChunks] = {};
         uint32_t visible = 0;
 #pragma unroll
         for (uint32_t ch = 0; ch < kFaceChunks; ++ch) {
+            if (ch * 64 >= nf)
+                break;
             const uint32_t f = ch * 64 + lane;
             bool v = false;
             if (f < nf) {
@@ -524,6 +534,8 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
         uint32_t out = 0;
 #pragma unroll
         for (uint32_t ch = 0; ch < kFaceChunks; ++ch) {
+            if (ch * 64 >= nf)
+                break;
             const uint32_t f = ch * 64 + lane;
             const bool valid = f < nf;
             uint32_t abc = 0;
@@ -946,6 +958,7 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         const WaveHull ha = wave_hull(B, sa), hb = wave_hull(B, sb);
         const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, simplex, opt.epa_max_iterations,
                                 opt.epa_tolerance, n, depth);
+        __builtin_amdgcn_s_setprio(0);
         if (st == 1) {
             const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
                                    support_wave(B, sb, tb, neg(n), hb)),
