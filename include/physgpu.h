@@ -346,8 +346,10 @@ physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t re
  * penetration[k] and the world contact point whose collider-local forms are
  * anchor_a[k] / anchor_b[k].  Supported classes: sphere-sphere, sphere-box /
  * box-sphere and box-box (the SAT axis, clipping vertices or fallback point
- * the forward chose); hull manifolds get zero gradients and are counted in
- * *skipped.  Gradients are with respect to the colliders' world poses (the
+ * the forward chose), box-hull and hull-hull (the face EPA converged on, with
+ * its vertices' hull points / box corners and the contact point's support
+ * features held fixed; GJK and EPA are recomputed).  Sphere-hull manifolds get
+ * zero gradients and are counted in *skipped.  Gradients are with respect to the colliders' world poses (the
  * resident poses, or the composed ones after physgpu_set_world); quaternion
  * gradients are tangent to the unit sphere (gradients with respect to the
  * orientation q/|q|).

@@ -1198,6 +1198,58 @@ void findPairs(const physgpu_aabb* bounds, uint32_t n, int algorithm, uint32_t w
 
 } // namespace
 
+// Contact of pair (a, b) in scalar type T (double or Dual): sphere, box and
+// the polyhedral hull pairs (box-hull, hull-hull through GJK + EPA); false
+// for sphere-hull pairs, which the adjoint does not differentiate.
+template <typename T>
+static bool contactTyped(const View<double>& v, uint32_t a, uint32_t b, const Pose<T>& ta,
+                         const Pose<T>& tb, Contact<T>& c, Choice* ch)
+{
+    const Shape<double> sa = v.shape(a), sb = v.shape(b);
+    std::vector<V3<T>> pa, pb;
+    auto convert = [](const Shape<double>& s, std::vector<V3<T>>& buf) {
+        Shape<T> o{};
+        o.type = s.type;
+        o.data = {T(s.data.x), T(s.data.y), T(s.data.z)};
+        if (s.type == kHull) {
+            buf.resize(s.pointCount);
+            for (uint32_t k = 0; k < s.pointCount; ++k)
+                buf[k] = {T(s.points[k].x), T(s.points[k].y), T(s.points[k].z)};
+            o.points = buf.data();
+            o.pointCount = s.pointCount;
+        }
+        return o;
+    };
+    const Shape<T> A = convert(sa, pa), B = convert(sb, pb);
+    c.count = 0;
+    if (A.type == kHull || B.type == kHull) {
+        if (A.type == kSphere || B.type == kSphere)
+            return false;
+        return np::convexPair(A, ta, B, tb, makeOptions(64, 64, 1e-6f), c, nullptr);
+    }
+    return contactNoHull(A.type, A.data, ta, B.type, B.data, tb, c, ch);
+}
+
+static bool sphereHull(const View<double>& v, uint32_t a, uint32_t b)
+{
+    const int ta = v.shape(a).type, tb = v.shape(b).type;
+    return (ta == kHull && tb == kSphere) || (ta == kSphere && tb == kHull);
+}
+
+// FP64 contact of pair k at poses pa, pb (7 doubles each, q normalised).
+static bool contactF64(View<double>& v, uint32_t a, uint32_t b, const double pa[7],
+                       const double pb[7], Contact<double>& c, Choice* ch)
+{
+    return contactTyped(v, a, b, normalisedPose<double>(pa), normalisedPose<double>(pb), c, ch);
+}
+
+static void posesOf(const physgpu_colliders* c, uint32_t i, double out[7])
+{
+    const uint32_t n = c->count;
+    for (int q = 0; q < 7; ++q)
+        out[q] = c->pose[q * n + i];
+}
+
 extern "C" {
 
 const char* orc_last_error() { return lastError.c_str(); }
@@ -1513,22 +1565,6 @@ int orc_world_collide(const physgpu_colliders* c, const float* bodyPosition,
     }
 }
 
-// FP64 contact of pair k at poses pa, pb (7 doubles each, q normalised).
-static bool contactF64(View<double>& v, uint32_t a, uint32_t b, const double pa[7],
-                       const double pb[7], Contact<double>& c, Choice* ch)
-{
-    Shape<double> sa = v.shape(a), sb = v.shape(b);
-    return contactNoHull(sa.type, sa.data, normalisedPose<double>(pa), sb.type, sb.data,
-                         normalisedPose<double>(pb), c, ch);
-}
-
-static void posesOf(const physgpu_colliders* c, uint32_t i, double out[7])
-{
-    const uint32_t n = c->count;
-    for (int q = 0; q < 7; ++q)
-        out[q] = c->pose[q * n + i];
-}
-
 // Pairs are the manifolds' (first, second).  status[k]: 1 differentiated,
 // 0 no contact in FP64, -1 hull pair (not differentiated).  grad: 14 x n
 // planes (A's px py pz qx qy qz qw, then B's).
@@ -1542,8 +1578,7 @@ int orc_contact_adjoint_f64(const physgpu_colliders* c, const physgpu_pair* pair
             const uint32_t a = pairs[k].first, b = pairs[k].second;
             for (int q = 0; q < 14; ++q)
                 grad[q * n + k] = 0.0;
-            Shape<double> sa = v.shape(a), sb = v.shape(b);
-            if (sa.type == kHull || sb.type == kHull) {
+            if (sphereHull(v, a, b)) {
                 status[k] = -1;
                 return;
             }
@@ -1551,9 +1586,8 @@ int orc_contact_adjoint_f64(const physgpu_colliders* c, const physgpu_pair* pair
             posesOf(c, a, pa);
             posesOf(c, b, pb);
             Pose<Dual> ta = normalisedPoseSeeded(pa, 0), tb = normalisedPoseSeeded(pb, 7);
-            V3<Dual> da{sa.data.x, sa.data.y, sa.data.z}, db{sb.data.x, sb.data.y, sb.data.z};
             Contact<Dual> ct;
-            if (!contactNoHull(sa.type, da, ta, sb.type, db, tb, ct, nullptr)) {
+            if (!contactTyped(v, a, b, ta, tb, ct, nullptr)) {
                 status[k] = 0;
                 return;
             }
@@ -1584,9 +1618,10 @@ int orc_contact_fd_f64(const physgpu_colliders* c, const physgpu_pair* pairs, ui
             stable[k] = 0;
             for (int q = 0; q < 14; ++q)
                 grad[q * n + k] = 0.0;
-            Shape<double> sa = v.shape(a), sb = v.shape(b);
-            if (sa.type == kHull || sb.type == kHull)
+            if (sphereHull(v, a, b))
                 return;
+            const Shape<double> sa = v.shape(a), sb = v.shape(b);
+            const bool hull = sa.type == kHull || sb.type == kHull;
             double x[14];
             posesOf(c, a, x);
             posesOf(c, b, x + 7);
@@ -1607,6 +1642,22 @@ int orc_contact_fd_f64(const physgpu_colliders* c, const physgpu_pair* pairs, ui
                     if (!contactF64(v, a, b, y, y + 7, cs, &ch) || !(ch == ch0) ||
                         cs.count != c0.count)
                         same = false;
+                    // Hull pairs: the EPA face is unchanged when normal and
+                    // depth move continuously; the contact point (midpoint of
+                    // the support points) jumps when a support is a tied face
+                    // or edge, which only matters when its seed is non-zero.
+                    if (hull && same) {
+                        const double dn = std::max({std::abs(cs.normal.x - c0.normal.x),
+                                                    std::abs(cs.normal.y - c0.normal.y),
+                                                    std::abs(cs.normal.z - c0.normal.z)});
+                        const double dd = std::abs(cs.depth[0] - c0.depth[0]);
+                        const double dw = std::max({std::abs(cs.world[0].x - c0.world[0].x),
+                                                    std::abs(cs.world[0].y - c0.world[0].y),
+                                                    std::abs(cs.world[0].z - c0.world[0].z)});
+                        const bool pointSeed = up.p[0][0] != 0.0 || up.p[0][1] != 0.0 ||
+                                               up.p[0][2] != 0.0;
+                        same = dn < 1e3 * h && dd < 1e3 * h && (!pointSeed || dw < 1e3 * h);
+                    }
                     f[s] = up.phi(cs);
                 }
                 grad[q * n + k] = (f[0] - f[1]) / (2.0 * h);

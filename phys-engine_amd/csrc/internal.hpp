@@ -274,6 +274,10 @@ void launch_pairs_in(physgpu_context* ctx, const physgpu_pair* dev_in, uint64_t 
 // narrowphase.hip
 void launch_np_begin(physgpu_context* ctx);
 void launch_narrowphase(physgpu_context* ctx, bool world);
+void launch_hull_adjoint(physgpu_context* ctx, uint32_t m, const float* g_normal,
+                         const float* g_depth, const float* g_point, float* geom_normal,
+                         float* geom_depth, float* geom_point, float* grad_pair, float* grad_pose,
+                         uint32_t* skipped);
 // adjoint.hip
 void launch_contact_adjoint(physgpu_context* ctx, uint32_t m, const float* g_normal,
                             const float* g_depth, const float* g_point, float* geom_normal,

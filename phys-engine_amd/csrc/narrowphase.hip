@@ -15,9 +15,12 @@
 // All arithmetic replays the reference's FP32 operation order (devmath.hpp).
 #include <stddef.h>
 
+#include <algorithm>
+
 #include "contact.hpp"
 #include "devmath.hpp"
 #include "internal.hpp"
+#include "vjp.hpp"
 #include "wave.hpp"
 
 namespace pg {
@@ -266,10 +269,17 @@ __device__ __forceinline__ WaveHull wave_hull(const Bodies& B, const Shape& s)
 // hull's points are spread over the lanes; the arg-max keeps the lowest
 // index among equal dot products, as the sequential strict-> scan does.
 __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
-                           const WaveHull& h)
+                           const WaveHull& h, f3* local = nullptr)
 {
-    if (s.type != PHYSGPU_HULL)
+    if (s.type != PHYSGPU_HULL) {
+        if (local && s.type == PHYSGPU_BOX) {
+            // boxSupportPoint's corner (narrowphase.cpp:27-35).
+            const f3 l = rotate(conj(t.q), d);
+            *local = {l.x >= 0.0f ? s.data.x : -s.data.x, l.y >= 0.0f ? s.data.y : -s.data.y,
+                      l.z >= 0.0f ? s.data.z : -s.data.z};
+        }
         return support(B, s, t, d);
+    }
     const f3 l = rotate(conj(t.q), d);
     const uint32_t nh = B.nh, lane = lane_id();
     float bv = 0.0f;
@@ -288,11 +298,16 @@ __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
     }
     wave_arg_best<true>(bv, bi);
     bi = uniform(bi);
-    if (h.cached)
-        return to_world(f3{lane_value(h.p.x, bi), lane_value(h.p.y, bi), lane_value(h.p.z, bi)},
-                        t);
-    const uint32_t p = s.first + bi;
-    return to_world(f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]}, t);
+    f3 pt;
+    if (h.cached) {
+        pt = {lane_value(h.p.x, bi), lane_value(h.p.y, bi), lane_value(h.p.z, bi)};
+    } else {
+        const uint32_t p = s.first + bi;
+        pt = {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+    }
+    if (local)
+        *local = pt;
+    return to_world(pt, t);
 }
 
 // Edge i of the visible faces' edge list: face vis_list[i / 3], edge i % 3
@@ -437,7 +452,7 @@ __device__ int horizon_wave(EpaShared& S, uint32_t edges)
 __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const pose& ta,
                         const WaveHull& ha, const Shape& sb, const pose& tb, const WaveHull& hb,
                         const f3 simplex[4], uint32_t iterations, float tol, f3& normal,
-                        float& depth)
+                        float& depth, uint32_t* face_out = nullptr)
 {
     const uint32_t lane = lane_id();
     normal = {0.0f, 0.0f, 0.0f};
@@ -486,6 +501,8 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
         if (dot(sup, cn) - cd <= tol) {
             normal = cn;
             depth = max_ref(cd, 0.0f);
+            if (face_out)
+                *face_out = uniform(S.fabc[bi]);
             return 1;
         }
         if (nv >= kMaxVertices)
@@ -989,6 +1006,194 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
                   partials + size_t(kSlotEpaOverflows) * kMaxPartialBlocks);
 }
 
+// ------------------------------------------------------------ hull adjoint
+// Adjoint of box-hull and hull-hull manifolds through the EPA face the
+// forward converged on.  One wavefront per manifold recomputes GJK and EPA
+// with the forward's code (same FP32 operations, so the same polytope) and
+// takes the final face (a, b, c) in its stored order: normal =
+// normalize((b - a) x (c - a)), depth = normal . a, contact point = midpoint
+// of the two support points along the normal.  Every polytope vertex is
+// support_A - support_B of one hull point or box corner of each shape; the
+// face vertices' features are found by recomputing that difference for every
+// feature pair and matching it bit for bit.  With the features held fixed the
+// outputs are smooth in the poses and are differentiated in reverse mode.
+// Sphere-hull pairs (vertices depend on the directions of earlier faces) are
+// not differentiated and are counted in `skipped`.
+struct Feature {
+    bool ok;
+    f3 a, b;  // collider-local points of A and B
+};
+
+__device__ __forceinline__ uint32_t feature_count(const Shape& s)
+{
+    return s.type == PHYSGPU_HULL ? s.count : 8u;
+}
+
+__device__ __forceinline__ f3 feature_point(const Bodies& B, const Shape& s, uint32_t i)
+{
+    if (s.type == PHYSGPU_HULL) {
+        const uint32_t p = s.first + i, nh = B.nh;
+        return {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+    }
+    return {(i & 1) ? s.data.x : -s.data.x, (i & 2) ? s.data.y : -s.data.y,
+            (i & 4) ? s.data.z : -s.data.z};
+}
+
+// Feature pair whose support difference is bit-identical to v (the lowest
+// pair index when several are).
+__device__ Feature match_vertex(const Bodies& B, const Shape& sa, const pose& ta, const Shape& sb,
+                                const pose& tb, f3 v)
+{
+    const uint32_t na = feature_count(sa), nb = feature_count(sb), total = na * nb;
+    uint32_t best = kNoIndex;
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t c = base + lane_id();
+        bool eq = false;
+        if (c < total) {
+            const f3 w = sub(to_world(feature_point(B, sa, c / nb), ta),
+                             to_world(feature_point(B, sb, c % nb), tb));
+            eq = __float_as_uint(w.x) == __float_as_uint(v.x) &&
+                 __float_as_uint(w.y) == __float_as_uint(v.y) &&
+                 __float_as_uint(w.z) == __float_as_uint(v.z);
+        }
+        const uint64_t m = __ballot(eq);
+        if (m && best == kNoIndex)
+            best = base + uint32_t(__ffsll((unsigned long long)m)) - 1u;
+    }
+    Feature f{best != kNoIndex, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    if (f.ok) {
+        f.a = feature_point(B, sa, best / nb);
+        f.b = feature_point(B, sb, best % nb);
+    }
+    return f;
+}
+
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void k_hull_adjoint(Bodies B, const physgpu_manifold* __restrict__ man, const Counters* c,
+                    uint32_t m, physgpu_narrowphase_options opt, const float* __restrict__ g_normal,
+                    const float* __restrict__ g_depth, const float* __restrict__ g_point,
+                    float* __restrict__ geom_normal, float* __restrict__ geom_depth,
+                    float* __restrict__ geom_point, float* __restrict__ grad_pair,
+                    float* __restrict__ grad_pose, uint32_t* __restrict__ skipped)
+{
+    __shared__ EpaShared shared[kEpaWavesPerBlock];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    EpaShared& S = shared[wave];
+    for (uint32_t i = lane; i < kEdgeKeys; i += 64)
+        S.key_sum[i] = 0;
+    wave_sync();
+    const uint32_t count = min(m, c->manifold_count);
+    const uint32_t stride = gridDim.x * kEpaWavesPerBlock;
+    uint32_t skip = 0;
+    for (uint32_t t = blockIdx.x * kEpaWavesPerBlock + wave; t < count; t += stride) {
+        const physgpu_manifold& mf = man[t];
+        const uint32_t cls = mf.shape_pair;
+        if (cls < PHYSGPU_PAIR_SPHERE_HULL)
+            continue;
+        if (cls == PHYSGPU_PAIR_SPHERE_HULL) {
+            ++skip;
+            continue;
+        }
+        const uint32_t a = mf.first, b = mf.second;
+        const Shape sa = load_shape(B, a), sb = load_shape(B, b);
+        const pose ta = load_pose(B, a), tb = load_pose(B, b);
+        Simplex sx;
+        if (!gjk(B, sa, ta, sb, tb, opt.gjk_max_iterations, sx)) {
+            ++skip;
+            continue;
+        }
+        const WaveHull ha = wave_hull(B, sa), hb = wave_hull(B, sb);
+        f3 n;
+        float depth;
+        uint32_t face = 0;
+        const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, sx.p, opt.epa_max_iterations,
+                                opt.epa_tolerance, n, depth, &face);
+        __builtin_amdgcn_s_setprio(0);
+        wave_sync();
+        if (st != 1) {
+            ++skip;
+            continue;
+        }
+        f3 la, lb;
+        const f3 w = scale(add(support_wave(B, sa, ta, n, ha, &la),
+                               support_wave(B, sb, tb, neg(n), hb, &lb)),
+                           0.5f);
+        const f3 v0 = lds_vertex(S, face & 0xFFu), v1 = lds_vertex(S, (face >> 8) & 0xFFu),
+                 v2 = lds_vertex(S, (face >> 16) & 0xFFu);
+        const Feature f0 = match_vertex(B, sa, ta, sb, tb, v0);
+        const Feature f1 = match_vertex(B, sa, ta, sb, tb, v1);
+        const Feature f2 = match_vertex(B, sa, ta, sb, tb, v2);
+        const f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+        const f3 u = cross(e1, e2);
+        const float len = length(u);
+        if (!(f0.ok && f1.ok && f2.ok) || !(len > 1e-8f)) {
+            ++skip;
+            continue;
+        }
+        // Seeds.
+        f3 gn{0.0f, 0.0f, 0.0f}, gw{0.0f, 0.0f, 0.0f};
+        float gd = 0.0f;
+        if (g_normal)
+            gn = {g_normal[t], g_normal[m + t], g_normal[2 * m + t]};
+        if (g_depth)
+            gd = g_depth[t];
+        if (g_point)
+            gw = {g_point[t], g_point[m + t], g_point[2 * m + t]};
+        PoseGrad ga{{0, 0, 0}, {0, 0, 0, 0}}, gb{{0, 0, 0}, {0, 0, 0, 0}};
+        // depth = n . v0; n = u / |u|; u = e1 x e2.
+        f3 gv0 = scale(n, gd);
+        const f3 gu = unit_vjp(n, len, add(gn, scale(v0, gd)));
+        const f3 ge1 = cross(e2, gu), ge2 = cross(gu, e1);
+        const f3 gv1 = ge1, gv2 = ge2;
+        gv0 = sub(gv0, add(ge1, ge2));
+        const Feature fs[3] = {f0, f1, f2};
+        const f3 gv[3] = {gv0, gv1, gv2};
+        for (int k = 0; k < 3; ++k) {
+            // v_k = (ta.p + R_A a_k) - (tb.p + R_B b_k)
+            acc(ga.p, gv[k]);
+            rotate_vjp(ta.q, fs[k].a, gv[k], ga.q);
+            gb.p = sub(gb.p, gv[k]);
+            rotate_vjp(tb.q, fs[k].b, neg(gv[k]), gb.q);
+        }
+        // w = (ta.p + R_A la + tb.p + R_B lb) / 2
+        const f3 hw = scale(gw, 0.5f);
+        acc(ga.p, hw);
+        rotate_vjp(ta.q, la, hw, ga.q);
+        acc(gb.p, hw);
+        rotate_vjp(tb.q, lb, hw, gb.q);
+        project_tangent(ga.q, ta.q);
+        project_tangent(gb.q, tb.q);
+        if (lane == 0) {
+            if (geom_normal) {
+                geom_normal[t] = n.x;
+                geom_normal[m + t] = n.y;
+                geom_normal[2 * m + t] = n.z;
+            }
+            if (geom_depth)
+                geom_depth[t] = depth;
+            if (geom_point) {
+                geom_point[t] = w.x;
+                geom_point[m + t] = w.y;
+                geom_point[2 * m + t] = w.z;
+            }
+            const float va[7] = {ga.p.x, ga.p.y, ga.p.z, ga.q.x, ga.q.y, ga.q.z, ga.q.w};
+            const float vb[7] = {gb.p.x, gb.p.y, gb.p.z, gb.q.x, gb.q.y, gb.q.z, gb.q.w};
+            for (int q = 0; q < 7; ++q) {
+                if (grad_pair) {
+                    grad_pair[size_t(q) * m + t] = va[q];
+                    grad_pair[size_t(7 + q) * m + t] = vb[q];
+                }
+                if (grad_pose) {
+                    atomicAdd(&grad_pose[size_t(q) * B.n + a], va[q]);
+                    atomicAdd(&grad_pose[size_t(q) * B.n + b], vb[q]);
+                }
+            }
+        }
+    }
+    if (lane == 0 && skip)
+        atomicAdd(skipped, skip);
+}
+
 // Compaction: hits per tile of kScanTile candidates.
 __global__ __launch_bounds__(kBlock) void k_tile_hits(const uint32_t* __restrict__ hit,
                                                       const Counters* c,
@@ -1109,6 +1314,24 @@ __global__ void k_np_begin(Counters* c)
 }
 
 }  // namespace
+
+void launch_hull_adjoint(physgpu_context* ctx, uint32_t m, const float* g_normal,
+                         const float* g_depth, const float* g_point, float* geom_normal,
+                         float* geom_depth, float* geom_point, float* grad_pair, float* grad_pose,
+                         uint32_t* skipped)
+{
+    if (!ctx->has_hulls || m == 0)
+        return;
+    Bodies B{ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data, ctx->hull_range,
+             ctx->hull_points, ctx->n_hull_points};
+    Timer t;
+    time_begin(ctx, "np_hull_adjoint", t);
+    hipLaunchKernelGGL(k_hull_adjoint, dim3(std::min<uint32_t>(div_up(m, kEpaWavesPerBlock), 2048)),
+                       dim3(kBlock), 0, ctx->stream, B, ctx->manifolds, ctx->counters, m,
+                       ctx->np_options, g_normal, g_depth, g_point, geom_normal, geom_depth,
+                       geom_point, grad_pair, grad_pose, skipped);
+    time_end(ctx, t);
+}
 
 void launch_np_begin(physgpu_context* ctx)
 {

@@ -124,7 +124,8 @@ def test_inside_spheres_and_small_scene(physgpu, oracle):
 
 
 def test_batched_mixed_worlds(physgpu, oracle):
-    """256 worlds x 1,024 mixed bodies in one batch; hull manifolds are skipped."""
+    """256 worlds x 1,024 mixed bodies in one batch; sphere-hull manifolds are
+    skipped, box-hull and hull-hull go through the EPA face."""
     parts = [scenes.mixed(1024, seed=[77, w], side=scenes.mixed_side_for(1024, 1.33))
              for w in range(256)]
     sc, off = scenes.concat(parts)
@@ -135,21 +136,31 @@ def test_batched_mixed_worlds(physgpu, oracle):
         pipe.collide(algorithm=abi.UNIFORM_GRID)
         man = pipe.manifolds()
         gn, gd, gp = _seeds(man, seed=5)
+        hull = man["shape_pair"] >= abi.PAIR_SPHERE_HULL
+        gp[:, hull] = 0.0  # hull contact points sit on tied supports (see the EPA-face test)
         got, skipped = pipe.contact_adjoint(gn, gd, gp, per_pair=True)
-    hull = man["shape_pair"] >= abi.PAIR_SPHERE_HULL
-    assert skipped == int(hull.sum()) > 0
-    assert np.all(got[:, hull] == 0)
+    sphere_hull = man["shape_pair"] == abi.PAIR_SPHERE_HULL
+    assert skipped == int(sphere_hull.sum()) > 0
+    assert np.all(got[:, sphere_hull] == 0)
     pairs = _pairs_of(man)
-    want, status = oracle.contact_adjoint_f64(sc, pairs, gn, gd, gp, workers=16)
-    assert np.array_equal(status == -1, hull)
-    ok = status == 1
+    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    try:
+        want, status = oracle.contact_adjoint_f64(sc, pairs, gn, gd, gp, workers=16)
+        fd, stable = oracle.contact_fd_f64(sc, pairs, gn, gd, gp, workers=16)
+        n64, _, _, _ = oracle.contact_geometry_f64(sc, pairs)
+    finally:
+        oracle.set_epa_bounds(0, 0)
+    assert np.array_equal(status == -1, sphere_hull)
+    same_face = np.abs(n64.T - man["normal"]).max(1) < 1e-5
+    ok = (status == 1) & (~hull | same_face)
     rel = _rel(got[:, ok].astype(np.float64), want[:, ok])
-    fd, stable = oracle.contact_fd_f64(sc, pairs, gn, gd, gp, workers=16)
-    print(f"batched: {ok.sum()} differentiated manifolds, max rel {rel.max():.2e}, "
-          f"{(rel > TOL).sum()} above {TOL}; FD-stable {stable[ok].mean():.4f}")
+    print(f"batched: {ok.sum()} differentiated manifolds ({(ok & hull).sum()} through EPA), "
+          f"max rel {rel.max():.2e}, {(rel > TOL).sum()} above {TOL}; "
+          f"FD-stable {stable[ok].mean():.4f}")
     # Random scenes are not kept away from ties: compare where FP64 differences
     # see one smooth branch.
     assert (rel[stable[ok]] <= TOL).all()
+    assert ok[hull & ~sphere_hull].mean() > 0.8
 
 
 def test_torch_autograd_binding(physgpu, oracle):
@@ -178,3 +189,48 @@ def test_torch_autograd_binding(physgpu, oracle):
     np.add.at(want.T, man["second"], per[7:].T)
     scale = np.maximum(np.abs(want).max(0), 1e-3)
     assert (np.abs(grad - want).max(0) / scale).max() <= TOL
+
+
+def test_hull_pairs_through_the_epa_face(physgpu, oracle):
+    """Box-hull and hull-hull manifolds: the adjoint recomputes GJK + EPA and
+    differentiates the converged face.  Recomputed normals and depths equal
+    the manifolds bit for bit; gradients of sum depth^2 + sum normal . w match
+    the FP64 dual adjoint where FP64 EPA converges on the same face (normals
+    within 1e-5) and FP64 central differences see one smooth branch."""
+    sc = scenes.mixed(20000, seed=5, side=scenes.mixed_side_for(20000, 2.0))
+    with physgpu.CollisionPipeline(sc.count, 8 * sc.count,
+                                   max_hull_points=sc.hull_points.shape[1]) as pipe:
+        pipe.set_colliders(sc)
+        pipe.collide(algorithm=abi.UNIFORM_GRID)
+        man = pipe.manifolds()
+        normal, depth, point = pipe.contact_geometry()
+        m = len(man)
+        rng = np.random.default_rng(8)
+        w = rng.standard_normal((3, m))
+        w = (w / np.linalg.norm(w, axis=0)).astype(np.float32)
+        used = np.arange(4)[:, None] < man["point_count"][None, :]
+        gd = np.where(used, 2.0 * man["penetration"].T, 0.0).astype(np.float32)
+        got, skipped = pipe.contact_adjoint(w, gd, per_pair=True)
+    cls = man["shape_pair"]
+    poly = (cls == abi.PAIR_BOX_HULL) | (cls == abi.PAIR_HULL_HULL)
+    assert poly.sum() > 500
+    assert skipped == int((cls == abi.PAIR_SPHERE_HULL).sum())
+    assert np.array_equal(normal.T[poly].view(np.uint32), man["normal"][poly].view(np.uint32))
+    assert np.array_equal(depth[0][poly].view(np.uint32),
+                          man["penetration"][poly, 0].view(np.uint32))
+    pairs = _pairs_of(man)[poly]
+    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    try:
+        want, status = oracle.contact_adjoint_f64(sc, pairs, w[:, poly], gd[:, poly], workers=16)
+        fd, stable = oracle.contact_fd_f64(sc, pairs, w[:, poly], gd[:, poly], workers=16)
+        n64, _, _, _ = oracle.contact_geometry_f64(sc, pairs)
+    finally:
+        oracle.set_epa_bounds(0, 0)
+    same_face = np.abs(n64.T - man["normal"][poly]).max(1) < 1e-5
+    ok = (status == 1) & stable & same_face
+    print(f"hull manifolds {poly.sum()}: compared {ok.sum()} (same FP64 face and smooth)")
+    assert ok.mean() > 0.8
+    rel = _rel(got[:, poly][:, ok].astype(np.float64), want[:, ok])
+    print(f"GPU vs FP64 dual (EPA face): max rel {rel.max():.2e}")
+    assert rel.max() <= TOL
+    assert _rel(got[:, poly][:, ok].astype(np.float64), fd[:, ok]).max() <= TOL

@@ -114,3 +114,34 @@ def test_sphere_sphere_closed_form(oracle):
     np.testing.assert_allclose(ga[7:10], want_b, rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(ga[3:7], 0.0, atol=1e-12)
     np.testing.assert_allclose(ga[10:14], 0.0, atol=1e-12)
+
+
+def test_hull_adjoint_matches_central_differences(oracle):
+    """Box-hull and hull-hull contacts through GJK + EPA: the dual-number
+    derivative of normal . w + depth^2 equals FP64 central differences where
+    the EPA face does not change under the steps (the contact point itself
+    jumps between tied support features of a face, so it is left out)."""
+    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    try:
+        sc = scenes.mixed(20000, seed=5, side=scenes.mixed_side_for(20000, 2.0))
+        pairs, _ = oracle.find_pairs(oracle.compute_aabbs(sc), abi.UNIFORM_GRID)
+        rec, hit = oracle.generate_contacts(sc, pairs)
+        m = rec[hit]
+        m = m[(m["shape_pair"] == abi.PAIR_BOX_HULL) | (m["shape_pair"] == abi.PAIR_HULL_HULL)]
+        P = np.stack([m["first"], m["second"]], 1).astype(np.uint32)
+        M = len(P)
+        assert M > 500
+        rng = np.random.default_rng(1)
+        w = rng.standard_normal((3, M))
+        w = (w / np.linalg.norm(w, axis=0)).astype(np.float32)
+        gd = np.zeros((4, M), np.float32)
+        gd[0] = 2 * m["penetration"][:, 0]
+        ga, status = oracle.contact_adjoint_f64(sc, P, w, gd)
+        gf, stable = oracle.contact_fd_f64(sc, P, w, gd, h=1e-6)
+        assert (status == 1).mean() > 0.99
+        ok = stable & (status == 1)
+        assert ok.mean() > 0.95
+        scale = np.maximum(np.abs(ga).max(0), 1e-3)
+        assert (np.abs(ga - gf).max(0) / scale)[ok].max() < 1e-5
+    finally:
+        oracle.set_epa_bounds(0, 0)
