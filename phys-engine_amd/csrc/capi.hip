@@ -94,7 +94,8 @@ void free_all(physgpu_context* c)
                     c->aabb, c->sorted_aabb, c->items_a, c->items_b,
                     c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_flag, c->node_left,
                     c->node_right, c->node_parent, c->node_first, c->node_last, c->node_box, c->level_box,
-                    c->pairs_raw, c->pairs_tmp, c->buckets, c->slots, c->hit, c->tile_sums,
+                    c->pairs_raw, c->pairs_tmp, c->seg_off, c->seg_tiles, c->pair_rank,
+                    c->seg_lists, c->buckets, c->slots, c->hit, c->tile_sums,
                     c->manifolds, c->body, c->local_pose, c->friction, c->restitution,
                     c->epa_list, c->epa_simplex,
                     c->counters, c->world_body_pose, c->world_offsets, c->world_of,
@@ -407,6 +408,10 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->level_box, 2 * (N / 32 + N / 1024 + N / 32768 + 3)));
     A(alloc(c->pairs_raw, P));
     A(alloc(c->pairs_tmp, P));
+    A(alloc(c->seg_off, N + 1));
+    A(alloc(c->seg_tiles, div_up(N + 1, 2048)));
+    A(alloc(c->pair_rank, P));
+    A(alloc(c->seg_lists, 2 * N));
     A(alloc(c->buckets, kClasses * P));
     A(alloc(c->slots, P));
     A(alloc(c->hit, P));

@@ -85,7 +85,9 @@ struct Counters {
     uint32_t sort_count;                // min(pair_count, capacity): pairs held
     uint32_t cell_bits;                 // grid cell size used (float bits)
     uint32_t tree_overflow;             // LBVH traversal stack overflows (must stay 0)
-    uint32_t pad[3];
+    uint32_t mid_segments;              // pair segments sorted by one wave
+    uint32_t long_segments;             // pair segments sorted by one workgroup
+    uint32_t pad[1];
 };
 
 struct Timer {
@@ -153,6 +155,13 @@ struct physgpu_context {
     uint64_t* pairs_raw = nullptr;
     uint64_t* pairs_tmp = nullptr;
     uint64_t* pairs = nullptr;
+    // Canonical-order scratch: per-body segment offsets (cap_n + 1), their
+    // scan tile sums, each pair's rank in its segment (cap_p), and the lists
+    // of segments too long for one thread (2 x cap_n).
+    uint32_t* seg_off = nullptr;
+    uint32_t* seg_tiles = nullptr;
+    uint32_t* pair_rank = nullptr;
+    uint32_t* seg_lists = nullptr;
 
     // Narrow phase.
     uint32_t* buckets = nullptr;       // kClasses x cap_p candidate indices

@@ -309,6 +309,57 @@ def contact_pairs(pairs=65536, seed=20261203, spacing=8.0, tie_margin=1e-3, insi
                  meta=dict(workload="contact_pairs", pairs=pairs, n=n, seed=seed))
 
 
+SEGMENT_ROWS = (2, 4, 5, 16, 17, 63, 64, 65, 100, 400)
+SEGMENT_SLABS = ((63, 65), (64, 64), (41, 100), (100, 250))
+
+
+def pair_segments(nx=400, ny=250, nz=3, seed=20261203, rows=SEGMENT_ROWS, slabs=SEGMENT_SLABS):
+    """Pairs per body of chosen sizes: spheres r = 0.4 on the unit lattice
+    nx x ny x nz (no two overlap) in shuffled order after 1 + len(rows) +
+    len(slabs) boxes: body 0 covers the whole lattice, row box k covers
+    rows[k] lattice points along x in layer 0, slab box k covers a
+    slabs[k][0] x slabs[k][1] patch (layer 1, the last one layer 2).  Boxes
+    overlap nothing but spheres and box 0, so box k's pairs as `first` are
+    exactly the points it covers and box 0 has one pair with every other body.
+    meta["segments"] lists the expected count per box."""
+    rng = np.random.default_rng(seed)
+    boxes = []  # (centre, half extents, expected pairs)
+    for k, length in enumerate(rows):
+        y = 2 * k
+        assert y < ny and length <= nx
+        boxes.append(((length - 1) / 2.0, y, 0.0, (length - 1) / 2.0 + 0.45, 0.45, 0.45, length))
+    x0 = 0
+    for k, (a, b) in enumerate(slabs):
+        layer = 1 if k + 1 < len(slabs) else 2
+        if layer == 2:
+            x0 = 0
+        assert x0 + a <= nx and b <= ny and layer < nz
+        boxes.append((x0 + (a - 1) / 2.0, (b - 1) / 2.0, float(layer),
+                      (a - 1) / 2.0 + 0.45, (b - 1) / 2.0 + 0.45, 0.45, a * b))
+        x0 += a + 1
+    lattice = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"),
+                       0).reshape(3, -1).astype(np.float32)
+    lattice = lattice[:, rng.permutation(lattice.shape[1])]
+    m = lattice.shape[1]
+    nb = 1 + len(boxes)
+    n = nb + m
+    pose = np.zeros((7, n), np.float32)
+    pose[6] = 1.0
+    data = np.zeros((3, n), np.float32)
+    types = np.zeros(n, np.uint8)
+    types[:nb] = abi.BOX
+    pose[:3, 0] = ((nx - 1) / 2.0, (ny - 1) / 2.0, (nz - 1) / 2.0)
+    data[:, 0] = (nx / 2.0, ny / 2.0, nz / 2.0)
+    for k, (cx, cy, cz, hx, hy, hz, _) in enumerate(boxes):
+        pose[:3, 1 + k] = (cx, cy, cz)
+        data[:, 1 + k] = (hx, hy, hz)
+    pose[:3, nb:] = lattice
+    data[0, nb:] = 0.4
+    segments = [n - 1] + [b[-1] for b in boxes]
+    return Scene(pose=pose, shape_type=types, shape_data=data,
+                 meta=dict(workload="pair_segments", n=n, seed=seed, segments=segments))
+
+
 def stack(columns=64, height=64, seed=20261203, spacing=1.1, sink=1e-3):
     """Resting stack: box k of a column at y = 0.5 + k (1 - sink) over a ground box."""
     rng = np.random.default_rng(seed)
