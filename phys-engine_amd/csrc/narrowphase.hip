@@ -284,20 +284,30 @@ __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
     const uint32_t nh = B.nh, lane = lane_id();
     float bv = 0.0f;
     uint32_t bi = kNoIndex;
-    for (uint32_t base = 0; base < s.count; base += 64) {
-        const uint32_t k = base + lane;
-        if (k < s.count) {
-            const uint32_t p = s.first + k;
-            const f3 pt = h.cached ? h.p : f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
-            const float v = dot(pt, l);
-            if (bi == kNoIndex || v > bv) {
-                bv = v;
-                bi = k;
+    if (h.cached && s.count <= 16) {
+        // All points in the first 16-lane row: its row arg-max is the answer.
+        if (lane < s.count) {
+            bv = dot(h.p, l);
+            bi = lane;
+        }
+        row_arg_best<true>(bv, bi);
+        bi = lane_value(bi, 0);
+    } else {
+        for (uint32_t base = 0; base < s.count; base += 64) {
+            const uint32_t k = base + lane;
+            if (k < s.count) {
+                const uint32_t p = s.first + k;
+                const f3 pt = h.cached ? h.p : f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+                const float v = dot(pt, l);
+                if (bi == kNoIndex || v > bv) {
+                    bv = v;
+                    bi = k;
+                }
             }
         }
+        wave_arg_best<true>(bv, bi);
+        bi = uniform(bi);
     }
-    wave_arg_best<true>(bv, bi);
-    bi = uniform(bi);
     f3 pt;
     if (h.cached) {
         pt = {lane_value(h.p.x, bi), lane_value(h.p.y, bi), lane_value(h.p.z, bi)};
@@ -319,7 +329,7 @@ struct EdgeLane {
     uint64_t same;   // lanes of this chunk holding the same vertex pair
 };
 
-__device__ __forceinline__ EdgeLane edge_lane(const EpaShared& S, uint32_t i, uint32_t edges)
+__device__ __forceinline__ EdgeLane edge_fields(const EpaShared& S, uint32_t i, uint32_t edges)
 {
     EdgeLane e{0, 0, 0, 0, i < edges, 0};
     if (e.valid) {
@@ -331,9 +341,20 @@ __device__ __forceinline__ EdgeLane edge_lane(const EpaShared& S, uint32_t i, ui
         e.key = edge_key(e.a, e.b);
         e.delta = e.a < e.b ? 1 : -1;
     }
+    return e;
+}
+
+__device__ __forceinline__ EdgeLane edge_lane(const EpaShared& S, uint32_t i, uint32_t edges)
+{
+    EdgeLane e = edge_fields(S, i, edges);
     e.same = match_lanes<kEdgeKeyBits>(e.key, __ballot(e.valid));
     return e;
 }
+
+// Horizons of at most this many edges (five visible faces, the common case)
+// match keys by reading each edge's key from its lane instead of one ballot
+// per key bit.
+constexpr uint32_t kSmallHorizon = 16;
 
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 
@@ -355,51 +376,26 @@ __device__ int horizon_wave(EpaShared& S, uint32_t edges)
     const uint32_t lane = lane_id();
     const uint64_t below = lanes_below(lane);
     const uint64_t upto = below | (1ull << lane);
-    const uint32_t steps = (edges + 63) / 64;
-    const bool multi = steps > 1;
-    // Forward: append flags and list size.
-    int list_size = 0;
-    uint64_t single_append = 0;
-    for (uint32_t c = 0; c < steps; ++c) {
-        const EdgeLane e = edge_lane(S, c * 64 + lane, edges);
+    if (edges <= 64) {
+        // One step: edge i in lane i; every sum is a popcount.
+        EdgeLane e = edge_fields(S, lane, edges);
+        if (edges <= kSmallHorizon) {
+            uint64_t same = 0;
+            for (uint32_t j = 0; j < edges; ++j)
+                same |= lane_value(e.key, j) == e.key ? (1ull << j) : 0ull;
+            e.same = same;
+        } else {
+            e.same = match_lanes<kEdgeKeyBits>(e.key, __ballot(e.valid));
+        }
         const uint64_t up = __ballot(e.valid && e.delta > 0);
-        const int carry = (multi && e.valid) ? int(S.key_sum[e.key]) : 0;
-        const int sum = carry + 2 * __popcll(e.same & up & upto) - __popcll(e.same & upto);
+        const int sum = 2 * __popcll(e.same & up & upto) - __popcll(e.same & upto);
         const bool append = e.valid && iabs(sum) > iabs(sum - e.delta);
         const uint64_t A = __ballot(append), V = __ballot(e.valid);
-        const int size_before = list_size + 2 * __popcll(A & below) - __popcll(V & below);
-        const bool over = __ballot(append && size_before >= int(kMaxEdges)) != 0;
-        if (multi) {
-            if (e.valid && (e.same & ~upto) == 0)
-                S.key_sum[e.key] =
-                    int16_t(carry + 2 * __popcll(e.same & up) - __popcll(e.same));
-            if (lane == 0) {
-                S.edge_bits[2 * c] = uint32_t(A);
-                S.edge_bits[2 * c + 1] = uint32_t(A >> 32);
-            }
-        }
-        single_append = A;
-        list_size += 2 * __popcll(A) - __popcll(V);
-        wave_sync();
-        if (over) {
-            if (multi) {
-                for (uint32_t z = 0; z <= c; ++z) {
-                    const EdgeLane ez = edge_lane(S, z * 64 + lane, edges);
-                    if (ez.valid)
-                        S.key_sum[ez.key] = 0;
-                }
-                wave_sync();
-            }
+        const int size_before = 2 * __popcll(A & below) - __popcll(V & below);
+        if (__ballot(append && size_before >= int(kMaxEdges)))
             return -1;
-        }
-    }
-    if (!multi) {
-        const EdgeLane e = edge_lane(S, lane, edges);
-        const uint64_t A = single_append;
-        const uint64_t up = __ballot(e.valid && e.delta > 0);
         const int total = 2 * __popcll(e.same & up) - __popcll(e.same);
-        const bool survive = ((A >> lane) & 1ull) &&
-                             __popcll(e.same & A & ~upto) < iabs(total);
+        const bool survive = append && __popcll(e.same & A & ~upto) < iabs(total);
         const uint64_t K = __ballot(survive);
         if (survive) {
             const uint32_t pos = (uint32_t)__popcll(K & below);
@@ -408,6 +404,36 @@ __device__ int horizon_wave(EpaShared& S, uint32_t edges)
         }
         wave_sync();
         return __popcll(K);
+    }
+    // Several steps: the LDS table key_sum carries each pair's sum forward.
+    const uint32_t steps = (edges + 63) / 64;
+    int list_size = 0;
+    for (uint32_t c = 0; c < steps; ++c) {
+        const EdgeLane e = edge_lane(S, c * 64 + lane, edges);
+        const uint64_t up = __ballot(e.valid && e.delta > 0);
+        const int carry = e.valid ? int(S.key_sum[e.key]) : 0;
+        const int sum = carry + 2 * __popcll(e.same & up & upto) - __popcll(e.same & upto);
+        const bool append = e.valid && iabs(sum) > iabs(sum - e.delta);
+        const uint64_t A = __ballot(append), V = __ballot(e.valid);
+        const int size_before = list_size + 2 * __popcll(A & below) - __popcll(V & below);
+        const bool over = __ballot(append && size_before >= int(kMaxEdges)) != 0;
+        if (e.valid && (e.same & ~upto) == 0)
+            S.key_sum[e.key] = int16_t(carry + 2 * __popcll(e.same & up) - __popcll(e.same));
+        if (lane == 0) {
+            S.edge_bits[2 * c] = uint32_t(A);
+            S.edge_bits[2 * c + 1] = uint32_t(A >> 32);
+        }
+        list_size += 2 * __popcll(A) - __popcll(V);
+        wave_sync();
+        if (over) {
+            for (uint32_t z = 0; z <= c; ++z) {
+                const EdgeLane ez = edge_lane(S, z * 64 + lane, edges);
+                if (ez.valid)
+                    S.key_sum[ez.key] = 0;
+            }
+            wave_sync();
+            return -1;
+        }
     }
     // Backward: the last |s_final| appended copies of each pair survive.
     for (int c = int(steps) - 1; c >= 0; --c) {
