@@ -31,10 +31,11 @@ physgpu_status check_hip(hipError_t e, const char* what)
     return PHYSGPU_DEVICE_ERROR;
 }
 
-void time_begin(physgpu_context* ctx, const char* name, Timer& t)
+void time_begin(physgpu_context* ctx, const char* name, Timer& t, hipStream_t stream)
 {
     t.name = name;
     t.start = t.stop = nullptr;
+    t.stream = stream ? stream : ctx->stream;
     if (!ctx->timing)
         return;
     auto take = [&]() {
@@ -49,14 +50,14 @@ void time_begin(physgpu_context* ctx, const char* name, Timer& t)
     };
     t.start = take();
     t.stop = take();
-    hipEventRecord(t.start, ctx->stream);
+    hipEventRecord(t.start, t.stream);
 }
 
 void time_end(physgpu_context* ctx, Timer& t)
 {
     if (!ctx->timing || !t.start)
         return;
-    hipEventRecord(t.stop, ctx->stream);
+    hipEventRecord(t.stop, t.stream);
     ctx->pending.push_back(t);
 }
 
@@ -116,6 +117,12 @@ void free_all(physgpu_context* c)
     }
     for (hipEvent_t e : c->event_pool)
         hipEventDestroy(e);
+    if (c->fork_event)
+        hipEventDestroy(c->fork_event);
+    if (c->join_event)
+        hipEventDestroy(c->join_event);
+    if (c->side_stream)
+        hipStreamDestroy(c->side_stream);
     if (c->stream && c->owns_stream)
         hipStreamDestroy(c->stream);
 }
@@ -384,6 +391,9 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     };
     A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->owns_stream = true;
+    A(hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
+    A(hipEventCreateWithFlags(&c->fork_event, hipEventDisableTiming));
+    A(hipEventCreateWithFlags(&c->join_event, hipEventDisableTiming));
     A(alloc(c->pose, 7 * N));
     A(alloc(c->world_body_pose, 7 * N));
     A(alloc(c->shape_type, N));

@@ -93,6 +93,7 @@ struct Counters {
 struct Timer {
     const char* name;
     hipEvent_t start, stop;
+    hipStream_t stream;
 };
 
 struct KernelTime {
@@ -108,6 +109,12 @@ struct physgpu_context {
     physgpu_tuning tuning{};
     hipStream_t stream = nullptr;
     bool owns_stream = false;
+    // Second stream of the narrow phase: the SS / SB / BB kernels run on it
+    // while the main stream runs EPA (fork and join by events, so the step
+    // stays one capturable sequence on the main stream).
+    hipStream_t side_stream = nullptr;
+    hipEvent_t fork_event = nullptr;
+    hipEvent_t join_event = nullptr;
     uint32_t cap_n = 0;
     uint64_t cap_p = 0;
     uint32_t cap_h = 0;
@@ -240,7 +247,8 @@ physgpu_status check_hip(hipError_t e, const char* what);
     } while (0)
 
 // Timing brackets around one launch (no-ops unless ctx->timing).
-void time_begin(physgpu_context* ctx, const char* name, Timer& t);
+// Kernel timers (when timing is on): events on `stream`, the main stream by default.
+void time_begin(physgpu_context* ctx, const char* name, Timer& t, hipStream_t stream = nullptr);
 void time_end(physgpu_context* ctx, Timer& t);
 void collect_times(physgpu_context* ctx);
 

@@ -1382,35 +1382,44 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
     hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0, ctx->stream, B,
                        W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit);
     time_end(ctx, t);
-    time_begin(ctx, "np_sphere_sphere", t);
-    hipLaunchKernelGGL(k_class<0>, dim3(tu.class_groups[0]), dim3(tu.class_block[0]), 0,
-                       ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
-                       ctx->slots, ctx->hit,
-                       ctx->partials + size_t(kSlotContacts + 0) * kMaxPartialBlocks);
-    time_end(ctx, t);
-    time_begin(ctx, "np_sphere_box", t);
-    hipLaunchKernelGGL(k_class<1>, dim3(tu.class_groups[1]), dim3(tu.class_block[1]), 0,
-                       ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
-                       ctx->slots, ctx->hit,
-                       ctx->partials + size_t(kSlotContacts + 1) * kMaxPartialBlocks);
-    time_end(ctx, t);
-    time_begin(ctx, "np_box_box", t);
-    hipLaunchKernelGGL(k_class<2>, dim3(tu.class_groups[2]), dim3(tu.class_block[2]), 0,
-                       ctx->stream, B, W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt,
-                       ctx->slots, ctx->hit,
-                       ctx->partials + size_t(kSlotContacts + 2) * kMaxPartialBlocks);
-    time_end(ctx, t);
-    if (ctx->has_hulls) {
+    // With hulls, the SS / SB / BB kernels run on the side stream while EPA
+    // runs here: EPA's long sphere-hull runs leave most SIMDs idle for much
+    // of its duration.
+    const bool fork = ctx->has_hulls;
+    hipStream_t cs = ctx->stream;
+    if (fork) {
         time_begin(ctx, "np_gjk", t);
         hipLaunchKernelGGL(k_gjk, dim3(tu.gjk_groups), dim3(tu.gjk_block), 0, ctx->stream, B,
                            ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->epa_list,
                            ctx->epa_simplex);
         time_end(ctx, t);
+        hipEventRecord(ctx->fork_event, ctx->stream);
+        hipStreamWaitEvent(ctx->side_stream, ctx->fork_event, 0);
+        cs = ctx->side_stream;
+    }
+    time_begin(ctx, "np_sphere_sphere", t, cs);
+    hipLaunchKernelGGL(k_class<0>, dim3(tu.class_groups[0]), dim3(tu.class_block[0]), 0, cs, B,
+                       W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
+                       ctx->hit, ctx->partials + size_t(kSlotContacts + 0) * kMaxPartialBlocks);
+    time_end(ctx, t);
+    time_begin(ctx, "np_sphere_box", t, cs);
+    hipLaunchKernelGGL(k_class<1>, dim3(tu.class_groups[1]), dim3(tu.class_block[1]), 0, cs, B,
+                       W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
+                       ctx->hit, ctx->partials + size_t(kSlotContacts + 1) * kMaxPartialBlocks);
+    time_end(ctx, t);
+    time_begin(ctx, "np_box_box", t, cs);
+    hipLaunchKernelGGL(k_class<2>, dim3(tu.class_groups[2]), dim3(tu.class_block[2]), 0, cs, B,
+                       W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
+                       ctx->hit, ctx->partials + size_t(kSlotContacts + 2) * kMaxPartialBlocks);
+    time_end(ctx, t);
+    if (fork) {
+        hipEventRecord(ctx->join_event, ctx->side_stream);
         time_begin(ctx, "np_epa", t);
         hipLaunchKernelGGL(k_epa, dim3(tu.epa_groups), dim3(kBlock), 0, ctx->stream, B, W,
                            ctx->pairs, ctx->counters, ctx->epa_list, ctx->epa_simplex, opt,
                            ctx->slots, ctx->hit, ctx->partials);
         time_end(ctx, t);
+        hipStreamWaitEvent(ctx->stream, ctx->join_event, 0);
     }
     // Contact and failure counts from the per-workgroup partials.
     FoldList fold{};
