@@ -67,6 +67,8 @@ def kernel_bytes(name, n, pairs, hull_bodies, hull_points, table_entries):
 
 
 def run_algorithm(pipe, algo, steps, warmup):
+    """Eager steps, then the same step replayed from a HIP graph (the form a
+    stepping loop uses: one launch per step), then per-kernel timings."""
     for _ in range(warmup):
         pipe.collide(algorithm=algo, sync=False)
     pipe.step_counts()
@@ -76,6 +78,16 @@ def run_algorithm(pipe, algo, steps, warmup):
     npairs, nman = pipe.step_counts()  # synchronises the stream
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / steps
+    for _ in range(warmup):
+        pipe.collide_graph(algorithm=algo, sync=False)
+    pipe.step_counts()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.collide_graph(algorithm=algo, sync=False)
+    gpairs, gman = pipe.step_counts()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / steps
+    if (gpairs, gman) != (npairs, nman):
+        raise RuntimeError(f"graph replay counts {gpairs, gman} differ from eager {npairs, nman}")
     # Second pass with per-kernel HIP events on the library's stream.
     pipe.set_timing(True)
     for _ in range(steps):
@@ -83,7 +95,7 @@ def run_algorithm(pipe, algo, steps, warmup):
     pipe.step_counts()
     kt = pipe.kernel_times()
     pipe.set_timing(False)
-    return ms, npairs, nman, kt
+    return ms, graph_ms, npairs, nman, kt
 
 
 def cpu_baseline(scene, seconds_budget=30.0):
@@ -302,10 +314,13 @@ def main():
     per = {}
     for name in args.algorithms.split(","):
         algo = ALGOS[name]
-        ms, npairs, nman, kt = run_algorithm(pipe, algo, args.steps, args.warmup)
-        per[name] = {"ms_per_step": round(ms, 4), "pairs": npairs, "manifolds": nman,
+        eager_ms, graph_ms, npairs, nman, kt = run_algorithm(pipe, algo, args.steps, args.warmup)
+        ms = min(eager_ms, graph_ms)
+        per[name] = {"ms_per_step": round(ms, 4), "eager_ms_per_step": round(eager_ms, 4),
+                     "graph_ms_per_step": round(graph_ms, 4), "pairs": npairs, "manifolds": nman,
                      "kernels": {k: round(v[0] * 1e3, 2) for k, v in kt.items()}}
-        log(f"{name}: {ms:.3f} ms/step, {npairs} pairs, {nman} manifolds")
+        log(f"{name}: {ms:.3f} ms/step (eager {eager_ms:.3f}, graph replay {graph_ms:.3f}), "
+            f"{npairs} pairs, {nman} manifolds")
         for kn, (avg, launches) in sorted(kt.items(), key=lambda x: -x[1][0] * x[1][1]):
             log(f"   {kn:24s} {avg * 1e3:9.2f} us x {launches // args.steps}")
     best = min(per, key=lambda k: per[k]["ms_per_step"])
@@ -343,7 +358,10 @@ def main():
         "config": {"workload": f"mixed-{n}: 50% spheres r U[0.2,0.5], 40% boxes h U[0.2,0.6], "
                                f"10% 16-point hulls, random orientations, {side:.0f} m cube",
                    "bodies": n, "broadphase": best, "candidate_pairs": npairs,
-                   "manifolds": per[best]["manifolds"], "parallelism": "1 GPU"},
+                   "manifolds": per[best]["manifolds"], "parallelism": "1 GPU",
+                   "execution": ("HIP graph replay of the full step"
+                                 if per[best]["graph_ms_per_step"] <= per[best]["eager_ms_per_step"]
+                                 else "eager launches")},
         "pairs_per_s": round(npairs / (ms * 1e-3), 1),
         "manifolds_per_s": round(per[best]["manifolds"] / (ms * 1e-3), 1),
         "per_algorithm": {k: {kk: vv for kk, vv in v.items() if kk != "kernels"}
