@@ -371,7 +371,7 @@ __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 // LDS table key_sum carries each pair's sum forward, then the count of
 // copies still to keep backward, ending all zero.  Writes the horizon to
 // S.ea/S.eb; returns its size, or -1 when the list would pass kMaxEdges.
-__device__ int horizon_wave(EpaShared& S, uint32_t edges)
+__device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges)
 {
     const uint32_t lane = lane_id();
     const uint64_t below = lanes_below(lane);
