@@ -194,6 +194,7 @@ __device__ bool gjk(const Bodies& B, const Shape& sa, const pose& ta, const Shap
 // storage is bounded.
 constexpr uint32_t kEpaWavesPerBlock = kBlock / 64;
 constexpr uint32_t kLongEpa = 12;
+constexpr uint32_t kVeryLongEpa = 40;
 constexpr uint32_t kFaceChunks = (kMaxFaces + 63) / 64;
 
 constexpr uint32_t kEdgeKeys = kMaxVertices * (kMaxVertices - 1) / 2;  // unordered vertex pairs
@@ -478,14 +479,15 @@ __device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges)
 __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const pose& ta,
                         const WaveHull& ha, const Shape& sb, const pose& tb, const WaveHull& hb,
                         const f3 simplex[4], uint32_t iterations, float tol, f3& normal,
-                        float& depth, uint32_t* face_out = nullptr)
+                        float& depth, uint32_t* face_out = nullptr, uint32_t* work = nullptr)
 {
     const uint32_t lane = lane_id();
     normal = {0.0f, 0.0f, 0.0f};
     depth = 0.0f;
     if (iterations == 0)
         return 0;
-    if (lane < 4) {
+    // The caller may stage the simplex in S itself (simplex == nullptr).
+    if (simplex && lane < 4) {
         S.vx[lane] = simplex[lane].x;
         S.vy[lane] = simplex[lane].y;
         S.vz[lane] = simplex[lane].z;
@@ -502,13 +504,21 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
     uint32_t nf = 4, nv = 4;
     for (uint32_t it = 0; it < iterations; ++it) {
         if (nf == 0)
-            return 0;
+            { if (work) *work += it + 1; return 0; }
         // Long runs (mostly sphere-hull pairs, whose polytope approaches a
-        // curved surface) bound the kernel's duration: past kLongEpa
-        // iterations the wavefront takes issue priority over the short runs
-        // sharing its SIMD (the caller resets it after the run).
-        if (it == kLongEpa)
+        // curved surface) bound the kernel's duration.  `work` counts the
+        // iterations this wavefront has run, earlier pairs included: past
+        // kLongEpa it takes issue priority over the wavefronts sharing its
+        // SIMD, past kVeryLongEpa the highest (the caller resets it).
+        if (work) {
+            const uint32_t done = *work + it;
+            if (done == kLongEpa || (it == 0 && done > kLongEpa && done < kVeryLongEpa))
+                __builtin_amdgcn_s_setprio(2);
+            if (done == kVeryLongEpa || (it == 0 && done > kVeryLongEpa))
+                __builtin_amdgcn_s_setprio(3);
+        } else if (it == kLongEpa) {
             __builtin_amdgcn_s_setprio(2);
+        }
         // Closest face: first minimum of the face distances.
         float bd = 0.0f;
         uint32_t bi = kNoIndex;
@@ -529,10 +539,10 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
             depth = max_ref(cd, 0.0f);
             if (face_out)
                 *face_out = uniform(S.fabc[bi]);
-            return 1;
+            { if (work) *work += it + 1; return 1; }
         }
         if (nv >= kMaxVertices)
-            return -1;
+            { if (work) *work += it + 1; return -1; }
         const uint32_t si = nv;
         if (lane == 0) {
             S.vx[si] = sup.x;
@@ -565,13 +575,13 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
         wave_sync();
         const int horizon = horizon_wave(S, 3 * visible);
         if (horizon < 0)
-            return -1;
+            { if (work) *work += it + 1; return -1; }
         const uint32_t ne = uint32_t(horizon);
         if (ne == 0)
-            return 0;
+            { if (work) *work += it + 1; return 0; }
         const uint32_t keep = nf - visible;
         if (keep + ne > kMaxFaces)
-            return -1;
+            { if (work) *work += it + 1; return -1; }
         // Kept faces move down in order; every write lands at or below the
         // slot the writing lane read, and after everything already placed.
         uint32_t out = 0;
@@ -608,6 +618,8 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
         nf = out + ne;
         wave_sync();
     }
+    if (work)
+        *work += iterations;
     return 0;
 }
 
@@ -975,8 +987,9 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
     if (threadIdx.x == 0)
         next = begin;
     __syncthreads();
-    uint32_t contacts[kClasses] = {0, 0, 0, 0, 0, 0};
+    uint32_t contacts3 = 0, contacts4 = 0, contacts5 = 0;
     uint32_t failures = 0, overflows = 0;
+    uint32_t work = 0;  // EPA iterations of this wavefront so far
     for (;;) {
         uint32_t e = 0;
         if (lane == 0)
@@ -992,16 +1005,18 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         const uint32_t a = uint32_t(pr >> 32), b = uint32_t(pr);
         const Shape sa = load_shape(B, a), sb = load_shape(B, b);
         const pose ta = load_pose(B, a), tb = load_pose(B, b);
-        f3 simplex[4];
-        const float* si = simplex_in + size_t(e) * 12;
-        for (int p = 0; p < 4; ++p)
-            simplex[p] = {si[3 * p], si[3 * p + 1], si[3 * p + 2]};
+        // Simplex vertex p straight from memory into the polytope's LDS.
+        if (lane < 4) {
+            const float* si = simplex_in + size_t(e) * 12 + 3 * lane;
+            S.vx[lane] = si[0];
+            S.vy[lane] = si[1];
+            S.vz[lane] = si[2];
+        }
         f3 n;
         float depth;
         const WaveHull ha = wave_hull(B, sa), hb = wave_hull(B, sb);
-        const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, simplex, opt.epa_max_iterations,
-                                opt.epa_tolerance, n, depth);
-        __builtin_amdgcn_s_setprio(0);
+        const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, nullptr, opt.epa_max_iterations,
+                                opt.epa_tolerance, n, depth, nullptr, &work);
         if (st == 1) {
             const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
                                    support_wave(B, sb, tb, neg(n), hb)),
@@ -1015,7 +1030,9 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
                 store_contact(ct, k, a, b, cls, W, B.n, slots);
                 hit[k] = 1;
             }
-            ++contacts[cls];
+            contacts3 += cls == 3 ? 1u : 0u;
+            contacts4 += cls == 4 ? 1u : 0u;
+            contacts5 += cls == 5 ? 1u : 0u;
         } else if (st == 0) {
             ++failures;
         } else {
@@ -1023,10 +1040,11 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         }
         wave_sync();
     }
+    __builtin_amdgcn_s_setprio(0);
     // Counts are wave-uniform: lane 0 carries the wavefront's share.
-    for (uint32_t q = 3; q < kClasses; ++q)
-        block_partial(lane == 0 ? contacts[q] : 0u,
-                      partials + size_t(kSlotContacts + q) * kMaxPartialBlocks);
+    block_partial(lane == 0 ? contacts3 : 0u, partials + size_t(kSlotContacts + 3) * kMaxPartialBlocks);
+    block_partial(lane == 0 ? contacts4 : 0u, partials + size_t(kSlotContacts + 4) * kMaxPartialBlocks);
+    block_partial(lane == 0 ? contacts5 : 0u, partials + size_t(kSlotContacts + 5) * kMaxPartialBlocks);
     block_partial(lane == 0 ? failures : 0u, partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks);
     block_partial(lane == 0 ? overflows : 0u,
                   partials + size_t(kSlotEpaOverflows) * kMaxPartialBlocks);
