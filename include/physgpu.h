@@ -69,6 +69,13 @@ typedef struct physgpu_pair {
     uint32_t second;
 } physgpu_pair;
 
+/* A pair of user indices from the persistent proxies (phys::BroadPhasePair
+ * with the reference's full-width indices, broadphase.hpp:15-19). */
+typedef struct physgpu_proxy_pair {
+    uint64_t first;
+    uint64_t second;
+} physgpu_proxy_pair;
+
 /* phys::BroadPhaseStats (broadphase.hpp:21-42), field for field.  Timings
  * are device time in ms measured with HIP events when timing is enabled on
  * the context, else 0. */
@@ -369,6 +376,54 @@ physgpu_status physgpu_contact_adjoint(physgpu_context* ctx, uint64_t m, const f
  * 4 x m, world points 12 x m (zero beyond point_count and for hulls). */
 physgpu_status physgpu_contact_geometry(physgpu_context* ctx, uint64_t m, float* normal,
                                         float* depth, float* point, physgpu_memory memory);
+
+/* Persistent proxies: DynamicAabbTree (include/phys/collision/
+ * dynamic_aabb_tree.hpp:25-31, src/collision/dynamic_aabb_tree.cpp:108-139,
+ * 268-336) with its proxy state on the device.  Each context holds one set of
+ * proxies (at most max_colliders); proxy ids are recycled like the
+ * reference's.  Bounds and ids are host arrays; every call validates all of
+ * its items before changing anything and returns PHYSGPU_INVALID_ARGUMENT
+ * (the reference's std::invalid_argument) for non-finite or unordered bounds,
+ * unknown or destroyed ids, or a user index already held by a live proxy. */
+
+/* createProxy for `count` proxies; ids to proxy_out. */
+physgpu_status physgpu_tree_create_proxies(physgpu_context* ctx, uint32_t count,
+                                           const physgpu_aabb* bounds,
+                                           const uint64_t* user_index, uint32_t* proxy_out);
+/* destroyProxy for `count` proxies. */
+physgpu_status physgpu_tree_destroy_proxies(physgpu_context* ctx, uint32_t count,
+                                            const uint32_t* proxy);
+/* updateProxy for `count` proxies: the tight bounds become `bounds`; the fat
+ * bounds follow the reference's rule, and reinserted[i] (may be NULL) is 1
+ * where the proxy is reinserted.  Repeated ids in one call: the last wins. */
+physgpu_status physgpu_tree_update_proxies(physgpu_context* ctx, uint32_t count,
+                                           const uint32_t* proxy, const physgpu_aabb* bounds,
+                                           uint8_t* reinserted);
+/* Live proxy count (DynamicAabbTree::size). */
+physgpu_status physgpu_tree_size(physgpu_context* ctx, uint32_t* live);
+
+typedef enum physgpu_tree_policy {
+    /* Build the LBVH over the live proxies' tight bounds for every query. */
+    PHYSGPU_TREE_REBUILD = 0,
+    /* Keep the last build's leaf order and nodes and recompute the node boxes
+     * while no proxy was created or destroyed since that build (and no other
+     * broad phase ran on the context); rebuild otherwise. */
+    PHYSGPU_TREE_REFIT = 1
+} physgpu_tree_policy;
+
+/* DynamicAabbTree::findCandidatePairs: every pair of live proxies whose tight
+ * bounds overlap, as (smaller, larger) user index, sorted and unique.  Writes
+ * up to `capacity` pairs to `out` (host, may be NULL) and the count to
+ * `*count`.  `stats` (may be NULL): aabb_pairs, tree_proxy_count, node visits
+ * and leaf checks of the LBVH query, its height, and the creations,
+ * destructions and reinsertions since the previous query.  The query uses
+ * the context's broad-phase buffers: the resident colliders stay, but the
+ * candidate pairs and resident AABBs of earlier calls are replaced, so
+ * physgpu_generate_contacts(NULL pairs) and physgpu_find_candidate_pairs(NULL
+ * bounds) need a new physgpu_collide / physgpu_compute_aabbs first. */
+physgpu_status physgpu_tree_find_pairs(physgpu_context* ctx, int policy, physgpu_proxy_pair* out,
+                                       uint64_t capacity, uint64_t* count,
+                                       physgpu_broadphase_stats* stats);
 
 /* Average device time (ms) per launch of each kernel since timing was
  * enabled.  Writes up to `capacity` entries; names are static strings. */

@@ -239,6 +239,7 @@ void launch_step_begin(physgpu_context* ctx)
 
 void launch_compute_aabbs(physgpu_context* ctx)
 {
+    ctx->resident_aabbs_valid = true;
     if (ctx->n == 0)
         return;
     Timer t;

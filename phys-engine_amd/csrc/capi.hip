@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -101,7 +102,9 @@ void free_all(physgpu_context* c)
                     c->epa_list, c->epa_simplex,
                     c->counters, c->world_body_pose, c->world_offsets, c->world_of,
                     c->world_pair_off, c->world_manifold_off, c->adjoint_scratch,
-                    c->adjoint_skipped, c->partials};
+                    c->adjoint_skipped, c->partials, c->tree_tight, c->tree_fat, c->tree_rank,
+                    c->tree_rank_user, c->tree_stage_ids, c->tree_stage_bounds,
+                    c->tree_stage_flags, c->tree_stage_pairs};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -204,6 +207,8 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
 // Broad phase on the resident AABB planes (ctx->n bodies).
 void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool window_stats)
 {
+    ctx->tree_build_current = false;
+    ctx->tree_pairs_loaded = false;
     launch_aabb_reductions(ctx);
     if (ctx->n >= 2) {
         if (algorithm == PHYSGPU_SWEEP_AND_PRUNE) {
@@ -613,8 +618,12 @@ physgpu_status physgpu_find_candidate_pairs(physgpu_context* ctx, const physgpu_
                                       hipMemcpyHostToDevice, ctx->stream));
             launch_convert_aabbs_in(ctx, staging);
         }
+        ctx->resident_aabbs_valid = true;
     } else if (n != ctx->n) {
         return fail(PHYSGPU_INVALID_ARGUMENT, "n must equal the resident collider count");
+    } else if (!ctx->resident_aabbs_valid) {
+        return fail(PHYSGPU_INVALID_ARGUMENT,
+                    "a proxy-tree query replaced the resident AABBs: compute them again first");
     } else if (ctx->n_worlds && algorithm != PHYSGPU_UNIFORM_GRID) {
         return fail(PHYSGPU_UNSUPPORTED, "a batch of worlds runs the uniform-grid broad phase");
     }
@@ -666,6 +675,9 @@ physgpu_status physgpu_generate_contacts(physgpu_context* ctx, const physgpu_pai
                                       hipMemcpyHostToDevice, ctx->stream));
         launch_pairs_in(ctx, nullptr, n_pairs);
     } else {
+        if (ctx->tree_pairs_loaded)
+            return fail(PHYSGPU_INVALID_ARGUMENT,
+                        "the last broad phase was a proxy-tree query: pass pairs or collide first");
         launch_np_begin(ctx);
     }
     launch_narrowphase(ctx, false);
@@ -1149,6 +1161,74 @@ physgpu_status contact_pass(physgpu_context* ctx, uint64_t m, const float* g_nor
 }  // namespace
 }  // namespace pg
 
+namespace pg {
+namespace {
+
+physgpu_status tree_alloc(physgpu_context* ctx)
+{
+    if (ctx->tree_tight)
+        return PHYSGPU_OK;
+    const size_t N = ctx->cap_n, P = ctx->cap_p;
+    hipError_t e = hipSuccess;
+    auto A = [&](hipError_t r) {
+        if (e == hipSuccess)
+            e = r;
+    };
+    A(alloc(ctx->tree_tight, 2 * N));
+    A(alloc(ctx->tree_fat, 2 * N));
+    A(alloc(ctx->tree_rank, N));
+    A(alloc(ctx->tree_rank_user, N));
+    A(alloc(ctx->tree_stage_ids, N));
+    A(alloc(ctx->tree_stage_bounds, N));
+    A(alloc(ctx->tree_stage_flags, N));
+    A(alloc(ctx->tree_stage_pairs, P));
+    if (e != hipSuccess) {
+        void* ptrs[] = {ctx->tree_tight, ctx->tree_fat, ctx->tree_rank, ctx->tree_rank_user,
+                        ctx->tree_stage_ids, ctx->tree_stage_bounds, ctx->tree_stage_flags,
+                        ctx->tree_stage_pairs};
+        for (void* p : ptrs)
+            if (p)
+                hipFree(p);
+        ctx->tree_tight = ctx->tree_fat = nullptr;
+        ctx->tree_rank = ctx->tree_stage_ids = nullptr;
+        ctx->tree_rank_user = nullptr;
+        ctx->tree_stage_bounds = nullptr;
+        ctx->tree_stage_flags = nullptr;
+        ctx->tree_stage_pairs = nullptr;
+        return check_hip(e, "proxy tree allocation");
+    }
+    return PHYSGPU_OK;
+}
+
+// validateBounds, dynamic_aabb_tree.cpp:34-42.
+bool finite_ordered(const physgpu_aabb& b)
+{
+    for (int a = 0; a < 3; ++a)
+        if (!(std::isfinite(b.min[a]) && std::isfinite(b.max[a]) && b.min[a] <= b.max[a]))
+            return false;
+    return true;
+}
+
+bool tree_proxy_live(const physgpu_context* ctx, uint32_t id)
+{
+    return id < ctx->tree_live.size() && ctx->tree_live[id];
+}
+
+// Ids (and bounds) of one call into the device staging buffers.
+physgpu_status tree_stage(physgpu_context* ctx, uint32_t count, const uint32_t* ids,
+                          const physgpu_aabb* bounds)
+{
+    PG_TRY_HIP(hipMemcpyAsync(ctx->tree_stage_ids, ids, sizeof(uint32_t) * count,
+                              hipMemcpyHostToDevice, ctx->stream));
+    if (bounds)
+        PG_TRY_HIP(hipMemcpyAsync(ctx->tree_stage_bounds, bounds, sizeof(physgpu_aabb) * count,
+                                  hipMemcpyHostToDevice, ctx->stream));
+    return PHYSGPU_OK;
+}
+
+}  // namespace
+}  // namespace pg
+
 extern "C" {
 
 physgpu_status physgpu_contact_adjoint(physgpu_context* ctx, uint64_t m, const float* g_normal,
@@ -1169,6 +1249,220 @@ physgpu_status physgpu_contact_geometry(physgpu_context* ctx, uint64_t m, float*
         return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
     return contact_pass(ctx, m, nullptr, nullptr, nullptr, nullptr, nullptr, normal, depth, point,
                         memory, nullptr);
+}
+
+physgpu_status physgpu_tree_create_proxies(physgpu_context* ctx, uint32_t count,
+                                           const physgpu_aabb* bounds,
+                                           const uint64_t* user_index, uint32_t* proxy_out)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    if (count == 0)
+        return PHYSGPU_OK;
+    if (!bounds || !user_index || !proxy_out)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "bounds, user_index and proxy_out are required");
+    if (ctx->tree_user_proxy.size() + count > ctx->cap_n)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "more proxies than the context holds");
+    std::unordered_map<uint64_t, uint32_t> batch;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!finite_ordered(bounds[i]))
+            return fail(PHYSGPU_INVALID_ARGUMENT, "Dynamic AABB tree requires finite, ordered bounds");
+        if (ctx->tree_user_proxy.count(user_index[i]) || !batch.emplace(user_index[i], i).second)
+            return fail(PHYSGPU_INVALID_ARGUMENT, "user index already held by a live proxy");
+    }
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = tree_alloc(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t id;
+        if (!ctx->tree_free.empty()) {
+            id = ctx->tree_free.back();
+            ctx->tree_free.pop_back();
+        } else {
+            id = uint32_t(ctx->tree_live.size());
+            ctx->tree_live.push_back(0);
+            ctx->tree_user.push_back(0);
+        }
+        ctx->tree_live[id] = 1;
+        ctx->tree_user[id] = user_index[i];
+        ctx->tree_user_proxy[user_index[i]] = id;
+        proxy_out[i] = id;
+    }
+    st = tree_stage(ctx, count, proxy_out, bounds);
+    if (st != PHYSGPU_OK)
+        return st;
+    launch_tree_create(ctx, count, ctx->tree_stage_ids, ctx->tree_stage_bounds);
+    PG_TRY_HIP(hipGetLastError());
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->tree_created += count;
+    ctx->tree_ranks_current = false;
+    ctx->tree_build_current = false;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_tree_destroy_proxies(physgpu_context* ctx, uint32_t count,
+                                            const uint32_t* proxy)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    if (count == 0)
+        return PHYSGPU_OK;
+    if (!proxy)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "proxy is NULL");
+    std::unordered_map<uint32_t, uint32_t> seen;
+    for (uint32_t i = 0; i < count; ++i)
+        if (!tree_proxy_live(ctx, proxy[i]) || !seen.emplace(proxy[i], i).second)
+            return fail(PHYSGPU_INVALID_ARGUMENT, "Invalid dynamic AABB tree proxy");
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t id = proxy[i];
+        ctx->tree_live[id] = 0;
+        ctx->tree_user_proxy.erase(ctx->tree_user[id]);
+        ctx->tree_free.push_back(id);
+    }
+    ctx->tree_destroyed += count;
+    ctx->tree_ranks_current = false;
+    ctx->tree_build_current = false;
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_tree_update_proxies(physgpu_context* ctx, uint32_t count,
+                                           const uint32_t* proxy, const physgpu_aabb* bounds,
+                                           uint8_t* reinserted)
+{
+    if (!ctx)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx is NULL");
+    if (count == 0)
+        return PHYSGPU_OK;
+    if (!proxy || !bounds)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "proxy and bounds are required");
+    std::unordered_map<uint32_t, uint32_t> seen;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!tree_proxy_live(ctx, proxy[i]))
+            return fail(PHYSGPU_INVALID_ARGUMENT, "Invalid dynamic AABB tree proxy");
+        if (!seen.emplace(proxy[i], i).second)
+            return fail(PHYSGPU_INVALID_ARGUMENT, "a proxy may be updated once per call");
+        if (!finite_ordered(bounds[i]))
+            return fail(PHYSGPU_INVALID_ARGUMENT, "Dynamic AABB tree requires finite, ordered bounds");
+    }
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = tree_stage(ctx, count, proxy, bounds);
+    if (st != PHYSGPU_OK)
+        return st;
+    launch_tree_update(ctx, count, ctx->tree_stage_ids, ctx->tree_stage_bounds,
+                       ctx->tree_stage_flags);
+    PG_TRY_HIP(hipGetLastError());
+    std::vector<uint8_t> flags(count);
+    PG_TRY_HIP(hipMemcpyAsync(flags.data(), ctx->tree_stage_flags, count, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint32_t i = 0; i < count; ++i) {
+        ctx->tree_reinserted += flags[i];
+        if (reinserted)
+            reinserted[i] = flags[i];
+    }
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_tree_size(physgpu_context* ctx, uint32_t* live)
+{
+    if (!ctx || !live)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or live is NULL");
+    *live = uint32_t(ctx->tree_user_proxy.size());
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_tree_find_pairs(physgpu_context* ctx, int policy, physgpu_proxy_pair* out,
+                                       uint64_t capacity, uint64_t* count,
+                                       physgpu_broadphase_stats* stats)
+{
+    if (!ctx || !count)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or count is NULL");
+    if (policy != PHYSGPU_TREE_REBUILD && policy != PHYSGPU_TREE_REFIT)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown tree policy");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = tree_alloc(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const uint32_t live = uint32_t(ctx->tree_user_proxy.size());
+    const uint32_t slots = uint32_t(ctx->tree_live.size());
+    if (!ctx->tree_ranks_current) {
+        // Ranks of the live user indices (ascending), per proxy id.
+        std::vector<std::pair<uint64_t, uint32_t>> order;
+        order.reserve(live);
+        for (uint32_t id = 0; id < slots; ++id)
+            if (ctx->tree_live[id])
+                order.emplace_back(ctx->tree_user[id], id);
+        std::sort(order.begin(), order.end());
+        std::vector<uint32_t> rank(slots, 0xFFFFFFFFu);
+        std::vector<uint64_t> user(live);
+        for (uint32_t r = 0; r < live; ++r) {
+            rank[order[r].second] = r;
+            user[r] = order[r].first;
+        }
+        if (slots)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->tree_rank, rank.data(), sizeof(uint32_t) * slots,
+                                      hipMemcpyHostToDevice, ctx->stream));
+        if (live)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->tree_rank_user, user.data(), sizeof(uint64_t) * live,
+                                      hipMemcpyHostToDevice, ctx->stream));
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->tree_ranks_current = true;
+    }
+    const uint32_t resident_n = ctx->n;
+    const bool refit = policy == PHYSGPU_TREE_REFIT && ctx->tree_build_current && live >= 2;
+    ctx->n = live;
+    launch_step_begin(ctx);
+    launch_tree_records(ctx, slots);
+    if (live >= 2) {
+        if (refit) {
+            launch_lbvh_refit(ctx);
+        } else {
+            launch_aabb_reductions(ctx);
+            launch_lbvh(ctx);
+        }
+    }
+    launch_canonical_pairs(ctx);
+    launch_tree_pairs_out(ctx, ctx->tree_stage_pairs);
+    ctx->n = resident_n;
+    ctx->resident_aabbs_valid = false;
+    ctx->tree_pairs_loaded = true;
+    ctx->tree_build_current = live >= 2;
+    PG_TRY_HIP(hipGetLastError());
+    st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const Counters& c = *ctx->host_counters;
+    if (live < 2)
+        ctx->tree_height = 0;
+    else if (!refit)
+        ctx->tree_height = c.tree_height;
+    *count = c.pair_count;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->aabb_pairs = c.pair_count;
+        stats->tree_proxy_count = live;
+        stats->tree_node_pair_visits = c.tree_visits;
+        stats->tree_leaf_checks = c.tree_leaf_checks;
+        stats->tree_height = ctx->tree_height;
+        stats->tree_insertions = ctx->tree_created;
+        stats->tree_removals = ctx->tree_destroyed;
+        stats->tree_reinsertions = ctx->tree_reinserted;
+    }
+    ctx->tree_created = ctx->tree_destroyed = ctx->tree_reinserted = 0;
+    if (c.tree_overflow)
+        return fail(PHYSGPU_DEVICE_ERROR, "LBVH traversal stack overflow");
+    if (c.pair_count > ctx->cap_p)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "candidate pairs exceed the context capacity");
+    const uint64_t n_out = std::min<uint64_t>(capacity, c.pair_count);
+    if (out && n_out) {
+        PG_TRY_HIP(hipMemcpyAsync(out, ctx->tree_stage_pairs, sizeof(physgpu_proxy_pair) * n_out,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    if (out && capacity < c.pair_count)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "output capacity below the pair count");
+    return PHYSGPU_OK;
 }
 
 physgpu_status physgpu_kernel_times(physgpu_context* ctx, const char** names, double* avg_ms,

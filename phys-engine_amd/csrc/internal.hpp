@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/physgpu.h"
@@ -156,6 +157,7 @@ struct physgpu_context {
     uint32_t* node_last = nullptr;    // last leaf position covered by an internal node
     float4* node_box = nullptr;       // internal node bounds, 2 float4 per node
     float4* level_box = nullptr;      // boxes of 32 / 1024 / 32768-leaf groups
+    uint64_t* lbvh_sorted = nullptr;  // leaf order of the last build (items_a or items_b)
 
     // Pairs as first << 32 | second: raw emission buffer, sort scratch, and
     // the canonical (sorted) list, which aliases one of the two.
@@ -169,6 +171,32 @@ struct physgpu_context {
     uint32_t* seg_tiles = nullptr;
     uint32_t* pair_rank = nullptr;
     uint32_t* seg_lists = nullptr;
+
+    // Persistent proxies (physgpu_tree_*), allocated on first use.  Device:
+    // tight and fat bounds per proxy id (2 float4 each, record layout), the
+    // rank of each live proxy's user index among the live ones (~0u when
+    // dead), the user index of each rank, and host-to-device staging.
+    float4* tree_tight = nullptr;
+    float4* tree_fat = nullptr;
+    uint32_t* tree_rank = nullptr;
+    uint64_t* tree_rank_user = nullptr;
+    uint32_t* tree_stage_ids = nullptr;
+    physgpu_aabb* tree_stage_bounds = nullptr;
+    uint8_t* tree_stage_flags = nullptr;
+    physgpu_proxy_pair* tree_stage_pairs = nullptr;
+    // Host bookkeeping: liveness and user index per proxy id, free ids
+    // (reused last-in first-out), live user indices, counts since the last
+    // query, and whether the ranks or the LBVH build are current.
+    std::vector<uint8_t> tree_live;
+    std::vector<uint64_t> tree_user;
+    std::vector<uint32_t> tree_free;
+    std::unordered_map<uint64_t, uint32_t> tree_user_proxy;
+    uint64_t tree_created = 0, tree_destroyed = 0, tree_reinserted = 0;
+    uint32_t tree_height = 0;
+    bool tree_ranks_current = false;
+    bool tree_build_current = false;  // last broad phase was this tree's LBVH build
+    bool resident_aabbs_valid = true; // ctx->aabb holds the resident colliders' bounds
+    bool tree_pairs_loaded = false;   // the candidate pairs are a tree query's (rank space)
 
     // Narrow phase.
     uint32_t* buckets = nullptr;       // kClasses x cap_p candidate indices
@@ -284,6 +312,15 @@ void launch_sap_window_tests(physgpu_context* ctx);
 void launch_grid(physgpu_context* ctx, double cell_size);
 // lbvh.hip
 void launch_lbvh(physgpu_context* ctx);
+void launch_lbvh_refit(physgpu_context* ctx);
+void launch_lbvh_boxes_and_query(physgpu_context* ctx);
+// tree.hip
+void launch_tree_create(physgpu_context* ctx, uint32_t count, const uint32_t* ids,
+                        const physgpu_aabb* bounds);
+void launch_tree_update(physgpu_context* ctx, uint32_t count, const uint32_t* ids,
+                        const physgpu_aabb* bounds, uint8_t* reinserted);
+void launch_tree_records(physgpu_context* ctx, uint32_t slots);
+void launch_tree_pairs_out(physgpu_context* ctx, physgpu_proxy_pair* out);
 // pairs.hip
 void launch_canonical_pairs(physgpu_context* ctx);
 void launch_pairs_out(physgpu_context* ctx, physgpu_pair* dev_out, uint64_t count);

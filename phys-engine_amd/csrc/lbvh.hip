@@ -289,6 +289,7 @@ void launch_lbvh(physgpu_context* ctx)
     time_end(ctx, t);
     uint64_t* sorted = nullptr;
     radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 30, 0, 0, &sorted);
+    ctx->lbvh_sorted = sorted;
     launch_gather_sorted(ctx, sorted);
     if (n < 2)
         return;
@@ -297,6 +298,24 @@ void launch_lbvh(physgpu_context* ctx)
                        sorted, ctx->node_left, ctx->node_right, ctx->node_parent, ctx->node_first,
                        ctx->node_last);
     time_end(ctx, t);
+    launch_lbvh_boxes_and_query(ctx);
+}
+
+// Refit: the leaf order and the nodes of the last build, with the records'
+// current bounds gathered in that order and every node box recomputed.  The
+// query stays exact (boxes bound their subtrees); only pruning can loosen.
+void launch_lbvh_refit(physgpu_context* ctx)
+{
+    launch_gather_sorted(ctx, ctx->lbvh_sorted);
+    if (ctx->n < 2)
+        return;
+    launch_lbvh_boxes_and_query(ctx);
+}
+
+void launch_lbvh_boxes_and_query(physgpu_context* ctx)
+{
+    const uint32_t n = ctx->n;
+    Timer t;
     // Group boxes: 32, 1024 and 32768 leaves.
     const uint32_t c1 = div_up(n, kSpan), c2 = div_up(c1, kSpan), c3 = div_up(c2, kSpan);
     float4* l1 = ctx->level_box;

@@ -81,6 +81,12 @@ def library():
         "physgpu_set_tuning": [vp, C.POINTER(abi.Tuning)],
         "physgpu_get_tuning": [vp, C.POINTER(abi.Tuning)],
         "physgpu_autotune": [vp, i32, u32, C.POINTER(abi.Tuning)],
+        "physgpu_tree_create_proxies": [vp, u32, vp, vp, vp],
+        "physgpu_tree_destroy_proxies": [vp, u32, vp],
+        "physgpu_tree_update_proxies": [vp, u32, vp, vp, vp],
+        "physgpu_tree_size": [vp, C.POINTER(u32)],
+        "physgpu_tree_find_pairs": [vp, i32, vp, u64, C.POINTER(u64),
+                                    C.POINTER(abi.BroadPhaseStats)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -432,80 +438,99 @@ class BroadPhase:
 
 
 class DynamicAabbTree:
-    """DynamicAabbTree (dynamic_aabb_tree.hpp:25-32): proxies with the
-    reference's fat-bounds bookkeeping (dynamic_aabb_tree.cpp:44-57, 69-81);
-    the pair query runs an LBVH built per query over the tight bounds."""
+    """DynamicAabbTree (dynamic_aabb_tree.hpp:25-32) with its proxies on the
+    device (physgpu_tree_*): the reference's fat-bounds rule per update
+    (dynamic_aabb_tree.cpp:44-57, 128-139), pairs of live proxies whose tight
+    bounds overlap by user index (:268-336), from an LBVH rebuilt per query or
+    refitted while the proxy set is unchanged (`policy`).  Proxy ids are
+    recycled; invalid ids and bounds raise ValueError (the reference's
+    std::invalid_argument)."""
 
-    def __init__(self):
-        self._bounds = {}
-        self._fat = {}
-        self._next = 0
-        self._free = []
+    def __init__(self, capacity=1 << 16, max_pairs=1 << 20, policy="rebuild"):
+        if policy not in ("rebuild", "refit"):
+            raise ValueError("policy is 'rebuild' or 'refit'")
+        self._pipe = CollisionPipeline(capacity, max_pairs)
+        self.capacity = capacity
+        self.policy = abi.TREE_REFIT if policy == "refit" else abi.TREE_REBUILD
+
+    def close(self):
+        if self._pipe is not None:
+            self._pipe.close()
+            self._pipe = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     @staticmethod
-    def _checked(bounds):
-        b = np.asarray(bounds, np.float32).reshape(6)
-        if not (np.all(np.isfinite(b)) and np.all(b[:3] <= b[3:])):
-            raise ValueError("bounds must be finite and ordered")
-        return b
+    def _aabbs(bounds):
+        b = np.ascontiguousarray(np.asarray(bounds, np.float32).reshape(-1, 6))
+        return b, b.ctypes.data
 
-    @staticmethod
-    def _expanded(b, factor):
-        """Padding max(0.01, 10% of the side) x factor per axis, in double."""
-        lo, hi = b[:3].astype(np.float64), b[3:].astype(np.float64)
-        pad = np.maximum(0.01, (hi - lo) * 0.1) * factor
-        big = float(np.finfo(np.float32).max)
-        return np.concatenate([np.maximum(-big, lo - pad), np.minimum(big, hi + pad)]).astype(
-            np.float32)
-
-    @staticmethod
-    def _contains(outer, inner):
-        return bool(np.all(outer[:3] <= inner[:3]) and np.all(outer[3:] >= inner[3:]))
+    def create_proxies(self, bounds, user_index):
+        """createProxy for each row of bounds (min xyz, max xyz); proxy ids."""
+        b, bp = self._aabbs(bounds)
+        users = np.ascontiguousarray(np.asarray(user_index, np.uint64).reshape(-1))
+        if len(users) != len(b):
+            raise ValueError("one user index per bounds row")
+        ids = np.zeros(len(b), np.uint32)
+        _check(library().physgpu_tree_create_proxies(self._pipe._ctx, len(b), bp,
+                                                      users.ctypes.data, ids.ctypes.data))
+        return ids
 
     def create_proxy(self, bounds, user_index):
-        b = self._checked(bounds)
-        pid = self._free.pop() if self._free else self._next
-        if pid == self._next:
-            self._next += 1
-        self._bounds[pid] = (b, int(user_index))
-        self._fat[pid] = self._expanded(b, 1.0)
-        return pid
+        return int(self.create_proxies([bounds], [user_index])[0])
+
+    @staticmethod
+    def _ids(proxies):
+        """Proxy ids as uint32; wider values are invalid ids, never truncated
+        onto live ones."""
+        raw = np.asarray(proxies, dtype=object).reshape(-1)
+        if any(not (0 <= int(v) < (1 << 32)) for v in raw):
+            raise ValueError("Invalid dynamic AABB tree proxy")
+        return np.ascontiguousarray(raw.astype(np.uint32))
+
+    def destroy_proxies(self, proxies):
+        ids = self._ids(proxies)
+        _check(library().physgpu_tree_destroy_proxies(self._pipe._ctx, len(ids), ids.ctypes.data))
 
     def destroy_proxy(self, proxy):
-        if proxy not in self._bounds:
-            raise ValueError("invalid proxy")
-        del self._bounds[proxy]
-        del self._fat[proxy]
-        self._free.append(proxy)
+        self.destroy_proxies([proxy])
+
+    def update_proxies(self, proxies, bounds):
+        """updateProxy per row; bool array, True where reinserted."""
+        ids = self._ids(proxies)
+        b, bp = self._aabbs(bounds)
+        if len(ids) != len(b):
+            raise ValueError("one bounds row per proxy")
+        flags = np.zeros(max(len(ids), 1), np.uint8)
+        _check(library().physgpu_tree_update_proxies(self._pipe._ctx, len(ids), ids.ctypes.data,
+                                                      bp, flags.ctypes.data))
+        return flags[:len(ids)].astype(bool)
 
     def update_proxy(self, proxy, bounds):
         """True when the proxy is reinserted (its bounds left the fat bounds,
         or the fat bounds are more than 4x padding too large)."""
-        if proxy not in self._bounds:
-            raise ValueError("invalid proxy")
-        b = self._checked(bounds)
-        self._bounds[proxy] = (b, self._bounds[proxy][1])
-        fat = self._fat[proxy]
-        if self._contains(fat, b) and self._contains(self._expanded(b, 4.0), fat):
-            return False
-        self._fat[proxy] = self._expanded(b, 1.0)
-        return True
+        return bool(self.update_proxies([proxy], [bounds])[0])
 
     def find_candidate_pairs(self, stats=None):
-        items = sorted(self._bounds.values(), key=lambda t: t[1])
-        if not items:
-            return np.zeros((0, 2), np.uint32)
-        bounds = np.stack([b for b, _ in items])
-        users = np.array([u for _, u in items], np.uint32)
-        pipe = BroadPhase._pipeline(len(items))
-        pairs, st = pipe.find_candidate_pairs(bounds, BroadPhaseAlgorithm.DynamicTree,
-                                              with_stats=True)
+        """Sorted unique (smaller, larger) user-index pairs, uint64 [P, 2]."""
+        L = library()
+        n = C.c_uint64(0)
+        st = abi.BroadPhaseStats()
+        _check(L.physgpu_tree_find_pairs(self._pipe._ctx, self.policy, None, 0, C.byref(n),
+                                         C.byref(st)))
+        out = np.zeros((max(n.value, 1), 2), np.uint64)
+        _check(L.physgpu_tree_find_pairs(self._pipe._ctx, self.policy, out.ctypes.data,
+                                         n.value, C.byref(n), C.byref(st)))
         if stats is not None:
             stats.clear()
-            stats.update(st)
-        mapped = np.sort(users[pairs], axis=1) if len(pairs) else pairs
-        order = np.lexsort((mapped[:, 1], mapped[:, 0])) if len(mapped) else []
-        return mapped[order]
+            stats.update({f: getattr(st, f) for f, _ in abi.BroadPhaseStats._fields_})
+        return out[:n.value]
 
     def size(self):
-        return len(self._bounds)
+        live = C.c_uint32(0)
+        _check(library().physgpu_tree_size(self._pipe._ctx, C.byref(live)))
+        return live.value

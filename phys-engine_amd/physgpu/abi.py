@@ -45,6 +45,14 @@ class Pair(C.Structure):
     _fields_ = [("first", C.c_uint32), ("second", C.c_uint32)]
 
 
+class ProxyPair(C.Structure):
+    _fields_ = [("first", C.c_uint64), ("second", C.c_uint64)]
+
+
+TREE_REBUILD = 0
+TREE_REFIT = 1
+
+
 class BroadPhaseStats(C.Structure):
     _fields_ = [
         ("record_build_ms", C.c_double),
