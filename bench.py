@@ -35,7 +35,7 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 # with its commands is profiles/grid_step_mixed1M_hbm_traffic.md.  Valid for
 # the 1,048,576-body mixed scene only.
 MEASURED_TRAFFIC_1M = {
-    "grid_query": 68.72e6,
+    "grid_query": 58.63e6,
     "compute_aabbs": 105.73e6,
     "gather_sorted": 167.91e6,
 }
@@ -259,6 +259,45 @@ def adjoint_bench(pairs, steps):
             "manifolds_per_s": round(m / (adj_us * 1e-6), 1)}
 
 
+def tree_bench(bounds, steps):
+    """Persistent proxies (physgpu_tree_*) over the step's AABBs: create every
+    proxy, then per step move every proxy (x + 0.01 m), update all of them
+    and query, with the LBVH rebuilt or refitted.  Host-API wall time of
+    synchronous calls (bounds staged from host memory, pairs copied back)."""
+    n = len(bounds)
+    t = physgpu.DynamicAabbTree(capacity=n, max_pairs=max(4 * n // 8, 1 << 16))
+    users = np.arange(n, dtype=np.uint64)
+    t0 = time.perf_counter()
+    ids = t.create_proxies(bounds, users)
+    create_ms = (time.perf_counter() - t0) * 1e3
+    out = {}
+    b = bounds.copy()
+    pairs = None
+    for policy in ("rebuild", "refit"):
+        t.policy = abi.TREE_REFIT if policy == "refit" else abi.TREE_REBUILD
+        t.find_candidate_pairs()
+        upd = qry = 0.0
+        for _ in range(steps):
+            b[:, 0] += np.float32(0.01)
+            b[:, 3] += np.float32(0.01)
+            t0 = time.perf_counter()
+            t.update_proxies(ids, b)
+            t1 = time.perf_counter()
+            got = t.find_candidate_pairs()
+            t2 = time.perf_counter()
+            upd += t1 - t0
+            qry += t2 - t1
+        pairs = len(got)
+        out[policy] = {"update_ms": round(upd * 1e3 / steps, 3),
+                       "query_ms": round(qry * 1e3 / steps, 3)}
+    t.close()
+    log(f"proxy tree: {n} proxies, {pairs} pairs; create {create_ms:.1f} ms; "
+        f"rebuild query {out['rebuild']['query_ms']:.2f} ms, refit query "
+        f"{out['refit']['query_ms']:.2f} ms, update {out['refit']['update_ms']:.2f} ms")
+    return {"workload": f"{n} proxies on the step's AABBs, all moved every step",
+            "proxies": n, "pairs": pairs, "create_ms": round(create_ms, 3), **out}
+
+
 def copy_probe(nbytes=256 << 20):
     """One device-to-device copy of known size (the HIP runtime's blit kernel),
     used to scale rocprof FETCH_SIZE / WRITE_SIZE readings."""
@@ -287,6 +326,8 @@ def main():
                     help="worlds of the batched-form comparison (0: skip it)")
     ap.add_argument("--adjoint-pairs", type=int, default=65536,
                     help="touching pairs of the adjoint timing (0: skip it)")
+    ap.add_argument("--tree-proxies", type=int, default=1048576,
+                    help="proxies of the persistent-tree timing (0: skip it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -373,6 +414,9 @@ def main():
         line["batched"] = batched_bench(args.batched_worlds, args.steps, args.warmup)
     if args.adjoint_pairs > 0:
         line["adjoint"] = adjoint_bench(args.adjoint_pairs, args.steps)
+    if args.tree_proxies > 0:
+        aabbs = pipe.compute_aabbs()[:args.tree_proxies]
+        line["proxy_tree"] = tree_bench(np.ascontiguousarray(aabbs, np.float32), min(args.steps, 5))
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(scene)

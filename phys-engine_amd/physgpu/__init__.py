@@ -487,8 +487,12 @@ class DynamicAabbTree:
     def _ids(proxies):
         """Proxy ids as uint32; wider values are invalid ids, never truncated
         onto live ones."""
-        raw = np.asarray(proxies, dtype=object).reshape(-1)
-        if any(not (0 <= int(v) < (1 << 32)) for v in raw):
+        raw = np.asarray(proxies).reshape(-1)
+        if raw.dtype.kind not in "iu":
+            raw = np.asarray(proxies, dtype=object).reshape(-1)
+            if any(not (0 <= int(v) < (1 << 32)) for v in raw):
+                raise ValueError("Invalid dynamic AABB tree proxy")
+        elif raw.size and (int(raw.min()) < 0 or int(raw.max()) >= (1 << 32)):
             raise ValueError("Invalid dynamic AABB tree proxy")
         return np.ascontiguousarray(raw.astype(np.uint32))
 
