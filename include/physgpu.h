@@ -395,7 +395,7 @@ physgpu_status physgpu_tree_destroy_proxies(physgpu_context* ctx, uint32_t count
                                             const uint32_t* proxy);
 /* updateProxy for `count` proxies: the tight bounds become `bounds`; the fat
  * bounds follow the reference's rule, and reinserted[i] (may be NULL) is 1
- * where the proxy is reinserted.  Repeated ids in one call: the last wins. */
+ * where the proxy is reinserted.  An id may appear once per call. */
 physgpu_status physgpu_tree_update_proxies(physgpu_context* ctx, uint32_t count,
                                            const uint32_t* proxy, const physgpu_aabb* bounds,
                                            uint8_t* reinserted);
