@@ -97,6 +97,7 @@ void free_all(physgpu_context* c)
                     c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_flag, c->node_left,
                     c->node_right, c->node_parent, c->node_first, c->node_last, c->node_box, c->level_box,
                     c->pairs_raw, c->pairs_tmp, c->seg_off, c->seg_tiles, c->pair_rank,
+                    c->pair_perm, c->sort_scratch, c->sort_scratch_perm,
                     c->seg_lists, c->buckets, c->slots, c->hit, c->tile_sums,
                     c->manifolds, c->body, c->local_pose, c->friction, c->restitution,
                     c->epa_list, c->epa_simplex,
@@ -120,6 +121,8 @@ void free_all(physgpu_context* c)
     }
     for (hipEvent_t e : c->event_pool)
         hipEventDestroy(e);
+    if (c->sort_event)
+        hipEventDestroy(c->sort_event);
     if (c->fork_event)
         hipEventDestroy(c->fork_event);
     if (c->join_event)
@@ -204,8 +207,11 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
     return PHYSGPU_OK;
 }
 
-// Broad phase on the resident AABB planes (ctx->n bodies).
-void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool window_stats)
+// Broad phase on the resident AABB planes (ctx->n bodies).  With defer_sort
+// the emitted pairs are left unordered (pairs_raw) for a narrow phase that
+// sorts them on its side stream.
+void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool window_stats,
+                    bool defer_sort = false)
 {
     ctx->tree_build_current = false;
     ctx->tree_pairs_loaded = false;
@@ -225,7 +231,10 @@ void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool windo
             launch_lbvh(ctx);
         }
     }
-    launch_canonical_pairs(ctx);
+    if (defer_sort)
+        launch_finish_pairs(ctx);
+    else
+        launch_canonical_pairs(ctx);
     ctx->last_algorithm = algorithm;
 }
 
@@ -303,8 +312,10 @@ void enqueue_step(physgpu_context* ctx, int algorithm, double cell)
     if (ctx->has_world)
         launch_compose_world(ctx);
     launch_compute_aabbs(ctx);
-    run_broadphase(ctx, algorithm, cell, false);
-    launch_narrowphase(ctx, ctx->has_world);
+    run_broadphase(ctx, algorithm, cell, false, true);
+    ctx->np_pairs = ctx->pairs_raw;
+    ctx->np_perm = ctx->pair_perm;
+    launch_narrowphase(ctx, ctx->has_world, true);
     launch_world_ranges(ctx);
 }
 
@@ -397,6 +408,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->owns_stream = true;
     A(hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
+    A(hipEventCreateWithFlags(&c->sort_event, hipEventDisableTiming));
     A(hipEventCreateWithFlags(&c->fork_event, hipEventDisableTiming));
     A(hipEventCreateWithFlags(&c->join_event, hipEventDisableTiming));
     A(alloc(c->pose, 7 * N));
@@ -426,6 +438,9 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->seg_off, N + 1));
     A(alloc(c->seg_tiles, div_up(N + 1, 2048)));
     A(alloc(c->pair_rank, P));
+    A(alloc(c->pair_perm, P));
+    A(alloc(c->sort_scratch, P));
+    A(alloc(c->sort_scratch_perm, P));
     A(alloc(c->seg_lists, 2 * N));
     A(alloc(c->buckets, kClasses * P));
     A(alloc(c->slots, P));
@@ -680,7 +695,11 @@ physgpu_status physgpu_generate_contacts(physgpu_context* ctx, const physgpu_pai
                         "the last broad phase was a proxy-tree query: pass pairs or collide first");
         launch_np_begin(ctx);
     }
-    launch_narrowphase(ctx, false);
+    // Given pairs come back in their own order, the last broad phase's in
+    // canonical order.
+    ctx->np_pairs = ctx->pairs;
+    ctx->np_perm = nullptr;
+    launch_narrowphase(ctx, false, false);
     PG_TRY_HIP(hipGetLastError());
     st = read_counters(ctx);
     if (st != PHYSGPU_OK)

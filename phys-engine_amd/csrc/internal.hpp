@@ -114,6 +114,7 @@ struct physgpu_context {
     // while the main stream runs EPA (fork and join by events, so the step
     // stays one capturable sequence on the main stream).
     hipStream_t side_stream = nullptr;
+    hipEvent_t sort_event = nullptr;  // pairs emitted: the canonical sort may start
     hipEvent_t fork_event = nullptr;
     hipEvent_t join_event = nullptr;
     uint32_t cap_n = 0;
@@ -169,6 +170,9 @@ struct physgpu_context {
     // of segments too long for one thread (2 x cap_n).
     uint32_t* seg_off = nullptr;
     uint32_t* seg_tiles = nullptr;
+    uint32_t* pair_perm = nullptr;          // emission index of each sorted pair (cap_p)
+    uint64_t* sort_scratch = nullptr;       // long-segment staging (cap_p)
+    uint32_t* sort_scratch_perm = nullptr;  // (cap_p)
     uint32_t* pair_rank = nullptr;
     uint32_t* seg_lists = nullptr;
 
@@ -198,7 +202,11 @@ struct physgpu_context {
     bool resident_aabbs_valid = true; // ctx->aabb holds the resident colliders' bounds
     bool tree_pairs_loaded = false;   // the candidate pairs are a tree query's (rank space)
 
-    // Narrow phase.
+    // Narrow phase.  Its input pairs and, when they are not in canonical
+    // order, the emission index of each canonical position (np_perm; null
+    // means the input order is the output order).
+    const uint64_t* np_pairs = nullptr;
+    const uint32_t* np_perm = nullptr;
     uint32_t* buckets = nullptr;       // kClasses x cap_p candidate indices
     physgpu_manifold* slots = nullptr; // one record per candidate
     uint32_t* hit = nullptr;           // per candidate
@@ -322,12 +330,16 @@ void launch_tree_update(physgpu_context* ctx, uint32_t count, const uint32_t* id
 void launch_tree_records(physgpu_context* ctx, uint32_t slots);
 void launch_tree_pairs_out(physgpu_context* ctx, physgpu_proxy_pair* out);
 // pairs.hip
-void launch_canonical_pairs(physgpu_context* ctx);
+void launch_canonical_pairs(physgpu_context* ctx);  // finish + sort on the main stream
+void launch_finish_pairs(physgpu_context* ctx);     // pair count clamp (main stream)
+void launch_sort_pairs(physgpu_context* ctx, hipStream_t stream);
 void launch_pairs_out(physgpu_context* ctx, physgpu_pair* dev_out, uint64_t count);
 void launch_pairs_in(physgpu_context* ctx, const physgpu_pair* dev_in, uint64_t count);
 // narrowphase.hip
 void launch_np_begin(physgpu_context* ctx);
-void launch_narrowphase(physgpu_context* ctx, bool world);
+// Narrow phase over ctx->np_pairs; sort_on_side also builds the canonical
+// pair order (and np_perm) on the side stream meanwhile.
+void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side);
 void launch_hull_adjoint(physgpu_context* ctx, uint32_t m, const float* g_normal,
                          const float* g_depth, const float* g_point, float* geom_normal,
                          float* geom_depth, float* geom_point, float* grad_pair, float* grad_pose,

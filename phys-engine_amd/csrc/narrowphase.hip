@@ -1239,7 +1239,18 @@ void k_hull_adjoint(Bodies B, const physgpu_manifold* __restrict__ man, const Co
 }
 
 // Compaction: hits per tile of kScanTile candidates.
+static_assert(sizeof(physgpu_manifold) == 160 && offsetof(physgpu_manifold, pair_index) == 144,
+              "k_compact copies manifolds as 10 x 16 bytes and rewrites word 36");
+
+// Compaction in canonical order: position k of the canonical list is
+// candidate perm[k] of the narrow phase's input (k itself without perm).
+__device__ __forceinline__ uint32_t candidate_at(const uint32_t* __restrict__ perm, uint32_t k)
+{
+    return perm ? perm[k] : k;
+}
+
 __global__ __launch_bounds__(kBlock) void k_tile_hits(const uint32_t* __restrict__ hit,
+                                                      const uint32_t* __restrict__ perm,
                                                       const Counters* c,
                                                       uint32_t* __restrict__ tile_sums)
 {
@@ -1251,7 +1262,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_hits(const uint32_t* __restrict
         for (uint32_t i = threadIdx.x; i < kScanTile; i += blockDim.x) {
             const uint32_t k = tile * kScanTile + i;
             if (k < count)
-                s += hit[k];
+                s += hit[candidate_at(perm, k)];
         }
         for (uint32_t off = 32; off > 0; off >>= 1)
             s += __shfl_xor(s, off);
@@ -1298,6 +1309,7 @@ __global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* __restrict__ tile_
 
 // Ordered gather: thread t of a tile owns 8 consecutive candidates.
 __global__ __launch_bounds__(kBlock) void k_compact(const uint32_t* __restrict__ hit,
+                                                    const uint32_t* __restrict__ perm,
                                                     const Counters* c,
                                                     const uint32_t* __restrict__ tile_sums,
                                                     const physgpu_manifold* __restrict__ slots,
@@ -1311,9 +1323,11 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint32_t* __restrict__
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint32_t k0 = tile * kScanTile + threadIdx.x * per;
         uint32_t flags = 0, mine = 0;
+        uint32_t src_k[per];
         for (uint32_t i = 0; i < per; ++i) {
             const uint32_t k = k0 + i;
-            const uint32_t f = (k < count) ? hit[k] : 0u;
+            src_k[i] = k < count ? candidate_at(perm, k) : 0u;
+            const uint32_t f = (k < count) ? hit[src_k[i]] : 0u;
             flags |= f << i;
             mine += f;
         }
@@ -1333,11 +1347,15 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint32_t* __restrict__
         uint32_t pos = base + incl - mine;
         for (uint32_t i = 0; i < per; ++i) {
             if (flags & (1u << i)) {
-                const uint4* src = reinterpret_cast<const uint4*>(slots + k0 + i);
+                const uint4* src = reinterpret_cast<const uint4*>(slots + src_k[i]);
                 uint4* dst = reinterpret_cast<uint4*>(out + pos);
 #pragma unroll
-                for (int q = 0; q < 10; ++q)
+                for (int q = 0; q < 9; ++q)
                     dst[q] = src[q];
+                // Word 36 is pair_index: the candidate's canonical position.
+                uint4 tail = src[9];
+                tail.x = k0 + i;
+                dst[9] = tail;
                 ++pos;
             }
         }
@@ -1382,8 +1400,9 @@ void launch_np_begin(physgpu_context* ctx)
     hipLaunchKernelGGL(k_np_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->counters);
 }
 
-void launch_narrowphase(physgpu_context* ctx, bool world)
+void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
 {
+    const uint64_t* pairs = ctx->np_pairs;
     Bodies B{ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data, ctx->hull_range,
              ctx->hull_points, ctx->n_hull_points};
     World W{nullptr, nullptr, nullptr, nullptr};
@@ -1396,9 +1415,16 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
     const uint32_t grid = kStrideGrid;
     const physgpu_tuning& tu = ctx->tuning;
     Timer t;
+    // The canonical pair order is built on the side stream meanwhile; the
+    // compaction reads the results through its permutation.
+    if (sort_on_side) {
+        hipEventRecord(ctx->sort_event, ctx->stream);
+        hipStreamWaitEvent(ctx->side_stream, ctx->sort_event, 0);
+        launch_sort_pairs(ctx, ctx->side_stream);
+    }
     time_begin(ctx, "np_classify", t);
     hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0, ctx->stream, B,
-                       W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit);
+                       W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit);
     time_end(ctx, t);
     // With hulls, the SS / SB / BB kernels run on the side stream while EPA
     // runs here: EPA's long sphere-hull runs leave most SIMDs idle for much
@@ -1408,7 +1434,7 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
     if (fork) {
         time_begin(ctx, "np_gjk", t);
         hipLaunchKernelGGL(k_gjk, dim3(tu.gjk_groups), dim3(tu.gjk_block), 0, ctx->stream, B,
-                           ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->epa_list,
+                           pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->epa_list,
                            ctx->epa_simplex);
         time_end(ctx, t);
         hipEventRecord(ctx->fork_event, ctx->stream);
@@ -1417,26 +1443,28 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
     }
     time_begin(ctx, "np_sphere_sphere", t, cs);
     hipLaunchKernelGGL(k_class<0>, dim3(tu.class_groups[0]), dim3(tu.class_block[0]), 0, cs, B,
-                       W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
+                       W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
                        ctx->hit, ctx->partials + size_t(kSlotContacts + 0) * kMaxPartialBlocks);
     time_end(ctx, t);
     time_begin(ctx, "np_sphere_box", t, cs);
     hipLaunchKernelGGL(k_class<1>, dim3(tu.class_groups[1]), dim3(tu.class_block[1]), 0, cs, B,
-                       W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
+                       W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
                        ctx->hit, ctx->partials + size_t(kSlotContacts + 1) * kMaxPartialBlocks);
     time_end(ctx, t);
     time_begin(ctx, "np_box_box", t, cs);
     hipLaunchKernelGGL(k_class<2>, dim3(tu.class_groups[2]), dim3(tu.class_block[2]), 0, cs, B,
-                       W, ctx->pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
+                       W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
                        ctx->hit, ctx->partials + size_t(kSlotContacts + 2) * kMaxPartialBlocks);
     time_end(ctx, t);
     if (fork) {
-        hipEventRecord(ctx->join_event, ctx->side_stream);
         time_begin(ctx, "np_epa", t);
         hipLaunchKernelGGL(k_epa, dim3(tu.epa_groups), dim3(kBlock), 0, ctx->stream, B, W,
-                           ctx->pairs, ctx->counters, ctx->epa_list, ctx->epa_simplex, opt,
+                           pairs, ctx->counters, ctx->epa_list, ctx->epa_simplex, opt,
                            ctx->slots, ctx->hit, ctx->partials);
         time_end(ctx, t);
+    }
+    if (fork || sort_on_side) {
+        hipEventRecord(ctx->join_event, ctx->side_stream);
         hipStreamWaitEvent(ctx->stream, ctx->join_event, 0);
     }
     // Contact and failure counts from the per-workgroup partials.
@@ -1458,11 +1486,11 @@ void launch_narrowphase(physgpu_context* ctx, bool world)
     launch_fold(ctx, fold);
     time_begin(ctx, "np_compact", t);
     hipLaunchKernelGGL(k_tile_hits, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->hit,
-                       ctx->counters, ctx->tile_sums);
+                       ctx->np_perm, ctx->counters, ctx->tile_sums);
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_sums,
                        ctx->counters);
     hipLaunchKernelGGL(k_compact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->hit,
-                       ctx->counters, ctx->tile_sums, ctx->slots, ctx->manifolds);
+                       ctx->np_perm, ctx->counters, ctx->tile_sums, ctx->slots, ctx->manifolds);
     time_end(ctx, t);
 }
 

@@ -17,7 +17,11 @@
 //      64, a bitonic sort in LDS by one workgroup up to 4096, and beyond that
 //      one workgroup marking `second` values in an LDS bitmap window by window
 //      (distinct keys: a key's position is the number of set bits before it).
-// The pair count is read on the device, so nothing waits for the host.
+// Alongside the pairs the sort writes perm: the emission index of the pair at
+// each sorted position, so a narrow phase that ran on the emitted (unsorted)
+// pairs can be read out in canonical order.  The pair count is read on the
+// device, so nothing waits for the host; every kernel runs on the stream the
+// caller names.
 #include "internal.hpp"
 #include "wave.hpp"
 
@@ -27,7 +31,7 @@ namespace {
 constexpr uint32_t kRegSegment = 16;
 constexpr uint32_t kWaveSegment = 64;
 constexpr uint32_t kLdsSegment = 4096;
-constexpr uint32_t kBitmapWords = 8192;  // 2^18 keys per window
+constexpr uint32_t kBitmapWords = 4096;  // 2^17 keys per window
 constexpr uint32_t kTile = 2048;         // scan tile: 256 threads x 8
 constexpr uint32_t kRestGroups = 512;
 
@@ -151,13 +155,16 @@ __global__ __launch_bounds__(kBlock) void k_pair_scatter(const uint64_t* __restr
                                                          const Counters* __restrict__ c,
                                                          const uint32_t* __restrict__ seg,
                                                          const uint32_t* __restrict__ rank,
-                                                         uint64_t* __restrict__ out)
+                                                         uint64_t* __restrict__ out,
+                                                         uint32_t* __restrict__ perm)
 {
     const uint32_t count = c->sort_count;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < count;
          p += gridDim.x * blockDim.x) {
         const uint64_t v = pairs[p];
-        out[seg[uint32_t(v >> 32)] + rank[p]] = v;
+        const uint32_t pos = seg[uint32_t(v >> 32)] + rank[p];
+        out[pos] = v;
+        perm[pos] = p;
     }
 }
 
@@ -179,16 +186,19 @@ __device__ __forceinline__ void wave_push(bool pred, uint32_t value, uint32_t* _
         list[base + (uint32_t)__popcll(mask & lanes_below(lane))] = value;
 }
 
-__device__ __forceinline__ void compare_swap(uint32_t& a, uint32_t& b)
+__device__ __forceinline__ void compare_swap(uint64_t& a, uint64_t& b)
 {
-    const uint32_t lo = min(a, b), hi = max(a, b);
+    const bool swap = b < a;
+    const uint64_t lo = swap ? b : a, hi = swap ? a : b;
     a = lo;
     b = hi;
 }
 
 // Bitonic sorting network over K registers (K a power of two), ascending.
+// Entries are second << 32 | emission index: seconds are distinct within a
+// segment, so this orders by second and carries the index along.
 template <uint32_t K>
-__device__ __forceinline__ void reg_sort(uint32_t (&k)[K])
+__device__ __forceinline__ void reg_sort(uint64_t (&k)[K])
 {
 #pragma unroll
     for (uint32_t size = 2; size <= K; size <<= 1)
@@ -206,26 +216,31 @@ __device__ __forceinline__ void reg_sort(uint32_t (&k)[K])
             }
 }
 
-// Sort out[b, b + len) (len <= K, all with the same `first`) by `second`.
+// Sort out[b, b + len) (len <= K, all with the same `first`) by `second`,
+// perm alongside.
 template <uint32_t K>
-__device__ __forceinline__ void sort_in_registers(uint64_t* __restrict__ out, uint32_t b,
+__device__ __forceinline__ void sort_in_registers(uint64_t* __restrict__ out,
+                                                  uint32_t* __restrict__ perm, uint32_t b,
                                                   uint32_t len, uint64_t hi)
 {
-    uint32_t k[K];
+    uint64_t k[K];
 #pragma unroll
     for (uint32_t x = 0; x < K; ++x)
-        k[x] = x < len ? uint32_t(out[b + x]) : 0xFFFFFFFFu;
+        k[x] = x < len ? (out[b + x] << 32) | perm[b + x] : ~0ull;
     reg_sort<K>(k);
 #pragma unroll
     for (uint32_t x = 0; x < K; ++x)
-        if (x < len)
-            out[b + x] = hi | k[x];
+        if (x < len) {
+            out[b + x] = hi | (k[x] >> 32);
+            perm[b + x] = uint32_t(k[x]);
+        }
 }
 
 // One thread per body: segments of up to kRegSegment pairs are sorted in
 // registers; longer ones are listed for k_sort_rest.
 __global__ __launch_bounds__(kBlock) void k_sort_short(uint32_t n, const uint32_t* __restrict__ seg,
-                                                       uint64_t* __restrict__ out, Counters* c,
+                                                       uint64_t* __restrict__ out,
+                                                       uint32_t* __restrict__ perm, Counters* c,
                                                        uint32_t* __restrict__ mid_list,
                                                        uint32_t* __restrict__ long_list)
 {
@@ -241,9 +256,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_short(uint32_t n, const uint32_
         const bool reg = len > 1 && len <= kRegSegment;
         if (__ballot(reg && len > 4)) {
             if (reg)
-                sort_in_registers<kRegSegment>(out, b, len, hi);
+                sort_in_registers<kRegSegment>(out, perm, b, len, hi);
         } else if (reg) {
-            sort_in_registers<4>(out, b, len, hi);
+            sort_in_registers<4>(out, perm, b, len, hi);
         }
         wave_push(len > kRegSegment && len <= kWaveSegment, i, mid_list, &c->mid_segments);
         wave_push(len > kWaveSegment, i, long_list, &c->long_segments);
@@ -252,18 +267,24 @@ __global__ __launch_bounds__(kBlock) void k_sort_short(uint32_t n, const uint32_
 
 // Segments listed by k_sort_short.  Waves take the mid segments (rank sort by
 // one wave: lane x holds key x, its rank is the number of smaller keys), then
-// workgroups take the long ones (bitonic sort in LDS, or the bitmap windows
-// beyond kLdsSegment, writing through `scratch` at the segment's own range).
+// workgroups take the long ones: a bitonic sort of second << 32 | index in
+// LDS, or beyond kLdsSegment bitmap windows (a key's rank is the number of
+// set bits below it: per-word prefix counts plus a popcount), written through
+// the scratch arrays at the segment's own range.
 __global__ __launch_bounds__(256) void k_sort_rest(const uint32_t* __restrict__ seg,
                                                    const Counters* __restrict__ c,
                                                    const uint32_t* __restrict__ mid_list,
                                                    const uint32_t* __restrict__ long_list,
                                                    uint64_t* __restrict__ out,
-                                                   uint64_t* __restrict__ scratch)
+                                                   uint32_t* __restrict__ perm,
+                                                   uint64_t* __restrict__ scratch,
+                                                   uint32_t* __restrict__ scratch_perm)
 {
-    __shared__ uint32_t lds[kBitmapWords];
+    __shared__ uint64_t lds[kLdsSegment];  // bitonic keys, or bitmap words + their prefixes
     __shared__ uint32_t wave_tot[4];
     __shared__ uint32_t bounds[8];
+    uint32_t* bits = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* prefix = bits + kBitmapWords;
 
     const uint32_t lane = lane_id();
     const uint32_t mids = c->mid_segments;
@@ -272,12 +293,15 @@ __global__ __launch_bounds__(256) void k_sort_rest(const uint32_t* __restrict__ 
         const uint32_t i = mid_list[s];
         const uint32_t b = seg[i], len = seg[i + 1] - b;
         const uint32_t key = lane < len ? uint32_t(out[b + lane]) : 0xFFFFFFFFu;
+        const uint32_t idx = lane < len ? perm[b + lane] : 0u;
         uint32_t r = 0;
 #pragma unroll
         for (uint32_t j = 0; j < 64; ++j)
             r += lane_value(key, j) < key ? 1u : 0u;
-        if (lane < len)
+        if (lane < len) {
             out[b + r] = (uint64_t(i) << 32) | key;
+            perm[b + r] = idx;
+        }
     }
 
     const uint32_t longs = c->long_segments;
@@ -290,14 +314,14 @@ __global__ __launch_bounds__(256) void k_sort_rest(const uint32_t* __restrict__ 
             while (size < len)
                 size <<= 1;
             for (uint32_t k = threadIdx.x; k < size; k += blockDim.x)
-                lds[k] = k < len ? uint32_t(out[b + k]) : 0xFFFFFFFFu;
+                lds[k] = k < len ? (out[b + k] << 32) | perm[b + k] : ~0ull;
             __syncthreads();
             for (uint32_t k = 2; k <= size; k <<= 1)
                 for (uint32_t j = k >> 1; j > 0; j >>= 1) {
                     for (uint32_t x = threadIdx.x; x < size; x += blockDim.x) {
                         const uint32_t y = x ^ j;
                         if (y > x) {
-                            const uint32_t kx = lds[x], ky = lds[y];
+                            const uint64_t kx = lds[x], ky = lds[y];
                             if ((kx > ky) == ((x & k) == 0)) {
                                 lds[x] = ky;
                                 lds[y] = kx;
@@ -306,8 +330,10 @@ __global__ __launch_bounds__(256) void k_sort_rest(const uint32_t* __restrict__ 
                     }
                     __syncthreads();
                 }
-            for (uint32_t k = threadIdx.x; k < len; k += blockDim.x)
-                out[b + k] = hi | lds[k];
+            for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) {
+                out[b + k] = hi | (lds[k] >> 32);
+                perm[b + k] = uint32_t(lds[k]);
+            }
             __syncthreads();
             continue;
         }
@@ -335,34 +361,44 @@ __global__ __launch_bounds__(256) void k_sort_rest(const uint32_t* __restrict__ 
         uint32_t placed = 0;
         for (uint64_t w0 = mn; w0 <= mx; w0 += kWindow) {
             for (uint32_t k = threadIdx.x; k < kBitmapWords; k += blockDim.x)
-                lds[k] = 0;
+                bits[k] = 0;
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) {
                 const uint64_t rel = uint64_t(uint32_t(out[b + k])) - w0;
                 if (rel < kWindow)
-                    atomicOr(&lds[rel >> 5], 1u << (rel & 31u));
+                    atomicOr(&bits[rel >> 5], 1u << (rel & 31u));
             }
             __syncthreads();
+            // Exclusive prefix of the set bits per word.
             uint32_t cnt = 0;
 #pragma unroll
             for (uint32_t q = 0; q < kPer; ++q)
-                cnt += __popc(lds[threadIdx.x * kPer + q]);
+                cnt += __popc(bits[threadIdx.x * kPer + q]);
             uint32_t total;
-            uint32_t pos = placed + block_exclusive(cnt, wave_tot, total);
+            uint32_t run = placed + block_exclusive(cnt, wave_tot, total);
+#pragma unroll
             for (uint32_t q = 0; q < kPer; ++q) {
-                uint32_t bits = lds[threadIdx.x * kPer + q];
-                while (bits) {
-                    const uint32_t bit = __ffs(bits) - 1u;
-                    bits &= bits - 1u;
-                    const uint64_t key = w0 + (threadIdx.x * kPer + q) * 32u + bit;
-                    scratch[b + pos++] = hi | key;
+                prefix[threadIdx.x * kPer + q] = run;
+                run += __popc(bits[threadIdx.x * kPer + q]);
+            }
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) {
+                const uint32_t key = uint32_t(out[b + k]);
+                const uint64_t rel = uint64_t(key) - w0;
+                if (rel < kWindow) {
+                    const uint32_t w = uint32_t(rel >> 5), bit = uint32_t(rel & 31u);
+                    const uint32_t pos = prefix[w] + __popc(bits[w] & ((1u << bit) - 1u));
+                    scratch[b + pos] = hi | key;
+                    scratch_perm[b + pos] = perm[b + k];
                 }
             }
             placed += total;
             __syncthreads();
         }
-        for (uint32_t k = threadIdx.x; k < len; k += blockDim.x)
+        for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) {
             out[b + k] = scratch[b + k];
+            perm[b + k] = scratch_perm[b + k];
+        }
         __syncthreads();
     }
 }
@@ -384,41 +420,60 @@ __global__ void k_set_pair_count(Counters* c, uint64_t count)
 
 }  // namespace
 
-void launch_canonical_pairs(physgpu_context* ctx)
+namespace {
+
+// In-place exclusive scan of n u32 values on `stream` (tile sums in seg_tiles).
+void scan_u32(physgpu_context* ctx, uint32_t* v, uint32_t n, hipStream_t stream)
 {
-    Timer t;
+    const uint32_t tiles = div_up(n, kTile);
+    hipLaunchKernelGGL(k_tile_sum, dim3(tiles), dim3(256), 0, stream, v, n, ctx->seg_tiles);
+    hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, stream, ctx->seg_tiles, tiles);
+    hipLaunchKernelGGL(k_tile_apply, dim3(tiles), dim3(256), 0, stream, v, n, ctx->seg_tiles);
+}
+
+}  // namespace
+
+void launch_finish_pairs(physgpu_context* ctx)
+{
     hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, ctx->stream, ctx->counters,
                        ctx->cap_p);
+}
+
+void launch_sort_pairs(physgpu_context* ctx, hipStream_t stream)
+{
+    Timer t;
     const uint32_t n = ctx->n;
     const uint32_t segs = n + 1;
-    const uint32_t tiles = div_up(segs, kTile);
     const uint32_t pgrid = stride_grid(ctx->cap_p);
-    time_begin(ctx, "pair_rank", t);
-    hipLaunchKernelGGL(k_zero_u32, dim3(stride_grid(segs)), dim3(kBlock), 0, ctx->stream,
+    time_begin(ctx, "pair_rank", t, stream);
+    hipLaunchKernelGGL(k_zero_u32, dim3(stride_grid(segs)), dim3(kBlock), 0, stream,
                        ctx->seg_off, uint64_t(segs));
-    hipLaunchKernelGGL(k_pair_rank, dim3(pgrid), dim3(kBlock), 0, ctx->stream, ctx->pairs_raw,
+    hipLaunchKernelGGL(k_pair_rank, dim3(pgrid), dim3(kBlock), 0, stream, ctx->pairs_raw,
                        ctx->counters, ctx->seg_off, ctx->pair_rank);
     time_end(ctx, t);
-    time_begin(ctx, "pair_scan", t);
-    hipLaunchKernelGGL(k_tile_sum, dim3(tiles), dim3(256), 0, ctx->stream, ctx->seg_off, segs,
-                       ctx->seg_tiles);
-    hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, ctx->stream, ctx->seg_tiles, tiles);
-    hipLaunchKernelGGL(k_tile_apply, dim3(tiles), dim3(256), 0, ctx->stream, ctx->seg_off, segs,
-                       ctx->seg_tiles);
+    time_begin(ctx, "pair_scan", t, stream);
+    scan_u32(ctx, ctx->seg_off, segs, stream);
     time_end(ctx, t);
-    time_begin(ctx, "pair_scatter", t);
-    hipLaunchKernelGGL(k_pair_scatter, dim3(pgrid), dim3(kBlock), 0, ctx->stream, ctx->pairs_raw,
-                       ctx->counters, ctx->seg_off, ctx->pair_rank, ctx->pairs_tmp);
+    time_begin(ctx, "pair_scatter", t, stream);
+    hipLaunchKernelGGL(k_pair_scatter, dim3(pgrid), dim3(kBlock), 0, stream, ctx->pairs_raw,
+                       ctx->counters, ctx->seg_off, ctx->pair_rank, ctx->pairs_tmp,
+                       ctx->pair_perm);
     time_end(ctx, t);
-    time_begin(ctx, "pair_segment_sort", t);
-    hipLaunchKernelGGL(k_sort_short, dim3(stride_grid(n)), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->seg_off, ctx->pairs_tmp, ctx->counters, ctx->seg_lists,
-                       ctx->seg_lists + ctx->cap_n);
-    hipLaunchKernelGGL(k_sort_rest, dim3(kRestGroups), dim3(256), 0, ctx->stream, ctx->seg_off,
+    time_begin(ctx, "pair_segment_sort", t, stream);
+    hipLaunchKernelGGL(k_sort_short, dim3(stride_grid(n)), dim3(kBlock), 0, stream, n,
+                       ctx->seg_off, ctx->pairs_tmp, ctx->pair_perm, ctx->counters,
+                       ctx->seg_lists, ctx->seg_lists + ctx->cap_n);
+    hipLaunchKernelGGL(k_sort_rest, dim3(kRestGroups), dim3(256), 0, stream, ctx->seg_off,
                        ctx->counters, ctx->seg_lists, ctx->seg_lists + ctx->cap_n, ctx->pairs_tmp,
-                       ctx->pairs_raw);
+                       ctx->pair_perm, ctx->sort_scratch, ctx->sort_scratch_perm);
     time_end(ctx, t);
     ctx->pairs = ctx->pairs_tmp;
+}
+
+void launch_canonical_pairs(physgpu_context* ctx)
+{
+    launch_finish_pairs(ctx);
+    launch_sort_pairs(ctx, ctx->stream);
 }
 
 // physgpu_pair records (first, second) already copied into pairs_raw as raw
