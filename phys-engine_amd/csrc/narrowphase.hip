@@ -248,36 +248,56 @@ __device__ __forceinline__ void make_face_lds(EpaShared& S, uint32_t f, uint32_t
     S.fd[f] = d;
 }
 
-// A hull's points held one per lane for the whole EPA (hulls of at most 64
-// points), so support evaluations read no memory.
+// A hull's points (at most 64) or a box's 8 corners held one per lane for the
+// whole EPA, in collider space (p) and in world space (w = to_world(p), the
+// same function of the same inputs the support map would evaluate, so the
+// same bits), so support evaluations read no memory and rotate only the
+// direction.  Box corner c has the +extent on the axes of its set bits
+// (bit 0 x, bit 1 y, bit 2 z).
 struct WaveHull {
-    f3 p;
-    bool cached;
+    f3 p, w;
+    bool cached;  // hull of at most 64 points
+    bool box;
 };
 
-__device__ __forceinline__ WaveHull wave_hull(const Bodies& B, const Shape& s)
+__device__ __forceinline__ WaveHull wave_hull(const Bodies& B, const Shape& s, const pose& t)
 {
-    WaveHull h{{0.0f, 0.0f, 0.0f}, s.type == PHYSGPU_HULL && s.count <= 64};
+    WaveHull h{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, s.type == PHYSGPU_HULL && s.count <= 64,
+               s.type == PHYSGPU_BOX};
     const uint32_t lane = lane_id();
     if (h.cached && lane < s.count) {
         const uint32_t p = s.first + lane, nh = B.nh;
         h.p = {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+        h.w = to_world(h.p, t);
+    } else if (h.box && lane < 8) {
+        h.p = {(lane & 1u) ? s.data.x : -s.data.x, (lane & 2u) ? s.data.y : -s.data.y,
+               (lane & 4u) ? s.data.z : -s.data.z};
+        h.w = to_world(h.p, t);
     }
     return h;
+}
+
+__device__ __forceinline__ f3 lane_point(const f3& v, uint32_t src)
+{
+    return {lane_value(v.x, src), lane_value(v.y, src), lane_value(v.z, src)};
 }
 
 // Support map evaluated by the whole wave (same result in every lane).  A
 // hull's points are spread over the lanes; the arg-max keeps the lowest
 // index among equal dot products, as the sequential strict-> scan does.
+// `local` (optional) receives the collider-space point (the box corner).
 __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
                            const WaveHull& h, f3* local = nullptr)
 {
     if (s.type != PHYSGPU_HULL) {
-        if (local && s.type == PHYSGPU_BOX) {
+        if (h.box) {
             // boxSupportPoint's corner (narrowphase.cpp:27-35).
             const f3 l = rotate(conj(t.q), d);
-            *local = {l.x >= 0.0f ? s.data.x : -s.data.x, l.y >= 0.0f ? s.data.y : -s.data.y,
-                      l.z >= 0.0f ? s.data.z : -s.data.z};
+            const uint32_t c = (l.x >= 0.0f ? 1u : 0u) | (l.y >= 0.0f ? 2u : 0u) |
+                               (l.z >= 0.0f ? 4u : 0u);
+            if (local)
+                *local = lane_point(h.p, c);
+            return lane_point(h.w, c);
         }
         return support(B, s, t, d);
     }
@@ -286,13 +306,15 @@ __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
     float bv = 0.0f;
     uint32_t bi = kNoIndex;
     if (h.cached && s.count <= 16) {
-        // All points in the first 16-lane row: its row arg-max is the answer.
+        // All points in the first 16-lane row.
         if (lane < s.count) {
             bv = dot(h.p, l);
             bi = lane;
         }
-        row_arg_best<true>(bv, bi);
-        bi = lane_value(bi, 0);
+        if (!wave_arg_best_lanes<true>(bv, bi, 1)) {
+            row_arg_best<true>(bv, bi);
+            bi = lane_value(bi, 0);
+        }
     } else {
         for (uint32_t base = 0; base < s.count; base += 64) {
             const uint32_t k = base + lane;
@@ -306,16 +328,18 @@ __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
                 }
             }
         }
-        wave_arg_best<true>(bv, bi);
-        bi = uniform(bi);
+        if (!wave_arg_best_lanes<true>(bv, bi, (s.count + 63) / 64)) {
+            wave_arg_best<true>(bv, bi);
+            bi = uniform(bi);
+        }
     }
-    f3 pt;
     if (h.cached) {
-        pt = {lane_value(h.p.x, bi), lane_value(h.p.y, bi), lane_value(h.p.z, bi)};
-    } else {
-        const uint32_t p = s.first + bi;
-        pt = {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+        if (local)
+            *local = lane_point(h.p, bi);
+        return lane_point(h.w, bi);
     }
+    const uint32_t p = s.first + bi;
+    const f3 pt{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
     if (local)
         *local = pt;
     return to_world(pt, t);
@@ -529,8 +553,10 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
                 bi = f;
             }
         }
-        wave_arg_best<false>(bd, bi);
-        bi = uniform(bi);
+        if (!wave_arg_best_lanes<false>(bd, bi, (nf + 63) / 64)) {
+            wave_arg_best<false>(bd, bi);
+            bi = uniform(bi);
+        }
         const f3 cn{S.fnx[bi], S.fny[bi], S.fnz[bi]};
         const float cd = S.fd[bi];
         const f3 sup = sub(support_wave(B, sa, ta, cn, ha), support_wave(B, sb, tb, neg(cn), hb));
@@ -1014,7 +1040,7 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         }
         f3 n;
         float depth;
-        const WaveHull ha = wave_hull(B, sa), hb = wave_hull(B, sb);
+        const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
         const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, nullptr, opt.epa_max_iterations,
                                 opt.epa_tolerance, n, depth, nullptr, &work);
         if (st == 1) {
@@ -1146,7 +1172,7 @@ void k_hull_adjoint(Bodies B, const physgpu_manifold* __restrict__ man, const Co
             ++skip;
             continue;
         }
-        const WaveHull ha = wave_hull(B, sa), hb = wave_hull(B, sb);
+        const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
         f3 n;
         float depth;
         uint32_t face = 0;

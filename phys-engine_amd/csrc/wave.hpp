@@ -327,6 +327,44 @@ __device__ __forceinline__ void wave_arg_best(float& v, uint32_t& i)
     i = bi;
 }
 
+template <int Ctrl>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xF, 0xF, false));
+}
+
+// wave_arg_best for candidates whose index is chunk * 64 + lane (each lane
+// holding the first extreme of its own chunks, scanned in order): the
+// extreme value by DPP min / max, then the lowest chunk and lane holding it
+// by ballots, over `chunks` chunks.  Returns false, leaving v and i as they
+// were, when a candidate value is NaN (the caller then runs wave_arg_best,
+// whose strict comparisons give NaN the sequential scan's treatment).
+template <bool Max>
+__device__ __forceinline__ bool wave_arg_best_lanes(float& v, uint32_t& i, uint32_t chunks)
+{
+    const bool has = i != kNoIndex;
+    if (__ballot(has && v != v))
+        return false;
+    const float none = Max ? -__builtin_inff() : __builtin_inff();
+    float x = has ? v : none;
+    auto pick = [](float a, float b) { return Max ? fmaxf(a, b) : fminf(a, b); };
+    x = pick(x, dpp_f32<0xB1>(x));
+    x = pick(x, dpp_f32<0x4E>(x));
+    x = pick(x, dpp_f32<0x141>(x));
+    x = pick(x, dpp_f32<0x140>(x));
+    const float m = pick(pick(lane_value(x, 0), lane_value(x, 16)),
+                         pick(lane_value(x, 32), lane_value(x, 48)));
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const uint64_t at = __ballot(has && v == m && (i >> 6) == c);
+        if (at) {
+            i = c * 64 + (uint32_t)__ffsll((unsigned long long)at) - 1u;
+            v = m;
+            return true;
+        }
+    }
+    return false;
+}
+
 // Lanes whose `key` (below 2^Bits) equals this lane's, among the lanes of
 // `active`: one ballot per key bit.
 template <uint32_t Bits>
