@@ -898,7 +898,8 @@ __device__ f3 support_row(const Bodies& B, const Shape& s, const pose& t, f3 d, 
             }
         }
     }
-    row_arg_best<true>(bv, bi);
+    if (!row_arg_best_lanes<true>(bv, bi, (s.count + kGjkLanes - 1) / kGjkLanes, row_base))
+        row_arg_best<true>(bv, bi);
     if (bi == kNoIndex)
         bi = 0;  // no points: the first slot, as the sequential scan starts there
     f3 pt;

@@ -365,6 +365,38 @@ __device__ __forceinline__ bool wave_arg_best_lanes(float& v, uint32_t& i, uint3
     return false;
 }
 
+// row_arg_best for candidates whose index is chunk * 16 + (lane within the
+// 16-lane row), each lane holding the first extreme of its own chunks: the
+// row's extreme by DPP min / max, then its lowest chunk and lane by ballots.
+// Every lane of a row ends with the row's result.  Returns false, leaving v
+// and i as they were, when a candidate of the active lanes is NaN or the row
+// has no candidate (the caller then runs row_arg_best).
+template <bool Max>
+__device__ __forceinline__ bool row_arg_best_lanes(float& v, uint32_t& i, uint32_t chunks,
+                                                   uint32_t row_base)
+{
+    const bool has = i != kNoIndex;
+    if (__ballot(has && v != v))
+        return false;
+    const float none = Max ? -__builtin_inff() : __builtin_inff();
+    float x = has ? v : none;
+    auto pick = [](float a, float b) { return Max ? fmaxf(a, b) : fminf(a, b); };
+    x = pick(x, dpp_f32<0xB1>(x));
+    x = pick(x, dpp_f32<0x4E>(x));
+    x = pick(x, dpp_f32<0x141>(x));
+    x = pick(x, dpp_f32<0x140>(x));
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const uint32_t at =
+            uint32_t(__ballot(has && v == x && (i >> 4) == c) >> row_base) & 0xFFFFu;
+        if (at) {
+            i = c * 16 + (uint32_t)__ffs(at) - 1u;
+            v = x;
+            return true;
+        }
+    }
+    return false;
+}
+
 // Lanes whose `key` (below 2^Bits) equals this lane's, among the lanes of
 // `active`: one ballot per key bit.
 template <uint32_t Bits>
