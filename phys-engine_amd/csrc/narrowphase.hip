@@ -857,28 +857,54 @@ __device__ __forceinline__ Shape load_shape(const Bodies& B, uint32_t i)
 constexpr uint32_t kGjkLanes = 16;
 constexpr uint32_t kNoEpa = 0xFFFFFFFFu;
 
+// A hull's points (at most 16) or a box's 8 corners held one per row lane,
+// in collider space (p) and in world space (w = to_world(p): the function the
+// support map applies to the chosen point, so the same bits).  Box corner c
+// has the +extent on the axes of its set bits (bit 0 x, bit 1 y, bit 2 z).
 struct RowHull {
-    f3 p;         // point `rl` (lane within the row) of a hull with at most 16 points
-    bool cached;
+    f3 p, w;
+    bool cached;  // hull of at most 16 points
+    bool box;
 };
 
-__device__ __forceinline__ RowHull row_hull(const Bodies& B, const Shape& s, uint32_t rl)
+__device__ __forceinline__ RowHull row_hull(const Bodies& B, const Shape& s, const pose& t,
+                                            uint32_t rl)
 {
-    RowHull h{{0.0f, 0.0f, 0.0f}, s.type == PHYSGPU_HULL && s.count <= kGjkLanes};
+    RowHull h{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f},
+              s.type == PHYSGPU_HULL && s.count <= kGjkLanes, s.type == PHYSGPU_BOX};
     if (h.cached && rl < s.count) {
         const uint32_t p = s.first + rl, nh = B.nh;
         h.p = {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
+        h.w = to_world(h.p, t);
+    } else if (h.box && rl < 8) {
+        h.p = {(rl & 1u) ? s.data.x : -s.data.x, (rl & 2u) ? s.data.y : -s.data.y,
+               (rl & 4u) ? s.data.z : -s.data.z};
+        h.w = to_world(h.p, t);
     }
     return h;
 }
 
-// Support map evaluated by a row (same result in its 16 lanes).  All lanes of
-// the wave must call it together.
+__device__ __forceinline__ f3 row_point(const f3& v, uint32_t row_base, uint32_t src)
+{
+    const int at = int(row_base + src);
+    return {__shfl(v.x, at), __shfl(v.y, at), __shfl(v.z, at)};
+}
+
+// Support map evaluated by a row (same result in its 16 lanes).  All 16
+// lanes of a row call it together.
 __device__ f3 support_row(const Bodies& B, const Shape& s, const pose& t, f3 d, const RowHull& h,
                           uint32_t rl, uint32_t row_base)
 {
-    if (s.type != PHYSGPU_HULL)
+    if (s.type != PHYSGPU_HULL) {
+        if (h.box) {
+            // boxSupportPoint's corner (narrowphase.cpp:27-35).
+            const f3 l = rotate(conj(t.q), d);
+            const uint32_t c = (l.x >= 0.0f ? 1u : 0u) | (l.y >= 0.0f ? 2u : 0u) |
+                               (l.z >= 0.0f ? 4u : 0u);
+            return row_point(h.w, row_base, c);
+        }
         return support(B, s, t, d);
+    }
     const f3 l = rotate(conj(t.q), d);
     const uint32_t nh = B.nh;
     float bv = 0.0f;
@@ -902,15 +928,10 @@ __device__ f3 support_row(const Bodies& B, const Shape& s, const pose& t, f3 d, 
         row_arg_best<true>(bv, bi);
     if (bi == kNoIndex)
         bi = 0;  // no points: the first slot, as the sequential scan starts there
-    f3 pt;
-    if (h.cached) {
-        const int src = int(row_base + bi);
-        pt = {__shfl(h.p.x, src), __shfl(h.p.y, src), __shfl(h.p.z, src)};
-    } else {
-        const uint32_t p = s.first + bi;
-        pt = {B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]};
-    }
-    return to_world(pt, t);
+    if (h.cached)
+        return row_point(h.w, row_base, bi);
+    const uint32_t p = s.first + bi;
+    return to_world(f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]}, t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __restrict__ pairs,
@@ -947,7 +968,7 @@ __global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __rest
         }
         const Shape sa = load_shape(B, a), sb = load_shape(B, b);
         const pose ta = load_pose(B, a), tb = load_pose(B, b);
-        const RowHull ha = row_hull(B, sa, rl), hb = row_hull(B, sb, rl);
+        const RowHull ha = row_hull(B, sa, ta, rl), hb = row_hull(B, sb, tb, rl);
         Simplex s;
         s.size = 0;
         f3 dir{1.0f, 0.0f, 0.0f};
