@@ -62,19 +62,122 @@ __global__ __launch_bounds__(kBlock) void k_compute_aabbs(
         mn = sub(t.p, e);
         mx = add(t.p, e);
     } else {
-        const uint32_t first = hrange[i], count = hrange[n + i];
-        f3 w = to_world(f3{hpts[first], hpts[nh + first], hpts[2 * nh + first]}, t);
-        mn = w;
-        mx = w;
-        for (uint32_t k = 1; k < count; ++k) {
-            const uint32_t p = first + k;
-            w = to_world(f3{hpts[p], hpts[nh + p], hpts[2 * nh + p]}, t);
-            mn = {min_ref(mn.x, w.x), min_ref(mn.y, w.y), min_ref(mn.z, w.z)};
-            mx = {max_ref(mx.x, w.x), max_ref(mx.y, w.y), max_ref(mx.z, w.z)};
-        }
+        return;  // hulls: k_hull_aabbs
     }
     const float b[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
     store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t count, uint32_t value)
+{
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x)
+        p[i] = value;
+}
+
+// Hull bodies (listed once per upload by k_hull_list).
+__global__ __launch_bounds__(kBlock) void k_hull_list(uint32_t n, const uint8_t* __restrict__ type,
+                                                      uint32_t* __restrict__ list,
+                                                      uint32_t* __restrict__ count)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool hull = i < n && type[i] == PHYSGPU_HULL;
+    const uint64_t mask = __ballot(hull);
+    if (!mask)
+        return;
+    const uint32_t lane = lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    if (hull)
+        list[base + (uint32_t)__popcll(mask & lanes_below(lane))] = i;
+}
+
+// First extreme (lowest index among equal values, as the sequential min_ref /
+// max_ref scan keeps the earlier point) of one component over a 16-lane row
+// whose lane holds the first extreme of its points rl, rl + 16, ...: false when
+// a value is NaN.
+template <bool Max>
+__device__ __forceinline__ bool row_first_extreme(float& v, uint32_t count, uint32_t rl,
+                                                  uint32_t row_base)
+{
+    uint32_t i = rl < count ? rl : kNoIndex;
+    return row_arg_best_lanes<Max>(v, i, 1, row_base);
+}
+
+// Hull AABBs (min / max of the transformed points) with one 16-lane row per
+// hull: each lane transforms points rl, rl + 16, ... and keeps their first
+// extremes, then each component is reduced over the row.  A NaN coordinate
+// sends the hull to the sequential scan on the row's first lane.
+__global__ __launch_bounds__(kBlock) void k_hull_aabbs(const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ list_count,
+                                                       uint32_t n, const float* __restrict__ pose,
+                                                       const uint32_t* __restrict__ hrange,
+                                                       const float* __restrict__ hpts, uint32_t nh,
+                                                       const uint32_t* __restrict__ world_of,
+                                                       float4* __restrict__ aabb)
+{
+    constexpr uint32_t kLanes = 16;
+    const uint32_t hulls = *list_count;
+    const uint32_t lane = lane_id(), rl = lane % kLanes, row_base = lane - rl;
+    const uint32_t rows = gridDim.x * blockDim.x / kLanes;
+    const uint32_t rounds = (hulls + rows - 1) / rows;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t h = r * rows + (blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+        if (h >= hulls)
+            continue;  // whole rows (h is row-uniform)
+        const uint32_t i = list[h];
+        const pg::pose t{{pose[i], pose[n + i], pose[2 * n + i]},
+                         {pose[3 * n + i], pose[4 * n + i], pose[5 * n + i], pose[6 * n + i]}};
+        const uint32_t first = hrange[i], count = hrange[n + i];
+        f3 mn{0.0f, 0.0f, 0.0f}, mx{0.0f, 0.0f, 0.0f};
+        for (uint32_t k = rl; k < count; k += kLanes) {
+            const uint32_t p = first + k;
+            const f3 w = to_world(f3{hpts[p], hpts[nh + p], hpts[2 * nh + p]}, t);
+            if (k == rl) {
+                mn = w;
+                mx = w;
+            } else {
+                mn = {min_ref(mn.x, w.x), min_ref(mn.y, w.y), min_ref(mn.z, w.z)};
+                mx = {max_ref(mx.x, w.x), max_ref(mx.y, w.y), max_ref(mx.z, w.z)};
+            }
+        }
+        // Chunk order within a lane is index order, so a lane's first extreme
+        // precedes every later equal value of that lane; across lanes the row
+        // reduction takes the lowest lane, i.e. the lowest index among the
+        // lanes' first extremes only when count <= 16.  Longer hulls take
+        // the sequential scan.
+        bool ok = count <= kLanes;
+        float b[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
+        if (ok) {
+            ok = row_first_extreme<false>(b[0], count, rl, row_base) &&
+                 row_first_extreme<false>(b[1], count, rl, row_base) &&
+                 row_first_extreme<false>(b[2], count, rl, row_base) &&
+                 row_first_extreme<true>(b[3], count, rl, row_base) &&
+                 row_first_extreme<true>(b[4], count, rl, row_base) &&
+                 row_first_extreme<true>(b[5], count, rl, row_base);
+        }
+        if (rl == 0) {
+            if (!ok) {
+                f3 w = to_world(f3{hpts[first], hpts[nh + first], hpts[2 * nh + first]}, t);
+                f3 a = w, c = w;
+                for (uint32_t k = 1; k < count; ++k) {
+                    const uint32_t p = first + k;
+                    w = to_world(f3{hpts[p], hpts[nh + p], hpts[2 * nh + p]}, t);
+                    a = {min_ref(a.x, w.x), min_ref(a.y, w.y), min_ref(a.z, w.z)};
+                    c = {max_ref(c.x, w.x), max_ref(c.y, w.y), max_ref(c.z, w.z)};
+                }
+                b[0] = a.x;
+                b[1] = a.y;
+                b[2] = a.z;
+                b[3] = c.x;
+                b[4] = c.y;
+                b[5] = c.z;
+            }
+            store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
+        }
+    }
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
@@ -248,7 +351,22 @@ void launch_compute_aabbs(physgpu_context* ctx)
                        ctx->stream, ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data,
                        ctx->hull_range, ctx->hull_points, ctx->n_hull_points,
                        ctx->n_worlds ? ctx->world_of : nullptr, ctx->aabb);
+    if (ctx->has_hulls)
+        hipLaunchKernelGGL(k_hull_aabbs, dim3(stride_grid(16ull * ctx->n)), dim3(kBlock), 0,
+                           ctx->stream, ctx->hull_list, ctx->hull_list_count, ctx->n, ctx->pose,
+                           ctx->hull_range, ctx->hull_points, ctx->n_hull_points,
+                           ctx->n_worlds ? ctx->world_of : nullptr, ctx->aabb);
     time_end(ctx, t);
+}
+
+void launch_hull_list(physgpu_context* ctx)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->hull_list_count, 1u,
+                       0u);
+    if (ctx->n)
+        hipLaunchKernelGGL(k_hull_list, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0,
+                           ctx->stream, ctx->n, ctx->shape_type, ctx->hull_list,
+                           ctx->hull_list_count);
 }
 
 void launch_aabb_reductions(physgpu_context* ctx)

@@ -93,7 +93,8 @@ hipError_t alloc(T*& p, size_t count)
 void free_all(physgpu_context* c)
 {
     void* ptrs[] = {c->pose, c->shape_type, c->shape_data, c->hull_range, c->hull_points,
-                    c->aabb, c->sorted_aabb, c->items_a, c->items_b,
+                    c->aabb, c->sorted_aabb, c->items_a, c->items_b, c->hull_list,
+                    c->hull_list_count,
                     c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_flag, c->node_left,
                     c->node_right, c->node_parent, c->node_first, c->node_last, c->node_box, c->level_box,
                     c->pairs_raw, c->pairs_tmp, c->seg_off, c->seg_tiles, c->pair_rank,
@@ -417,6 +418,8 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->shape_data, 3 * N));
     A(alloc(c->hull_range, 2 * N));
     A(alloc(c->hull_points, 3 * size_t(std::max<uint32_t>(max_hull_points, 1))));
+    A(alloc(c->hull_list, N));
+    A(alloc(c->hull_list_count, 1));
     A(alloc(c->aabb, 2 * N));
     A(alloc(c->sorted_aabb, 2 * N));
     A(alloc(c->items_a, N));
@@ -553,8 +556,11 @@ physgpu_status physgpu_set_colliders(physgpu_context* ctx, const physgpu_collide
             PG_TRY_HIP(hipMemcpyAsync(ctx->hull_points, col->hull_points,
                                       3ull * col->hull_point_count * 4, kind, ctx->stream));
     }
-    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
     ctx->n = n;
+    if (hulls)
+        launch_hull_list(ctx);
+    PG_TRY_HIP(hipGetLastError());
+    PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
     ctx->n_hull_points = col->hull_point_count;
     ctx->has_hulls = hulls;
     ctx->colliders_valid = true;

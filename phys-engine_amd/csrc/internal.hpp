@@ -130,6 +130,8 @@ struct physgpu_context {
     uint32_t* hull_range = nullptr;  // 2 x cap_n
     float* hull_points = nullptr;    // 3 x cap_h
     bool has_hulls = false;
+    uint32_t* hull_list = nullptr;        // hull collider indices (cap_n), built per upload
+    uint32_t* hull_list_count = nullptr;  // their count (device)
     bool colliders_valid = false;
 
     // AABB records (2 float4 per body, see wave.hpp Rec) and their sorted copy
@@ -301,6 +303,7 @@ inline uint32_t stride_grid(uint64_t work)
 // aabb.hip
 void launch_step_begin(physgpu_context* ctx);
 void launch_compute_aabbs(physgpu_context* ctx);
+void launch_hull_list(physgpu_context* ctx);
 void launch_aabb_reductions(physgpu_context* ctx);
 void launch_convert_aabbs_out(physgpu_context* ctx, physgpu_aabb* dev_out);
 void launch_convert_aabbs_in(physgpu_context* ctx, const physgpu_aabb* dev_in);
