@@ -164,7 +164,7 @@ physgpu_tuning default_tuning()
     }
     t.gjk_block = kBlock;
     t.gjk_groups = kStrideGrid;
-    t.epa_groups = 2048;  // measured at 1M bodies: 1024 -> 546 us, 2048 -> 452 us, 8192 -> 603 us
+    t.epa_groups = 65536;  // one wavefront each; at 1M bodies (34k entries) 2048 / 16384 / 65536: 638 / 310 / 263 us
     t.query_block = kBlock;
     t.query_groups = 4096;  // at 1M bodies, 256 x 1024 / 2048 / 4096: 119 / 114 / 98 us
     t.sort_items = 8;
@@ -173,6 +173,7 @@ physgpu_tuning default_tuning()
 
 bool valid_block(uint32_t b) { return b == 64 || b == 128 || b == 256; }
 bool valid_groups(uint32_t g) { return g >= 1 && g <= kMaxPartialBlocks; }
+bool valid_epa_groups(uint32_t g) { return g >= 1 && g <= kMaxEpaGroups; }
 
 // Zero fields take the defaults; anything else must be a supported shape.
 physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
@@ -200,7 +201,7 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
     if (!out.sort_items)
         out.sort_items = d.sort_items;
     if (!valid_block(out.gjk_block) || !valid_groups(out.gjk_groups) ||
-        !valid_groups(out.epa_groups) || !valid_block(out.query_block) ||
+        !valid_epa_groups(out.epa_groups) || !valid_block(out.query_block) ||
         !valid_groups(out.query_groups))
         return fail(PHYSGPU_INVALID_ARGUMENT, "kernel shape out of range");
     if (out.sort_items != 4 && out.sort_items != 8 && out.sort_items != 16)
@@ -955,7 +956,7 @@ physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t re
     if (st == PHYSGPU_OK && ctx->has_hulls)
         st = tune_shape(&ctx->tuning.gjk_block, &ctx->tuning.gjk_groups, "np_gjk");
     if (st == PHYSGPU_OK && ctx->has_hulls) {
-        static const uint32_t epa_groups[] = {256, 512, 768, 1024, 2048};
+        static const uint32_t epa_groups[] = {4096, 8192, 16384, 32768, 65536};
         uint32_t best_g = ctx->tuning.epa_groups;
         double best = 1e30;
         for (uint32_t g : epa_groups) {

@@ -32,6 +32,8 @@ constexpr uint32_t kScanTile = 2048;
 // by k_fold (one small launch) instead.  Kernels that write partials are
 // launched with at most kMaxPartialBlocks workgroups.
 constexpr uint32_t kMaxPartialBlocks = 8192;
+// EPA workgroups (one wavefront each) at most.
+constexpr uint32_t kMaxEpaGroups = 1u << 20;
 enum PartialSlot : uint32_t {
     kSlotGridTests = 0,
     kSlotGridOverflowTests,

@@ -394,9 +394,10 @@ __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 // with the same pair find each other by ballots over the key bits, so sums
 // and counts inside a step are popcounts.  With more than one step, the
 // LDS table key_sum carries each pair's sum forward, then the count of
-// copies still to keep backward, ending all zero.  Writes the horizon to
-// S.ea/S.eb; returns its size, or -1 when the list would pass kMaxEdges.
-__device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges)
+// copies still to keep backward, ending all zero.  `keys_ready` says whether
+// the table holds zeros yet (it is cleared on first use).  Writes the horizon
+// to S.ea/S.eb; returns its size, or -1 when the list would pass kMaxEdges.
+__device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges, bool& keys_ready)
 {
     const uint32_t lane = lane_id();
     const uint64_t below = lanes_below(lane);
@@ -431,6 +432,12 @@ __device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges)
         return __popcll(K);
     }
     // Several steps: the LDS table key_sum carries each pair's sum forward.
+    if (!keys_ready) {
+        for (uint32_t i = lane; i < kEdgeKeys; i += 64)
+            S.key_sum[i] = 0;
+        wave_sync();
+        keys_ready = true;
+    }
     const uint32_t steps = (edges + 63) / 64;
     int list_size = 0;
     for (uint32_t c = 0; c < steps; ++c) {
@@ -499,11 +506,13 @@ __device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges)
 }
 
 // Returns 1 converged, 0 failed (the reference's failure), -1 storage bound
-// exceeded.  `normal`/`depth` are uniform across the wave.
+// exceeded.  `normal`/`depth` are uniform across the wave.  `keys_ready`
+// tracks whether S.key_sum holds zeros (see horizon_wave).
 __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const pose& ta,
                         const WaveHull& ha, const Shape& sb, const pose& tb, const WaveHull& hb,
                         const f3 simplex[4], uint32_t iterations, float tol, f3& normal,
-                        float& depth, uint32_t* face_out = nullptr, uint32_t* work = nullptr)
+                        float& depth, bool& keys_ready, uint32_t* face_out = nullptr,
+                        uint32_t* work = nullptr)
 {
     const uint32_t lane = lane_id();
     normal = {0.0f, 0.0f, 0.0f};
@@ -599,7 +608,7 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
             visible += (uint32_t)__popcll(vis[ch]);
         }
         wave_sync();
-        const int horizon = horizon_wave(S, 3 * visible);
+        const int horizon = horizon_wave(S, 3 * visible, keys_ready);
         if (horizon < 0)
             { if (work) *work += it + 1; return -1; }
         const uint32_t ne = uint32_t(horizon);
@@ -856,6 +865,11 @@ __device__ __forceinline__ Shape load_shape(const Bodies& B, uint32_t i)
 // class << 29 for a hit (with its final simplex), kNoEpa otherwise.
 constexpr uint32_t kGjkLanes = 16;
 constexpr uint32_t kNoEpa = 0xFFFFFFFFu;
+// EPA's partial slots: contacts of classes 3..5, failures, overflows.
+constexpr uint32_t kEpaPartialFirst = kSlotContacts + 3;
+constexpr uint32_t kEpaPartialSlots = 5;
+static_assert(kSlotGjkFailures == kEpaPartialFirst + 3 && kSlotEpaOverflows == kEpaPartialFirst + 4,
+              "EPA partial slots are contiguous");
 
 // A hull's points (at most 16) or a box's 8 corners held one per row lane,
 // in collider space (p) and in world space (w = to_world(p): the function the
@@ -938,8 +952,13 @@ __global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __rest
                                                 Counters* c, const uint32_t* __restrict__ buckets,
                                                 uint64_t cap, physgpu_narrowphase_options opt,
                                                 uint32_t* __restrict__ list,
-                                                float* __restrict__ simplex_out)
+                                                float* __restrict__ simplex_out,
+                                                uint64_t* __restrict__ epa_partials)
 {
+    // The EPA kernel that follows adds its counts into these partial slots.
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kEpaPartialSlots * kMaxPartialBlocks;
+         i += gridDim.x * blockDim.x)
+        epa_partials[i] = 0;
     const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
     const uint32_t total = c3 + c4 + c->class_count[5];
     const uint32_t rows = gridDim.x * blockDim.x / kGjkLanes;
@@ -1013,38 +1032,36 @@ __global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __rest
 
 // Hull pairs, step 2: EPA, one wavefront per intersecting pair; the contact
 // point is the midpoint of the two deepest support points along the EPA
-// normal (A toward B).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+// normal (A toward B).  Each workgroup is one wavefront and takes list entry
+// blockIdx.x (then every gridDim.x-th after it): with at least as many
+// workgroups as entries the dispatcher hands out the entries in list order
+// as wavefront slots free up, which keeps long sphere-hull runs (the list
+// starts with the sphere-hull class) from queueing behind one another.
+// Counts go to the EPA partial slots (zeroed by k_gjk) with one atomic add per
+// nonzero count, spread over kMaxPartialBlocks addresses.
+constexpr uint32_t kEpaBlock = 64;
+
+__device__ __forceinline__ void epa_partial_add(uint64_t* slot, uint32_t v)
+{
+    if (v)
+        atomicAdd(reinterpret_cast<unsigned long long*>(slot + (blockIdx.x & (kMaxPartialBlocks - 1))),
+                  (unsigned long long)v);
+}
+
+__global__ __launch_bounds__(kEpaBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
            const uint32_t* __restrict__ list, const float* __restrict__ simplex_in,
            physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
            uint32_t* __restrict__ hit, uint64_t* __restrict__ partials)
 {
-    __shared__ EpaShared shared[kEpaWavesPerBlock];
-    __shared__ uint32_t next;
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    EpaShared& S = shared[wave];
-    for (uint32_t i = lane; i < kEdgeKeys; i += 64)
-        S.key_sum[i] = 0;
-    // Hull pairs are split evenly over the workgroups; inside a workgroup the
-    // wavefronts take pairs from an LDS counter, so long EPA runs do not pile
-    // up on one wavefront and no device atomic is needed.
+    __shared__ EpaShared S;
+    const uint32_t lane = lane_id();
     const uint32_t total = c->class_count[3] + c->class_count[4] + c->class_count[5];
-    const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
-    const uint32_t begin = min(blockIdx.x * per, total), end = min(begin + per, total);
-    if (threadIdx.x == 0)
-        next = begin;
-    __syncthreads();
+    bool keys_ready = false;
     uint32_t contacts3 = 0, contacts4 = 0, contacts5 = 0;
     uint32_t failures = 0, overflows = 0;
     uint32_t work = 0;  // EPA iterations of this wavefront so far
-    for (;;) {
-        uint32_t e = 0;
-        if (lane == 0)
-            e = atomicAdd(&next, 1u);
-        e = uniform(e);
-        if (e >= end)
-            break;
+    for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
         const uint32_t entry = list[e];
         if (entry == kNoEpa)
             continue;
@@ -1064,7 +1081,7 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         float depth;
         const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
         const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, nullptr, opt.epa_max_iterations,
-                                opt.epa_tolerance, n, depth, nullptr, &work);
+                                opt.epa_tolerance, n, depth, keys_ready, nullptr, &work);
         if (st == 1) {
             const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
                                    support_wave(B, sb, tb, neg(n), hb)),
@@ -1089,13 +1106,14 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         wave_sync();
     }
     __builtin_amdgcn_s_setprio(0);
-    // Counts are wave-uniform: lane 0 carries the wavefront's share.
-    block_partial(lane == 0 ? contacts3 : 0u, partials + size_t(kSlotContacts + 3) * kMaxPartialBlocks);
-    block_partial(lane == 0 ? contacts4 : 0u, partials + size_t(kSlotContacts + 4) * kMaxPartialBlocks);
-    block_partial(lane == 0 ? contacts5 : 0u, partials + size_t(kSlotContacts + 5) * kMaxPartialBlocks);
-    block_partial(lane == 0 ? failures : 0u, partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks);
-    block_partial(lane == 0 ? overflows : 0u,
-                  partials + size_t(kSlotEpaOverflows) * kMaxPartialBlocks);
+    // Counts are wave-uniform: lane 0 adds the wavefront's share.
+    if (lane == 0) {
+        epa_partial_add(partials + size_t(kSlotContacts + 3) * kMaxPartialBlocks, contacts3);
+        epa_partial_add(partials + size_t(kSlotContacts + 4) * kMaxPartialBlocks, contacts4);
+        epa_partial_add(partials + size_t(kSlotContacts + 5) * kMaxPartialBlocks, contacts5);
+        epa_partial_add(partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks, failures);
+        epa_partial_add(partials + size_t(kSlotEpaOverflows) * kMaxPartialBlocks, overflows);
+    }
 }
 
 // ------------------------------------------------------------ hull adjoint
@@ -1171,9 +1189,7 @@ void k_hull_adjoint(Bodies B, const physgpu_manifold* __restrict__ man, const Co
     __shared__ EpaShared shared[kEpaWavesPerBlock];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     EpaShared& S = shared[wave];
-    for (uint32_t i = lane; i < kEdgeKeys; i += 64)
-        S.key_sum[i] = 0;
-    wave_sync();
+    bool keys_ready = false;
     const uint32_t count = min(m, c->manifold_count);
     const uint32_t stride = gridDim.x * kEpaWavesPerBlock;
     uint32_t skip = 0;
@@ -1199,7 +1215,7 @@ void k_hull_adjoint(Bodies B, const physgpu_manifold* __restrict__ man, const Co
         float depth;
         uint32_t face = 0;
         const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, sx.p, opt.epa_max_iterations,
-                                opt.epa_tolerance, n, depth, &face);
+                                opt.epa_tolerance, n, depth, keys_ready, &face);
         __builtin_amdgcn_s_setprio(0);
         wave_sync();
         if (st != 1) {
@@ -1483,7 +1499,8 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
         time_begin(ctx, "np_gjk", t);
         hipLaunchKernelGGL(k_gjk, dim3(tu.gjk_groups), dim3(tu.gjk_block), 0, ctx->stream, B,
                            pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->epa_list,
-                           ctx->epa_simplex);
+                           ctx->epa_simplex,
+                           ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks);
         time_end(ctx, t);
         hipEventRecord(ctx->fork_event, ctx->stream);
         hipStreamWaitEvent(ctx->side_stream, ctx->fork_event, 0);
@@ -1506,7 +1523,7 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
     time_end(ctx, t);
     if (fork) {
         time_begin(ctx, "np_epa", t);
-        hipLaunchKernelGGL(k_epa, dim3(tu.epa_groups), dim3(kBlock), 0, ctx->stream, B, W,
+        hipLaunchKernelGGL(k_epa, dim3(tu.epa_groups), dim3(kEpaBlock), 0, ctx->stream, B, W,
                            pairs, ctx->counters, ctx->epa_list, ctx->epa_simplex, opt,
                            ctx->slots, ctx->hit, ctx->partials);
         time_end(ctx, t);
@@ -1522,12 +1539,13 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
                      uint32_t(offsetof(Counters, class_contacts) + 4 * q)};
     fold.count = 3;
     if (ctx->has_hulls) {
+        const uint32_t epa_slots = std::min(tu.epa_groups, kMaxPartialBlocks);
         for (uint32_t q = 3; q < kClasses; ++q)
-            fold.e[q] = {kSlotContacts + q, tu.epa_groups, kFoldSum32,
+            fold.e[q] = {kSlotContacts + q, epa_slots, kFoldSum32,
                          uint32_t(offsetof(Counters, class_contacts) + 4 * q)};
-        fold.e[6] = {kSlotGjkFailures, tu.epa_groups, kFoldSum32,
+        fold.e[6] = {kSlotGjkFailures, epa_slots, kFoldSum32,
                      uint32_t(offsetof(Counters, gjk_failures))};
-        fold.e[7] = {kSlotEpaOverflows, tu.epa_groups, kFoldSum32,
+        fold.e[7] = {kSlotEpaOverflows, epa_slots, kFoldSum32,
                      uint32_t(offsetof(Counters, epa_overflow))};
         fold.count = 8;
     }

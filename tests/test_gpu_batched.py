@@ -154,9 +154,10 @@ def test_autotune_keeps_results(bpipe, batch):
     bpipe.collide(algorithm=abi.UNIFORM_GRID)
     assert np.array_equal(bpipe.pairs(), want_p)
     assert np.array_equal(words(bpipe.manifolds()), words(want_m))
-    for items in (4, 16):
+    for items, epa in ((4, 7), (16, 10000)):
         bpipe.set_tuning(dict(t, sort_items=items, class_block=[64, 128, 64],
-                              class_groups=[7, 300, 5000], query_block=64, query_groups=33))
+                              class_groups=[7, 300, 5000], query_block=64, query_groups=33,
+                              epa_groups=epa))
         bpipe.collide_graph(algorithm=abi.UNIFORM_GRID)
         assert np.array_equal(bpipe.pairs(), want_p)
         assert np.array_equal(words(bpipe.manifolds()), words(want_m))
