@@ -210,7 +210,6 @@ struct EpaShared {
     uint8_t ea[kMaxEdges], eb[kMaxEdges];
     uint8_t vis_list[kMaxFaces];                  // visible faces, in index order
     uint32_t edge_bits[kMaxVisibleEdges / 32];    // per-edge flags of the horizon passes
-    int16_t key_sum[kEdgeKeys];                   // per vertex pair, all zero between uses
 };
 
 __device__ __forceinline__ uint32_t edge_key(uint32_t a, uint32_t b)
@@ -392,12 +391,12 @@ __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 // follow from s alone; the survivors are the last |s_final| appended copies,
 // kept in arrival order.  Edges are taken 64 per step in arrival order; lanes
 // with the same pair find each other by ballots over the key bits, so sums
-// and counts inside a step are popcounts.  With more than one step, the
-// LDS table key_sum carries each pair's sum forward, then the count of
-// copies still to keep backward, ending all zero.  `keys_ready` says whether
-// the table holds zeros yet (it is cleared on first use).  Writes the horizon
-// to S.ea/S.eb; returns its size, or -1 when the list would pass kMaxEdges.
-__device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges, bool& keys_ready)
+// and counts inside a step are popcounts.  With more than one step (over 21
+// visible faces, rare), each lane also matches its key against every other
+// step's keys: earlier steps give its running sum, all steps its final sum,
+// later steps the appended copies that follow it.  Writes the horizon to
+// S.ea/S.eb; returns its size, or -1 when the list would pass kMaxEdges.
+__device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges)
 {
     const uint32_t lane = lane_id();
     const uint64_t below = lanes_below(lane);
@@ -431,54 +430,59 @@ __device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges, bool& 
         wave_sync();
         return __popcll(K);
     }
-    // Several steps: the LDS table key_sum carries each pair's sum forward.
-    if (!keys_ready) {
-        for (uint32_t i = lane; i < kEdgeKeys; i += 64)
-            S.key_sum[i] = 0;
-        wave_sync();
-        keys_ready = true;
-    }
+    // Several steps.  A lane's sums over the edges of another step come from
+    // matching its key against that step's keys, one ballot per key bit.
     const uint32_t steps = (edges + 63) / 64;
     int list_size = 0;
     for (uint32_t c = 0; c < steps; ++c) {
         const EdgeLane e = edge_lane(S, c * 64 + lane, edges);
+        int carry = 0;  // running sum of this lane's pair over the earlier steps
+        for (uint32_t z = 0; z < c; ++z) {
+            const EdgeLane f = edge_fields(S, z * 64 + lane, edges);
+            const uint64_t m = match_other<kEdgeKeyBits>(e.key, f.key, __ballot(f.valid));
+            const uint64_t fup = __ballot(f.valid && f.delta > 0);
+            carry += 2 * __popcll(m & fup) - __popcll(m);
+        }
         const uint64_t up = __ballot(e.valid && e.delta > 0);
-        const int carry = e.valid ? int(S.key_sum[e.key]) : 0;
         const int sum = carry + 2 * __popcll(e.same & up & upto) - __popcll(e.same & upto);
         const bool append = e.valid && iabs(sum) > iabs(sum - e.delta);
         const uint64_t A = __ballot(append), V = __ballot(e.valid);
         const int size_before = list_size + 2 * __popcll(A & below) - __popcll(V & below);
-        const bool over = __ballot(append && size_before >= int(kMaxEdges)) != 0;
-        if (e.valid && (e.same & ~upto) == 0)
-            S.key_sum[e.key] = int16_t(carry + 2 * __popcll(e.same & up) - __popcll(e.same));
+        if (__ballot(append && size_before >= int(kMaxEdges)))
+            return -1;
         if (lane == 0) {
             S.edge_bits[2 * c] = uint32_t(A);
             S.edge_bits[2 * c + 1] = uint32_t(A >> 32);
         }
         list_size += 2 * __popcll(A) - __popcll(V);
-        wave_sync();
-        if (over) {
-            for (uint32_t z = 0; z <= c; ++z) {
-                const EdgeLane ez = edge_lane(S, z * 64 + lane, edges);
-                if (ez.valid)
-                    S.key_sum[ez.key] = 0;
-            }
-            wave_sync();
-            return -1;
-        }
     }
-    // Backward: the last |s_final| appended copies of each pair survive.
-    for (int c = int(steps) - 1; c >= 0; --c) {
-        const EdgeLane e = edge_lane(S, uint32_t(c) * 64 + lane, edges);
+    wave_sync();
+    // An appended edge survives while fewer than |final sum of its pair|
+    // appended copies of the pair follow it.  Step c's flags become its
+    // survivor flags; later steps still read the append flags of steps > c.
+    for (uint32_t c = 0; c < steps; ++c) {
+        const EdgeLane e = edge_lane(S, c * 64 + lane, edges);
         const uint64_t A =
             uint64_t(S.edge_bits[2 * c]) | (uint64_t(S.edge_bits[2 * c + 1]) << 32);
-        const int keep = e.valid ? iabs(int(S.key_sum[e.key])) : 0;
-        const bool survive = ((A >> lane) & 1ull) && __popcll(e.same & A & ~upto) < keep;
-        const uint64_t K = __ballot(survive);
-        if (e.valid && (e.same & below) == 0) {
-            const int left = keep - __popcll(e.same & A);
-            S.key_sum[e.key] = int16_t(left > 0 ? left : 0);
+        const uint64_t up = __ballot(e.valid && e.delta > 0);
+        int total = 2 * __popcll(e.same & up) - __popcll(e.same);
+        int after = __popcll(e.same & A & ~upto);
+        for (uint32_t z = 0; z < steps; ++z) {
+            if (z == c)
+                continue;
+            const EdgeLane f = edge_fields(S, z * 64 + lane, edges);
+            const uint64_t m = match_other<kEdgeKeyBits>(e.key, f.key, __ballot(f.valid));
+            const uint64_t fup = __ballot(f.valid && f.delta > 0);
+            total += 2 * __popcll(m & fup) - __popcll(m);
+            if (z > c) {
+                const uint64_t Az =
+                    uint64_t(S.edge_bits[2 * z]) | (uint64_t(S.edge_bits[2 * z + 1]) << 32);
+                after += __popcll(m & Az);
+            }
         }
+        const bool survive = ((A >> lane) & 1ull) && after < iabs(total);
+        const uint64_t K = __ballot(survive);
+        wave_sync();
         if (lane == 0) {
             S.edge_bits[2 * c] = uint32_t(K);
             S.edge_bits[2 * c + 1] = uint32_t(K >> 32);
@@ -506,13 +510,11 @@ __device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges, bool& 
 }
 
 // Returns 1 converged, 0 failed (the reference's failure), -1 storage bound
-// exceeded.  `normal`/`depth` are uniform across the wave.  `keys_ready`
-// tracks whether S.key_sum holds zeros (see horizon_wave).
+// exceeded.  `normal`/`depth` are uniform across the wave.
 __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const pose& ta,
                         const WaveHull& ha, const Shape& sb, const pose& tb, const WaveHull& hb,
                         const f3 simplex[4], uint32_t iterations, float tol, f3& normal,
-                        float& depth, bool& keys_ready, uint32_t* face_out = nullptr,
-                        uint32_t* work = nullptr)
+                        float& depth, uint32_t* face_out = nullptr, uint32_t* work = nullptr)
 {
     const uint32_t lane = lane_id();
     normal = {0.0f, 0.0f, 0.0f};
@@ -608,7 +610,7 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
             visible += (uint32_t)__popcll(vis[ch]);
         }
         wave_sync();
-        const int horizon = horizon_wave(S, 3 * visible, keys_ready);
+        const int horizon = horizon_wave(S, 3 * visible);
         if (horizon < 0)
             { if (work) *work += it + 1; return -1; }
         const uint32_t ne = uint32_t(horizon);
@@ -1048,7 +1050,7 @@ __device__ __forceinline__ void epa_partial_add(uint64_t* slot, uint32_t v)
                   (unsigned long long)v);
 }
 
-__global__ __launch_bounds__(kEpaBlock) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(kEpaBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
            const uint32_t* __restrict__ list, const float* __restrict__ simplex_in,
            physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
@@ -1057,7 +1059,6 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
     __shared__ EpaShared S;
     const uint32_t lane = lane_id();
     const uint32_t total = c->class_count[3] + c->class_count[4] + c->class_count[5];
-    bool keys_ready = false;
     uint32_t contacts3 = 0, contacts4 = 0, contacts5 = 0;
     uint32_t failures = 0, overflows = 0;
     uint32_t work = 0;  // EPA iterations of this wavefront so far
@@ -1081,7 +1082,7 @@ void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
         float depth;
         const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
         const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, nullptr, opt.epa_max_iterations,
-                                opt.epa_tolerance, n, depth, keys_ready, nullptr, &work);
+                                opt.epa_tolerance, n, depth, nullptr, &work);
         if (st == 1) {
             const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
                                    support_wave(B, sb, tb, neg(n), hb)),
@@ -1189,7 +1190,6 @@ void k_hull_adjoint(Bodies B, const physgpu_manifold* __restrict__ man, const Co
     __shared__ EpaShared shared[kEpaWavesPerBlock];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     EpaShared& S = shared[wave];
-    bool keys_ready = false;
     const uint32_t count = min(m, c->manifold_count);
     const uint32_t stride = gridDim.x * kEpaWavesPerBlock;
     uint32_t skip = 0;
@@ -1215,7 +1215,7 @@ void k_hull_adjoint(Bodies B, const physgpu_manifold* __restrict__ man, const Co
         float depth;
         uint32_t face = 0;
         const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, sx.p, opt.epa_max_iterations,
-                                opt.epa_tolerance, n, depth, keys_ready, &face);
+                                opt.epa_tolerance, n, depth, &face);
         __builtin_amdgcn_s_setprio(0);
         wave_sync();
         if (st != 1) {

@@ -412,4 +412,18 @@ __device__ __forceinline__ uint64_t match_lanes(uint32_t key, uint64_t active)
     return same;
 }
 
+// Lanes of `active` whose `theirs` equals this lane's `mine` (both below
+// 2^Bits): one ballot per key bit.
+template <uint32_t Bits>
+__device__ __forceinline__ uint64_t match_other(uint32_t mine, uint32_t theirs, uint64_t active)
+{
+    uint64_t same = active;
+#pragma unroll
+    for (uint32_t b = 0; b < Bits; ++b) {
+        const uint64_t m = __ballot((theirs >> b) & 1u);
+        same &= ((mine >> b) & 1u) ? m : ~m;
+    }
+    return same;
+}
+
 }  // namespace pg
