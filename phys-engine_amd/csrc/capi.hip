@@ -101,7 +101,6 @@ void free_all(physgpu_context* c)
                     c->pair_perm, c->sort_scratch, c->sort_scratch_perm,
                     c->seg_lists, c->buckets, c->slots, c->hit, c->tile_sums,
                     c->manifolds, c->body, c->local_pose, c->friction, c->restitution,
-                    c->epa_list, c->epa_simplex,
                     c->counters, c->world_body_pose, c->world_offsets, c->world_of,
                     c->world_pair_off, c->world_manifold_off, c->adjoint_scratch,
                     c->adjoint_skipped, c->partials, c->tree_tight, c->tree_fat, c->tree_rank,
@@ -162,8 +161,6 @@ physgpu_tuning default_tuning()
         t.class_block[c] = kBlock;
         t.class_groups[c] = kStrideGrid;
     }
-    t.gjk_block = kBlock;
-    t.gjk_groups = kStrideGrid;
     t.epa_groups = 65536;  // one wavefront each; at 1M bodies (34k entries) 2048 / 16384 / 65536: 638 / 310 / 263 us
     t.query_block = kBlock;
     t.query_groups = 4096;  // at 1M bodies, 256 x 1024 / 2048 / 4096: 119 / 114 / 98 us
@@ -188,10 +185,6 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
         if (!valid_block(out.class_block[c]) || !valid_groups(out.class_groups[c]))
             return fail(PHYSGPU_INVALID_ARGUMENT, "class kernel shape out of range");
     }
-    if (!out.gjk_block)
-        out.gjk_block = d.gjk_block;
-    if (!out.gjk_groups)
-        out.gjk_groups = d.gjk_groups;
     if (!out.epa_groups)
         out.epa_groups = d.epa_groups;
     if (!out.query_block)
@@ -200,8 +193,7 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
         out.query_groups = d.query_groups;
     if (!out.sort_items)
         out.sort_items = d.sort_items;
-    if (!valid_block(out.gjk_block) || !valid_groups(out.gjk_groups) ||
-        !valid_epa_groups(out.epa_groups) || !valid_block(out.query_block) ||
+    if (!valid_epa_groups(out.epa_groups) || !valid_block(out.query_block) ||
         !valid_groups(out.query_groups))
         return fail(PHYSGPU_INVALID_ARGUMENT, "kernel shape out of range");
     if (out.sort_items != 4 && out.sort_items != 8 && out.sort_items != 16)
@@ -451,8 +443,6 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->hit, P));
     A(alloc(c->tile_sums, div_up(P, kScanTile) + 1));
     A(alloc(c->manifolds, P));
-    A(alloc(c->epa_list, P));
-    A(alloc(c->epa_simplex, 12 * P));
     A(alloc(c->body, N));
     A(alloc(c->local_pose, 7 * N));
     A(alloc(c->friction, N));
@@ -953,8 +943,6 @@ physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t re
     static const char* class_names[3] = {"np_sphere_sphere", "np_sphere_box", "np_box_box"};
     for (int c = 0; c < 3 && st == PHYSGPU_OK; ++c)
         st = tune_shape(&ctx->tuning.class_block[c], &ctx->tuning.class_groups[c], class_names[c]);
-    if (st == PHYSGPU_OK && ctx->has_hulls)
-        st = tune_shape(&ctx->tuning.gjk_block, &ctx->tuning.gjk_groups, "np_gjk");
     if (st == PHYSGPU_OK && ctx->has_hulls) {
         static const uint32_t epa_groups[] = {4096, 8192, 16384, 32768, 65536};
         uint32_t best_g = ctx->tuning.epa_groups;
@@ -962,7 +950,7 @@ physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t re
         for (uint32_t g : epa_groups) {
             ctx->tuning.epa_groups = g;
             double ms = 0.0;
-            st = measure({"np_epa"}, ms);
+            st = measure({"np_gjk_epa"}, ms);
             if (st != PHYSGPU_OK)
                 break;
             if (ms < best) {

@@ -215,8 +215,6 @@ struct physgpu_context {
     physgpu_manifold* slots = nullptr; // one record per candidate
     uint32_t* hit = nullptr;           // per candidate
     uint32_t* tile_sums = nullptr;     // compaction
-    uint32_t* epa_list = nullptr;      // EPA work: candidate | class << 29
-    float* epa_simplex = nullptr;      // 12 floats per EPA work item
     physgpu_manifold* manifolds = nullptr;  // compacted, candidate order
     physgpu_narrowphase_options np_options{};
 

@@ -31,6 +31,11 @@ constexpr uint32_t kEpaMaxIterations = PHYSGPU_EPA_MAX_ITERATIONS;
 constexpr uint32_t kMaxVertices = 4 + kEpaMaxIterations;
 constexpr uint32_t kMaxFaces = PHYSGPU_EPA_MAX_FACES;
 constexpr uint32_t kMaxEdges = PHYSGPU_EPA_MAX_HORIZON;
+// EPA's partial slots: contacts of classes 3..5, failures, overflows.
+constexpr uint32_t kEpaPartialFirst = kSlotContacts + 3;
+constexpr uint32_t kEpaPartialSlots = 5;
+static_assert(kSlotGjkFailures == kEpaPartialFirst + 3 && kSlotEpaOverflows == kEpaPartialFirst + 4,
+              "EPA partial slots are contiguous");
 
 struct Shape {
     uint32_t type;
@@ -688,8 +693,13 @@ __global__ __launch_bounds__(kClassifyThreads) void k_classify(Bodies B, World W
                                                                Counters* c,
                                                                uint32_t* __restrict__ buckets,
                                                                uint64_t cap,
-                                                               uint32_t* __restrict__ hit)
+                                                               uint32_t* __restrict__ hit,
+                                                               uint64_t* __restrict__ epa_partials)
 {
+    // The EPA kernels that follow add their counts into these partial slots.
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kEpaPartialSlots * kMaxPartialBlocks;
+         i += gridDim.x * blockDim.x)
+        epa_partials[i] = 0;
     constexpr uint32_t kWavesPerBlock = kClassifyThreads / 64;
     __shared__ uint32_t wave_count[kWavesPerBlock][kClasses];
     __shared__ uint32_t block_base[kClasses];
@@ -858,20 +868,12 @@ __device__ __forceinline__ Shape load_shape(const Bodies& B, uint32_t i)
     return s;
 }
 
-// Hull pairs, step 1: GJK (intersectGjk, gjk.cpp:128-158) with 16 lanes per
-// pair (one 16-lane DPP row).  Each lane holds one point of each hull, so a
-// hull support map is one dot product and a 4-step row arg-max (first maximum
-// in index order, as the sequential strict-> scan) instead of a loop of
-// dependent loads; the simplex logic runs replicated in the row's lanes.
-// Hull pair t's result goes to slot t of the EPA work list: candidate |
-// class << 29 for a hit (with its final simplex), kNoEpa otherwise.
+// GJK (intersectGjk, gjk.cpp:128-158) on 16 lanes (one 16-lane DPP row).
+// Each lane holds one point of each hull, so a hull support map is one dot
+// product and a 4-step row arg-max (first maximum in index order, as the
+// sequential strict-> scan) instead of a loop of dependent loads; the simplex
+// logic runs replicated in the row's lanes.
 constexpr uint32_t kGjkLanes = 16;
-constexpr uint32_t kNoEpa = 0xFFFFFFFFu;
-// EPA's partial slots: contacts of classes 3..5, failures, overflows.
-constexpr uint32_t kEpaPartialFirst = kSlotContacts + 3;
-constexpr uint32_t kEpaPartialSlots = 5;
-static_assert(kSlotGjkFailures == kEpaPartialFirst + 3 && kSlotEpaOverflows == kEpaPartialFirst + 4,
-              "EPA partial slots are contiguous");
 
 // A hull's points (at most 16) or a box's 8 corners held one per row lane,
 // in collider space (p) and in world space (w = to_world(p): the function the
@@ -950,97 +952,18 @@ __device__ f3 support_row(const Bodies& B, const Shape& s, const pose& t, f3 d, 
     return to_world(f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]}, t);
 }
 
-__global__ __launch_bounds__(kBlock) void k_gjk(Bodies B, const uint64_t* __restrict__ pairs,
-                                                Counters* c, const uint32_t* __restrict__ buckets,
-                                                uint64_t cap, physgpu_narrowphase_options opt,
-                                                uint32_t* __restrict__ list,
-                                                float* __restrict__ simplex_out,
-                                                uint64_t* __restrict__ epa_partials)
-{
-    // The EPA kernel that follows adds its counts into these partial slots.
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kEpaPartialSlots * kMaxPartialBlocks;
-         i += gridDim.x * blockDim.x)
-        epa_partials[i] = 0;
-    const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
-    const uint32_t total = c3 + c4 + c->class_count[5];
-    const uint32_t rows = gridDim.x * blockDim.x / kGjkLanes;
-    const uint32_t rounds = (total + rows - 1) / rows;
-    const uint32_t lane = lane_id(), rl = lane % kGjkLanes, row_base = lane - rl;
-    for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t t = r * rows + (blockIdx.x * blockDim.x + threadIdx.x) / kGjkLanes;
-        // Rows past the end run along with dummy data so the row reductions
-        // stay converged; they never report a hit.
-        bool done = t >= total;
-        uint32_t entry = 0, a = 0, b = 0;
-        if (!done) {
-            uint32_t cls = 3, q = t;
-            if (q >= c3 + c4) {
-                cls = 5;
-                q -= c3 + c4;
-            } else if (q >= c3) {
-                cls = 4;
-                q -= c3;
-            }
-            const uint32_t k = buckets[cls * cap + q];
-            const uint64_t pr = pairs[k];
-            a = uint32_t(pr >> 32);
-            b = uint32_t(pr);
-            entry = k | (cls << 29);
-        }
-        const Shape sa = load_shape(B, a), sb = load_shape(B, b);
-        const pose ta = load_pose(B, a), tb = load_pose(B, b);
-        const RowHull ha = row_hull(B, sa, ta, rl), hb = row_hull(B, sb, tb, rl);
-        Simplex s;
-        s.size = 0;
-        f3 dir{1.0f, 0.0f, 0.0f};
-        bool intersect = false;
-        for (uint32_t it = 0; it < opt.gjk_max_iterations; ++it) {
-            if (__ballot(!done) == 0)
-                break;
-            done = done || dot(dir, dir) < 1e-12f;
-            const f3 pa = support_row(B, sa, ta, dir, ha, rl, row_base);
-            const f3 pb = support_row(B, sb, tb, neg(dir), hb, rl, row_base);
-            if (!done) {
-                const f3 p = sub(pa, pb);
-                if (dot(p, dir) <= 0.0f) {
-                    done = true;
-                } else {
-                    for (uint32_t k = min(s.size, 3u); k > 0; --k)
-                        s.p[k] = s.p[k - 1];
-                    s.p[0] = p;
-                    s.size = min(s.size + 1, 4u);
-                    if (gjk_contains(s, dir)) {
-                        intersect = true;
-                        done = true;
-                    }
-                }
-            }
-        }
-        // The row's first lane records the result in the pair's own slot:
-        // the EPA entry (or kNoEpa) and, for a hit, the final simplex.
-        if (rl == 0 && t < total) {
-            list[t] = intersect ? entry : kNoEpa;
-            if (intersect) {
-                float* o = simplex_out + size_t(t) * 12;
-                for (int p = 0; p < 4; ++p) {
-                    o[3 * p] = s.p[p].x;
-                    o[3 * p + 1] = s.p[p].y;
-                    o[3 * p + 2] = s.p[p].z;
-                }
-            }
-        }
-    }
-}
-
-// Hull pairs, step 2: EPA, one wavefront per intersecting pair; the contact
-// point is the midpoint of the two deepest support points along the EPA
-// normal (A toward B).  Each workgroup is one wavefront and takes list entry
-// blockIdx.x (then every gridDim.x-th after it): with at least as many
-// workgroups as entries the dispatcher hands out the entries in list order
-// as wavefront slots free up, which keeps long sphere-hull runs (the list
-// starts with the sphere-hull class) from queueing behind one another.
-// Counts go to the EPA partial slots (zeroed by k_gjk) with one atomic add per
-// nonzero count, spread over kMaxPartialBlocks addresses.
+// Hull pairs in one wavefront each: GJK (the row code above, the four rows
+// on the same pair, so every lane holds the same result) and, for an
+// intersecting pair, EPA; the contact point is the midpoint of the two
+// deepest support points along the EPA normal (A toward B).  Each workgroup
+// is one wavefront and takes hull candidate blockIdx.x (then every
+// gridDim.x-th after it): with at least as many workgroups as candidates the
+// dispatcher hands them out in order as wavefront slots free up, a work queue
+// with no atomics.  The sphere-hull bucket comes first, so its long GJK + EPA
+// runs start first, and a pair's EPA follows its own GJK with no grid-wide
+// wait in between.  Counts go to the EPA partial slots (zeroed by
+// k_classify) with one atomic add per nonzero count, spread over
+// kMaxPartialBlocks addresses.
 constexpr uint32_t kEpaBlock = 64;
 
 __device__ __forceinline__ void epa_partial_add(uint64_t* slot, uint32_t v)
@@ -1051,32 +974,71 @@ __device__ __forceinline__ void epa_partial_add(uint64_t* slot, uint32_t v)
 }
 
 __global__ __launch_bounds__(kEpaBlock) __attribute__((amdgpu_waves_per_eu(4)))
-void k_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
-           const uint32_t* __restrict__ list, const float* __restrict__ simplex_in,
-           physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
-           uint32_t* __restrict__ hit, uint64_t* __restrict__ partials)
+void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
+               const uint32_t* __restrict__ buckets, uint64_t cap,
+               physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
+               uint32_t* __restrict__ hit, uint64_t* __restrict__ partials)
 {
     __shared__ EpaShared S;
-    const uint32_t lane = lane_id();
-    const uint32_t total = c->class_count[3] + c->class_count[4] + c->class_count[5];
+    const uint32_t lane = lane_id(), rl = lane % kGjkLanes, row_base = lane - rl;
+    const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
+    const uint32_t total = c3 + c4 + c->class_count[5];
     uint32_t contacts3 = 0, contacts4 = 0, contacts5 = 0;
     uint32_t failures = 0, overflows = 0;
     uint32_t work = 0;  // EPA iterations of this wavefront so far
-    for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
-        const uint32_t entry = list[e];
-        if (entry == kNoEpa)
-            continue;
-        const uint32_t k = entry & 0x1FFFFFFFu, cls = entry >> 29;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        uint32_t cls = 3, q = t;
+        if (q >= c3 + c4) {
+            cls = 5;
+            q -= c3 + c4;
+        } else if (q >= c3) {
+            cls = 4;
+            q -= c3;
+        }
+        const uint32_t k = buckets[cls * cap + q];
         const uint64_t pr = pairs[k];
         const uint32_t a = uint32_t(pr >> 32), b = uint32_t(pr);
         const Shape sa = load_shape(B, a), sb = load_shape(B, b);
         const pose ta = load_pose(B, a), tb = load_pose(B, b);
-        // Simplex vertex p straight from memory into the polytope's LDS.
-        if (lane < 4) {
-            const float* si = simplex_in + size_t(e) * 12 + 3 * lane;
-            S.vx[lane] = si[0];
-            S.vy[lane] = si[1];
-            S.vz[lane] = si[2];
+        Simplex sx;
+        bool intersect = false;
+        {
+            const RowHull ra = row_hull(B, sa, ta, rl), rb = row_hull(B, sb, tb, rl);
+            sx.size = 0;
+            f3 dir{1.0f, 0.0f, 0.0f};
+            for (uint32_t it = 0; it < opt.gjk_max_iterations; ++it) {
+                if (dot(dir, dir) < 1e-12f)
+                    break;
+                const f3 pa = support_row(B, sa, ta, dir, ra, rl, row_base);
+                const f3 pb = support_row(B, sb, tb, neg(dir), rb, rl, row_base);
+                const f3 p = sub(pa, pb);
+                if (dot(p, dir) <= 0.0f)
+                    break;
+                // Shift the simplex up by one (constant indices keep it in
+                // registers).
+                if (sx.size >= 3)
+                    sx.p[3] = sx.p[2];
+                if (sx.size >= 2)
+                    sx.p[2] = sx.p[1];
+                if (sx.size >= 1)
+                    sx.p[1] = sx.p[0];
+                sx.p[0] = p;
+                sx.size = min(sx.size + 1, 4u);
+                if (gjk_contains(sx, dir)) {
+                    intersect = true;
+                    break;
+                }
+            }
+        }
+        if (!intersect)
+            continue;
+        if (lane == 0) {
+#pragma unroll
+            for (uint32_t v = 0; v < 4; ++v) {
+                S.vx[v] = sx.p[v].x;
+                S.vy[v] = sx.p[v].y;
+                S.vz[v] = sx.p[v].z;
+            }
         }
         f3 n;
         float depth;
@@ -1488,23 +1450,22 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
     }
     time_begin(ctx, "np_classify", t);
     hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0, ctx->stream, B,
-                       W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit);
+                       W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit,
+                       ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks);
     time_end(ctx, t);
-    // With hulls, the SS / SB / BB kernels run on the side stream while EPA
-    // runs here: EPA's long sphere-hull runs leave most SIMDs idle for much
-    // of its duration.
+    // With hulls, GJK + EPA run here while the SS / SB / BB kernels run on the
+    // side stream.
     const bool fork = ctx->has_hulls;
     hipStream_t cs = ctx->stream;
     if (fork) {
-        time_begin(ctx, "np_gjk", t);
-        hipLaunchKernelGGL(k_gjk, dim3(tu.gjk_groups), dim3(tu.gjk_block), 0, ctx->stream, B,
-                           pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->epa_list,
-                           ctx->epa_simplex,
-                           ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks);
-        time_end(ctx, t);
         hipEventRecord(ctx->fork_event, ctx->stream);
         hipStreamWaitEvent(ctx->side_stream, ctx->fork_event, 0);
         cs = ctx->side_stream;
+        time_begin(ctx, "np_gjk_epa", t);
+        hipLaunchKernelGGL(k_gjk_epa, dim3(tu.epa_groups), dim3(kEpaBlock), 0, ctx->stream, B,
+                           W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
+                           ctx->hit, ctx->partials);
+        time_end(ctx, t);
     }
     time_begin(ctx, "np_sphere_sphere", t, cs);
     hipLaunchKernelGGL(k_class<0>, dim3(tu.class_groups[0]), dim3(tu.class_block[0]), 0, cs, B,
@@ -1521,13 +1482,6 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
                        W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
                        ctx->hit, ctx->partials + size_t(kSlotContacts + 2) * kMaxPartialBlocks);
     time_end(ctx, t);
-    if (fork) {
-        time_begin(ctx, "np_epa", t);
-        hipLaunchKernelGGL(k_epa, dim3(tu.epa_groups), dim3(kEpaBlock), 0, ctx->stream, B, W,
-                           pairs, ctx->counters, ctx->epa_list, ctx->epa_simplex, opt,
-                           ctx->slots, ctx->hit, ctx->partials);
-        time_end(ctx, t);
-    }
     if (fork || sort_on_side) {
         hipEventRecord(ctx->join_event, ctx->side_stream);
         hipStreamWaitEvent(ctx->stream, ctx->join_event, 0);

@@ -152,8 +152,6 @@ class Tuning(C.Structure):
     _fields_ = [
         ("class_block", C.c_uint32 * 3),
         ("class_groups", C.c_uint32 * 3),
-        ("gjk_block", C.c_uint32),
-        ("gjk_groups", C.c_uint32),
         ("epa_groups", C.c_uint32),
         ("query_block", C.c_uint32),
         ("query_groups", C.c_uint32),
