@@ -35,25 +35,23 @@ __global__ void k_step_begin(Counters* c)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_compute_aabbs(
-    uint32_t n, const float* __restrict__ pose, const uint8_t* __restrict__ type,
-    const float* __restrict__ sdata, const uint32_t* __restrict__ hrange,
-    const float* __restrict__ hpts, uint32_t nh, const uint32_t* __restrict__ world_of,
-    float4* __restrict__ aabb)
+// AABB of sphere or box i (fromCollider, aabb.cpp); false for a hull.
+__device__ __forceinline__ bool body_aabb(uint32_t i, uint32_t n, const float* __restrict__ pose,
+                                          const uint8_t* __restrict__ type,
+                                          const float* __restrict__ sdata, float b[6])
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
+    const uint8_t ty = type[i];
+    if (ty != PHYSGPU_SPHERE && ty != PHYSGPU_BOX)
+        return false;
     pg::pose t{{pose[i], pose[n + i], pose[2 * n + i]},
                {pose[3 * n + i], pose[4 * n + i], pose[5 * n + i], pose[6 * n + i]}};
-    const uint8_t ty = type[i];
     f3 mn, mx;
     if (ty == PHYSGPU_SPHERE) {
         const float r = sdata[i];
         f3 rr{r, r, r};
         mn = sub(t.p, rr);
         mx = add(t.p, rr);
-    } else if (ty == PHYSGPU_BOX) {
+    } else {
         const f3 h{sdata[i], sdata[n + i], sdata[2 * n + i]};
         const m3 m = to_mat(t.q);
         f3 e{fabsf(m.m00) * h.x + fabsf(m.m01) * h.y + fabsf(m.m02) * h.z,
@@ -61,11 +59,25 @@ __global__ __launch_bounds__(kBlock) void k_compute_aabbs(
              fabsf(m.m20) * h.x + fabsf(m.m21) * h.y + fabsf(m.m22) * h.z};
         mn = sub(t.p, e);
         mx = add(t.p, e);
-    } else {
-        return;  // hulls: k_hull_aabbs
     }
-    const float b[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
-    store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
+    b[0] = mn.x;
+    b[1] = mn.y;
+    b[2] = mn.z;
+    b[3] = mx.x;
+    b[4] = mx.y;
+    b[5] = mx.z;
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_compute_aabbs(
+    uint32_t n, const float* __restrict__ pose, const uint8_t* __restrict__ type,
+    const float* __restrict__ sdata, const uint32_t* __restrict__ world_of,
+    float4* __restrict__ aabb)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float b[6];
+    if (i < n && body_aabb(i, n, pose, type, sdata, b))  // hulls: k_hull_aabbs
+        store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t count, uint32_t value)
@@ -106,10 +118,72 @@ __device__ __forceinline__ bool row_first_extreme(float& v, uint32_t count, uint
     return row_arg_best_lanes<Max>(v, i, 1, row_base);
 }
 
-// Hull AABBs (min / max of the transformed points) with one 16-lane row per
-// hull: each lane transforms points rl, rl + 16, ... and keeps their first
-// extremes, then each component is reduced over the row.  A NaN coordinate
-// sends the hull to the sequential scan on the row's first lane.
+// AABB (min / max of the transformed points) of hull i with one 16-lane row
+// (all 16 lanes call it): each lane transforms points rl, rl + 16, ... and
+// keeps their first extremes, then each component is reduced over the row.
+// A NaN coordinate, or more than 16 points, sends the hull to the sequential
+// scan on the row's first lane.  The bounds are valid in lane rl == 0.
+constexpr uint32_t kHullRowLanes = 16;
+
+__device__ __forceinline__ void hull_row_aabb(uint32_t i, uint32_t n, const float* __restrict__ pose,
+                                              const uint32_t* __restrict__ hrange,
+                                              const float* __restrict__ hpts, uint32_t nh,
+                                              uint32_t rl, uint32_t row_base, float b[6])
+{
+    const pg::pose t{{pose[i], pose[n + i], pose[2 * n + i]},
+                     {pose[3 * n + i], pose[4 * n + i], pose[5 * n + i], pose[6 * n + i]}};
+    const uint32_t first = hrange[i], count = hrange[n + i];
+    f3 mn{0.0f, 0.0f, 0.0f}, mx{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = rl; k < count; k += kHullRowLanes) {
+        const uint32_t p = first + k;
+        const f3 w = to_world(f3{hpts[p], hpts[nh + p], hpts[2 * nh + p]}, t);
+        if (k == rl) {
+            mn = w;
+            mx = w;
+        } else {
+            mn = {min_ref(mn.x, w.x), min_ref(mn.y, w.y), min_ref(mn.z, w.z)};
+            mx = {max_ref(mx.x, w.x), max_ref(mx.y, w.y), max_ref(mx.z, w.z)};
+        }
+    }
+    // Chunk order within a lane is index order, so a lane's first extreme
+    // precedes every later equal value of that lane; across lanes the row
+    // reduction takes the lowest lane, i.e. the lowest index among the
+    // lanes' first extremes only when count <= 16.  Longer hulls take
+    // the sequential scan.
+    bool ok = count <= kHullRowLanes;
+    b[0] = mn.x;
+    b[1] = mn.y;
+    b[2] = mn.z;
+    b[3] = mx.x;
+    b[4] = mx.y;
+    b[5] = mx.z;
+    if (ok) {
+        ok = row_first_extreme<false>(b[0], count, rl, row_base) &&
+             row_first_extreme<false>(b[1], count, rl, row_base) &&
+             row_first_extreme<false>(b[2], count, rl, row_base) &&
+             row_first_extreme<true>(b[3], count, rl, row_base) &&
+             row_first_extreme<true>(b[4], count, rl, row_base) &&
+             row_first_extreme<true>(b[5], count, rl, row_base);
+    }
+    if (rl == 0 && !ok) {
+        f3 w = to_world(f3{hpts[first], hpts[nh + first], hpts[2 * nh + first]}, t);
+        f3 a = w, c = w;
+        for (uint32_t k = 1; k < count; ++k) {
+            const uint32_t p = first + k;
+            w = to_world(f3{hpts[p], hpts[nh + p], hpts[2 * nh + p]}, t);
+            a = {min_ref(a.x, w.x), min_ref(a.y, w.y), min_ref(a.z, w.z)};
+            c = {max_ref(c.x, w.x), max_ref(c.y, w.y), max_ref(c.z, w.z)};
+        }
+        b[0] = a.x;
+        b[1] = a.y;
+        b[2] = a.z;
+        b[3] = c.x;
+        b[4] = c.y;
+        b[5] = c.z;
+    }
+}
+
+// Hull AABBs (listed by k_hull_list), one 16-lane row per hull.
 __global__ __launch_bounds__(kBlock) void k_hull_aabbs(const uint32_t* __restrict__ list,
                                                        const uint32_t* __restrict__ list_count,
                                                        uint32_t n, const float* __restrict__ pose,
@@ -118,65 +192,19 @@ __global__ __launch_bounds__(kBlock) void k_hull_aabbs(const uint32_t* __restric
                                                        const uint32_t* __restrict__ world_of,
                                                        float4* __restrict__ aabb)
 {
-    constexpr uint32_t kLanes = 16;
     const uint32_t hulls = *list_count;
-    const uint32_t lane = lane_id(), rl = lane % kLanes, row_base = lane - rl;
-    const uint32_t rows = gridDim.x * blockDim.x / kLanes;
+    const uint32_t lane = lane_id(), rl = lane % kHullRowLanes, row_base = lane - rl;
+    const uint32_t rows = gridDim.x * blockDim.x / kHullRowLanes;
     const uint32_t rounds = (hulls + rows - 1) / rows;
     for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t h = r * rows + (blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+        const uint32_t h = r * rows + (blockIdx.x * blockDim.x + threadIdx.x) / kHullRowLanes;
         if (h >= hulls)
             continue;  // whole rows (h is row-uniform)
         const uint32_t i = list[h];
-        const pg::pose t{{pose[i], pose[n + i], pose[2 * n + i]},
-                         {pose[3 * n + i], pose[4 * n + i], pose[5 * n + i], pose[6 * n + i]}};
-        const uint32_t first = hrange[i], count = hrange[n + i];
-        f3 mn{0.0f, 0.0f, 0.0f}, mx{0.0f, 0.0f, 0.0f};
-        for (uint32_t k = rl; k < count; k += kLanes) {
-            const uint32_t p = first + k;
-            const f3 w = to_world(f3{hpts[p], hpts[nh + p], hpts[2 * nh + p]}, t);
-            if (k == rl) {
-                mn = w;
-                mx = w;
-            } else {
-                mn = {min_ref(mn.x, w.x), min_ref(mn.y, w.y), min_ref(mn.z, w.z)};
-                mx = {max_ref(mx.x, w.x), max_ref(mx.y, w.y), max_ref(mx.z, w.z)};
-            }
-        }
-        // Chunk order within a lane is index order, so a lane's first extreme
-        // precedes every later equal value of that lane; across lanes the row
-        // reduction takes the lowest lane, i.e. the lowest index among the
-        // lanes' first extremes only when count <= 16.  Longer hulls take
-        // the sequential scan.
-        bool ok = count <= kLanes;
-        float b[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
-        if (ok) {
-            ok = row_first_extreme<false>(b[0], count, rl, row_base) &&
-                 row_first_extreme<false>(b[1], count, rl, row_base) &&
-                 row_first_extreme<false>(b[2], count, rl, row_base) &&
-                 row_first_extreme<true>(b[3], count, rl, row_base) &&
-                 row_first_extreme<true>(b[4], count, rl, row_base) &&
-                 row_first_extreme<true>(b[5], count, rl, row_base);
-        }
-        if (rl == 0) {
-            if (!ok) {
-                f3 w = to_world(f3{hpts[first], hpts[nh + first], hpts[2 * nh + first]}, t);
-                f3 a = w, c = w;
-                for (uint32_t k = 1; k < count; ++k) {
-                    const uint32_t p = first + k;
-                    w = to_world(f3{hpts[p], hpts[nh + p], hpts[2 * nh + p]}, t);
-                    a = {min_ref(a.x, w.x), min_ref(a.y, w.y), min_ref(a.z, w.z)};
-                    c = {max_ref(c.x, w.x), max_ref(c.y, w.y), max_ref(c.z, w.z)};
-                }
-                b[0] = a.x;
-                b[1] = a.y;
-                b[2] = a.z;
-                b[3] = c.x;
-                b[4] = c.y;
-                b[5] = c.z;
-            }
+        float b[6];
+        hull_row_aabb(i, n, pose, hrange, hpts, nh, rl, row_base, b);
+        if (rl == 0)
             store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
-        }
     }
 }
 
@@ -199,18 +227,19 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __restrict__ aabb,
-                                                   uint64_t* __restrict__ partials)
-{
+// Statistics behind the automatic grid cell: largest x side and largest side
+// (float keys), sum of largest sides (2^-16 m fixed point, clamped), centre
+// min / max per axis (float keys).  Every reduction is order-independent.
+struct WidthStats {
     uint32_t wx = 0, wall = 0;
     uint32_t cmn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, cmx[3] = {0u, 0u, 0u};
     unsigned long long wsum = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += gridDim.x * blockDim.x) {
+
+    __device__ __forceinline__ void add(const float b[6])
+    {
         float w[3];
-        const Rec r = load_rec(aabb, i);
         for (int a = 0; a < 3; ++a) {
-            const float lo = r.b[a], hi = r.b[3 + a];
+            const float lo = b[a], hi = b[3 + a];
             w[a] = hi - lo;
             // Centre as lo/2 + hi/2 cannot overflow for finite bounds.
             const uint32_t ck = float_key(lo * 0.5f + hi * 0.5f);
@@ -220,51 +249,106 @@ __global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __r
         const float wm = fmaxf(w[0], fmaxf(w[1], w[2]));
         wx = max(wx, float_key(w[0]));
         wall = max(wall, float_key(wm));
-        // Sum of sides in 2^-16 m fixed point, clamped, for the mean side.
         const float fx = fminf(wm, 65536.0f) * 65536.0f;
         wsum += (unsigned long long)fx;
     }
-    wx = wave_max_u32(wx);
-    wall = wave_max_u32(wall);
-    wsum = wave_sum_u64(wsum);
-    for (int a = 0; a < 3; ++a) {
-        cmn[a] = wave_min_u32(cmn[a]);
-        cmx[a] = wave_max_u32(cmx[a]);
-    }
-    // Workgroup reduction in LDS, then one partial per value and workgroup.
-    __shared__ uint32_t s_u[kBlock / 64][8];
-    __shared__ unsigned long long s_sum[kBlock / 64];
-    const uint32_t wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63u) == 0) {
-        s_u[wave][0] = wx;
-        s_u[wave][1] = wall;
+
+    // Workgroup reduction; thread 0 writes the workgroup's partials.  Every
+    // thread of the workgroup calls it.
+    __device__ void flush(uint64_t* __restrict__ partials)
+    {
+        wx = wave_max_u32(wx);
+        wall = wave_max_u32(wall);
+        wsum = wave_sum_u64(wsum);
         for (int a = 0; a < 3; ++a) {
-            s_u[wave][2 + a] = cmn[a];
-            s_u[wave][5 + a] = cmx[a];
+            cmn[a] = wave_min_u32(cmn[a]);
+            cmx[a] = wave_max_u32(cmx[a]);
         }
-        s_sum[wave] = wsum;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < kBlock / 64; ++w) {
-            s_u[0][0] = max(s_u[0][0], s_u[w][0]);
-            s_u[0][1] = max(s_u[0][1], s_u[w][1]);
+        __shared__ uint32_t s_u[kBlock / 64][8];
+        __shared__ unsigned long long s_sum[kBlock / 64];
+        const uint32_t wave = threadIdx.x >> 6;
+        if ((threadIdx.x & 63u) == 0) {
+            s_u[wave][0] = wx;
+            s_u[wave][1] = wall;
             for (int a = 0; a < 3; ++a) {
-                s_u[0][2 + a] = min(s_u[0][2 + a], s_u[w][2 + a]);
-                s_u[0][5 + a] = max(s_u[0][5 + a], s_u[w][5 + a]);
+                s_u[wave][2 + a] = cmn[a];
+                s_u[wave][5 + a] = cmx[a];
             }
-            s_sum[0] += s_sum[w];
+            s_sum[wave] = wsum;
         }
-        // One partial per value and workgroup (k_fold combines them).
-        uint64_t* P = partials + blockIdx.x;
-        P[size_t(kSlotWidthMaxX) * kMaxPartialBlocks] = s_u[0][0];
-        P[size_t(kSlotWidthMax) * kMaxPartialBlocks] = s_u[0][1];
-        P[size_t(kSlotWidthSum) * kMaxPartialBlocks] = s_sum[0];
-        for (int a = 0; a < 3; ++a) {
-            P[size_t(kSlotCentreMin + a) * kMaxPartialBlocks] = s_u[0][2 + a];
-            P[size_t(kSlotCentreMax + a) * kMaxPartialBlocks] = s_u[0][5 + a];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+                s_u[0][0] = max(s_u[0][0], s_u[w][0]);
+                s_u[0][1] = max(s_u[0][1], s_u[w][1]);
+                for (int a = 0; a < 3; ++a) {
+                    s_u[0][2 + a] = min(s_u[0][2 + a], s_u[w][2 + a]);
+                    s_u[0][5 + a] = max(s_u[0][5 + a], s_u[w][5 + a]);
+                }
+                s_sum[0] += s_sum[w];
+            }
+            // One partial per value and workgroup (k_fold combines them).
+            uint64_t* P = partials + blockIdx.x;
+            P[size_t(kSlotWidthMaxX) * kMaxPartialBlocks] = s_u[0][0];
+            P[size_t(kSlotWidthMax) * kMaxPartialBlocks] = s_u[0][1];
+            P[size_t(kSlotWidthSum) * kMaxPartialBlocks] = s_sum[0];
+            for (int a = 0; a < 3; ++a) {
+                P[size_t(kSlotCentreMin + a) * kMaxPartialBlocks] = s_u[0][2 + a];
+                P[size_t(kSlotCentreMax + a) * kMaxPartialBlocks] = s_u[0][5 + a];
+            }
         }
     }
+};
+
+__global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __restrict__ aabb,
+                                                   uint64_t* __restrict__ partials)
+{
+    WidthStats st;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x) {
+        const Rec r = load_rec(aabb, i);
+        st.add(r.b);
+    }
+    st.flush(partials);
+}
+
+// One launch for a step: AABBs of every body and the width statistics.
+// Workgroups [0, body_groups) take the spheres and boxes (grid-stride), the
+// rest take the hulls, one 16-lane row each; every workgroup writes its
+// partials.
+__global__ __launch_bounds__(kBlock) void k_aabbs_step(
+    uint32_t n, const float* __restrict__ pose, const uint8_t* __restrict__ type,
+    const float* __restrict__ sdata, const uint32_t* __restrict__ hrange,
+    const float* __restrict__ hpts, uint32_t nh, const uint32_t* __restrict__ world_of,
+    const uint32_t* __restrict__ hull_list, const uint32_t* __restrict__ hull_count,
+    uint32_t body_groups, float4* __restrict__ aabb, uint64_t* __restrict__ partials)
+{
+    WidthStats st;
+    if (blockIdx.x < body_groups) {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += body_groups * blockDim.x) {
+            float b[6];
+            if (body_aabb(i, n, pose, type, sdata, b)) {
+                store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
+                st.add(b);
+            }
+        }
+    } else {
+        const uint32_t hulls = *hull_count;
+        const uint32_t lane = lane_id(), rl = lane % kHullRowLanes, row_base = lane - rl;
+        const uint32_t rows = (gridDim.x - body_groups) * blockDim.x / kHullRowLanes;
+        const uint32_t row = ((blockIdx.x - body_groups) * blockDim.x + threadIdx.x) / kHullRowLanes;
+        for (uint32_t h = row; h < hulls; h += rows) {
+            const uint32_t i = hull_list[h];
+            float b[6];
+            hull_row_aabb(i, n, pose, hrange, hpts, nh, rl, row_base, b);
+            if (rl == 0) {
+                store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
+                st.add(b);
+            }
+        }
+    }
+    st.flush(partials);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rec_to_aabb(uint32_t n, const float4* __restrict__ rec,
@@ -340,22 +424,59 @@ void launch_step_begin(physgpu_context* ctx)
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->counters);
 }
 
-void launch_compute_aabbs(physgpu_context* ctx)
+namespace {
+
+void fold_width_stats(physgpu_context* ctx, uint32_t groups)
+{
+    FoldList fold{};
+    fold.e[0] = {kSlotWidthMaxX, groups, kFoldMax32, uint32_t(offsetof(Counters, wmax_x_key))};
+    fold.e[1] = {kSlotWidthMax, groups, kFoldMax32, uint32_t(offsetof(Counters, wmax_key))};
+    fold.e[2] = {kSlotWidthSum, groups, kFoldSum64, uint32_t(offsetof(Counters, width_sum_fx))};
+    for (uint32_t a = 0; a < 3; ++a) {
+        fold.e[3 + a] = {kSlotCentreMin + a, groups, kFoldMin32,
+                         uint32_t(offsetof(Counters, cmin_key) + 4 * a)};
+        fold.e[6 + a] = {kSlotCentreMax + a, groups, kFoldMax32,
+                         uint32_t(offsetof(Counters, cmax_key) + 4 * a)};
+    }
+    fold.count = 9;
+    launch_fold(ctx, fold);
+}
+
+}  // namespace
+
+void launch_compute_aabbs(physgpu_context* ctx, bool with_stats)
 {
     ctx->resident_aabbs_valid = true;
     if (ctx->n == 0)
         return;
     Timer t;
     time_begin(ctx, "aabb_from_pose", t);
+    const uint32_t* world_of = ctx->n_worlds ? ctx->world_of : nullptr;
+    if (with_stats) {
+        // Spheres and boxes: up to 2,048 workgroups; hulls: the rest of the
+        // partial slots, one row per hull and round.
+        const uint32_t body_groups = std::min<uint32_t>(div_up(ctx->n, kBlock), 2048);
+        const uint32_t hull_groups =
+            ctx->has_hulls ? std::min<uint32_t>(div_up(16ull * ctx->n, kBlock),
+                                                kMaxPartialBlocks - body_groups)
+                           : 0u;
+        hipLaunchKernelGGL(k_aabbs_step, dim3(body_groups + hull_groups), dim3(kBlock), 0,
+                           ctx->stream, ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data,
+                           ctx->hull_range, ctx->hull_points, ctx->n_hull_points, world_of,
+                           ctx->hull_list, ctx->hull_list_count, body_groups, ctx->aabb,
+                           ctx->partials);
+        fold_width_stats(ctx, body_groups + hull_groups);
+        time_end(ctx, t);
+        return;
+    }
     hipLaunchKernelGGL(k_compute_aabbs, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0,
                        ctx->stream, ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data,
-                       ctx->hull_range, ctx->hull_points, ctx->n_hull_points,
-                       ctx->n_worlds ? ctx->world_of : nullptr, ctx->aabb);
+                       world_of, ctx->aabb);
     if (ctx->has_hulls)
         hipLaunchKernelGGL(k_hull_aabbs, dim3(stride_grid(16ull * ctx->n)), dim3(kBlock), 0,
                            ctx->stream, ctx->hull_list, ctx->hull_list_count, ctx->n, ctx->pose,
-                           ctx->hull_range, ctx->hull_points, ctx->n_hull_points,
-                           ctx->n_worlds ? ctx->world_of : nullptr, ctx->aabb);
+                           ctx->hull_range, ctx->hull_points, ctx->n_hull_points, world_of,
+                           ctx->aabb);
     time_end(ctx, t);
 }
 
@@ -378,18 +499,7 @@ void launch_aabb_reductions(physgpu_context* ctx)
     const uint32_t grid = std::min<uint32_t>(div_up(ctx->n, kBlock), 1024);
     hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->aabb,
                        ctx->partials);
-    FoldList fold{};
-    fold.e[0] = {kSlotWidthMaxX, grid, kFoldMax32, uint32_t(offsetof(Counters, wmax_x_key))};
-    fold.e[1] = {kSlotWidthMax, grid, kFoldMax32, uint32_t(offsetof(Counters, wmax_key))};
-    fold.e[2] = {kSlotWidthSum, grid, kFoldSum64, uint32_t(offsetof(Counters, width_sum_fx))};
-    for (uint32_t a = 0; a < 3; ++a) {
-        fold.e[3 + a] = {kSlotCentreMin + a, grid, kFoldMin32,
-                         uint32_t(offsetof(Counters, cmin_key) + 4 * a)};
-        fold.e[6 + a] = {kSlotCentreMax + a, grid, kFoldMax32,
-                         uint32_t(offsetof(Counters, cmax_key) + 4 * a)};
-    }
-    fold.count = 9;
-    launch_fold(ctx, fold);
+    fold_width_stats(ctx, grid);
     time_end(ctx, t);
 }
 

@@ -205,11 +205,14 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
 // the emitted pairs are left unordered (pairs_raw) for a narrow phase that
 // sorts them on its side stream.
 void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool window_stats,
-                    bool defer_sort = false)
+                    bool defer_sort = false, bool stats_done = false)
 {
     ctx->tree_build_current = false;
     ctx->tree_pairs_loaded = false;
-    launch_aabb_reductions(ctx);
+    if (!stats_done)
+        launch_aabb_reductions(ctx);
+    // The broad phase's closing fold (if any) also finishes the pair count.
+    ctx->finish_pending = ctx->n >= 2;
     if (ctx->n >= 2) {
         if (algorithm == PHYSGPU_SWEEP_AND_PRUNE) {
             launch_sap_keys(ctx);
@@ -225,10 +228,11 @@ void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool windo
             launch_lbvh(ctx);
         }
     }
-    if (defer_sort)
+    if (ctx->n < 2 || ctx->finish_pending)
         launch_finish_pairs(ctx);
-    else
-        launch_canonical_pairs(ctx);
+    ctx->finish_pending = false;
+    if (!defer_sort)
+        launch_sort_pairs(ctx, ctx->stream);
     ctx->last_algorithm = algorithm;
 }
 
@@ -305,8 +309,8 @@ void enqueue_step(physgpu_context* ctx, int algorithm, double cell)
     launch_step_begin(ctx);
     if (ctx->has_world)
         launch_compose_world(ctx);
-    launch_compute_aabbs(ctx);
-    run_broadphase(ctx, algorithm, cell, false, true);
+    launch_compute_aabbs(ctx, true);
+    run_broadphase(ctx, algorithm, cell, false, true, true);
     ctx->np_pairs = ctx->pairs_raw;
     ctx->np_perm = ctx->pair_perm;
     launch_narrowphase(ctx, ctx->has_world, true);

@@ -8,10 +8,21 @@
 namespace pg {
 namespace {
 
-__global__ __launch_bounds__(256) void k_fold(const uint64_t* __restrict__ partials, FoldList list,
-                                              Counters* counters)
+constexpr uint32_t kFoldThreads = 1024;  // up to 8 partials per thread
+
+__global__ __launch_bounds__(kFoldThreads) void k_fold(const uint64_t* __restrict__ partials,
+                                                       FoldList list, Counters* counters)
 {
-    __shared__ uint64_t part[256 / 64];
+    __shared__ uint64_t part[kFoldThreads / 64];
+    if (list.finish_pairs && blockIdx.x == 0 && threadIdx.x == 0) {
+        // launch_finish_pairs' work, after the broad phase's last pair.
+        const unsigned long long p = counters->pair_count;
+        counters->sort_count = uint32_t(p < list.finish_cap ? p : list.finish_cap);
+        counters->mid_segments = 0;
+        counters->long_segments = 0;
+    }
+    if (blockIdx.x >= list.count)
+        return;
     const FoldEntry e = list.e[blockIdx.x];
     const uint64_t* src = partials + size_t(e.slot) * kMaxPartialBlocks;
     const bool is_min = e.op == kFoldMin32, is_max = e.op == kFoldMax32;
@@ -45,11 +56,17 @@ __global__ __launch_bounds__(256) void k_fold(const uint64_t* __restrict__ parti
 
 }  // namespace
 
-void launch_fold(physgpu_context* ctx, const FoldList& list)
+void launch_fold(physgpu_context* ctx, const FoldList& in)
 {
-    if (list.count)
-        hipLaunchKernelGGL(k_fold, dim3(list.count), dim3(256), 0, ctx->stream, ctx->partials,
-                           list, ctx->counters);
+    FoldList list = in;
+    if (ctx->finish_pending) {
+        list.finish_pairs = 1;
+        list.finish_cap = ctx->cap_p;
+        ctx->finish_pending = false;
+    }
+    if (list.count || list.finish_pairs)
+        hipLaunchKernelGGL(k_fold, dim3(list.count ? list.count : 1), dim3(kFoldThreads), 0, ctx->stream,
+                           ctx->partials, list, ctx->counters);
 }
 
 }  // namespace pg

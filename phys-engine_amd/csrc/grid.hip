@@ -128,11 +128,6 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* 
 }
 
 // Bodies in cells = all bodies minus the overflow ones.
-__global__ void k_grid_entries(uint32_t n, Counters* counters)
-{
-    counters->grid_entries = n - counters->overflow_count;
-}
-
 __global__ __launch_bounds__(kBlock) void k_fill(uint32_t* __restrict__ p, uint64_t count,
                                                  uint32_t value)
 {
@@ -143,12 +138,15 @@ __global__ __launch_bounds__(kBlock) void k_fill(uint32_t* __restrict__ p, uint6
 
 // Cell ranges over the sorted in-grid prefix [0, entries): table[2 key] =
 // first position, table[2 key + 1] = one past the last (0, 0 = empty).
+// In-grid bodies sort before the overflow ones (their keys carry the top bit).
 __global__ __launch_bounds__(kBlock) void k_cell_ranges(uint32_t n, const uint64_t* __restrict__ items,
-                                                        const Counters* counters,
+                                                        Counters* counters,
                                                         uint32_t* __restrict__ table)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t entries = counters->grid_entries;
+    const uint32_t entries = n - counters->overflow_count;
+    if (p == 0)
+        counters->grid_entries = entries;
     if (p >= entries)
         return;
     const uint32_t key = uint32_t(items[p]);
@@ -170,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
 {
     __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
     WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
-    const uint32_t entries = counters->grid_entries;
+    const uint32_t entries = n - counters->overflow_count;
     const float cell = __uint_as_float(counters->cell_bits);
     const double inv = 1.0 / double(cell);
     const uint32_t mask = (1u << bits_per_axis) - 1u;
@@ -293,7 +291,6 @@ void launch_grid(physgpu_context* ctx, double cell_size)
                        ctx->aabb, ctx->counters, float(cell_size), bpa, ctx->items_a,
                        ctx->overflow_list, ctx->grid_flag);
     time_end(ctx, t);
-    hipLaunchKernelGGL(k_grid_entries, dim3(1), dim3(1), 0, ctx->stream, n, ctx->counters);
     uint64_t* sorted = nullptr;
     radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 3 * bpa + 1, 0, 0, &sorted);
     time_begin(ctx, "grid_table_clear", t);

@@ -61,6 +61,8 @@ constexpr uint32_t kMaxFold = 12;
 struct FoldList {
     FoldEntry e[kMaxFold];
     uint32_t count;
+    uint32_t finish_pairs;  // also clamp the pair count to finish_cap (see launch_finish_pairs)
+    uint64_t finish_cap;
 };
 
 // Device-side counters of one collision step.  Zeroed by the step-begin
@@ -204,6 +206,7 @@ struct physgpu_context {
     bool tree_ranks_current = false;
     bool tree_build_current = false;  // last broad phase was this tree's LBVH build
     bool resident_aabbs_valid = true; // ctx->aabb holds the resident colliders' bounds
+    bool finish_pending = false;      // the next fold also finishes the broad phase's pairs
     bool tree_pairs_loaded = false;   // the candidate pairs are a tree query's (rank space)
 
     // Narrow phase.  Its input pairs and, when they are not in canonical
@@ -302,7 +305,9 @@ inline uint32_t stride_grid(uint64_t work)
 // ---- launchers implemented in the kernel translation units -------------
 // aabb.hip
 void launch_step_begin(physgpu_context* ctx);
-void launch_compute_aabbs(physgpu_context* ctx);
+// With with_stats, also the width statistics behind the automatic grid cell
+// (what launch_aabb_reductions computes), in the same launch.
+void launch_compute_aabbs(physgpu_context* ctx, bool with_stats = false);
 void launch_hull_list(physgpu_context* ctx);
 void launch_aabb_reductions(physgpu_context* ctx);
 void launch_convert_aabbs_out(physgpu_context* ctx, physgpu_aabb* dev_out);

@@ -31,9 +31,11 @@ constexpr uint32_t kEpaMaxIterations = PHYSGPU_EPA_MAX_ITERATIONS;
 constexpr uint32_t kMaxVertices = 4 + kEpaMaxIterations;
 constexpr uint32_t kMaxFaces = PHYSGPU_EPA_MAX_FACES;
 constexpr uint32_t kMaxEdges = PHYSGPU_EPA_MAX_HORIZON;
-// EPA's partial slots: contacts of classes 3..5, failures, overflows.
+// EPA's partial slots: contacts of classes 3..5, failures, overflows, written
+// by atomic adds spread over kEpaPartials addresses each.
 constexpr uint32_t kEpaPartialFirst = kSlotContacts + 3;
 constexpr uint32_t kEpaPartialSlots = 5;
+constexpr uint32_t kEpaPartials = 1024;
 static_assert(kSlotGjkFailures == kEpaPartialFirst + 3 && kSlotEpaOverflows == kEpaPartialFirst + 4,
               "EPA partial slots are contiguous");
 
@@ -697,9 +699,9 @@ __global__ __launch_bounds__(kClassifyThreads) void k_classify(Bodies B, World W
                                                                uint64_t* __restrict__ epa_partials)
 {
     // The EPA kernels that follow add their counts into these partial slots.
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kEpaPartialSlots * kMaxPartialBlocks;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kEpaPartialSlots * kEpaPartials;
          i += gridDim.x * blockDim.x)
-        epa_partials[i] = 0;
+        epa_partials[(i / kEpaPartials) * kMaxPartialBlocks + i % kEpaPartials] = 0;
     constexpr uint32_t kWavesPerBlock = kClassifyThreads / 64;
     __shared__ uint32_t wave_count[kWavesPerBlock][kClasses];
     __shared__ uint32_t block_base[kClasses];
@@ -962,14 +964,13 @@ __device__ f3 support_row(const Bodies& B, const Shape& s, const pose& t, f3 d, 
 // with no atomics.  The sphere-hull bucket comes first, so its long GJK + EPA
 // runs start first, and a pair's EPA follows its own GJK with no grid-wide
 // wait in between.  Counts go to the EPA partial slots (zeroed by
-// k_classify) with one atomic add per nonzero count, spread over
-// kMaxPartialBlocks addresses.
+// k_classify) with one atomic add per nonzero count.
 constexpr uint32_t kEpaBlock = 64;
 
 __device__ __forceinline__ void epa_partial_add(uint64_t* slot, uint32_t v)
 {
     if (v)
-        atomicAdd(reinterpret_cast<unsigned long long*>(slot + (blockIdx.x & (kMaxPartialBlocks - 1))),
+        atomicAdd(reinterpret_cast<unsigned long long*>(slot + (blockIdx.x & (kEpaPartials - 1))),
                   (unsigned long long)v);
 }
 
@@ -1493,7 +1494,7 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
                      uint32_t(offsetof(Counters, class_contacts) + 4 * q)};
     fold.count = 3;
     if (ctx->has_hulls) {
-        const uint32_t epa_slots = std::min(tu.epa_groups, kMaxPartialBlocks);
+        const uint32_t epa_slots = std::min(tu.epa_groups, kEpaPartials);
         for (uint32_t q = 3; q < kClasses; ++q)
             fold.e[q] = {kSlotContacts + q, epa_slots, kFoldSum32,
                          uint32_t(offsetof(Counters, class_contacts) + 4 * q)};
