@@ -23,7 +23,8 @@
 namespace pg {
 namespace {
 
-__global__ void k_step_begin(Counters* c)
+// Step counters to their initial values (the calling workgroup's threads).
+__device__ __forceinline__ void reset_counters(Counters* c)
 {
     uint32_t* w = reinterpret_cast<uint32_t*>(c);
     for (uint32_t i = threadIdx.x; i < sizeof(Counters) / 4; i += blockDim.x)
@@ -33,6 +34,11 @@ __global__ void k_step_begin(Counters* c)
         c->cmin_key[threadIdx.x] = 0xFFFFFFFFu;
         c->cmax_key[threadIdx.x] = 0u;
     }
+}
+
+__global__ void k_step_begin(Counters* c)
+{
+    reset_counters(c);
 }
 
 // AABB of sphere or box i (fromCollider, aabb.cpp); false for a hull.
@@ -312,7 +318,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __r
     st.flush(partials);
 }
 
-// One launch for a step: AABBs of every body and the width statistics.
+// One launch for a step: the step counters' reset (workgroup 0; no other
+// workgroup touches them), AABBs of every body and the width statistics.
 // Workgroups [0, body_groups) take the spheres and boxes (grid-stride), the
 // rest take the hulls, one 16-lane row each; every workgroup writes its
 // partials.
@@ -321,8 +328,11 @@ __global__ __launch_bounds__(kBlock) void k_aabbs_step(
     const float* __restrict__ sdata, const uint32_t* __restrict__ hrange,
     const float* __restrict__ hpts, uint32_t nh, const uint32_t* __restrict__ world_of,
     const uint32_t* __restrict__ hull_list, const uint32_t* __restrict__ hull_count,
-    uint32_t body_groups, float4* __restrict__ aabb, uint64_t* __restrict__ partials)
+    uint32_t body_groups, float4* __restrict__ aabb, uint64_t* __restrict__ partials,
+    Counters* counters)
 {
+    if (blockIdx.x == 0)
+        reset_counters(counters);
     WidthStats st;
     if (blockIdx.x < body_groups) {
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -447,8 +457,11 @@ void fold_width_stats(physgpu_context* ctx, uint32_t groups)
 void launch_compute_aabbs(physgpu_context* ctx, bool with_stats)
 {
     ctx->resident_aabbs_valid = true;
-    if (ctx->n == 0)
+    if (ctx->n == 0) {
+        if (with_stats)
+            launch_step_begin(ctx);
         return;
+    }
     Timer t;
     time_begin(ctx, "aabb_from_pose", t);
     const uint32_t* world_of = ctx->n_worlds ? ctx->world_of : nullptr;
@@ -464,7 +477,7 @@ void launch_compute_aabbs(physgpu_context* ctx, bool with_stats)
                            ctx->stream, ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data,
                            ctx->hull_range, ctx->hull_points, ctx->n_hull_points, world_of,
                            ctx->hull_list, ctx->hull_list_count, body_groups, ctx->aabb,
-                           ctx->partials);
+                           ctx->partials, ctx->counters);
         fold_width_stats(ctx, body_groups + hull_groups);
         time_end(ctx, t);
         return;

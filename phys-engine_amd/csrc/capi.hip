@@ -306,7 +306,7 @@ physgpu_status check_step(physgpu_context* ctx, int algorithm)
 // without any host synchronisation (so it can be captured into a graph).
 void enqueue_step(physgpu_context* ctx, int algorithm, double cell)
 {
-    launch_step_begin(ctx);
+    // (The AABB launch resets the step counters.)
     if (ctx->has_world)
         launch_compose_world(ctx);
     launch_compute_aabbs(ctx, true);

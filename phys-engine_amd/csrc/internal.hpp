@@ -24,7 +24,7 @@ constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kSortMinTile = kSortThreads * 4;
 
 // Compaction tiling.
-constexpr uint32_t kScanTile = 2048;
+constexpr uint32_t kScanTile = 256;  // candidates per compaction tile
 
 // Per-workgroup partial results.  Device-scope atomics on one address are
 // serialised (about 11 ns each on MI355X), so statistics and reductions are
@@ -305,8 +305,9 @@ inline uint32_t stride_grid(uint64_t work)
 // ---- launchers implemented in the kernel translation units -------------
 // aabb.hip
 void launch_step_begin(physgpu_context* ctx);
-// With with_stats, also the width statistics behind the automatic grid cell
-// (what launch_aabb_reductions computes), in the same launch.
+// With with_stats, also the step counters' reset (launch_step_begin) and the
+// width statistics behind the automatic grid cell (what
+// launch_aabb_reductions computes), in the same launch.
 void launch_compute_aabbs(physgpu_context* ctx, bool with_stats = false);
 void launch_hull_list(physgpu_context* ctx);
 void launch_aabb_reductions(physgpu_context* ctx);

@@ -685,7 +685,7 @@ __device__ __forceinline__ void wave_append_u32(bool pred, uint32_t value, uint3
 }
 
 constexpr uint32_t kClassifyThreads = 1024;
-constexpr uint32_t kClassifyGrid = 64;
+constexpr uint32_t kClassifyGrid = 256;
 
 // Shape-pair classification and bucketing.  Counts are aggregated per
 // workgroup of 1024 candidates in LDS, so each class costs one global atomic
@@ -1276,6 +1276,10 @@ __device__ __forceinline__ uint32_t candidate_at(const uint32_t* __restrict__ pe
     return perm ? perm[k] : k;
 }
 
+// Compaction tiles are kScanTile = kBlock candidates in canonical order, one
+// per thread.
+static_assert(kScanTile == kBlock, "one compaction tile per workgroup pass");
+
 __global__ __launch_bounds__(kBlock) void k_tile_hits(const uint32_t* __restrict__ hit,
                                                       const uint32_t* __restrict__ perm,
                                                       const Counters* c,
@@ -1285,14 +1289,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_hits(const uint32_t* __restrict
     const uint32_t count = c->sort_count;
     const uint32_t tiles = (count + kScanTile - 1) / kScanTile;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        uint32_t s = 0;
-        for (uint32_t i = threadIdx.x; i < kScanTile; i += blockDim.x) {
-            const uint32_t k = tile * kScanTile + i;
-            if (k < count)
-                s += hit[candidate_at(perm, k)];
-        }
-        for (uint32_t off = 32; off > 0; off >>= 1)
-            s += __shfl_xor(s, off);
+        const uint32_t k = tile * kScanTile + threadIdx.x;
+        const bool f = k < count && hit[candidate_at(perm, k)] != 0;
+        const uint32_t s = (uint32_t)__popcll(__ballot(f));
         __syncthreads();
         if (lane_id() == 0)
             part[threadIdx.x >> 6] = s;
@@ -1334,7 +1333,9 @@ __global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* __restrict__ tile_
         c->manifold_count = sums[1023];
 }
 
-// Ordered gather: thread t of a tile owns 8 consecutive candidates.
+// Ordered gather: thread t of a tile owns candidate t; each wavefront lists
+// its hits in LDS and copies their manifolds with all 64 lanes, one 16-byte
+// word per lane (10 words per manifold).
 __global__ __launch_bounds__(kBlock) void k_compact(const uint32_t* __restrict__ hit,
                                                     const uint32_t* __restrict__ perm,
                                                     const Counters* c,
@@ -1342,49 +1343,44 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint32_t* __restrict__
                                                     const physgpu_manifold* __restrict__ slots,
                                                     physgpu_manifold* __restrict__ out)
 {
+    constexpr uint32_t kWords = sizeof(physgpu_manifold) / sizeof(uint4);
+    static_assert(kWords == 10, "manifold = 10 x 16 bytes");
     __shared__ uint32_t wave_tot[kBlock / 64];
-    constexpr uint32_t per = kScanTile / kBlock;
+    __shared__ uint32_t src_of[kBlock], pos_of[kBlock];
     const uint32_t count = c->sort_count;
     const uint32_t tiles = (count + kScanTile - 1) / kScanTile;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    uint32_t* my_src = src_of + wave * 64;
+    uint32_t* my_pos = pos_of + wave * 64;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const uint32_t k0 = tile * kScanTile + threadIdx.x * per;
-        uint32_t flags = 0, mine = 0;
-        uint32_t src_k[per];
-        for (uint32_t i = 0; i < per; ++i) {
-            const uint32_t k = k0 + i;
-            src_k[i] = k < count ? candidate_at(perm, k) : 0u;
-            const uint32_t f = (k < count) ? hit[src_k[i]] : 0u;
-            flags |= f << i;
-            mine += f;
+        const uint32_t k = tile * kScanTile + threadIdx.x;
+        uint32_t src = 0;
+        bool f = false;
+        if (k < count) {
+            src = candidate_at(perm, k);
+            f = hit[src] != 0;
         }
-        uint32_t incl = mine;
-        for (uint32_t off = 1; off < 64; off <<= 1) {
-            const uint32_t t = __shfl_up(incl, off);
-            if (lane >= off)
-                incl += t;
-        }
+        const uint64_t m = __ballot(f);
+        const uint32_t hits = (uint32_t)__popcll(m);
         __syncthreads();
-        if (lane == 63)
-            wave_tot[wave] = incl;
+        if (lane == 0)
+            wave_tot[wave] = hits;
+        if (f) {
+            const uint32_t r = (uint32_t)__popcll(m & lanes_below(lane));
+            my_src[r] = src;
+            my_pos[r] = k;  // pair_index: the candidate's canonical position
+        }
         __syncthreads();
         uint32_t base = tile_sums[tile];
         for (uint32_t w = 0; w < wave; ++w)
             base += wave_tot[w];
-        uint32_t pos = base + incl - mine;
-        for (uint32_t i = 0; i < per; ++i) {
-            if (flags & (1u << i)) {
-                const uint4* src = reinterpret_cast<const uint4*>(slots + src_k[i]);
-                uint4* dst = reinterpret_cast<uint4*>(out + pos);
-#pragma unroll
-                for (int q = 0; q < 9; ++q)
-                    dst[q] = src[q];
-                // Word 36 is pair_index: the candidate's canonical position.
-                uint4 tail = src[9];
-                tail.x = k0 + i;
-                dst[9] = tail;
-                ++pos;
-            }
+        for (uint32_t i = lane; i < hits * kWords; i += 64) {
+            const uint32_t j = i / kWords, q = i - kWords * j;
+            uint4 v = reinterpret_cast<const uint4*>(slots + my_src[j])[q];
+            // Word 36 (x of the last 16 bytes) is pair_index.
+            if (q == kWords - 1)
+                v.x = my_pos[j];
+            reinterpret_cast<uint4*>(out + base + j)[q] = v;
         }
     }
 }
