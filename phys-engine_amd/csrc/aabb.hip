@@ -42,23 +42,22 @@ __global__ void k_step_begin(Counters* c)
 }
 
 // AABB of sphere or box i (fromCollider, aabb.cpp); false for a hull.
-__device__ __forceinline__ bool body_aabb(uint32_t i, uint32_t n, const float* __restrict__ pose,
-                                          const uint8_t* __restrict__ type,
-                                          const float* __restrict__ sdata, float b[6])
+__device__ __forceinline__ bool body_aabb(uint32_t i, const float4* __restrict__ pose,
+                                          const float4* __restrict__ shape, float b[6])
 {
-    const uint8_t ty = type[i];
+    const float4 s0 = shape[2 * size_t(i)];
+    const uint32_t ty = __float_as_uint(s0.w);
     if (ty != PHYSGPU_SPHERE && ty != PHYSGPU_BOX)
         return false;
-    pg::pose t{{pose[i], pose[n + i], pose[2 * n + i]},
-               {pose[3 * n + i], pose[4 * n + i], pose[5 * n + i], pose[6 * n + i]}};
+    const pg::pose t = load_pose4(pose, i);
     f3 mn, mx;
     if (ty == PHYSGPU_SPHERE) {
-        const float r = sdata[i];
+        const float r = s0.x;
         f3 rr{r, r, r};
         mn = sub(t.p, rr);
         mx = add(t.p, rr);
     } else {
-        const f3 h{sdata[i], sdata[n + i], sdata[2 * n + i]};
+        const f3 h{s0.x, s0.y, s0.z};
         const m3 m = to_mat(t.q);
         f3 e{fabsf(m.m00) * h.x + fabsf(m.m01) * h.y + fabsf(m.m02) * h.z,
              fabsf(m.m10) * h.x + fabsf(m.m11) * h.y + fabsf(m.m12) * h.z,
@@ -76,13 +75,12 @@ __device__ __forceinline__ bool body_aabb(uint32_t i, uint32_t n, const float* _
 }
 
 __global__ __launch_bounds__(kBlock) void k_compute_aabbs(
-    uint32_t n, const float* __restrict__ pose, const uint8_t* __restrict__ type,
-    const float* __restrict__ sdata, const uint32_t* __restrict__ world_of,
-    float4* __restrict__ aabb)
+    uint32_t n, const float4* __restrict__ pose, const float4* __restrict__ shape,
+    const uint32_t* __restrict__ world_of, float4* __restrict__ aabb)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float b[6];
-    if (i < n && body_aabb(i, n, pose, type, sdata, b))  // hulls: k_hull_aabbs
+    if (i < n && body_aabb(i, pose, shape, b))  // hulls: k_hull_aabbs
         store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
 }
 
@@ -131,14 +129,14 @@ __device__ __forceinline__ bool row_first_extreme(float& v, uint32_t count, uint
 // scan on the row's first lane.  The bounds are valid in lane rl == 0.
 constexpr uint32_t kHullRowLanes = 16;
 
-__device__ __forceinline__ void hull_row_aabb(uint32_t i, uint32_t n, const float* __restrict__ pose,
-                                              const uint32_t* __restrict__ hrange,
+__device__ __forceinline__ void hull_row_aabb(uint32_t i, const float4* __restrict__ pose,
+                                              const float4* __restrict__ shape,
                                               const float* __restrict__ hpts, uint32_t nh,
                                               uint32_t rl, uint32_t row_base, float b[6])
 {
-    const pg::pose t{{pose[i], pose[n + i], pose[2 * n + i]},
-                     {pose[3 * n + i], pose[4 * n + i], pose[5 * n + i], pose[6 * n + i]}};
-    const uint32_t first = hrange[i], count = hrange[n + i];
+    const pg::pose t = load_pose4(pose, i);
+    uint32_t first, count;
+    hull_range(shape, i, first, count);
     f3 mn{0.0f, 0.0f, 0.0f}, mx{0.0f, 0.0f, 0.0f};
     for (uint32_t k = rl; k < count; k += kHullRowLanes) {
         const uint32_t p = first + k;
@@ -192,8 +190,8 @@ __device__ __forceinline__ void hull_row_aabb(uint32_t i, uint32_t n, const floa
 // Hull AABBs (listed by k_hull_list), one 16-lane row per hull.
 __global__ __launch_bounds__(kBlock) void k_hull_aabbs(const uint32_t* __restrict__ list,
                                                        const uint32_t* __restrict__ list_count,
-                                                       uint32_t n, const float* __restrict__ pose,
-                                                       const uint32_t* __restrict__ hrange,
+                                                       const float4* __restrict__ pose,
+                                                       const float4* __restrict__ shape,
                                                        const float* __restrict__ hpts, uint32_t nh,
                                                        const uint32_t* __restrict__ world_of,
                                                        float4* __restrict__ aabb)
@@ -208,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_hull_aabbs(const uint32_t* __restric
             continue;  // whole rows (h is row-uniform)
         const uint32_t i = list[h];
         float b[6];
-        hull_row_aabb(i, n, pose, hrange, hpts, nh, rl, row_base, b);
+        hull_row_aabb(i, pose, shape, hpts, nh, rl, row_base, b);
         if (rl == 0)
             store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
     }
@@ -324,8 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(uint32_t n, const float4* __r
 // rest take the hulls, one 16-lane row each; every workgroup writes its
 // partials.
 __global__ __launch_bounds__(kBlock) void k_aabbs_step(
-    uint32_t n, const float* __restrict__ pose, const uint8_t* __restrict__ type,
-    const float* __restrict__ sdata, const uint32_t* __restrict__ hrange,
+    uint32_t n, const float4* __restrict__ pose, const float4* __restrict__ shape,
     const float* __restrict__ hpts, uint32_t nh, const uint32_t* __restrict__ world_of,
     const uint32_t* __restrict__ hull_list, const uint32_t* __restrict__ hull_count,
     uint32_t body_groups, float4* __restrict__ aabb, uint64_t* __restrict__ partials,
@@ -338,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_aabbs_step(
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
              i += body_groups * blockDim.x) {
             float b[6];
-            if (body_aabb(i, n, pose, type, sdata, b)) {
+            if (body_aabb(i, pose, shape, b)) {
                 store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
                 st.add(b);
             }
@@ -351,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_aabbs_step(
         for (uint32_t h = row; h < hulls; h += rows) {
             const uint32_t i = hull_list[h];
             float b[6];
-            hull_row_aabb(i, n, pose, hrange, hpts, nh, rl, row_base, b);
+            hull_row_aabb(i, pose, shape, hpts, nh, rl, row_base, b);
             if (rl == 0) {
                 store_rec(aabb, i, b, i, world_of ? world_of[i] : 0u);
                 st.add(b);
@@ -391,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_aabb_to_rec(uint32_t n, const physgp
 // transform, as physics_world.cpp:660-667 and aabb.cpp:10-13 compose it.
 __global__ __launch_bounds__(kBlock) void k_compose(uint32_t n, const float* __restrict__ body,
                                                     const float* __restrict__ local,
-                                                    float* __restrict__ pose_out)
+                                                    float4* __restrict__ pose_out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -406,13 +403,35 @@ __global__ __launch_bounds__(kBlock) void k_compose(uint32_t n, const float* __r
         p = add(bp, rotate(bq, lp));
         q = qmul(bq, lq);
     }
-    pose_out[i] = p.x;
-    pose_out[n + i] = p.y;
-    pose_out[2 * n + i] = p.z;
-    pose_out[3 * n + i] = q.x;
-    pose_out[4 * n + i] = q.y;
-    pose_out[5 * n + i] = q.z;
-    pose_out[6 * n + i] = q.w;
+    store_pose4(pose_out, i, pg::pose{p, q});
+}
+
+// Resident layouts from the uploaded planes: poses (7 x n) to 2 float4 per
+// body; types, shape data (3 x n) and hull ranges (2 x n) to 2 float4 per
+// body (see load_pose4 / shape_data).
+__global__ __launch_bounds__(kBlock) void k_pose_aos(uint32_t n, const float* __restrict__ p,
+                                                     float4* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    store_pose4(out, i, pg::pose{{p[i], p[n + i], p[2 * n + i]},
+                                 {p[3 * n + i], p[4 * n + i], p[5 * n + i], p[6 * n + i]}});
+}
+
+__global__ __launch_bounds__(kBlock) void k_shape_aos(uint32_t n, const uint8_t* __restrict__ type,
+                                                      const float* __restrict__ sdata,
+                                                      const uint32_t* __restrict__ hrange,
+                                                      float4* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t ty = type[i];
+    const uint32_t first = ty == PHYSGPU_HULL ? hrange[i] : 0u;
+    const uint32_t count = ty == PHYSGPU_HULL ? hrange[n + i] : 0u;
+    out[2 * size_t(i)] = make_float4(sdata[i], sdata[n + i], sdata[2 * n + i], __uint_as_float(ty));
+    out[2 * size_t(i) + 1] = make_float4(__uint_as_float(first), __uint_as_float(count), 0.0f, 0.0f);
 }
 
 }  // namespace
@@ -425,7 +444,7 @@ void launch_compose_world(physgpu_context* ctx)
     time_begin(ctx, "compose_world", t);
     hipLaunchKernelGGL(k_compose, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0, ctx->stream,
                        ctx->n, ctx->world_body_pose,
-                       ctx->world_has_local ? ctx->local_pose : nullptr, ctx->pose);
+                       ctx->world_has_local ? ctx->local_pose : nullptr, ctx->pose4);
     time_end(ctx, t);
 }
 
@@ -474,8 +493,8 @@ void launch_compute_aabbs(physgpu_context* ctx, bool with_stats)
                                                 kMaxPartialBlocks - body_groups)
                            : 0u;
         hipLaunchKernelGGL(k_aabbs_step, dim3(body_groups + hull_groups), dim3(kBlock), 0,
-                           ctx->stream, ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data,
-                           ctx->hull_range, ctx->hull_points, ctx->n_hull_points, world_of,
+                           ctx->stream, ctx->n, ctx->pose4, ctx->shape4, ctx->hull_points,
+                           ctx->n_hull_points, world_of,
                            ctx->hull_list, ctx->hull_list_count, body_groups, ctx->aabb,
                            ctx->partials, ctx->counters);
         fold_width_stats(ctx, body_groups + hull_groups);
@@ -483,14 +502,26 @@ void launch_compute_aabbs(physgpu_context* ctx, bool with_stats)
         return;
     }
     hipLaunchKernelGGL(k_compute_aabbs, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0,
-                       ctx->stream, ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data,
-                       world_of, ctx->aabb);
+                       ctx->stream, ctx->n, ctx->pose4, ctx->shape4, world_of, ctx->aabb);
     if (ctx->has_hulls)
         hipLaunchKernelGGL(k_hull_aabbs, dim3(stride_grid(16ull * ctx->n)), dim3(kBlock), 0,
-                           ctx->stream, ctx->hull_list, ctx->hull_list_count, ctx->n, ctx->pose,
-                           ctx->hull_range, ctx->hull_points, ctx->n_hull_points, world_of,
-                           ctx->aabb);
+                           ctx->stream, ctx->hull_list, ctx->hull_list_count, ctx->pose4,
+                           ctx->shape4, ctx->hull_points, ctx->n_hull_points, world_of, ctx->aabb);
     time_end(ctx, t);
+}
+
+void launch_pose_aos(physgpu_context* ctx)
+{
+    if (ctx->n)
+        hipLaunchKernelGGL(k_pose_aos, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0, ctx->stream,
+                           ctx->n, ctx->pose, ctx->pose4);
+}
+
+void launch_shape_aos(physgpu_context* ctx)
+{
+    if (ctx->n)
+        hipLaunchKernelGGL(k_shape_aos, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0, ctx->stream,
+                           ctx->n, ctx->shape_type, ctx->shape_data, ctx->hull_range, ctx->shape4);
 }
 
 void launch_hull_list(physgpu_context* ctx)

@@ -355,8 +355,8 @@ __device__ void box_box_adjoint(f3 ha, const pose& ta, f3 hb, const pose& tb, co
 
 // Gradient planes: pose 7 x n (px py pz qx qy qz qw); pair 14 x m (A's 7, B's 7).
 __global__ __launch_bounds__(kBlock) void k_contact_adjoint(
-    uint32_t n, const float* __restrict__ pose_planes, const uint8_t* __restrict__ type,
-    const float* __restrict__ sdata, const physgpu_manifold* __restrict__ man, uint32_t m,
+    uint32_t n, const float4* __restrict__ pose4, const float4* __restrict__ shape4,
+    const physgpu_manifold* __restrict__ man, uint32_t m,
     const Counters* __restrict__ counters, const float* __restrict__ g_normal,
     const float* __restrict__ g_depth, const float* __restrict__ g_point,
     float* __restrict__ geom_normal, float* __restrict__ geom_depth,
@@ -390,24 +390,19 @@ __global__ __launch_bounds__(kBlock) void k_contact_adjoint(
             g.depth[k] = 0.0f;
             g.point[k] = {0.0f, 0.0f, 0.0f};
         }
-        auto load = [&](uint32_t i) {
-            return pose{{pose_planes[i], pose_planes[n + i], pose_planes[2 * n + i]},
-                        {pose_planes[3 * n + i], pose_planes[4 * n + i], pose_planes[5 * n + i],
-                         pose_planes[6 * n + i]}};
-        };
-        auto data = [&](uint32_t i) { return f3{sdata[i], sdata[n + i], sdata[2 * n + i]}; };
-        const pose ta = load(a), tb = load(b);
+        auto data = [&](uint32_t i) { return shape_data(shape4, i); };
+        const pose ta = load_pose4(pose4, a), tb = load_pose4(pose4, b);
         if (cls == PHYSGPU_PAIR_SPHERE_SPHERE) {
-            sphere_sphere_adjoint(sdata[a], ta, sdata[b], tb, g, out, ga, gb);
+            sphere_sphere_adjoint(data(a).x, ta, data(b).x, tb, g, out, ga, gb);
             done = true;
         } else if (cls == PHYSGPU_PAIR_SPHERE_BOX) {
-            if (type[a] == PHYSGPU_SPHERE) {
-                sphere_box_adjoint(sdata[a], ta, data(b), tb, g, out, ga, gb);
+            if (shape_type(shape4, a) == PHYSGPU_SPHERE) {
+                sphere_box_adjoint(data(a).x, ta, data(b), tb, g, out, ga, gb);
             } else {
                 // Box-sphere: the forward swaps and negates the normal (:353-365).
                 Seeds gs = g;
                 gs.normal = neg(g.normal);
-                sphere_box_adjoint(sdata[b], tb, data(a), ta, gs, out, gb, ga);
+                sphere_box_adjoint(data(b).x, tb, data(a), ta, gs, out, gb, ga);
                 out.normal = neg(out.normal);
             }
             done = true;
@@ -482,7 +477,7 @@ void launch_contact_adjoint(physgpu_context* ctx, uint32_t m, const float* g_nor
     Timer t;
     time_begin(ctx, "np_adjoint", t);
     hipLaunchKernelGGL(k_contact_adjoint, dim3(div_up(m, kBlock)), dim3(kBlock), 0, ctx->stream,
-                       ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data, ctx->manifolds, m,
+                       ctx->n, ctx->pose4, ctx->shape4, ctx->manifolds, m,
                        ctx->counters, g_normal, g_depth, g_point, geom_normal, geom_depth,
                        geom_point, grad_pair, grad_pose, skipped);
     time_end(ctx, t);

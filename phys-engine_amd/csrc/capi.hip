@@ -92,7 +92,7 @@ hipError_t alloc(T*& p, size_t count)
 
 void free_all(physgpu_context* c)
 {
-    void* ptrs[] = {c->pose, c->shape_type, c->shape_data, c->hull_range, c->hull_points,
+    void* ptrs[] = {c->pose, c->pose4, c->shape4, c->shape_type, c->shape_data, c->hull_range, c->hull_points,
                     c->aabb, c->sorted_aabb, c->items_a, c->items_b, c->hull_list,
                     c->hull_list_count,
                     c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_flag, c->node_left,
@@ -410,6 +410,8 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(hipEventCreateWithFlags(&c->fork_event, hipEventDisableTiming));
     A(hipEventCreateWithFlags(&c->join_event, hipEventDisableTiming));
     A(alloc(c->pose, 7 * N));
+    A(alloc(c->pose4, 2 * N));
+    A(alloc(c->shape4, 2 * N));
     A(alloc(c->world_body_pose, 7 * N));
     A(alloc(c->shape_type, N));
     A(alloc(c->shape_data, 3 * N));
@@ -552,6 +554,8 @@ physgpu_status physgpu_set_colliders(physgpu_context* ctx, const physgpu_collide
                                       3ull * col->hull_point_count * 4, kind, ctx->stream));
     }
     ctx->n = n;
+    launch_pose_aos(ctx);
+    launch_shape_aos(ctx);
     if (hulls)
         launch_hull_list(ctx);
     PG_TRY_HIP(hipGetLastError());
@@ -580,6 +584,7 @@ physgpu_status physgpu_update_poses(physgpu_context* ctx, const float* pose,
                                   memory == PHYSGPU_HOST ? hipMemcpyHostToDevice
                                                          : hipMemcpyDeviceToDevice,
                                   ctx->stream));
+    launch_pose_aos(ctx);
     if (memory == PHYSGPU_HOST)
         PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
     return PHYSGPU_OK;

@@ -13,11 +13,9 @@ namespace {
 
 struct Bodies {
     uint32_t n;
-    const float* pose;
-    const uint8_t* type;
-    const float* sdata;
-    const uint32_t* hrange;
-    const float* hpts;
+    const float4* pose;   // 2 float4 per body (load_pose4)
+    const float4* shape;  // 2 float4 per body (shape_data, shape_type, hull_range)
+    const float* hpts;    // hull points, 3 planes of nh
     uint32_t nh;
 };
 
@@ -38,13 +36,15 @@ struct Contact {
 
 __device__ __forceinline__ pose load_pose(const Bodies& B, uint32_t i)
 {
-    const uint32_t n = B.n;
-    return {{B.pose[i], B.pose[n + i], B.pose[2 * n + i]},
-            {B.pose[3 * n + i], B.pose[4 * n + i], B.pose[5 * n + i], B.pose[6 * n + i]}};
+    return load_pose4(B.pose, i);
 }
 __device__ __forceinline__ f3 load_data(const Bodies& B, uint32_t i)
 {
-    return {B.sdata[i], B.sdata[B.n + i], B.sdata[2 * B.n + i]};
+    return shape_data(B.shape, i);
+}
+__device__ __forceinline__ uint32_t load_type(const Bodies& B, uint32_t i)
+{
+    return shape_type(B.shape, i);
 }
 
 __device__ __forceinline__ uint32_t pair_class(uint32_t ta, uint32_t tb)

@@ -98,4 +98,33 @@ PG_D uint32_t float_key(float f)
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// Resident body layout: pose = 2 float4 per body (p.xyz | 0, q.xyzw); shape
+// = 2 float4 per body (data.xyz | type bits, hull first bits | hull count
+// bits | 0 | 0).  A body's fields cost one or two cache lines.
+PG_D pose load_pose4(const float4* __restrict__ p, uint32_t i)
+{
+    const float4 a = p[2 * size_t(i)], b = p[2 * size_t(i) + 1];
+    return {{a.x, a.y, a.z}, {b.x, b.y, b.z, b.w}};
+}
+PG_D void store_pose4(float4* __restrict__ p, uint32_t i, const pose& t)
+{
+    p[2 * size_t(i)] = make_float4(t.p.x, t.p.y, t.p.z, 0.0f);
+    p[2 * size_t(i) + 1] = make_float4(t.q.x, t.q.y, t.q.z, t.q.w);
+}
+PG_D f3 shape_data(const float4* __restrict__ s, uint32_t i)
+{
+    const float4 a = s[2 * size_t(i)];
+    return {a.x, a.y, a.z};
+}
+PG_D uint32_t shape_type(const float4* __restrict__ s, uint32_t i)
+{
+    return __float_as_uint(s[2 * size_t(i)].w);
+}
+PG_D void hull_range(const float4* __restrict__ s, uint32_t i, uint32_t& first, uint32_t& count)
+{
+    const float4 b = s[2 * size_t(i) + 1];
+    first = __float_as_uint(b.x);
+    count = __float_as_uint(b.y);
+}
+
 } // namespace pg

@@ -125,13 +125,18 @@ struct physgpu_context {
     uint64_t cap_p = 0;
     uint32_t cap_h = 0;
 
-    // Colliders (SoA planes).
+    // Colliders: the uploaded planes (pose, shape_type, shape_data,
+    // hull_range) and the resident layouts every kernel reads, 2 float4 per
+    // body each (pose4: load_pose4; shape4: shape_data / shape_type /
+    // hull_range), rebuilt by launch_pose_aos / launch_shape_aos on upload.
     uint32_t n = 0;
     uint32_t n_hull_points = 0;
     float* pose = nullptr;        // 7 x cap_n
     uint8_t* shape_type = nullptr;
     float* shape_data = nullptr;  // 3 x cap_n
     uint32_t* hull_range = nullptr;  // 2 x cap_n
+    float4* pose4 = nullptr;         // 2 x cap_n
+    float4* shape4 = nullptr;        // 2 x cap_n
     float* hull_points = nullptr;    // 3 x cap_h
     bool has_hulls = false;
     uint32_t* hull_list = nullptr;        // hull collider indices (cap_n), built per upload
@@ -309,6 +314,8 @@ void launch_step_begin(physgpu_context* ctx);
 // width statistics behind the automatic grid cell (what
 // launch_aabb_reductions computes), in the same launch.
 void launch_compute_aabbs(physgpu_context* ctx, bool with_stats = false);
+void launch_pose_aos(physgpu_context* ctx);   // ctx->pose -> ctx->pose4
+void launch_shape_aos(physgpu_context* ctx);  // shape planes -> ctx->shape4
 void launch_hull_list(physgpu_context* ctx);
 void launch_aabb_reductions(physgpu_context* ctx);
 void launch_convert_aabbs_out(physgpu_context* ctx, physgpu_aabb* dev_out);

@@ -719,7 +719,7 @@ __global__ __launch_bounds__(kClassifyThreads) void k_classify(Bodies B, World W
             // Same-body pairs are removed before the narrow phase
             // (physics_world.cpp:630-633).
             if (!(W.body && W.body[a] == W.body[b]))
-                cls = pair_class(B.type[a], B.type[b]);
+                cls = pair_class(load_type(B, a), load_type(B, b));
         }
         uint64_t mine = 0;
 #pragma unroll
@@ -833,13 +833,13 @@ __global__ __launch_bounds__(kBlock) void k_class(Bodies B, World W,
             const pose ta = load_pose(B, a), tb = load_pose(B, b);
             Contact ct;
             if (kClass == PHYSGPU_PAIR_SPHERE_SPHERE) {
-                h = sphere_sphere(B.sdata[a], ta, B.sdata[b], tb, ct);
+                h = sphere_sphere(load_data(B, a).x, ta, load_data(B, b).x, tb, ct);
             } else if (kClass == PHYSGPU_PAIR_SPHERE_BOX) {
-                if (B.type[a] == PHYSGPU_SPHERE) {
-                    h = sphere_box(B.sdata[a], ta, load_data(B, b), tb, ct);
+                if (load_type(B, a) == PHYSGPU_SPHERE) {
+                    h = sphere_box(load_data(B, a).x, ta, load_data(B, b), tb, ct);
                 } else {
                     // Box-sphere: swap, then negate the normal and swap anchors (:353-365).
-                    h = sphere_box(B.sdata[b], tb, load_data(B, a), ta, ct);
+                    h = sphere_box(load_data(B, b).x, tb, load_data(B, a), ta, ct);
                     if (h) {
                         ct.normal = neg(ct.normal);
                         const f3 tmp = ct.a[0];
@@ -862,11 +862,9 @@ __global__ __launch_bounds__(kBlock) void k_class(Bodies B, World W,
 
 __device__ __forceinline__ Shape load_shape(const Bodies& B, uint32_t i)
 {
-    Shape s{B.type[i], load_data(B, i), 0, 0};
-    if (s.type == PHYSGPU_HULL) {
-        s.first = B.hrange[i];
-        s.count = B.hrange[B.n + i];
-    }
+    Shape s{load_type(B, i), load_data(B, i), 0, 0};
+    if (s.type == PHYSGPU_HULL)
+        hull_range(B.shape, i, s.first, s.count);
     return s;
 }
 
@@ -1407,8 +1405,7 @@ void launch_hull_adjoint(physgpu_context* ctx, uint32_t m, const float* g_normal
 {
     if (!ctx->has_hulls || m == 0)
         return;
-    Bodies B{ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data, ctx->hull_range,
-             ctx->hull_points, ctx->n_hull_points};
+    Bodies B{ctx->n, ctx->pose4, ctx->shape4, ctx->hull_points, ctx->n_hull_points};
     Timer t;
     time_begin(ctx, "np_hull_adjoint", t);
     hipLaunchKernelGGL(k_hull_adjoint, dim3(std::min<uint32_t>(div_up(m, kEpaWavesPerBlock), 2048)),
@@ -1426,8 +1423,7 @@ void launch_np_begin(physgpu_context* ctx)
 void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
 {
     const uint64_t* pairs = ctx->np_pairs;
-    Bodies B{ctx->n, ctx->pose, ctx->shape_type, ctx->shape_data, ctx->hull_range,
-             ctx->hull_points, ctx->n_hull_points};
+    Bodies B{ctx->n, ctx->pose4, ctx->shape4, ctx->hull_points, ctx->n_hull_points};
     World W{nullptr, nullptr, nullptr, nullptr};
     if (world && ctx->has_world)
         W = World{ctx->world_has_body ? ctx->body : nullptr,
