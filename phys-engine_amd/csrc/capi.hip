@@ -217,8 +217,8 @@ void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool windo
         if (algorithm == PHYSGPU_SWEEP_AND_PRUNE) {
             launch_sap_keys(ctx);
             uint64_t* sorted = nullptr;
-            radix_sort(ctx, ctx->items_a, ctx->items_b, ctx->n, nullptr, 0, 32, 0, 0, &sorted);
-            launch_gather_sorted(ctx, sorted);
+            radix_sort(ctx, ctx->items_a, ctx->items_b, ctx->n, nullptr, 0, 32, 0, 0, &sorted,
+                       ctx->aabb, ctx->sorted_aabb);
             launch_sap_sweep(ctx);
             if (window_stats)
                 launch_sap_window_tests(ctx);

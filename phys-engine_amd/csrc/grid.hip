@@ -292,7 +292,8 @@ void launch_grid(physgpu_context* ctx, double cell_size)
                        ctx->overflow_list, ctx->grid_flag);
     time_end(ctx, t);
     uint64_t* sorted = nullptr;
-    radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 3 * bpa + 1, 0, 0, &sorted);
+    radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 3 * bpa + 1, 0, 0, &sorted,
+               ctx->aabb, ctx->sorted_aabb);
     time_begin(ctx, "grid_table_clear", t);
     hipLaunchKernelGGL(k_fill, dim3(stride_grid(2ull * table_size)), dim3(kBlock), 0, ctx->stream,
                        ctx->cell_table, 2ull * table_size, 0u);
@@ -301,7 +302,6 @@ void launch_grid(physgpu_context* ctx, double cell_size)
     hipLaunchKernelGGL(k_cell_ranges, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
                        sorted, ctx->counters, ctx->cell_table);
     time_end(ctx, t);
-    launch_gather_sorted(ctx, sorted);
     time_begin(ctx, "grid_query", t);
     hipLaunchKernelGGL(k_grid_query, dim3(ctx->tuning.query_groups),
                        dim3(ctx->tuning.query_block), 0, ctx->stream, n,

@@ -323,10 +323,13 @@ void launch_convert_aabbs_in(physgpu_context* ctx, const physgpu_aabb* dev_in);
 void launch_compose_world(physgpu_context* ctx);
 // radix_sort.hip: stable sort of n u64 items by `bits` bits starting at bit
 // `shift0` (and, when bits1 > 0, then by `bits1` bits at `shift1`, which is the
-// more significant field).  Result ends in *result (one of a, b).
+// more significant field).  Result ends in *result (one of a, b).  With
+// gather_dst, the final pass also writes gather_dst[p] = gather_src[index]
+// (2 float4 records) for the item at sorted position p, index = item >> 32.
 void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                 const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
-                uint32_t bits1, uint64_t** result);
+                uint32_t bits1, uint64_t** result, const float4* gather_src = nullptr,
+                float4* gather_dst = nullptr);
 // sap.hip
 void launch_sap_keys(physgpu_context* ctx);
 void launch_gather_sorted(physgpu_context* ctx, const uint64_t* sorted_items);

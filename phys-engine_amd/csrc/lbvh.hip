@@ -288,9 +288,9 @@ void launch_lbvh(physgpu_context* ctx)
                        ctx->aabb, ctx->counters, ctx->items_a);
     time_end(ctx, t);
     uint64_t* sorted = nullptr;
-    radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 30, 0, 0, &sorted);
+    radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 30, 0, 0, &sorted, ctx->aabb,
+               ctx->sorted_aabb);
     ctx->lbvh_sorted = sorted;
-    launch_gather_sorted(ctx, sorted);
     if (n < 2)
         return;
     time_begin(ctx, "lbvh_build", t);

@@ -116,7 +116,9 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
                                                             const uint32_t* dev_count,
                                                             uint32_t shift,
                                                             const uint32_t* __restrict__ hist,
-                                                            const uint32_t* __restrict__ digit_total)
+                                                            const uint32_t* __restrict__ digit_total,
+                                                            const float4* __restrict__ gsrc,
+                                                            float4* __restrict__ gdst)
 {
     constexpr uint32_t kSortTile = kSortThreads * kItems;
     constexpr uint32_t kWaveItems = kSortTile / kWaves;  // contiguous sub-tile per wave
@@ -228,7 +230,14 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
         if (s < valid) {
             const uint64_t it = tile_items[s];
             const uint32_t d = digit_of(it, shift);
-            out[global_start[d] + (s - tile_start[d])] = it;
+            const uint32_t pos = global_start[d] + (s - tile_start[d]);
+            out[pos] = it;
+            if (gdst) {
+                // Final pass: the record of the item's index (high word) too.
+                const size_t idx = size_t(it >> 32);
+                gdst[2 * size_t(pos)] = gsrc[2 * idx];
+                gdst[2 * size_t(pos) + 1] = gsrc[2 * idx + 1];
+            }
         }
     }
 }
@@ -236,12 +245,16 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
 template <uint32_t kItems>
 void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                       const uint32_t* device_count, uint32_t shift0, uint32_t bits0,
-                      uint32_t shift1, uint32_t bits1, uint64_t** result)
+                      uint32_t shift1, uint32_t bits1, uint64_t** result,
+                      const float4* gather_src, float4* gather_dst)
 {
     const uint32_t tiles = div_up(n, kSortThreads * kItems);
+    const uint32_t passes = (bits0 + 7) / 8 + (bits1 + 7) / 8;
+    uint32_t done = 0;
     uint64_t* src = a;
     uint64_t* dst = b;
     auto pass = [&](uint32_t shift) {
+        const bool last = ++done == passes;
         Timer t;
         time_begin(ctx, "radix_upsweep", t);
         hipLaunchKernelGGL(k_upsweep<kItems>, dim3(tiles), dim3(kSortThreads), 0, ctx->stream,
@@ -253,7 +266,8 @@ void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n
         time_end(ctx, t);
         time_begin(ctx, "radix_downsweep", t);
         hipLaunchKernelGGL(k_downsweep<kItems>, dim3(tiles), dim3(kSortThreads), 0, ctx->stream,
-                           src, dst, n, device_count, shift, ctx->histogram, ctx->digit_total);
+                           src, dst, n, device_count, shift, ctx->histogram, ctx->digit_total,
+                           last ? gather_src : nullptr, last ? gather_dst : nullptr);
         time_end(ctx, t);
         uint64_t* tmp = src;
         src = dst;
@@ -272,19 +286,22 @@ void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n
 
 void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                 const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
-                uint32_t bits1, uint64_t** result)
+                uint32_t bits1, uint64_t** result, const float4* gather_src, float4* gather_dst)
 {
     // Tile = 256 threads x sort_items (4, 8 or 16); the histogram is sized for
     // the smallest tile.
     switch (ctx->tuning.sort_items) {
     case 4:
-        radix_sort_tiled<4>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result);
+        radix_sort_tiled<4>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
+                                gather_src, gather_dst);
         break;
     case 16:
-        radix_sort_tiled<16>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result);
+        radix_sort_tiled<16>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
+                                gather_src, gather_dst);
         break;
     default:
-        radix_sort_tiled<8>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result);
+        radix_sort_tiled<8>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
+                                gather_src, gather_dst);
         break;
     }
 }
