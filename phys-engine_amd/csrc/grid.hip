@@ -3,7 +3,7 @@
 //
 // The reference inserts each AABB into up to 64 cells and de-duplicates with
 // sort + unique.  The gfx950 form inserts each body once, into the cell of its
-// AABB's min corner, and queries the neighbour cells (at most 27):
+// AABB's min corner, and queries half of the neighbour cells (at most 14):
 //   * cell side c >= every in-grid body's largest side (widened by 2^-9), so
 //     two overlapping in-grid bodies have min corners less than one cell
 //     apart on every axis and their home cells are neighbours;
@@ -14,9 +14,13 @@
 //     k bits per axis (table of 2^3k entries, 2^3k >= 2N): spatially close
 //     cells are close in memory; far cells that wrap onto the same key only
 //     add AABB tests, never change the result;
-//   * a pair is emitted by the query of its lower original index, so each
-//     unordered pair appears once (the 27 neighbour keys are distinct for
-//     k >= 2, and every body sits in exactly one cell).
+//   * of two overlapping bodies with different home cells, the one whose cell
+//     comes first in (z, y, x) order finds the pair (the other cell is at an
+//     offset after (0, 0, 0) in that order, one of the 13 queried); in a
+//     shared home cell the lower original index does; so each unordered pair
+//     appears once (the 27 neighbour keys are distinct for k >= 2, so a
+//     wrapped key never brings back the other half, and every body sits in
+//     exactly one cell).
 // Cell coordinates use floor(min / c) in double precision, a monotone map.
 // Batches of worlds: every world has its own cells.  A world's keys are its
 // Morton keys XOR a per-world hash (a bijection, so one world's 27 neighbour
@@ -191,18 +195,25 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
             home_cell(me.b[3], me.b[4], me.b[5], inv, top);
             wk = world_key(me.world, bits_per_axis);
         }
-        // One z-slab of 3 x 3 neighbour cells at a time: the 9 range loads are
-        // issued together, then each lane walks the candidates of all 9 cells
-        // in one loop, so the wave waits for one table round trip and
+        // Half neighbourhood: of two overlapping bodies in different cells,
+        // the one whose home cell comes first in (z, y, x) order finds the
+        // pair (the other's cell is at offset (dx, dy, dz) > (0, 0, 0) in
+        // that order, and within its [home - 1, top] range); in the home
+        // cell itself the lower index does.  So the slab dz = -1 and the
+        // first four cells of the slab dz = 0 are never read.
+        // One z-slab of 3 x 3 neighbour cells at a time: the range loads are
+        // issued together, then each lane walks the candidates of the slab's
+        // cells in one loop, so the wave waits for one table round trip and
         // max(candidates) record loads per slab instead of one round trip and
         // one loop per cell.
-        for (int dz = -1; dz <= 1; ++dz) {
+        for (int dz = 0; dz <= 1; ++dz) {
             uint2 r[9];
 #pragma unroll
             for (int j = 0; j < 9; ++j) {
                 const int dx = j % 3 - 1, dy = j / 3 - 1;
                 r[j] = make_uint2(0u, 0u);
-                if (active && c[0] + dx <= top[0] && c[1] + dy <= top[1] && c[2] + dz <= top[2])
+                if (active && (dz > 0 || j >= 4) && c[0] + dx <= top[0] && c[1] + dy <= top[1] &&
+                    c[2] + dz <= top[2])
                     r[j] = range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask) ^ wk];
             }
             uint32_t q = 0, qe = 0, next = 0;
@@ -229,7 +240,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                 if (more) {
                     const Rec o = load_rec(s, q);
                     b = o.index;
-                    if (me.index < b && me.world == o.world) {
+                    const bool home = dz == 0 && next == 5;  // the walk is in cell j = 4
+                    if ((!home || me.index < b) && me.world == o.world) {
                         ++tests;
                         hit = rec_overlap(me.b, o.b);
                     }
