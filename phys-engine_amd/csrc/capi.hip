@@ -163,7 +163,7 @@ physgpu_tuning default_tuning()
     }
     t.epa_groups = 65536;  // one wavefront each; at 1M bodies (34k entries) 2048 / 16384 / 65536: 638 / 310 / 263 us
     t.query_block = kBlock;
-    t.query_groups = 4096;  // at 1M bodies, 256 x 1024 / 2048 / 4096: 119 / 114 / 98 us
+    t.query_groups = 2048;  // at 1M bodies, 256 x 1024 / 2048 / 4096: 71 / 63 / 78 us
     t.sort_items = 8;
     return t;
 }
