@@ -87,17 +87,15 @@ __device__ __forceinline__ bool home_cell(float x, float y, float z, double inv,
     return true;
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* __restrict__ aabb,
-                                                      Counters* counters, float requested,
-                                                      uint32_t bits_per_axis,
-                                                      uint64_t* __restrict__ items,
-                                                      uint32_t* __restrict__ overflow,
-                                                      uint32_t* __restrict__ flag)
+// Key of body i (i < n): its home cell's key, or 1 << 3k for the overflow
+// list; writes the sort item, the in-grid flag and, for overflow bodies,
+// their list entry.  Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t grid_key(uint32_t n, const float4* __restrict__ aabb,
+                                             Counters* counters, float cell,
+                                             uint32_t bits_per_axis, uint64_t* __restrict__ items,
+                                             uint32_t* __restrict__ overflow,
+                                             uint32_t* __restrict__ flag, uint32_t i)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const float cell = step_cell(counters, requested, n);
-    if (i == 0)
-        counters->cell_bits = __float_as_uint(cell);
     bool in_grid = false;
     uint32_t key = 1u << (3 * bits_per_axis);  // sorts after every cell key
     if (i < n) {
@@ -129,6 +127,38 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* 
         if (over)
             overflow[base + __popcll(om & lanes_below(lane))] = i;
     }
+    return key;
+}
+
+// One workgroup per radix-sort tile (tile_items keys): the keys, and the
+// tile's histogram of the sort's first digit (key bits 0..7) in the sort's
+// layout (hist[digit * tiles + tile]), so the sort skips its first upsweep.
+__global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* __restrict__ aabb,
+                                                      Counters* counters, float requested,
+                                                      uint32_t bits_per_axis,
+                                                      uint64_t* __restrict__ items,
+                                                      uint32_t* __restrict__ overflow,
+                                                      uint32_t* __restrict__ flag,
+                                                      uint32_t tile_items,
+                                                      uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t counts[256];
+    static_assert(kBlock == 256, "one histogram bin per thread");
+    const float cell = step_cell(counters, requested, n);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        counters->cell_bits = __float_as_uint(cell);
+    counts[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t r = 0; r < tile_items; r += kBlock) {
+        const uint32_t i = blockIdx.x * tile_items + r + threadIdx.x;
+        const uint32_t key = grid_key(n, aabb, counters, cell, bits_per_axis, items, overflow,
+                                      flag, i);
+        if (i < n)
+            atomicAdd(&counts[key & 0xFFu], 1u);
+    }
+    __syncthreads();
+    const uint32_t tiles = (n + tile_items - 1) / tile_items;
+    hist[threadIdx.x * tiles + blockIdx.x] = counts[threadIdx.x];
 }
 
 // Bodies in cells = all bodies minus the overflow ones.
@@ -299,13 +329,14 @@ void launch_grid(physgpu_context* ctx, double cell_size)
     const uint32_t table_size = 1u << ctx->table_bits;
     Timer t;
     time_begin(ctx, "grid_keys", t);
-    hipLaunchKernelGGL(k_grid_keys, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
+    const uint32_t tile_items = kSortThreads * ctx->tuning.sort_items;
+    hipLaunchKernelGGL(k_grid_keys, dim3(div_up(n, tile_items)), dim3(kBlock), 0, ctx->stream, n,
                        ctx->aabb, ctx->counters, float(cell_size), bpa, ctx->items_a,
-                       ctx->overflow_list, ctx->grid_flag);
+                       ctx->overflow_list, ctx->grid_flag, tile_items, ctx->histogram);
     time_end(ctx, t);
     uint64_t* sorted = nullptr;
     radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 3 * bpa + 1, 0, 0, &sorted,
-               ctx->aabb, ctx->sorted_aabb);
+               ctx->aabb, ctx->sorted_aabb, true);
     time_begin(ctx, "grid_table_clear", t);
     hipLaunchKernelGGL(k_fill, dim3(stride_grid(2ull * table_size)), dim3(kBlock), 0, ctx->stream,
                        ctx->cell_table, 2ull * table_size, 0u);

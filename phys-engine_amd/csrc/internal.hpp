@@ -326,10 +326,13 @@ void launch_compose_world(physgpu_context* ctx);
 // more significant field).  Result ends in *result (one of a, b).  With
 // gather_dst, the final pass also writes gather_dst[p] = gather_src[index]
 // (2 float4 records) for the item at sorted position p, index = item >> 32.
+// With first_hist_done the caller has written the first pass's per-tile
+// histogram (ctx->histogram[digit * tiles + tile], tiles of
+// kSortThreads x ctx->tuning.sort_items items, n items on the host).
 void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                 const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
                 uint32_t bits1, uint64_t** result, const float4* gather_src = nullptr,
-                float4* gather_dst = nullptr);
+                float4* gather_dst = nullptr, bool first_hist_done = false);
 // sap.hip
 void launch_sap_keys(physgpu_context* ctx);
 void launch_gather_sorted(physgpu_context* ctx, const uint64_t* sorted_items);

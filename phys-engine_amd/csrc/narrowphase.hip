@@ -19,6 +19,7 @@
 
 #include "contact.hpp"
 #include "devmath.hpp"
+#include "fold.hpp"
 #include "internal.hpp"
 #include "vjp.hpp"
 #include "wave.hpp"
@@ -1303,8 +1304,18 @@ __global__ __launch_bounds__(kBlock) void k_tile_hits(const uint32_t* __restrict
     }
 }
 
-__global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* __restrict__ tile_sums, Counters* c)
+// Workgroup 0: exclusive scan of the tile hit counts and the manifold count;
+// workgroups 1.. fold the narrow phase's partial slots (list entry b - 1).
+__global__ __launch_bounds__(kFoldThreads) void k_tile_scan(uint32_t* __restrict__ tile_sums,
+                                                            Counters* c,
+                                                            const uint64_t* __restrict__ partials,
+                                                            FoldList fold)
 {
+    static_assert(kFoldThreads == 1024, "one scan entry per thread of 1024");
+    if (blockIdx.x > 0) {
+        fold_entry(partials, fold.e[blockIdx.x - 1], c);
+        return;
+    }
     __shared__ uint32_t sums[1024];
     const uint32_t count = c->sort_count;
     const uint32_t tiles = (count + kScanTile - 1) / kScanTile;
@@ -1496,12 +1507,11 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
                      uint32_t(offsetof(Counters, epa_overflow))};
         fold.count = 8;
     }
-    launch_fold(ctx, fold);
     time_begin(ctx, "np_compact", t);
     hipLaunchKernelGGL(k_tile_hits, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->hit,
                        ctx->np_perm, ctx->counters, ctx->tile_sums);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_sums,
-                       ctx->counters);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1 + fold.count), dim3(kFoldThreads), 0, ctx->stream,
+                       ctx->tile_sums, ctx->counters, ctx->partials, fold);
     hipLaunchKernelGGL(k_compact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->hit,
                        ctx->np_perm, ctx->counters, ctx->tile_sums, ctx->slots, ctx->manifolds);
     time_end(ctx, t);

@@ -246,7 +246,7 @@ template <uint32_t kItems>
 void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                       const uint32_t* device_count, uint32_t shift0, uint32_t bits0,
                       uint32_t shift1, uint32_t bits1, uint64_t** result,
-                      const float4* gather_src, float4* gather_dst)
+                      const float4* gather_src, float4* gather_dst, bool first_hist_done)
 {
     const uint32_t tiles = div_up(n, kSortThreads * kItems);
     const uint32_t passes = (bits0 + 7) / 8 + (bits1 + 7) / 8;
@@ -254,12 +254,15 @@ void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n
     uint64_t* src = a;
     uint64_t* dst = b;
     auto pass = [&](uint32_t shift) {
+        const bool first = done == 0;
         const bool last = ++done == passes;
         Timer t;
-        time_begin(ctx, "radix_upsweep", t);
-        hipLaunchKernelGGL(k_upsweep<kItems>, dim3(tiles), dim3(kSortThreads), 0, ctx->stream,
-                           src, n, device_count, shift, ctx->histogram);
-        time_end(ctx, t);
+        if (!(first && first_hist_done)) {
+            time_begin(ctx, "radix_upsweep", t);
+            hipLaunchKernelGGL(k_upsweep<kItems>, dim3(tiles), dim3(kSortThreads), 0,
+                               ctx->stream, src, n, device_count, shift, ctx->histogram);
+            time_end(ctx, t);
+        }
         time_begin(ctx, "radix_scan", t);
         hipLaunchKernelGGL(k_scan_rows<kItems>, dim3(256), dim3(kSortThreads), 0, ctx->stream,
                            ctx->histogram, n, device_count, ctx->digit_total);
@@ -286,22 +289,23 @@ void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n
 
 void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                 const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
-                uint32_t bits1, uint64_t** result, const float4* gather_src, float4* gather_dst)
+                uint32_t bits1, uint64_t** result, const float4* gather_src, float4* gather_dst,
+                bool first_hist_done)
 {
     // Tile = 256 threads x sort_items (4, 8 or 16); the histogram is sized for
     // the smallest tile.
     switch (ctx->tuning.sort_items) {
     case 4:
         radix_sort_tiled<4>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
-                                gather_src, gather_dst);
+                                gather_src, gather_dst, first_hist_done);
         break;
     case 16:
         radix_sort_tiled<16>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
-                                gather_src, gather_dst);
+                                gather_src, gather_dst, first_hist_done);
         break;
     default:
         radix_sort_tiled<8>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
-                                gather_src, gather_dst);
+                                gather_src, gather_dst, first_hist_done);
         break;
     }
 }
