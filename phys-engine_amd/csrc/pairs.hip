@@ -80,32 +80,6 @@ __global__ __launch_bounds__(256) void k_tile_sum(const uint32_t* __restrict__ v
         tiles[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-// Exclusive scan of `count` tile sums by one workgroup: a sequential run per
-// thread, then a scan of the 1024 run totals.
-__global__ __launch_bounds__(1024) void k_tile_offsets(uint32_t* __restrict__ tiles, uint32_t count)
-{
-    __shared__ uint32_t sums[1024];
-    const uint32_t per = (count + 1023) / 1024;
-    const uint32_t begin = min(threadIdx.x * per, count), end = min(begin + per, count);
-    uint32_t s = 0;
-    for (uint32_t i = begin; i < end; ++i)
-        s += tiles[i];
-    sums[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? sums[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sums[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
-    for (uint32_t i = begin; i < end; ++i) {
-        const uint32_t v = tiles[i];
-        tiles[i] = run;
-        run += v;
-    }
-}
-
 // Exclusive scan of one value per thread over a 256-thread workgroup;
 // `total` gets the sum.  Every thread must call it.
 __device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* wave_tot, uint32_t& total)
@@ -129,11 +103,18 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* wave_t
 }
 
 // In-place exclusive scan of each tile plus its tile offset.
+// Exclusive scan of tile blockIdx.x; its offset is the sum of the earlier
+// tiles' sums (at most a few hundred), reduced by the workgroup itself.
 __global__ __launch_bounds__(256) void k_tile_apply(uint32_t* __restrict__ v, uint32_t n,
                                                     const uint32_t* __restrict__ tiles)
 {
     __shared__ uint32_t wave_tot[4];
     constexpr uint32_t kPer = kTile / 256;
+    uint32_t before = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256)
+        before += tiles[t];
+    uint32_t all;
+    block_exclusive(before, wave_tot, all);
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
     uint32_t x[kPer], mine = 0;
 #pragma unroll
@@ -142,7 +123,7 @@ __global__ __launch_bounds__(256) void k_tile_apply(uint32_t* __restrict__ v, ui
         mine += x[i];
     }
     uint32_t total;
-    uint32_t run = tiles[blockIdx.x] + block_exclusive(mine, wave_tot, total);
+    uint32_t run = all + block_exclusive(mine, wave_tot, total);
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
         if (base + i < n)
@@ -427,7 +408,6 @@ void scan_u32(physgpu_context* ctx, uint32_t* v, uint32_t n, hipStream_t stream)
 {
     const uint32_t tiles = div_up(n, kTile);
     hipLaunchKernelGGL(k_tile_sum, dim3(tiles), dim3(256), 0, stream, v, n, ctx->seg_tiles);
-    hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, stream, ctx->seg_tiles, tiles);
     hipLaunchKernelGGL(k_tile_apply, dim3(tiles), dim3(256), 0, stream, v, n, ctx->seg_tiles);
 }
 
