@@ -318,7 +318,7 @@ __device__ f3 support_wave(const Bodies& B, const Shape& s, const pose& t, f3 d,
             bv = dot(h.p, l);
             bi = lane;
         }
-        if (!wave_arg_best_lanes<true>(bv, bi, 1)) {
+        if (!row_arg_best_lanes<true>(bv, bi, 1, 0)) {
             row_arg_best<true>(bv, bi);
             bi = lane_value(bi, 0);
         }

@@ -352,8 +352,15 @@ __device__ __forceinline__ bool wave_arg_best_lanes(float& v, uint32_t& i, uint3
     x = pick(x, dpp_f32<0x4E>(x));
     x = pick(x, dpp_f32<0x141>(x));
     x = pick(x, dpp_f32<0x140>(x));
-    const float m = pick(pick(lane_value(x, 0), lane_value(x, 16)),
-                         pick(lane_value(x, 32), lane_value(x, 48)));
+    // Rows to wave: v_permlane16_swap pairs rows 0-1 and 2-3, then
+    // v_permlane32_swap the two halves; both results of each swap hold the
+    // two partners' values, so every lane ends with the wave's extreme.
+    const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_int(x), __float_as_int(x), false,
+                                                      false);
+    x = pick(__int_as_float(r16[0]), __int_as_float(r16[1]));
+    const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(x), false,
+                                                      false);
+    const float m = pick(__int_as_float(r32[0]), __int_as_float(r32[1]));
     for (uint32_t c = 0; c < chunks; ++c) {
         const uint64_t at = __ballot(has && v == m && (i >> 6) == c);
         if (at) {
