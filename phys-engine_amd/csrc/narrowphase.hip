@@ -973,6 +973,30 @@ __device__ __forceinline__ void epa_partial_add(uint64_t* slot, uint32_t v)
                   (unsigned long long)v);
 }
 
+constexpr uint32_t kRowsPerWave = 64 / kGjkLanes;
+
+// Hull candidate t of the class 3, 4, 5 buckets taken in that order.
+struct Candidate {
+    uint32_t cls, k, a, b;
+};
+
+__device__ __forceinline__ Candidate candidate(uint32_t t, uint32_t c3, uint32_t c4,
+                                               const uint32_t* __restrict__ buckets, uint64_t cap,
+                                               const uint64_t* __restrict__ pairs)
+{
+    uint32_t cls = 3, q = t;
+    if (q >= c3 + c4) {
+        cls = 5;
+        q -= c3 + c4;
+    } else if (q >= c3) {
+        cls = 4;
+        q -= c3;
+    }
+    const uint32_t k = buckets[cls * cap + q];
+    const uint64_t pr = pairs[k];
+    return {cls, k, uint32_t(pr >> 32), uint32_t(pr)};
+}
+
 __global__ __launch_bounds__(kEpaBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
                const uint32_t* __restrict__ buckets, uint64_t cap,
@@ -980,32 +1004,36 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
                uint32_t* __restrict__ hit, uint64_t* __restrict__ partials)
 {
     __shared__ EpaShared S;
+    // GJK simplices (4 points x 3) and candidate positions of the rows that
+    // found an intersection, for the EPA runs that follow.
+    __shared__ float stage_p[kRowsPerWave][12];
+    __shared__ uint32_t stage_t[kRowsPerWave];
     const uint32_t lane = lane_id(), rl = lane % kGjkLanes, row_base = lane - rl;
+    const uint32_t row = lane / kGjkLanes;
     const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
     const uint32_t total = c3 + c4 + c->class_count[5];
+    const uint32_t items = c3 + (c4 + c->class_count[5] + kRowsPerWave - 1) / kRowsPerWave;
     uint32_t contacts3 = 0, contacts4 = 0, contacts5 = 0;
     uint32_t failures = 0, overflows = 0;
     uint32_t work = 0;  // EPA iterations of this wavefront so far
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        uint32_t cls = 3, q = t;
-        if (q >= c3 + c4) {
-            cls = 5;
-            q -= c3 + c4;
-        } else if (q >= c3) {
-            cls = 4;
-            q -= c3;
-        }
-        const uint32_t k = buckets[cls * cap + q];
-        const uint64_t pr = pairs[k];
-        const uint32_t a = uint32_t(pr >> 32), b = uint32_t(pr);
-        const Shape sa = load_shape(B, a), sb = load_shape(B, b);
-        const pose ta = load_pose(B, a), tb = load_pose(B, b);
+    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+        // Sphere-hull items are one candidate (all four rows on it); the
+        // box-hull and hull-hull candidates that follow go four to an item,
+        // one per row.
+        const bool solo = item < c3;
+        const uint32_t tr = solo ? item : c3 + kRowsPerWave * (item - c3) + row;
+        const bool valid = tr < total;
         Simplex sx;
         bool intersect = false;
         {
+            const uint32_t t = valid ? tr : total - 1;
+            const Candidate cd = candidate(t, c3, c4, buckets, cap, pairs);
+            const Shape sa = load_shape(B, cd.a), sb = load_shape(B, cd.b);
+            const pose ta = load_pose(B, cd.a), tb = load_pose(B, cd.b);
             const RowHull ra = row_hull(B, sa, ta, rl), rb = row_hull(B, sb, tb, rl);
             sx.size = 0;
-            f3 dir{1.0f, 0.0f, 0.0f};
+            // A row past the last candidate stops at once (zero direction).
+            f3 dir{valid ? 1.0f : 0.0f, 0.0f, 0.0f};
             for (uint32_t it = 0; it < opt.gjk_max_iterations; ++it) {
                 if (dot(dir, dir) < 1e-12f)
                     break;
@@ -1030,43 +1058,63 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
                 }
             }
         }
-        if (!intersect)
+        // Bit 16 r: row r intersects.
+        uint64_t rows_hit = __ballot(intersect && valid && rl == 0);
+        if (solo)
+            rows_hit &= 1ull;
+        if (!rows_hit)
             continue;
-        if (lane == 0) {
+        if (intersect && valid && rl == 0 && (!solo || row == 0)) {
 #pragma unroll
             for (uint32_t v = 0; v < 4; ++v) {
-                S.vx[v] = sx.p[v].x;
-                S.vy[v] = sx.p[v].y;
-                S.vz[v] = sx.p[v].z;
+                stage_p[row][3 * v] = sx.p[v].x;
+                stage_p[row][3 * v + 1] = sx.p[v].y;
+                stage_p[row][3 * v + 2] = sx.p[v].z;
             }
-        }
-        f3 n;
-        float depth;
-        const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
-        const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, nullptr, opt.epa_max_iterations,
-                                opt.epa_tolerance, n, depth, nullptr, &work);
-        if (st == 1) {
-            const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
-                                   support_wave(B, sb, tb, neg(n), hb)),
-                               0.5f);
-            if (lane == 0) {
-                Contact ct;
-                ct.normal = n;
-                ct.count = 1;
-                ct.depth[0] = depth;
-                set_anchors(ct, 0, w, ta, tb);
-                store_contact(ct, k, a, b, cls, W, B.n, slots);
-                hit[k] = 1;
-            }
-            contacts3 += cls == 3 ? 1u : 0u;
-            contacts4 += cls == 4 ? 1u : 0u;
-            contacts5 += cls == 5 ? 1u : 0u;
-        } else if (st == 0) {
-            ++failures;
-        } else {
-            ++overflows;
+            stage_t[row] = tr;
         }
         wave_sync();
+        while (rows_hit) {
+            const uint32_t r = uint32_t(__ffsll((unsigned long long)rows_hit) - 1) / kGjkLanes;
+            rows_hit &= rows_hit - 1;
+            const Candidate cd = candidate(uniform(stage_t[r]), c3, c4, buckets, cap, pairs);
+            const uint32_t a = cd.a, b = cd.b, k = cd.k, cls = cd.cls;
+            const Shape sa = load_shape(B, a), sb = load_shape(B, b);
+            const pose ta = load_pose(B, a), tb = load_pose(B, b);
+            if (lane < 4) {
+                S.vx[lane] = stage_p[r][3 * lane];
+                S.vy[lane] = stage_p[r][3 * lane + 1];
+                S.vz[lane] = stage_p[r][3 * lane + 2];
+            }
+            f3 n;
+            float depth;
+            const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
+            const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, nullptr,
+                                    opt.epa_max_iterations, opt.epa_tolerance, n, depth, nullptr,
+                                    &work);
+            if (st == 1) {
+                const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
+                                       support_wave(B, sb, tb, neg(n), hb)),
+                                   0.5f);
+                if (lane == 0) {
+                    Contact ct;
+                    ct.normal = n;
+                    ct.count = 1;
+                    ct.depth[0] = depth;
+                    set_anchors(ct, 0, w, ta, tb);
+                    store_contact(ct, k, a, b, cls, W, B.n, slots);
+                    hit[k] = 1;
+                }
+                contacts3 += cls == 3 ? 1u : 0u;
+                contacts4 += cls == 4 ? 1u : 0u;
+                contacts5 += cls == 5 ? 1u : 0u;
+            } else if (st == 0) {
+                ++failures;
+            } else {
+                ++overflows;
+            }
+            wave_sync();
+        }
     }
     __builtin_amdgcn_s_setprio(0);
     // Counts are wave-uniform: lane 0 adds the wavefront's share.
