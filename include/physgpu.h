@@ -204,6 +204,9 @@ typedef struct physgpu_tuning {
     uint32_t query_block;      /* grid query workgroup size: 64, 128, 256 [256] */
     uint32_t query_groups;     /* [2048] */
     uint32_t sort_items;       /* radix sort items per thread: 4, 8, 16 (tile 256 x items) [8] */
+    uint32_t sphere_hull_per_wave; /* sphere-hull candidates per GJK + EPA wavefront: 1 (its EPA
+                                      starts at once) or 4 (one GJK per 16-lane row, then their
+                                      EPA runs in turn, as for box-hull and hull-hull) [1] */
 } physgpu_tuning;
 
 typedef struct physgpu_context physgpu_context;

@@ -165,6 +165,9 @@ physgpu_tuning default_tuning()
     t.query_block = kBlock;
     t.query_groups = 2048;  // at 1M bodies, 256 x 1024 / 2048 / 4096: 71 / 63 / 78 us
     t.sort_items = 8;
+    // At 1M bodies the last long sphere-hull EPA runs end the kernel: four to a
+    // wavefront, runs queue behind each other (GJK + EPA 292 -> 427 us).
+    t.sphere_hull_per_wave = 1;
     return t;
 }
 
@@ -193,11 +196,15 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
         out.query_groups = d.query_groups;
     if (!out.sort_items)
         out.sort_items = d.sort_items;
+    if (!out.sphere_hull_per_wave)
+        out.sphere_hull_per_wave = d.sphere_hull_per_wave;
     if (!valid_epa_groups(out.epa_groups) || !valid_block(out.query_block) ||
         !valid_groups(out.query_groups))
         return fail(PHYSGPU_INVALID_ARGUMENT, "kernel shape out of range");
     if (out.sort_items != 4 && out.sort_items != 8 && out.sort_items != 16)
         return fail(PHYSGPU_INVALID_ARGUMENT, "sort_items must be 4, 8 or 16");
+    if (out.sphere_hull_per_wave != 1 && out.sphere_hull_per_wave != 4)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "sphere_hull_per_wave must be 1 or 4");
     return PHYSGPU_OK;
 }
 
@@ -968,6 +975,20 @@ physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t re
             }
         }
         ctx->tuning.epa_groups = best_g;
+        uint32_t best_w = ctx->tuning.sphere_hull_per_wave;
+        best = 1e30;
+        for (uint32_t w : {1u, 4u}) {
+            if (st != PHYSGPU_OK)
+                break;
+            ctx->tuning.sphere_hull_per_wave = w;
+            double ms = 0.0;
+            st = measure({"np_gjk_epa"}, ms);
+            if (st == PHYSGPU_OK && ms < best) {
+                best = ms;
+                best_w = w;
+            }
+        }
+        ctx->tuning.sphere_hull_per_wave = best_w;
     }
     if (st == PHYSGPU_OK && algorithm == PHYSGPU_UNIFORM_GRID)
         st = tune_shape(&ctx->tuning.query_block, &ctx->tuning.query_groups, "grid_query");

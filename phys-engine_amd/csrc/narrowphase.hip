@@ -1001,7 +1001,7 @@ __global__ __launch_bounds__(kEpaBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
                const uint32_t* __restrict__ buckets, uint64_t cap,
                physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
-               uint32_t* __restrict__ hit, uint64_t* __restrict__ partials)
+               uint32_t* __restrict__ hit, uint64_t* __restrict__ partials, bool solo_sphere_hull)
 {
     __shared__ EpaShared S;
     // GJK simplices (4 points x 3) and candidate positions of the rows that
@@ -1012,16 +1012,17 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
     const uint32_t row = lane / kGjkLanes;
     const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
     const uint32_t total = c3 + c4 + c->class_count[5];
-    const uint32_t items = c3 + (c4 + c->class_count[5] + kRowsPerWave - 1) / kRowsPerWave;
+    const uint32_t solo_items = solo_sphere_hull ? c3 : 0;
+    const uint32_t items = solo_items + (total - solo_items + kRowsPerWave - 1) / kRowsPerWave;
     uint32_t contacts3 = 0, contacts4 = 0, contacts5 = 0;
     uint32_t failures = 0, overflows = 0;
     uint32_t work = 0;  // EPA iterations of this wavefront so far
     for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
-        // Sphere-hull items are one candidate (all four rows on it); the
-        // box-hull and hull-hull candidates that follow go four to an item,
-        // one per row.
-        const bool solo = item < c3;
-        const uint32_t tr = solo ? item : c3 + kRowsPerWave * (item - c3) + row;
+        // Sphere-hull items are one candidate (all four rows on it) when
+        // solo_sphere_hull is set; the other candidates go four to an item,
+        // one per row, and run their EPAs in turn.
+        const bool solo = item < solo_items;
+        const uint32_t tr = solo ? item : solo_items + kRowsPerWave * (item - solo_items) + row;
         const bool valid = tr < total;
         Simplex sx;
         bool intersect = false;
@@ -1516,7 +1517,7 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
         time_begin(ctx, "np_gjk_epa", t);
         hipLaunchKernelGGL(k_gjk_epa, dim3(tu.epa_groups), dim3(kEpaBlock), 0, ctx->stream, B,
                            W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
-                           ctx->hit, ctx->partials);
+                           ctx->hit, ctx->partials, tu.sphere_hull_per_wave == 1);
         time_end(ctx, t);
     }
     time_begin(ctx, "np_sphere_sphere", t, cs);

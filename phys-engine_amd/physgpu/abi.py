@@ -156,6 +156,7 @@ class Tuning(C.Structure):
         ("query_block", C.c_uint32),
         ("query_groups", C.c_uint32),
         ("sort_items", C.c_uint32),
+        ("sphere_hull_per_wave", C.c_uint32),
     ]
 
     def as_dict(self):

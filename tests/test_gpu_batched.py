@@ -154,15 +154,18 @@ def test_autotune_keeps_results(bpipe, batch):
     bpipe.collide(algorithm=abi.UNIFORM_GRID)
     assert np.array_equal(bpipe.pairs(), want_p)
     assert np.array_equal(words(bpipe.manifolds()), words(want_m))
-    for items, epa in ((4, 7), (16, 10000)):
+    assert t["sphere_hull_per_wave"] in (1, 4)
+    for items, epa, shw in ((4, 7, 4), (16, 10000, 1)):
         bpipe.set_tuning(dict(t, sort_items=items, class_block=[64, 128, 64],
                               class_groups=[7, 300, 5000], query_block=64, query_groups=33,
-                              epa_groups=epa))
+                              epa_groups=epa, sphere_hull_per_wave=shw))
         bpipe.collide_graph(algorithm=abi.UNIFORM_GRID)
         assert np.array_equal(bpipe.pairs(), want_p)
         assert np.array_equal(words(bpipe.manifolds()), words(want_m))
     with pytest.raises(ValueError):
         bpipe.set_tuning(dict(sort_items=5))
+    with pytest.raises(ValueError):
+        bpipe.set_tuning(dict(sphere_hull_per_wave=2))
     bpipe.set_tuning(None)
 
 
