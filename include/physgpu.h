@@ -199,8 +199,9 @@ typedef struct physgpu_narrowphase_options {
 typedef struct physgpu_tuning {
     uint32_t class_block[3];   /* workgroup size of the SS, SB/BS, BB kernels: 64, 128, 256 [256] */
     uint32_t class_groups[3];  /* workgroups of those grid-stride kernels: 1..8192 [2048] */
-    uint32_t epa_groups;       /* GJK + EPA workgroups of one wavefront; hull candidate e goes
-                                  to workgroup e mod epa_groups: 1..1048576 [65536] */
+    uint32_t epa_groups;       /* GJK + EPA workgroups of one wavefront; work item e (one
+                                  sphere-hull candidate, or four hull candidates) goes to
+                                  workgroup e mod epa_groups: 1..1048576 [65536] */
     uint32_t query_block;      /* grid query workgroup size: 64, 128, 256 [256] */
     uint32_t query_groups;     /* [2048] */
     uint32_t sort_items;       /* radix sort items per thread: 4, 8, 16 (tile 256 x items) [8] */

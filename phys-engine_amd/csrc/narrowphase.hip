@@ -953,16 +953,17 @@ __device__ f3 support_row(const Bodies& B, const Shape& s, const pose& t, f3 d, 
     return to_world(f3{B.hpts[p], B.hpts[nh + p], B.hpts[2 * nh + p]}, t);
 }
 
-// Hull pairs in one wavefront each: GJK (the row code above, the four rows
-// on the same pair, so every lane holds the same result) and, for an
-// intersecting pair, EPA; the contact point is the midpoint of the two
-// deepest support points along the EPA normal (A toward B).  Each workgroup
-// is one wavefront and takes hull candidate blockIdx.x (then every
-// gridDim.x-th after it): with at least as many workgroups as candidates the
-// dispatcher hands them out in order as wavefront slots free up, a work queue
-// with no atomics.  The sphere-hull bucket comes first, so its long GJK + EPA
-// runs start first, and a pair's EPA follows its own GJK with no grid-wide
-// wait in between.  Counts go to the EPA partial slots (zeroed by
+// Hull pairs: GJK (the row code above) and, for an intersecting pair, EPA
+// by the whole wavefront; the contact point is the midpoint of the two
+// deepest support points along the EPA normal (A toward B).  A work item is
+// one sphere-hull candidate (the four rows on the same pair, so its possibly
+// long EPA run starts at once) or four candidates, one per row, whose EPA
+// runs follow one another.  Each workgroup is one wavefront and takes item
+// blockIdx.x (then every gridDim.x-th after it): with at least as many
+// workgroups as items the dispatcher hands them out in order as wavefront
+// slots free up, a work queue with no atomics.  The sphere-hull bucket comes
+// first, so its long runs start first, and a pair's EPA follows its own GJK
+// with no grid-wide wait in between.  Counts go to the EPA partial slots (zeroed by
 // k_classify) with one atomic add per nonzero count.
 constexpr uint32_t kEpaBlock = 64;
 
