@@ -35,8 +35,8 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 # with its commands is profiles/grid_step_mixed1M_hbm_traffic.md.  Valid for
 # the 1,048,576-body mixed scene only.
 MEASURED_TRAFFIC_1M = {
-    "grid_query": 58.74e6,
-    "aabb_from_pose": 152.59e6,
+    "grid_query": 58.75e6,
+    "aabb_from_pose": 152.53e6,
 }
 
 
