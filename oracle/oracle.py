@@ -71,6 +71,8 @@ class CpuCollision:
         if which == "oracle":
             L.orc_set_epa_bounds.argtypes = [C.c_uint32, C.c_uint32]
             L.orc_set_epa_bounds.restype = None
+            L.orc_set_test_mutation.argtypes = [C.c_int]
+            L.orc_set_test_mutation.restype = None
             L.orc_contact_adjoint_f64.argtypes = [C.POINTER(abi.Colliders), vp, C.c_uint64, vp,
                                                   vp, vp, C.c_uint32, vp, vp]
             L.orc_contact_fd_f64.argtypes = [C.POINTER(abi.Colliders), vp, C.c_uint64, vp, vp, vp,
@@ -87,6 +89,13 @@ class CpuCollision:
         unbounded behaviour."""
         assert self.which == "oracle"
         self.lib.orc_set_epa_bounds(max_faces, max_edges)
+
+    def set_test_mutation(self, mutation=0):
+        """Plant a known mistake in the oracle's hull contact point (tests of
+        the tests only): 1 supports along the wrong signs, 2 A's support
+        point instead of the midpoint; 0 restores the oracle."""
+        assert self.which == "oracle"
+        self.lib.orc_set_test_mutation(mutation)
 
     def _check(self, rc, what):
         if rc not in (abi.OK,):

@@ -879,6 +879,11 @@ int epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, const N
     return 0;
 }
 
+// Planted mistakes for the tests of the tests (orc_set_test_mutation): 0 none,
+// 1 the hull contact point from the supports along the wrong signs, 2 the
+// contact point at A's support point (weight 1 instead of 1/2 on each side).
+std::atomic<int> g_mutation{0};
+
 // Hull-involving pairs: GJK then EPA; one contact point at the midpoint of
 // the two deepest support points along the EPA normal (A toward B).
 template <typename T>
@@ -906,6 +911,11 @@ bool convexPair(const Shape<T>& a, const Pose<T>& ta, const Shape<T>& b, const P
     c.count = 1;
     c.depth[0] = depth;
     V3<T> w = scale(add(sa(n), sb(neg(n))), T(0.5f));
+    const int mutation = g_mutation.load(std::memory_order_relaxed);
+    if (mutation == 1)
+        w = scale(add(sa(neg(n)), sb(n)), T(0.5f));
+    else if (mutation == 2)
+        w = sa(n);
     setAnchors(c, 0, w, ta, tb);
     return true;
 }
@@ -1268,6 +1278,9 @@ void orc_epa_histograms(uint64_t* iters, uint64_t* visible, int reset)
             np::g_epa_visible[i] = 0;
     }
 }
+
+// Test hook: plant one of the mistakes of g_mutation (0 restores the oracle).
+void orc_set_test_mutation(int mutation) { np::g_mutation.store(mutation); }
 
 // EPA polytope bounds used by the following calls (0, 0 = unbounded, the
 // reference's behaviour).  Test infrastructure: not thread-safe.
