@@ -33,7 +33,11 @@ typedef enum physgpu_status {
     PHYSGPU_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument */
     PHYSGPU_CAPACITY_EXCEEDED = 2, /* output or context capacity too small */
     PHYSGPU_DEVICE_ERROR = 3,      /* HIP runtime failure */
-    PHYSGPU_UNSUPPORTED = 4
+    PHYSGPU_UNSUPPORTED = 4,
+    /* Every result of the call is written, but some hull pairs got no EPA
+     * result (no contact, counted in epa_unresolved); physgpu_epa_unresolved
+     * lists them.  Returned by the calls that read the step's counts. */
+    PHYSGPU_EPA_UNRESOLVED = 5
 } physgpu_status;
 
 /* phys::BroadPhaseAlgorithm, include/phys/collision/broadphase.hpp:8-13.
@@ -110,17 +114,47 @@ typedef struct physgpu_narrowphase_stats {
     uint64_t hull_candidates;   /* pairs with at least one hull (GJK/EPA) */
     uint64_t hull_contacts;
     uint64_t gjk_epa_failures;  /* GJK hit but EPA did not converge */
-    uint64_t epa_overflows;     /* EPA polytope exceeded the bounds below */
+    uint64_t epa_overflows;     /* EPA runs continued past the LDS bounds below */
+    uint64_t epa_unresolved;    /* hull pairs left without an EPA result */
 } physgpu_narrowphase_stats;
 
-/* Bounded EPA polytope of the GPU narrow phase.  The reference's EPA
- * (src/collision/epa.cpp:64-130) is replayed exactly while it holds at most
- * PHYSGPU_EPA_MAX_FACES faces and its horizon at most PHYSGPU_EPA_MAX_HORIZON
- * edges; a pair that would exceed them gets no contact and is counted in
- * epa_overflows.  (In FP32 the reference's polytope can duplicate faces
- * without bound on near-degenerate pairs.) */
+/* The collision fields of phys::StepStats (include/phys/world/
+ * step_stats.hpp:33-54) for the last device-resident step (physgpu_collide /
+ * physgpu_collide_graph).  Stage times are device time between HIP events
+ * around the broad phase (after the AABB refresh) and the narrow phase when
+ * timing is enabled on the context (physgpu_set_timing), else 0; the
+ * reference's collect and filter sub-stages have no separate device stage
+ * (bounds are written by the AABB pass, the same-body filter runs in the
+ * narrow phase's classification) and stay 0. */
+typedef struct physgpu_step_stats {
+    double broad_phase_ms;
+    double narrow_phase_ms;
+    double broad_phase_collect_ms;
+    double broad_phase_filter_ms;
+    uint64_t possible_pairs;   /* collider pairs on different bodies, per world
+                                  (physics_world.cpp:571-577) */
+    uint64_t candidate_pairs;  /* broad-phase pairs left after the same-body
+                                  filter (:630-636) */
+    uint64_t contact_count;    /* manifolds (:739) */
+    physgpu_broadphase_stats broad_phase;
+    physgpu_narrowphase_stats narrow_phase;
+} physgpu_step_stats;
+
+/* EPA polytope of the GPU narrow phase.  The reference's EPA
+ * (src/collision/epa.cpp:64-130) is replayed exactly in LDS while it holds
+ * at most PHYSGPU_EPA_MAX_FACES faces and its horizon at most
+ * PHYSGPU_EPA_MAX_HORIZON edges.  In FP32 the reference's polytope can fill
+ * with copies of the same faces on near-degenerate pairs (hundreds of
+ * thousands of faces); a run that outgrows those bounds (counted in
+ * epa_overflows) is continued over distinct faces with copy counts, up to
+ * PHYSGPU_EPA_MAX_DISTINCT_FACES of them, which gives the reference's result
+ * except where the reference's closest-face choice would depend on the order
+ * of copies it does not keep: such pairs, and pairs past that bound, get no
+ * contact and are counted in epa_unresolved (and listed by
+ * physgpu_epa_unresolved). */
 #define PHYSGPU_EPA_MAX_FACES 256
 #define PHYSGPU_EPA_MAX_HORIZON 128
+#define PHYSGPU_EPA_MAX_DISTINCT_FACES 1024
 #define PHYSGPU_EPA_MAX_ITERATIONS 64
 
 /* Shape-pair classes.  The reference's shapePair = index(A) + index(B)
@@ -294,11 +328,24 @@ physgpu_status physgpu_download_pairs(physgpu_context* ctx, physgpu_pair* out, u
 physgpu_status physgpu_download_manifolds(physgpu_context* ctx, physgpu_manifold* out,
                                           uint64_t capacity);
 /* Counts of the last step: candidate pairs found (may exceed the capacity)
- * and manifolds produced.  Synchronises the context's stream. */
+ * and manifolds produced.  Synchronises the context's stream.  Returns
+ * PHYSGPU_EPA_UNRESOLVED (counts written) when some hull pairs of the step
+ * got no EPA result. */
 physgpu_status physgpu_step_counts(physgpu_context* ctx, uint64_t* pair_count,
                                    uint64_t* manifold_count);
+/* The hull pairs of the last narrow phase left without an EPA result
+ * (epa_unresolved), as (first, second) collider indices in increasing order:
+ * up to `capacity` of them to `out` (may be NULL), their number to *count.
+ * No counterpart in the reference, whose unbounded list reaches a result
+ * (after minutes on some pairs); a caller may run its CPU EPA on these.
+ * Synchronises the context's stream. */
+physgpu_status physgpu_epa_unresolved(physgpu_context* ctx, physgpu_pair* out, uint64_t capacity,
+                                      uint64_t* count);
 physgpu_status physgpu_narrowphase_stats_get(physgpu_context* ctx,
                                              physgpu_narrowphase_stats* stats);
+/* StepStats collision fields of the last step (see physgpu_step_stats).
+ * Synchronises the context's stream. */
+physgpu_status physgpu_step_stats_get(physgpu_context* ctx, physgpu_step_stats* stats);
 
 /* ---- Batches of worlds ---------------------------------------------------
  * No counterpart in the reference, which steps one PhysicsWorld at a time;
@@ -358,8 +405,10 @@ physgpu_status physgpu_autotune(physgpu_context* ctx, int algorithm, uint32_t re
  * box-sphere and box-box (the SAT axis, clipping vertices or fallback point
  * the forward chose), box-hull and hull-hull (the face EPA converged on, with
  * its vertices' hull points / box corners and the contact point's support
- * features held fixed; GJK and EPA are recomputed).  Sphere-hull manifolds get
- * zero gradients and are counted in *skipped.  Gradients are with respect to the colliders' world poses (the
+ * features held fixed; GJK and EPA are recomputed).  Sphere-hull manifolds,
+ * and hull manifolds whose EPA ran past the LDS polytope bounds (touching
+ * contacts of degenerate polytopes), get zero gradients and are counted in
+ * *skipped.  Gradients are with respect to the colliders' world poses (the
  * resident poses, or the composed ones after physgpu_set_world); quaternion
  * gradients are tangent to the unit sphere (gradients with respect to the
  * orientation q/|q|).

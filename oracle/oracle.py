@@ -71,6 +71,8 @@ class CpuCollision:
         if which == "oracle":
             L.orc_set_epa_bounds.argtypes = [C.c_uint32, C.c_uint32]
             L.orc_set_epa_bounds.restype = None
+            L.orc_set_epa_continuation.argtypes = [C.c_uint32, C.c_int]
+            L.orc_set_epa_continuation.restype = None
             L.orc_set_test_mutation.argtypes = [C.c_int]
             L.orc_set_test_mutation.restype = None
             L.orc_contact_adjoint_f64.argtypes = [C.POINTER(abi.Colliders), vp, C.c_uint64, vp,
@@ -83,12 +85,36 @@ class CpuCollision:
         getattr(L, p + "world_collide").argtypes = [
             C.POINTER(abi.Colliders), vp, vp, vp, vp, C.c_int, C.c_uint32, C.c_uint32,
             C.c_uint32, C.c_uint32, vp, C.c_uint64, C.POINTER(C.c_uint64), vp, vp]
+        getattr(L, p + "world_collide_bodies").argtypes = [
+            C.POINTER(abi.Colliders), vp, vp, vp, vp, vp, C.c_int, C.c_uint32, C.c_uint32,
+            C.c_uint32, C.c_uint32, vp, C.c_uint64, C.POINTER(C.c_uint64), vp, vp]
 
     def set_epa_bounds(self, max_faces=0, max_edges=0):
         """Bounded EPA polytope (the GPU's limits) or 0, 0 for the reference's
         unbounded behaviour."""
         assert self.which == "oracle"
         self.lib.orc_set_epa_bounds(max_faces, max_edges)
+
+    def mirror_gpu_epa(self, on=True, continuation=True):
+        """EPA as the GPU narrow phase runs it: the reference's list within
+        the LDS bounds, then (with continuation) the multiplicity form over
+        at most EPA_MAX_DISTINCT_FACES distinct faces, with the order of new
+        faces within an iteration untracked.  on=False restores the
+        reference's unbounded EPA."""
+        if on:
+            self.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+            self.set_epa_continuation(abi.EPA_MAX_DISTINCT_FACES if continuation else 0,
+                                      exact_order=False)
+        else:
+            self.set_epa_bounds(0, 0)
+            self.set_epa_continuation(0)
+
+    def set_epa_continuation(self, multi_faces=0, exact_order=True):
+        """Past the polytope bounds, continue EPA over distinct faces with
+        multiplicities (at most multi_faces of them; 0: stop with the
+        overflow status), as the GPU narrow phase does."""
+        assert self.which == "oracle"
+        self.lib.orc_set_epa_continuation(multi_faces, int(exact_order))
 
     def set_test_mutation(self, mutation=0):
         """Plant a known mistake in the oracle's hull contact point (tests of
@@ -174,18 +200,31 @@ class CpuCollision:
 
     def world_collide(self, scene, algorithm=abi.SWEEP_AND_PRUNE, bp_workers=1, np_workers=1,
                       warmup=0, steps=1, capacity=None):
-        """World-level collision section; returns (manifolds, timing dict, np stats)."""
+        """World-level collision section; returns (manifolds, timing dict, np stats).
+
+        With scene.body set, colliders with equal body values share one body
+        (body_position, friction and restitution then hold each collider's
+        body's values); manifold first / second are collider indices in the
+        oracle and body indices in the reference (ContactManifold::bodyA / B)."""
         c = abi.colliders_struct(scene)
         count = C.c_uint64(0)
         timing = np.zeros(6, np.float64)
         np_stats = np.zeros(6, np.uint64)
         ptr = (lambda a: None if a is None else a.ctypes.data)
-        fn = getattr(self.lib, self.p + "world_collide")
         cap = capacity if capacity is not None else 4 * scene.count + 16
         out = np.zeros(cap, abi.MANIFOLD_DTYPE)
-        rc = fn(C.byref(c), ptr(scene.body_position), ptr(scene.local_pose), ptr(scene.friction),
-                ptr(scene.restitution), algorithm, bp_workers, np_workers, warmup, steps,
-                out.ctypes.data, cap, C.byref(count), timing.ctypes.data, np_stats.ctypes.data)
+        common = (algorithm, bp_workers, np_workers, warmup, steps, out.ctypes.data, cap,
+                  C.byref(count), timing.ctypes.data, np_stats.ctypes.data)
+        body = getattr(scene, "body", None)
+        if body is not None:
+            body = np.ascontiguousarray(body, np.uint32)
+            fn = getattr(self.lib, self.p + "world_collide_bodies")
+            rc = fn(C.byref(c), ptr(scene.body_position), body.ctypes.data, ptr(scene.local_pose),
+                    ptr(scene.friction), ptr(scene.restitution), *common)
+        else:
+            fn = getattr(self.lib, self.p + "world_collide")
+            rc = fn(C.byref(c), ptr(scene.body_position), ptr(scene.local_pose),
+                    ptr(scene.friction), ptr(scene.restitution), *common)
         self._check(rc, "world_collide")
         t = dict(broadphase_ms=timing[0], narrowphase_ms=timing[1], aabb_ms=timing[2],
                  total_ms=timing[3], candidate_pairs=int(timing[4]), possible_pairs=timing[5])

@@ -19,6 +19,8 @@
 
 #include <atomic>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -798,8 +800,221 @@ inline void note_peak(std::atomic<uint32_t>& a, size_t v)
     }
 }
 
+// computeEpa (epa.cpp:64-130) over a polytope of distinct faces with
+// multiplicities.  The reference's face list can hold many copies of one face
+// (a horizon edge that survives |s| times gives |s| copies of its new face);
+// all copies share their visibility, so the list's evolution is fixed by:
+//   - the closest face: the first minimum in list order;
+//   - the horizon: for each unordered vertex pair the calls of
+//     addHorizonEdge (epa.cpp:49-60) in its two directions form a running sum
+//     s of +1 / -1, and the list ends holding |s| copies of the direction
+//     sign(s) (a call against the held direction erases the oldest copy); s
+//     is a sum, so it does not depend on the order of the calls;
+//   - the new faces: one per surviving directed edge, |s| copies each.
+// Only the closest-face choice needs the order of the copies.  Distinct faces
+// are kept in the order of their first copies: faces of older iterations come
+// first (kept faces keep their order, new ones are appended) and the initial
+// four are in the reference's order.  Within a later iteration, the order of
+// the new faces is the order of the first surviving copy of each directed
+// edge in the horizon list; it is known exactly when every visible face had
+// one copy and a known order (then the horizon list is replayed as the
+// reference builds it) and unknown otherwise.  A minimum shared by faces of
+// one iteration whose order is unknown, with different normals, is reported
+// as undecided (-2); every other run equals the reference's.
+struct MultiStats {
+    uint32_t peakFaces = 0, peakEdges = 0;
+    uint64_t peakMultiplicity = 0;
+    uint32_t iterations = 0;
+};
+
+template <typename T>
+struct MFace {
+    Face<T> f;
+    uint64_t mult;
+    uint32_t gen;  // iteration that made it (0: the faces the run started from)
+};
+
+// Runs iterations [first, opt.epaIterations) over the distinct faces `faces`
+// (first-copy order, every face of generation 0 in known order) and vertices
+// `v`.  Returns 1 converged, 0 the reference's failure, -1 more than
+// opt.multiFaces distinct faces, -2 undecided closest face.  With
+// opt.multiExactOrder the order of an iteration's new faces is derived where
+// it is known (see above); without it every later iteration's order counts as
+// unknown.
+template <typename T>
+int epaMultiRun(const Support<T>& sa, const Support<T>& sb, const NarrowOptions& opt,
+                std::vector<V3<T>>& v, std::vector<MFace<T>>& faces, uint32_t first,
+                V3<T>& normal, T& depth, MultiStats* ms)
+{
+    const uint32_t iterations = opt.epaIterations;
+    const T tol = T(opt.epaTolerance);
+    std::vector<uint8_t> ordered(iterations + 2, 0);  // order of each generation's faces known
+    ordered[0] = 1;
+    const uint32_t V = 4 + iterations;
+    std::vector<int64_t> sum(size_t(V) * V);
+    std::vector<uint32_t> touched;
+    struct Edge {
+        uint32_t a, b;
+    };
+    for (uint32_t it = first; it < iterations; ++it) {
+        if (ms)
+            ms->iterations = it + 1;
+        if (faces.empty())
+            return 0;
+        size_t best = 0;
+        for (size_t f = 1; f < faces.size(); ++f)
+            if (faces[f].f.d < faces[best].f.d)
+                best = f;
+        const uint32_t g0 = faces[best].gen;
+        if (!ordered[g0]) {
+            for (size_t f = best + 1; f < faces.size() && faces[f].gen == g0; ++f) {
+                const Face<T>& o = faces[f].f;
+                if (o.d == faces[best].f.d &&
+                    (o.n.x != faces[best].f.n.x || o.n.y != faces[best].f.n.y ||
+                     o.n.z != faces[best].f.n.z))
+                    return -2;
+            }
+        }
+        Face<T> cf = faces[best].f;
+        V3<T> sup = sub(sa(cf.n), sb(neg(cf.n)));
+        T sd = dot(sup, cf.n);
+        if (sd - cf.d <= tol) {
+            normal = cf.n;
+            depth = maxOf(cf.d, T(0));
+            return 1;
+        }
+        uint32_t si = uint32_t(v.size());
+        v.push_back(sup);
+        std::vector<MFace<T>> keep, added;
+        std::vector<const MFace<T>*> visible;
+        bool exact = opt.multiExactOrder;
+        for (const MFace<T>& mf : faces) {
+            if (dot(mf.f.n, sub(sup, v[mf.f.a])) > tol) {
+                visible.push_back(&mf);
+                exact = exact && mf.mult == 1 && ordered[mf.gen];
+            } else {
+                keep.push_back(mf);
+            }
+        }
+        const uint32_t gen = it + 1 - first + 1;  // generations after the start are 2, 3, ...
+        if (exact) {
+            // The reference's horizon list itself (one copy of each face).
+            std::vector<Edge> horizon;
+            auto addEdge = [&](uint32_t a, uint32_t b) {
+                for (size_t e = 0; e < horizon.size(); ++e)
+                    if (horizon[e].a == b && horizon[e].b == a) {
+                        horizon.erase(horizon.begin() + e);
+                        return;
+                    }
+                horizon.push_back({a, b});
+            };
+            for (const MFace<T>* mf : visible) {
+                addEdge(mf->f.a, mf->f.b);
+                addEdge(mf->f.b, mf->f.c);
+                addEdge(mf->f.c, mf->f.a);
+            }
+            for (const Edge& e : horizon) {
+                bool merged = false;
+                for (MFace<T>& m : added)  // a repeated directed edge: one more copy
+                    if (m.f.a == e.a && m.f.b == e.b) {
+                        ++m.mult;
+                        merged = true;
+                        break;
+                    }
+                if (!merged)
+                    added.push_back({makeFace(v, e.a, e.b, si), 1, gen});
+            }
+        } else {
+            // Running sums per unordered vertex pair, weighted by multiplicity;
+            // new faces in the order of each pair's first call.
+            for (const MFace<T>* mf : visible) {
+                const Face<T>& f = mf->f;
+                const uint32_t e[3][2] = {{f.a, f.b}, {f.b, f.c}, {f.c, f.a}};
+                for (auto& ed : e) {
+                    uint32_t lo = std::min(ed[0], ed[1]), hi = std::max(ed[0], ed[1]);
+                    size_t key = size_t(lo) * V + hi;
+                    bool fresh = true;
+                    for (uint32_t t : touched)
+                        fresh = fresh && t != key;
+                    if (fresh)
+                        touched.push_back(uint32_t(key));
+                    const int64_t m = int64_t(mf->mult);
+                    sum[key] += ed[0] < ed[1] ? m : -m;
+                }
+            }
+            for (uint32_t key : touched) {
+                int64_t sk = sum[key];
+                sum[key] = 0;
+                if (sk == 0)
+                    continue;
+                uint32_t lo = key / V, hi = key % V;
+                uint32_t a = sk > 0 ? lo : hi, b = sk > 0 ? hi : lo;
+                added.push_back({makeFace(v, a, b, si), uint64_t(sk > 0 ? sk : -sk), gen});
+            }
+            touched.clear();
+        }
+        ordered[gen] = exact ? 1 : 0;
+        if (added.empty())
+            return 0;
+        if (opt.multiFaces && keep.size() + added.size() > opt.multiFaces)
+            return -1;
+        faces = std::move(keep);
+        faces.insert(faces.end(), added.begin(), added.end());
+        if (ms) {
+            ms->peakEdges = std::max<uint32_t>(ms->peakEdges, uint32_t(added.size()));
+            ms->peakFaces = std::max<uint32_t>(ms->peakFaces, uint32_t(faces.size()));
+            for (const MFace<T>& f : faces)
+                ms->peakMultiplicity = std::max(ms->peakMultiplicity, f.mult);
+        }
+    }
+    return 0;
+}
+
+// The multiplicity form from the start (four initial faces).
+template <typename T>
+int epaMulti(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s,
+             const NarrowOptions& opt, V3<T>& normal, T& depth, MultiStats* ms)
+{
+    normal = {T(0), T(0), T(0)};
+    depth = T(0);
+    if (s.size < 4 || opt.epaIterations == 0)
+        return 0;
+    std::vector<V3<T>> v(s.p, s.p + 4);
+    std::vector<MFace<T>> faces = {{makeFace(v, 0, 1, 2), 1, 0}, {makeFace(v, 0, 3, 1), 1, 0},
+                                   {makeFace(v, 0, 2, 3), 1, 0}, {makeFace(v, 1, 3, 2), 1, 0}};
+    return epaMultiRun(sa, sb, opt, v, faces, 0, normal, depth, ms);
+}
+
+// Continuation of the bounded exact EPA (epa below) at iteration `it`, where
+// its polytope outgrew opt.maxFaces / opt.maxEdges: the exact list `list`
+// (before that iteration) becomes distinct faces in first-copy order with
+// their copy counts, all in known order, and the run goes on from `it`.
+template <typename T>
+int epaContinue(const Support<T>& sa, const Support<T>& sb, const NarrowOptions& opt,
+                std::vector<V3<T>>& v, const std::vector<Face<T>>& list, uint32_t it,
+                V3<T>& normal, T& depth, MultiStats* ms)
+{
+    std::vector<MFace<T>> faces;
+    for (const Face<T>& f : list) {
+        bool merged = false;
+        for (MFace<T>& m : faces)
+            if (m.f.a == f.a && m.f.b == f.b && m.f.c == f.c) {
+                ++m.mult;
+                merged = true;
+                break;
+            }
+        if (!merged)
+            faces.push_back({f, 1, 0});
+    }
+    if (opt.multiFaces && faces.size() > opt.multiFaces)
+        return -1;
+    return epaMultiRun(sa, sb, opt, v, faces, it, normal, depth, ms);
+}
+
 // computeEpa, epa.cpp:64-130.  Returns 1 (converged), 0 (the reference's
-// failure) or -1 (a bound of opt was exceeded; never with the defaults).
+// failure), -1 (a bound of opt was exceeded; never with the defaults) or,
+// with opt.multiFaces, what the multiplicity continuation returns from the
+// iteration where a bound was exceeded (1, 0, -1 its own bound, -2 undecided).
 template <typename T>
 int epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, const NarrowOptions& opt,
         V3<T>& normal, T& depth)
@@ -865,12 +1080,15 @@ int epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, const N
             size_t vis = faces.size() - keep.size();
             g_epa_visible[vis < 256 ? vis : 256]++;
         }
-        if (overflow)
-            return -1;
-        if (horizon.empty())
+        if (!overflow && horizon.empty())
             return 0;
-        if (opt.maxFaces && keep.size() + horizon.size() > opt.maxFaces)
-            return -1;
+        if (overflow || (opt.maxFaces && keep.size() + horizon.size() > opt.maxFaces)) {
+            if (!opt.multiFaces)
+                return -1;
+            v.pop_back();  // the continuation takes this iteration again
+            return epaContinue(sa, sb, opt, v, faces, it, normal, depth,
+                               static_cast<MultiStats*>(nullptr));
+        }
         faces = std::move(keep);
         for (const Edge& e : horizon)
             faces.push_back(makeFace(v, e.a, e.b, si));
@@ -1141,13 +1359,16 @@ struct View {
     }
 };
 
-uint32_t g_max_faces = 0, g_max_edges = 0;
+uint32_t g_max_faces = 0, g_max_edges = 0, g_multi_faces = 0;
+bool g_multi_exact = true;
 
 NarrowOptions makeOptions(uint32_t gjkIt, uint32_t epaIt, float tol)
 {
     NarrowOptions o;
     o.maxFaces = g_max_faces;
     o.maxEdges = g_max_edges;
+    o.multiFaces = g_multi_faces;
+    o.multiExactOrder = g_multi_exact;
     if (gjkIt)
         o.gjkIterations = gjkIt;
     if (epaIt)
@@ -1290,6 +1511,15 @@ void orc_set_epa_bounds(uint32_t maxFaces, uint32_t maxEdges)
     g_max_edges = maxEdges;
 }
 
+// Continuation past the bounds in multiplicity form with at most
+// `multiFaces` distinct faces (0: none, the bounded run stops with the
+// overflow status); exactOrder as NarrowOptions::multiExactOrder.
+void orc_set_epa_continuation(uint32_t multiFaces, int exactOrder)
+{
+    g_multi_faces = multiFaces;
+    g_multi_exact = exactOrder != 0;
+}
+
 // Peak EPA polytope sizes since the last reset (faces, edges, vertices).
 void orc_epa_peak(uint32_t* out, int reset)
 {
@@ -1410,6 +1640,51 @@ int orc_generate_contacts_f64(const physgpu_colliders* c, const physgpu_pair* pa
     }
 }
 
+// GJK then the multiplicity form of EPA (np::epaMulti), unbounded.
+// status: 0 GJK miss, 1 converged, 2 failure, 4 undecided closest face;
+// stats[k*4 + 0..3] = peak distinct faces, peak distinct new faces in one
+// iteration, iterations, peak multiplicity (saturated to 2^32-1).
+int orc_gjk_epa_multi(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t n,
+                      uint32_t gjkIterations, uint32_t epaIterations, float tolerance,
+                      int32_t* status, float* normal, float* depth, uint32_t* stats)
+{
+    try {
+        View<float> v(c);
+        for (uint64_t k = 0; k < n; ++k) {
+            Shape<float> sa = v.shape(pairs[k].first), sb = v.shape(pairs[k].second);
+            Pose<float> ta = v.pose(pairs[k].first), tb = v.pose(pairs[k].second);
+            np::Support<float> fa{&sa, &ta}, fb{&sb, &tb};
+            np::Simplex<float> s;
+            normal[3 * k] = normal[3 * k + 1] = normal[3 * k + 2] = 0.0f;
+            depth[k] = 0.0f;
+            np::MultiStats ms;
+            if (!np::gjk(fa, fb, s, gjkIterations)) {
+                status[k] = 0;
+            } else {
+                NarrowOptions opt = makeOptions(gjkIterations, epaIterations, tolerance);
+                opt.maxFaces = opt.maxEdges = 0;
+                V3<float> nn;
+                float d;
+                const int st = np::epaMulti(fa, fb, s, opt, nn, d, &ms);
+                status[k] = st == 1 ? 1 : (st == 0 ? 2 : 4);
+                normal[3 * k] = nn.x;
+                normal[3 * k + 1] = nn.y;
+                normal[3 * k + 2] = nn.z;
+                depth[k] = d;
+            }
+            if (stats) {
+                stats[k * 4 + 0] = ms.peakFaces;
+                stats[k * 4 + 1] = ms.peakEdges;
+                stats[k * 4 + 2] = ms.iterations;
+                stats[k * 4 + 3] = uint32_t(std::min<uint64_t>(ms.peakMultiplicity, 0xffffffffu));
+            }
+        }
+        return PHYSGPU_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 // Same contract as ref_gjk_epa.
 int orc_gjk_epa(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t n,
                 uint32_t gjkIterations, uint32_t epaIterations, float tolerance, int32_t* status,
@@ -1442,7 +1717,7 @@ int orc_gjk_epa(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t 
             NarrowOptions opt = makeOptions(gjkIterations, epaIterations, tolerance);
             opt.epaIterations = epaIterations;
             const int st = np::epa(fa, fb, s, opt, nn, d);
-            status[k] = st == 1 ? 1 : (st == 0 ? 2 : 3);
+            status[k] = st == 1 ? 1 : (st == 0 ? 2 : (st == -1 ? 3 : 4));
             normal[3 * k] = nn.x;
             normal[3 * k + 1] = nn.y;
             normal[3 * k + 2] = nn.z;
@@ -1460,7 +1735,8 @@ int orc_gjk_epa(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t 
 // same-body filter (one body per collider here, so nothing is removed),
 // narrow phase, anchors to body-local, material combination, compaction in
 // candidate order.
-int orc_world_collide(const physgpu_colliders* c, const float* bodyPosition,
+static int worldCollide(const physgpu_colliders* c, const float* bodyPosition,
+                        const uint32_t* colliderBody,
                       const float* localPose, const float* friction, const float* restitution,
                       int algorithm, uint32_t bpWorkers, uint32_t npWorkers, uint32_t warmup,
                       uint32_t steps, physgpu_manifold* out, uint64_t capacity, uint64_t* count,
@@ -1504,6 +1780,15 @@ int orc_world_collide(const physgpu_colliders* c, const float* bodyPosition,
             auto t1 = Clock::now();
             physgpu_broadphase_stats st;
             findPairs(bounds.data(), n, algorithm, bpWorkers, pairs, st);
+            // Pairs of colliders on one body are erased, order kept
+            // (physics_world.cpp:630-633).
+            if (colliderBody)
+                pairs.erase(std::remove_if(pairs.begin(), pairs.end(),
+                                           [&](const Pair& q) {
+                                               return colliderBody[q.first] ==
+                                                      colliderBody[q.second];
+                                           }),
+                            pairs.end());
             auto t2 = Clock::now();
             results.resize(pairs.size());
             hits.assign(pairs.size(), 0);
@@ -1564,7 +1849,16 @@ int orc_world_collide(const physgpu_colliders* c, const float* bodyPosition,
             timing[2] = median(ta);
             timing[3] = timing[0] + timing[1] + timing[2];
             timing[4] = double(pairs.size());
-            timing[5] = double(n) * (double(n) - 1.0) / 2.0;
+            // possiblePairs (physics_world.cpp:571-577): the earlier
+            // colliders on other bodies, summed over colliders.
+            uint64_t possible = 0;
+            std::vector<uint64_t> perBody(colliderBody ? n : 0);
+            for (uint32_t i = 0; i < n; ++i) {
+                possible += i;
+                if (colliderBody)
+                    possible -= perBody[colliderBody[i]]++;
+            }
+            timing[5] = double(possible);
         }
         if (npStats)
             std::copy(stats6, stats6 + 6, npStats);
@@ -1576,6 +1870,36 @@ int orc_world_collide(const physgpu_colliders* c, const float* bodyPosition,
     } catch (const std::exception& e) {
         return fail(e);
     }
+}
+
+// One body per collider (bodyPosition, friction and restitution per collider).
+int orc_world_collide(const physgpu_colliders* c, const float* bodyPosition,
+                      const float* localPose, const float* friction, const float* restitution,
+                      int algorithm, uint32_t bpWorkers, uint32_t npWorkers, uint32_t warmup,
+                      uint32_t steps, physgpu_manifold* out, uint64_t capacity, uint64_t* count,
+                      double* timing, uint64_t* npStats)
+{
+    return worldCollide(c, bodyPosition, nullptr, localPose, friction, restitution, algorithm,
+                        bpWorkers, npWorkers, warmup, steps, out, capacity, count, timing,
+                        npStats);
+}
+
+// Colliders sharing bodies: colliderBody[i] is the body of collider i (index
+// < n); bodyPosition, friction and restitution are given per collider (its
+// body's values), as the HIP library's physgpu_set_world takes them.
+int orc_world_collide_bodies(const physgpu_colliders* c, const float* bodyPosition,
+                             const uint32_t* colliderBody, const float* localPose,
+                             const float* friction, const float* restitution, int algorithm,
+                             uint32_t bpWorkers, uint32_t npWorkers, uint32_t warmup,
+                             uint32_t steps, physgpu_manifold* out, uint64_t capacity,
+                             uint64_t* count, double* timing, uint64_t* npStats)
+{
+    for (uint32_t i = 0; i < c->count; ++i)
+        if (colliderBody[i] >= c->count)
+            return fail(std::invalid_argument("body index out of range"));
+    return worldCollide(c, bodyPosition, colliderBody, localPose, friction, restitution,
+                        algorithm, bpWorkers, npWorkers, warmup, steps, out, capacity, count,
+                        timing, npStats);
 }
 
 // Pairs are the manifolds' (first, second).  status[k]: 1 differentiated,

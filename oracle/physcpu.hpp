@@ -187,6 +187,10 @@ struct NarrowOptions {
     // stops with an overflow status, exactly where the HIP kernel stops.
     uint32_t maxFaces = 0;
     uint32_t maxEdges = 0;
+    // Past those bounds, continue in multiplicity form (epaContinue) with at
+    // most multiFaces distinct faces (0: stop with the overflow status).
+    uint32_t multiFaces = 0;
+    bool multiExactOrder = true;
 };
 
 // Narrow phase entry (NarrowPhase::generateContact, narrowphase.cpp:326-374,

@@ -299,7 +299,8 @@ int ref_gjk_epa(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t 
 // timing[0..5] = median broadPhaseMs, narrowPhaseMs, aabb-refresh ms (total
 // minus the other stages), totalMs, candidatePairs, possiblePairs.
 // npStats = 6 NarrowPhaseStats counters.
-int ref_world_collide(const physgpu_colliders* c, const float* bodyPosition,
+static int worldCollide(const physgpu_colliders* c, const float* bodyPosition,
+                        const uint32_t* colliderBody,
                       const float* localPose, const float* friction, const float* restitution,
                       int algorithm, uint32_t bpWorkers, uint32_t npWorkers, uint32_t warmup,
                       uint32_t steps, physgpu_manifold* out, uint64_t capacity, uint64_t* count,
@@ -313,13 +314,24 @@ int ref_world_collide(const physgpu_colliders* c, const float* bodyPosition,
         world.broadPhaseAlgorithm = static_cast<BroadPhaseAlgorithm>(algorithm);
         world.setBroadPhaseWorkerCount(bpWorkers);
         world.setNarrowPhaseWorkerCount(npWorkers);
+        // One body per collider, or per distinct colliderBody value (added in
+        // the order of each body's first collider, with that collider's
+        // position and materials).
+        std::vector<RigidBodyHandle> handle(n);
+        std::vector<int64_t> made(n, -1);
         for (uint32_t i = 0; i < n; ++i) {
-            RigidBody body({bodyPosition[i], bodyPosition[n + i], bodyPosition[2 * n + i]}, 1.0f,
-                           restitution ? restitution[i] : 0.0f, friction ? friction[i] : 0.0f,
-                           true);
-            RigidBodyHandle h = world.addBody(body);
+            const uint32_t key = colliderBody ? colliderBody[i] : i;
+            if (made[key] < 0) {
+                RigidBody body({bodyPosition[i], bodyPosition[n + i], bodyPosition[2 * n + i]},
+                               1.0f, restitution ? restitution[i] : 0.0f,
+                               friction ? friction[i] : 0.0f, true);
+                made[key] = int64_t(i);
+                handle[key] = world.addBody(body);
+            }
+        }
+        for (uint32_t i = 0; i < n; ++i) {
             Collider col = v.collider(i);
-            col.body = h;
+            col.body = handle[colliderBody ? colliderBody[i] : i];
             if (localPose) {
                 col.localTransform.position = {localPose[i], localPose[n + i],
                                                localPose[2 * n + i]};
@@ -373,6 +385,35 @@ int ref_world_collide(const physgpu_colliders* c, const float* bodyPosition,
     } catch (const std::exception& e) {
         return fail(e);
     }
+}
+
+int ref_world_collide(const physgpu_colliders* c, const float* bodyPosition,
+                      const float* localPose, const float* friction, const float* restitution,
+                      int algorithm, uint32_t bpWorkers, uint32_t npWorkers, uint32_t warmup,
+                      uint32_t steps, physgpu_manifold* out, uint64_t capacity, uint64_t* count,
+                      double* timing, uint64_t* npStats)
+{
+    return worldCollide(c, bodyPosition, nullptr, localPose, friction, restitution, algorithm,
+                        bpWorkers, npWorkers, warmup, steps, out, capacity, count, timing,
+                        npStats);
+}
+
+// Colliders sharing bodies (colliderBody[i] < n); per-collider arrays carry
+// the body's values.  Manifold first/second are body indices (the order in
+// which the bodies were added), as ContactManifold::bodyA / bodyB.
+int ref_world_collide_bodies(const physgpu_colliders* c, const float* bodyPosition,
+                             const uint32_t* colliderBody, const float* localPose,
+                             const float* friction, const float* restitution, int algorithm,
+                             uint32_t bpWorkers, uint32_t npWorkers, uint32_t warmup,
+                             uint32_t steps, physgpu_manifold* out, uint64_t capacity,
+                             uint64_t* count, double* timing, uint64_t* npStats)
+{
+    for (uint32_t i = 0; i < c->count; ++i)
+        if (colliderBody[i] >= c->count)
+            return fail(std::invalid_argument("body index out of range"));
+    return worldCollide(c, bodyPosition, colliderBody, localPose, friction, restitution,
+                        algorithm, bpWorkers, npWorkers, warmup, steps, out, capacity, count,
+                        timing, npStats);
 }
 
 } // extern "C"

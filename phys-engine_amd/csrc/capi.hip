@@ -32,6 +32,9 @@ physgpu_status check_hip(hipError_t e, const char* what)
     return PHYSGPU_DEVICE_ERROR;
 }
 
+const char* const kStageBroadphase = "stage_broadphase";
+const char* const kStageNarrowphase = "stage_narrowphase";
+
 void time_begin(physgpu_context* ctx, const char* name, Timer& t, hipStream_t stream)
 {
     t.name = name;
@@ -76,6 +79,10 @@ void collect_times(physgpu_context* ctx)
         }
         it->total_ms += ms;
         it->launches += 1;
+        if (std::strcmp(t.name, kStageBroadphase) == 0)
+            ctx->last_broadphase_ms = ms;
+        else if (std::strcmp(t.name, kStageNarrowphase) == 0)
+            ctx->last_narrowphase_ms = ms;
         ctx->event_pool.push_back(t.start);
         ctx->event_pool.push_back(t.stop);
     }
@@ -105,7 +112,8 @@ void free_all(physgpu_context* c)
                     c->world_pair_off, c->world_manifold_off, c->adjoint_scratch,
                     c->adjoint_skipped, c->partials, c->tree_tight, c->tree_fat, c->tree_rank,
                     c->tree_rank_user, c->tree_stage_ids, c->tree_stage_bounds,
-                    c->tree_stage_flags, c->tree_stage_pairs};
+                    c->tree_stage_flags, c->tree_stage_pairs, c->epa_records,
+                    c->epa_unresolved_pairs};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -278,6 +286,18 @@ void fill_np_stats(const Counters& c, physgpu_narrowphase_stats* s)
     s->hull_contacts = uint64_t(c.class_contacts[3]) + c.class_contacts[4] + c.class_contacts[5];
     s->gjk_epa_failures = c.gjk_failures;
     s->epa_overflows = c.epa_overflow;
+    s->epa_unresolved = c.epa_unresolved;
+}
+
+// The step's results are complete except for hull pairs whose EPA result
+// could not be determined (physgpu_epa_unresolved lists them).
+physgpu_status unresolved_status(const Counters& c)
+{
+    if (!c.epa_unresolved)
+        return PHYSGPU_OK;
+    return fail(PHYSGPU_EPA_UNRESOLVED,
+                std::to_string(c.epa_unresolved) +
+                    " hull pair(s) without an EPA result (physgpu_epa_unresolved lists them)");
 }
 
 physgpu_status apply_options(physgpu_context* ctx, const physgpu_narrowphase_options* o)
@@ -317,10 +337,17 @@ void enqueue_step(physgpu_context* ctx, int algorithm, double cell)
     if (ctx->has_world)
         launch_compose_world(ctx);
     launch_compute_aabbs(ctx, true);
+    // Stage brackets as StepStats times them (physics_world.cpp:554-636,
+    // 640-738): the broad phase after the AABB refresh, then the narrow phase.
+    Timer bp, np;
+    time_begin(ctx, kStageBroadphase, bp);
     run_broadphase(ctx, algorithm, cell, false, true, true);
+    time_end(ctx, bp);
     ctx->np_pairs = ctx->pairs_raw;
     ctx->np_perm = ctx->pair_perm;
+    time_begin(ctx, kStageNarrowphase, np);
     launch_narrowphase(ctx, ctx->has_world, true);
+    time_end(ctx, np);
     launch_world_ranges(ctx);
 }
 
@@ -341,7 +368,7 @@ physgpu_status step_counts_checked(physgpu_context* ctx, uint64_t* pair_count,
         return fail(PHYSGPU_DEVICE_ERROR, "LBVH traversal stack overflow");
     if (c.pair_count > ctx->cap_p)
         return fail(PHYSGPU_CAPACITY_EXCEEDED, "candidate pairs exceed the context capacity");
-    return PHYSGPU_OK;
+    return unresolved_status(c);
 }
 
 physgpu_status copy_pairs_to_host(physgpu_context* ctx, physgpu_pair* out, uint64_t n)
@@ -463,6 +490,11 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->world_of, N));
     A(alloc(c->adjoint_skipped, 1));
     A(alloc(c->partials, size_t(kPartialSlots) * kMaxPartialBlocks));
+    // EPA hand-over records: one per thousand candidate pairs (a few per
+    // thousand hull pairs outgrow the LDS polytope in dense mixed scenes).
+    c->epa_record_cap = uint32_t(std::min<uint64_t>(std::max<uint64_t>(P / 1024, 64), 8192));
+    A(alloc(c->epa_records, size_t(c->epa_record_cap) * kEpaRecordWords));
+    A(alloc(c->epa_unresolved_pairs, c->epa_record_cap));
     A(alloc(c->counters, 1));
     A(hipHostMalloc(reinterpret_cast<void**>(&c->host_counters), sizeof(Counters)));
     if (e != hipSuccess) {
@@ -729,7 +761,7 @@ physgpu_status physgpu_generate_contacts(physgpu_context* ctx, const physgpu_pai
         if (m < c.manifold_count)
             return fail(PHYSGPU_CAPACITY_EXCEEDED, "output capacity below the manifold count");
     }
-    return PHYSGPU_OK;
+    return unresolved_status(c);
 }
 
 physgpu_status physgpu_collide(physgpu_context* ctx, int algorithm, double grid_cell_size,
@@ -807,6 +839,7 @@ physgpu_status physgpu_set_worlds(physgpu_context* ctx, uint32_t world_count,
     drop_graph(ctx);
     if (world_count == 0) {
         ctx->n_worlds = 0;
+        ctx->host_world_offsets.clear();
         return PHYSGPU_OK;
     }
     if (!offsets)
@@ -847,6 +880,7 @@ physgpu_status physgpu_set_worlds(physgpu_context* ctx, uint32_t world_count,
     PG_TRY_HIP(hipMemcpyAsync(ctx->world_offsets, host.data(), host.size() * 4,
                               hipMemcpyHostToDevice, ctx->stream));
     ctx->n_worlds = world_count;
+    ctx->host_world_offsets = host;
     launch_world_of(ctx);
     PG_TRY_HIP(hipGetLastError());
     PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
@@ -1048,6 +1082,12 @@ physgpu_status physgpu_set_world(physgpu_context* ctx, const float* body_pose,
                                       ctx->stream));
         ctx->world_has_local = has_local;
         ctx->world_has_body = extras && extras->body;
+        ctx->host_body.clear();
+        if (ctx->world_has_body) {
+            ctx->host_body.resize(n);
+            PG_TRY_HIP(hipMemcpyAsync(ctx->host_body.data(), ctx->body, 4ull * n,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+        }
         ctx->world_has_friction = extras && extras->friction;
         ctx->world_has_restitution = extras && extras->restitution;
     }
@@ -1107,6 +1147,36 @@ physgpu_status physgpu_step_counts(physgpu_context* ctx, uint64_t* pair_count,
         *pair_count = ctx->host_counters->pair_count;
     if (manifold_count)
         *manifold_count = ctx->host_counters->manifold_count;
+    return unresolved_status(*ctx->host_counters);
+}
+
+physgpu_status physgpu_epa_unresolved(physgpu_context* ctx, physgpu_pair* out, uint64_t capacity,
+                                      uint64_t* count)
+{
+    if (!ctx || !count)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or count is NULL");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    const uint32_t total = ctx->host_counters->epa_unresolved;
+    *count = total;
+    const uint64_t held = std::min<uint64_t>(total, ctx->epa_record_cap);
+    if (out && held && capacity) {
+        std::vector<uint64_t> tmp(held);
+        PG_TRY_HIP(hipMemcpyAsync(tmp.data(), ctx->epa_unresolved_pairs, held * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+        std::sort(tmp.begin(), tmp.end());
+        for (uint64_t k = 0; k < std::min<uint64_t>(held, capacity); ++k) {
+            out[k].first = uint32_t(tmp[k] >> 32);
+            out[k].second = uint32_t(tmp[k]);
+        }
+    }
+    if (held < total)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "more unresolved pairs than the context lists");
+    if (out && capacity < held)
+        return fail(PHYSGPU_CAPACITY_EXCEEDED, "output capacity below the unresolved count");
     return PHYSGPU_OK;
 }
 
@@ -1118,6 +1188,49 @@ physgpu_status physgpu_narrowphase_stats_get(physgpu_context* ctx, physgpu_narro
     if (st != PHYSGPU_OK)
         return st;
     fill_np_stats(*ctx->host_counters, s);
+    return PHYSGPU_OK;
+}
+
+physgpu_status physgpu_step_stats_get(physgpu_context* ctx, physgpu_step_stats* s)
+{
+    if (!ctx || !s)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "ctx or stats is NULL");
+    PG_TRY_HIP(hipSetDevice(ctx->device));
+    physgpu_status st = read_counters(ctx);
+    if (st != PHYSGPU_OK)
+        return st;
+    collect_times(ctx);
+    std::memset(s, 0, sizeof(*s));
+    const Counters& c = *ctx->host_counters;
+    fill_broad_stats(ctx, &s->broad_phase);
+    fill_np_stats(c, &s->narrow_phase);
+    if (ctx->timing) {
+        s->broad_phase_ms = ctx->last_broadphase_ms;
+        s->narrow_phase_ms = ctx->last_narrowphase_ms;
+    }
+    // possiblePairs (physics_world.cpp:571-577): each collider pairs with the
+    // earlier ones of its world that sit on other bodies.
+    const uint32_t n = ctx->n;
+    std::vector<uint32_t> off = ctx->host_world_offsets;
+    if (off.empty())
+        off = {0u, n};
+    std::unordered_map<uint32_t, uint64_t> per_body;
+    uint64_t possible = 0;
+    for (size_t w = 0; w + 1 < off.size(); ++w) {
+        const uint64_t len = off[w + 1] - off[w];
+        possible += len * (len - (len > 0 ? 1 : 0)) / 2;
+        if (!ctx->host_body.empty() && ctx->has_world) {
+            per_body.clear();
+            for (uint32_t i = off[w]; i < off[w + 1]; ++i)
+                possible -= per_body[ctx->host_body[i]]++;
+        }
+    }
+    s->possible_pairs = possible;
+    uint64_t candidates = 0;
+    for (int q = 0; q < kClasses; ++q)
+        candidates += c.class_count[q];
+    s->candidate_pairs = candidates;
+    s->contact_count = c.manifold_count;
     return PHYSGPU_OK;
 }
 

@@ -34,6 +34,10 @@ constexpr uint32_t kScanTile = 256;  // candidates per compaction tile
 constexpr uint32_t kMaxPartialBlocks = 8192;
 // EPA workgroups (one wavefront each) at most.
 constexpr uint32_t kMaxEpaGroups = 1u << 20;
+// EPA runs past the LDS polytope bounds: 32-bit words of one hand-over
+// record (iteration, vertices, faces), workgroups of the continuation kernel.
+constexpr uint32_t kEpaRecordWords = 1536;
+constexpr uint32_t kEpaContinueGroups = 256;
 enum PartialSlot : uint32_t {
     kSlotGridTests = 0,
     kSlotGridOverflowTests,
@@ -92,7 +96,8 @@ struct Counters {
     uint32_t tree_overflow;             // LBVH traversal stack overflows (must stay 0)
     uint32_t mid_segments;              // pair segments sorted by one wave
     uint32_t long_segments;             // pair segments sorted by one workgroup
-    uint32_t pad[1];
+    uint32_t epa_continued;             // EPA runs handed over past the LDS bounds
+    uint32_t epa_unresolved;            // hull pairs left without an EPA result
 };
 
 struct Timer {
@@ -225,6 +230,12 @@ struct physgpu_context {
     uint32_t* tile_sums = nullptr;     // compaction
     physgpu_manifold* manifolds = nullptr;  // compacted, candidate order
     physgpu_narrowphase_options np_options{};
+    // EPA runs past the LDS polytope bounds: hand-over records
+    // (kEpaRecordWords each) and the pairs left unresolved (first << 32 |
+    // second), epa_record_cap of each.
+    uint32_t* epa_records = nullptr;
+    uint64_t* epa_unresolved_pairs = nullptr;
+    uint32_t epa_record_cap = 0;
 
     // World-level extras (optional): per collider, the pose of its body, its
     // body index, its local transform and its body's materials.
@@ -270,6 +281,13 @@ struct physgpu_context {
     // Last step.
     int last_algorithm = -1;
     double last_cell_size = 0.0;
+    // Stage times of the last timed step (ms; physgpu_step_stats).
+    double last_broadphase_ms = 0.0;
+    double last_narrowphase_ms = 0.0;
+    // Host copies behind StepStats::possiblePairs: owning body per collider
+    // (empty: one body per collider) and world offsets (empty: one world).
+    std::vector<uint32_t> host_body;
+    std::vector<uint32_t> host_world_offsets;
 
     // Timing.
     bool timing = false;
@@ -295,6 +313,9 @@ physgpu_status check_hip(hipError_t e, const char* what);
 // Timing brackets around one launch (no-ops unless ctx->timing).
 // Kernel timers (when timing is on): events on `stream`, the main stream by default.
 void time_begin(physgpu_context* ctx, const char* name, Timer& t, hipStream_t stream = nullptr);
+// Names of the stage brackets of a device-resident step.
+extern const char* const kStageBroadphase;
+extern const char* const kStageNarrowphase;
 void time_end(physgpu_context* ctx, Timer& t);
 void collect_times(physgpu_context* ctx);
 

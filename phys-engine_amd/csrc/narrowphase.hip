@@ -226,13 +226,15 @@ __device__ __forceinline__ uint32_t edge_key(uint32_t a, uint32_t b)
     return v * (v - 1) / 2 + u;
 }
 
-__device__ __forceinline__ f3 lds_vertex(const EpaShared& S, uint32_t i)
+template <class Shared>
+__device__ __forceinline__ f3 lds_vertex(const Shared& S, uint32_t i)
 {
     return {S.vx[i], S.vy[i], S.vz[i]};
 }
 
 // makeFace, epa.cpp:29-47, over the LDS vertices; writes face slot f.
-__device__ __forceinline__ void make_face_lds(EpaShared& S, uint32_t f, uint32_t a, uint32_t b,
+template <class Shared>
+__device__ __forceinline__ void make_face_lds(Shared& S, uint32_t f, uint32_t a, uint32_t b,
                                               uint32_t c)
 {
     const f3 va = lds_vertex(S, a);
@@ -519,10 +521,15 @@ __device__ __forceinline__ int horizon_wave(EpaShared& S, uint32_t edges)
 
 // Returns 1 converged, 0 failed (the reference's failure), -1 storage bound
 // exceeded.  `normal`/`depth` are uniform across the wave.
+// On -1 from a full horizon or face list, *stop (optional) receives the
+// iteration and the face count before it (it | nf << 16): S then still holds
+// that iteration's faces, and its vertices up to 4 + it (plus the new support
+// point at 4 + it).
 __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const pose& ta,
                         const WaveHull& ha, const Shape& sb, const pose& tb, const WaveHull& hb,
                         const f3 simplex[4], uint32_t iterations, float tol, f3& normal,
-                        float& depth, uint32_t* face_out = nullptr, uint32_t* work = nullptr)
+                        float& depth, uint32_t* face_out = nullptr, uint32_t* work = nullptr,
+                        uint32_t* stop = nullptr)
 {
     const uint32_t lane = lane_id();
     normal = {0.0f, 0.0f, 0.0f};
@@ -619,14 +626,24 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
         }
         wave_sync();
         const int horizon = horizon_wave(S, 3 * visible);
-        if (horizon < 0)
-            { if (work) *work += it + 1; return -1; }
+        if (horizon < 0) {
+            if (work)
+                *work += it + 1;
+            if (stop)
+                *stop = it | (nf << 16);
+            return -1;
+        }
         const uint32_t ne = uint32_t(horizon);
         if (ne == 0)
             { if (work) *work += it + 1; return 0; }
         const uint32_t keep = nf - visible;
-        if (keep + ne > kMaxFaces)
-            { if (work) *work += it + 1; return -1; }
+        if (keep + ne > kMaxFaces) {
+            if (work)
+                *work += it + 1;
+            if (stop)
+                *stop = it | (nf << 16);
+            return -1;
+        }
         // Kept faces move down in order; every write lands at or below the
         // slot the writing lane read, and after everything already placed.
         uint32_t out = 0;
@@ -665,6 +682,313 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
     }
     if (work)
         *work += iterations;
+    return 0;
+}
+
+// ------------------------------------------------------------ EPA past the bounds
+// A pair whose polytope outgrows EpaShared (more than kMaxFaces faces or
+// kMaxEdges horizon edges) is handed over, at the iteration where that
+// happens, to a second kernel that continues computeEpa (epa.cpp:64-130)
+// over distinct faces with multiplicities.  In FP32 the reference's list
+// fills with copies of the same faces on near-degenerate pairs; all copies of
+// a face share their visibility, so the run is fixed by:
+//   - the closest face, the first minimum in list order;
+//   - the horizon: for one unordered vertex pair the calls of addHorizonEdge
+//     (epa.cpp:49-60) form a running sum s of +1 / -1 and the list ends with
+//     |s| copies of the direction sign(s); s is a sum, independent of the
+//     order of the calls, and a face with m copies contributes m per edge;
+//   - the new faces, one per surviving directed edge, with |s| copies.
+// Distinct faces are kept in the order of their first copies.  The list
+// handed over is exact, so its faces (generation 0) are in known order; the
+// faces of each later iteration follow all older ones, but their order among
+// themselves is not tracked.  A closest-face minimum shared by faces of one
+// later iteration with different normals is therefore undecided and reported
+// (-2), as is a polytope of more than kContFaces distinct faces or a
+// multiplicity past 2^32 - 1 (-1); every other run gives the reference's
+// result.
+constexpr uint32_t kContFaces = PHYSGPU_EPA_MAX_DISTINCT_FACES;
+constexpr uint32_t kContChunks = kContFaces / 64;
+
+// Hand-over record (32-bit words): header, vertices (SoA), faces (SoA).
+enum : uint32_t {
+    kRecK = 0, kRecCls, kRecA, kRecB, kRecIt, kRecNf, kRecHeader = 8,
+    kRecVx = kRecHeader, kRecVy = kRecVx + kMaxVertices, kRecVz = kRecVy + kMaxVertices,
+    kRecFabc = kRecVz + kMaxVertices, kRecFnx = kRecFabc + kMaxFaces,
+    kRecFny = kRecFnx + kMaxFaces, kRecFnz = kRecFny + kMaxFaces, kRecFd = kRecFnz + kMaxFaces,
+    kRecEnd = kRecFd + kMaxFaces
+};
+static_assert(kRecEnd <= kEpaRecordWords, "hand-over record layout");
+
+// One workgroup per handed-over run: kContThreads threads (one wavefront per
+// SIMD of a CU), thread t holding faces kFacesPerThread t .. + 3, so block
+// scans in thread order are scans in face order.
+constexpr uint32_t kContThreads = 256;
+constexpr uint32_t kContWaves = kContThreads / 64;
+constexpr uint32_t kFacesPerThread = kContFaces / kContThreads;
+static_assert(kFacesPerThread == 4, "four faces per thread (one float4 of distances)");
+static_assert(kMaxFaces <= kContThreads, "a handed-over list is staged one face per thread");
+
+struct ContShared {
+    float vx[kMaxVertices], vy[kMaxVertices], vz[kMaxVertices];
+    uint32_t fabc[kContFaces];  // a | b << 8 | c << 16
+    float fnx[kContFaces], fny[kContFaces], fnz[kContFaces];
+    alignas(16) float fd[kContFaces];
+    uint32_t mult[kContFaces];
+    uint8_t gen[kContFaces];       // 0: the handed-over faces; 1, 2, ...: later iterations
+    uint32_t vis_abc[kContFaces];  // visible faces in order (first copies while staging)
+    unsigned long long sum[kEdgeKeys];  // signed running sums per unordered vertex pair
+    uint32_t first[kEdgeKeys];          // first call index per unordered vertex pair
+    uint32_t wsum[2][kContWaves];       // per-wavefront partials of the block scans (2 buffers)
+    float wbest_v[kContWaves];
+    uint32_t wbest_i[kContWaves];
+    float sup[3];
+    uint32_t best;
+};
+
+// Edge e (0, 1, 2 = (a,b), (b,c), (c,a)) of a packed face.
+__device__ __forceinline__ void face_edge(uint32_t abc, uint32_t e, uint32_t& x, uint32_t& y)
+{
+    const uint32_t v0 = abc & 0xFFu, v1 = (abc >> 8) & 0xFFu, v2 = (abc >> 16) & 0xFFu;
+    x = e == 0 ? v0 : (e == 1 ? v1 : v2);
+    y = e == 0 ? v1 : (e == 1 ? v2 : v0);
+}
+
+__device__ __forceinline__ void horizon_call(ContShared& S, uint32_t x, uint32_t y, uint32_t m,
+                                             uint32_t call)
+{
+    const uint32_t key = edge_key(x, y);
+    const long long delta = x < y ? (long long)m : -(long long)m;
+    atomicAdd(&S.sum[key], (unsigned long long)delta);
+    atomicMin(&S.first[key], call);
+}
+
+// Exclusive prefix sum, in thread order, of per-thread counts below 2^Bits
+// over the workgroup, and its total.  Every thread calls it; one barrier.
+// Consecutive calls alternate between the two partial buffers (`buf`), and
+// any two calls on one buffer are separated by a barrier of the call between.
+template <uint32_t Bits>
+__device__ __forceinline__ uint32_t block_prefix(ContShared& S, uint32_t c, uint32_t& total,
+                                                 uint32_t buf)
+{
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t below = lanes_below(lane);
+    uint32_t in_wave = 0, wave_total = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < Bits; ++b) {
+        const uint64_t m = __ballot((c >> b) & 1u);
+        in_wave += (uint32_t)__popcll(m & below) << b;
+        wave_total += (uint32_t)__popcll(m) << b;
+    }
+    if (lane == 0)
+        S.wsum[buf][w] = wave_total;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kContWaves; ++q) {
+        const uint32_t x = S.wsum[buf][q];
+        off += q < w ? x : 0u;
+        tot += x;
+    }
+    total = tot;
+    return off + in_wave;
+}
+
+// Closest face: the first minimum of S.fd[0, nf) in index order, as the
+// reference's sequential scan (std::min_element) picks it.  Every thread
+// calls it and gets the index.
+__device__ uint32_t block_closest(ContShared& S, uint32_t nf)
+{
+    const uint32_t t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    const float4 d4 = reinterpret_cast<const float4*>(S.fd)[t];
+    const float d[4] = {d4.x, d4.y, d4.z, d4.w};
+    const uint32_t f0 = kFacesPerThread * t;
+    float v = 0.0f;
+    uint32_t i = kNoIndex;
+    bool nan = false;
+#pragma unroll
+    for (uint32_t j = 0; j < kFacesPerThread; ++j) {
+        if (f0 + j < nf) {
+            nan = nan || d[j] != d[j];
+            if (i == kNoIndex || d[j] < v) {
+                v = d[j];
+                i = f0 + j;
+            }
+        }
+    }
+    if (__syncthreads_or(nan)) {
+        // A NaN distance: the sequential scan itself (strict <).
+        if (t == 0) {
+            uint32_t best = 0;
+            for (uint32_t f = 1; f < nf; ++f)
+                if (S.fd[f] < S.fd[best])
+                    best = f;
+            S.best = best;
+        }
+        __syncthreads();
+        return S.best;
+    }
+    wave_arg_best<false>(v, i);
+    if (lane == 0) {
+        S.wbest_v[w] = v;
+        S.wbest_i[w] = i;
+    }
+    __syncthreads();
+    if (w == 0) {
+        float bv = lane < kContWaves ? S.wbest_v[lane] : 0.0f;
+        uint32_t bi = lane < kContWaves ? S.wbest_i[lane] : kNoIndex;
+        wave_arg_best<false>(bv, bi);
+        if (lane == 0)
+            S.best = bi;
+    }
+    __syncthreads();
+    return S.best;
+}
+
+// Iterations [first, iterations) over the nf distinct faces in S (first-copy
+// order, multiplicities in S.mult, generation 0) and the vertices up to
+// 4 + first.  Returns 1 converged, 0 the reference's failure, -1 storage
+// bound, -2 undecided closest face; normal / depth uniform across the block.
+// Every thread of the workgroup calls it; wavefront 0 evaluates the support
+// maps (ha, hb hold both shapes' points per lane).
+__device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, const pose& ta,
+                               const WaveHull& ha, const Shape& sb, const pose& tb,
+                               const WaveHull& hb, uint32_t first, uint32_t nf,
+                               uint32_t iterations, float tol, f3& normal, float& depth)
+{
+    const uint32_t t = threadIdx.x, w = t >> 6, f0 = kFacesPerThread * t;
+    normal = {0.0f, 0.0f, 0.0f};
+    depth = 0.0f;
+    uint32_t nv = 4 + first;
+    uint32_t gen = 1;
+    uint32_t buf = 0;
+    for (uint32_t it = first; it < iterations; ++it) {
+        if (nf == 0)
+            return 0;
+        const uint32_t bi = block_closest(S, nf);
+        const f3 cn{S.fnx[bi], S.fny[bi], S.fnz[bi]};
+        const float cd = S.fd[bi];
+        const uint32_t g0 = S.gen[bi];
+        const uint32_t si = nv;
+        if (w == 0) {
+            const f3 sp =
+                sub(support_wave(B, sa, ta, cn, ha), support_wave(B, sb, tb, neg(cn), hb));
+            if (t == 0) {
+                S.sup[0] = sp.x;
+                S.sup[1] = sp.y;
+                S.sup[2] = sp.z;
+                S.vx[si] = sp.x;
+                S.vy[si] = sp.y;
+                S.vz[si] = sp.z;
+            }
+        }
+        ++nv;
+        for (uint32_t q = t; q < kEdgeKeys; q += kContThreads) {
+            S.sum[q] = 0ull;
+            S.first[q] = kNoIndex;
+        }
+        // This thread's faces, read before anything moves, and whether one
+        // of them ties the closest face within its iteration (undecided).
+        uint32_t abc[kFacesPerThread], m[kFacesPerThread];
+        uint8_t g[kFacesPerThread];
+        float nx[kFacesPerThread], ny[kFacesPerThread], nz[kFacesPerThread], d[kFacesPerThread];
+        bool tie = false;
+#pragma unroll
+        for (uint32_t j = 0; j < kFacesPerThread; ++j) {
+            const uint32_t f = f0 + j;
+            abc[j] = S.fabc[f];
+            nx[j] = S.fnx[f];
+            ny[j] = S.fny[f];
+            nz[j] = S.fnz[f];
+            d[j] = S.fd[f];
+            m[j] = S.mult[f];
+            g[j] = S.gen[f];
+            tie = tie || (g0 != 0 && f < nf && f > bi && g[j] == g0 && d[j] == cd &&
+                          (nx[j] != cn.x || ny[j] != cn.y || nz[j] != cn.z));
+        }
+        if (__syncthreads_or(tie))
+            return -2;
+        const f3 sup{S.sup[0], S.sup[1], S.sup[2]};
+        if (dot(sup, cn) - cd <= tol) {
+            normal = cn;
+            depth = max_ref(cd, 0.0f);
+            return 1;
+        }
+        bool vis[kFacesPerThread];
+        uint32_t nvis = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kFacesPerThread; ++j) {
+            vis[j] = f0 + j < nf &&
+                     dot(f3{nx[j], ny[j], nz[j]}, sub(sup, lds_vertex(S, abc[j] & 0xFFu))) > tol;
+            nvis += vis[j] ? 1u : 0u;
+        }
+        uint32_t visible = 0;
+        uint32_t before = block_prefix<3>(S, nvis, visible, buf);
+        buf ^= 1u;
+#pragma unroll
+        for (uint32_t j = 0; j < kFacesPerThread; ++j) {
+            const uint32_t f = f0 + j;
+            if (vis[j]) {
+                S.vis_abc[before] = abc[j];
+#pragma unroll
+                for (uint32_t e = 0; e < 3; ++e) {
+                    uint32_t x, y;
+                    face_edge(abc[j], e, x, y);
+                    horizon_call(S, x, y, m[j], 3 * before + e);
+                }
+                ++before;
+            } else if (f < nf) {
+                // Kept faces move down in order.
+                const uint32_t pos = f - before;
+                S.fabc[pos] = abc[j];
+                S.fnx[pos] = nx[j];
+                S.fny[pos] = ny[j];
+                S.fnz[pos] = nz[j];
+                S.fd[pos] = d[j];
+                S.mult[pos] = m[j];
+                S.gen[pos] = g[j];
+            }
+        }
+        __syncthreads();
+        // New faces: one per unordered vertex pair with a nonzero sum, in the
+        // order of each pair's first call (call 3 r + e is edge e of visible
+        // face r), calls taken kContThreads at a time.
+        const uint32_t keep = nf - visible, calls = 3 * visible;
+        uint32_t ne = 0;
+        bool bound = false;
+        for (uint32_t base = 0; base < calls; base += kContThreads) {
+            const uint32_t call = base + t;
+            bool emit = false;
+            uint32_t x = 0, y = 0;
+            long long sk = 0;
+            if (call < calls) {
+                face_edge(S.vis_abc[call / 3], call % 3, x, y);
+                const uint32_t key = edge_key(x, y);
+                sk = (long long)S.sum[key];
+                emit = sk != 0 && S.first[key] == call;
+            }
+            uint32_t round = 0;
+            const uint32_t pos = keep + ne + block_prefix<1>(S, emit ? 1u : 0u, round, buf);
+            buf ^= 1u;
+            if (emit) {
+                const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
+                const unsigned long long mm = (unsigned long long)(sk > 0 ? sk : -sk);
+                if (pos < kContFaces && mm <= 0xFFFFFFFFull) {
+                    make_face_lds(S, pos, sk > 0 ? lo : hi, sk > 0 ? hi : lo, si);
+                    S.mult[pos] = uint32_t(mm);
+                    S.gen[pos] = uint8_t(gen);
+                } else {
+                    bound = true;
+                }
+            }
+            ne += round;
+        }
+        if (__syncthreads_or(bound))
+            return -1;
+        if (ne == 0)
+            return 0;
+        nf = keep + ne;
+        ++gen;
+    }
     return 0;
 }
 
@@ -998,11 +1322,44 @@ __device__ __forceinline__ Candidate candidate(uint32_t t, uint32_t c3, uint32_t
     return {cls, k, uint32_t(pr >> 32), uint32_t(pr)};
 }
 
+// Hand-over record of a run stopped by the LDS bounds (epa_wave's *stop).
+// Kept out of line: it runs on a few pairs per million and its registers
+// would otherwise count against the EPA loop's.
+__device__ __attribute__((noinline)) void write_handover(const EpaShared& S, uint32_t* R,
+                                                         uint32_t k, uint32_t cls, uint32_t a,
+                                                         uint32_t b, uint32_t stop)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t it = stop & 0xFFFFu, nf = stop >> 16, nv = 4 + it;
+    if (lane == 0) {
+        R[kRecK] = k;
+        R[kRecCls] = cls;
+        R[kRecA] = a;
+        R[kRecB] = b;
+        R[kRecIt] = it;
+        R[kRecNf] = nf;
+    }
+    for (uint32_t i = lane; i < nv; i += 64) {
+        R[kRecVx + i] = __float_as_uint(S.vx[i]);
+        R[kRecVy + i] = __float_as_uint(S.vy[i]);
+        R[kRecVz + i] = __float_as_uint(S.vz[i]);
+    }
+    for (uint32_t f = lane; f < nf; f += 64) {
+        R[kRecFabc + f] = S.fabc[f];
+        R[kRecFnx + f] = __float_as_uint(S.fnx[f]);
+        R[kRecFny + f] = __float_as_uint(S.fny[f]);
+        R[kRecFnz + f] = __float_as_uint(S.fnz[f]);
+        R[kRecFd + f] = __float_as_uint(S.fd[f]);
+    }
+}
+
 __global__ __launch_bounds__(kEpaBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* c,
                const uint32_t* __restrict__ buckets, uint64_t cap,
                physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
-               uint32_t* __restrict__ hit, uint64_t* __restrict__ partials, bool solo_sphere_hull)
+               uint32_t* __restrict__ hit, uint64_t* __restrict__ partials, bool solo_sphere_hull,
+               uint32_t* __restrict__ records, uint32_t record_cap,
+               uint64_t* __restrict__ unresolved)
 {
     __shared__ EpaShared S;
     // GJK simplices (4 points x 3) and candidate positions of the rows that
@@ -1091,9 +1448,10 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
             f3 n;
             float depth;
             const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
+            uint32_t stop = 0;
             const int st = epa_wave(S, B, sa, ta, ha, sb, tb, hb, nullptr,
                                     opt.epa_max_iterations, opt.epa_tolerance, n, depth, nullptr,
-                                    &work);
+                                    &work, &stop);
             if (st == 1) {
                 const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
                                        support_wave(B, sb, tb, neg(n), hb)),
@@ -1113,7 +1471,22 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
             } else if (st == 0) {
                 ++failures;
             } else {
+                // Past the LDS bounds: hand the exact state of this iteration
+                // over to k_epa_continue (or, with no record left, report the
+                // pair as unresolved).
                 ++overflows;
+                uint32_t slot = 0;
+                if (lane == 0)
+                    slot = atomicAdd(&c->epa_continued, 1u);
+                slot = uniform(slot);
+                if (slot < record_cap) {
+                    write_handover(S, records + size_t(slot) * kEpaRecordWords, k, cls, a, b,
+                                   stop);
+                } else if (lane == 0) {
+                    const uint32_t u = atomicAdd(&c->epa_unresolved, 1u);
+                    if (u < record_cap)
+                        unresolved[u] = (uint64_t(a) << 32) | b;
+                }
             }
             wave_sync();
         }
@@ -1126,6 +1499,110 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
         epa_partial_add(partials + size_t(kSlotContacts + 5) * kMaxPartialBlocks, contacts5);
         epa_partial_add(partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks, failures);
         epa_partial_add(partials + size_t(kSlotEpaOverflows) * kMaxPartialBlocks, overflows);
+    }
+}
+
+// EPA runs handed over by k_gjk_epa (one record each), continued over
+// distinct faces with multiplicities (epa_multi_block): one workgroup of
+// kContThreads threads per run, four faces per thread; workgroup g takes
+// records g, g + gridDim.x, ...  The exact list handed over becomes distinct
+// faces in first-copy order with their copy counts.  Results are stored like
+// k_gjk_epa's; undecided runs are listed as unresolved.
+__global__ __launch_bounds__(kContThreads) void k_epa_continue(
+    Bodies B, World W, Counters* c, const uint32_t* __restrict__ records, uint32_t record_cap,
+    physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
+    uint32_t* __restrict__ hit, uint64_t* __restrict__ partials, uint64_t* __restrict__ unresolved)
+{
+    __shared__ ContShared S;
+    const uint32_t t = threadIdx.x;
+    const uint32_t count = min(c->epa_continued, record_cap);
+    for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
+        const uint32_t* R = records + size_t(r) * kEpaRecordWords;
+        const uint32_t k = R[kRecK], cls = R[kRecCls], a = R[kRecA], b = R[kRecB];
+        const uint32_t it0 = R[kRecIt], nf0 = R[kRecNf];
+        const Shape sa = load_shape(B, a), sb = load_shape(B, b);
+        const pose ta = load_pose(B, a), tb = load_pose(B, b);
+        const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
+        if (t < 4 + it0) {
+            S.vx[t] = __uint_as_float(R[kRecVx + t]);
+            S.vy[t] = __uint_as_float(R[kRecVy + t]);
+            S.vz[t] = __uint_as_float(R[kRecVz + t]);
+        }
+        if (t < nf0) {  // nf0 <= kMaxFaces = kContThreads
+            S.fabc[t] = R[kRecFabc + t];
+            S.mult[t] = 0;
+        }
+        __syncthreads();
+        // First copy of every face (a face's copies have equal a, b, c and
+        // hence equal normal and distance bits); copies counted on it.
+        bool lead = false;
+        if (t < nf0) {
+            const uint32_t abc = S.fabc[t];
+            uint32_t firstc = t;
+            for (uint32_t g = 0; g < t; ++g)
+                if (S.fabc[g] == abc) {
+                    firstc = g;
+                    break;
+                }
+            lead = firstc == t;
+            atomicAdd(&S.mult[firstc], 1u);
+        }
+        __syncthreads();
+        uint32_t abc = 0, m = 0;
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f, d = 0.0f;
+        if (lead) {
+            abc = R[kRecFabc + t];
+            nx = __uint_as_float(R[kRecFnx + t]);
+            ny = __uint_as_float(R[kRecFny + t]);
+            nz = __uint_as_float(R[kRecFnz + t]);
+            d = __uint_as_float(R[kRecFd + t]);
+            m = S.mult[t];
+        }
+        uint32_t nf = 0;
+        const uint32_t pos = block_prefix<1>(S, lead ? 1u : 0u, nf, 0);
+        if (lead) {
+            S.fabc[pos] = abc;
+            S.fnx[pos] = nx;
+            S.fny[pos] = ny;
+            S.fnz[pos] = nz;
+            S.fd[pos] = d;
+            S.mult[pos] = m;
+            S.gen[pos] = 0;
+        }
+        __syncthreads();
+        f3 n;
+        float depth;
+        const int st = epa_multi_block(S, B, sa, ta, ha, sb, tb, hb, it0, nf,
+                                       opt.epa_max_iterations, opt.epa_tolerance, n, depth);
+        if (t < 64) {  // wavefront 0 stores the result
+            if (st == 1) {
+                const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
+                                       support_wave(B, sb, tb, neg(n), hb)),
+                                   0.5f);
+                if (t == 0) {
+                    Contact ct;
+                    ct.normal = n;
+                    ct.count = 1;
+                    ct.depth[0] = depth;
+                    set_anchors(ct, 0, w, ta, tb);
+                    store_contact(ct, k, a, b, cls, W, B.n, slots);
+                    hit[k] = 1;
+                    atomicAdd(reinterpret_cast<unsigned long long*>(
+                                  partials + size_t(kSlotContacts + cls) * kMaxPartialBlocks),
+                              1ull);
+                }
+            } else if (st == 0) {
+                if (t == 0)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(
+                                  partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks),
+                              1ull);
+            } else if (t == 0) {
+                const uint32_t u = atomicAdd(&c->epa_unresolved, 1u);
+                if (u < record_cap)
+                    unresolved[u] = (uint64_t(a) << 32) | b;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1453,6 +1930,8 @@ __global__ void k_np_begin(Counters* c)
     if (threadIdx.x == 0) {
         c->gjk_failures = 0;
         c->epa_overflow = 0;
+        c->epa_continued = 0;
+        c->epa_unresolved = 0;
         c->manifold_count = 0;
     }
 }
@@ -1518,7 +1997,16 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
         time_begin(ctx, "np_gjk_epa", t);
         hipLaunchKernelGGL(k_gjk_epa, dim3(tu.epa_groups), dim3(kEpaBlock), 0, ctx->stream, B,
                            W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
-                           ctx->hit, ctx->partials, tu.sphere_hull_per_wave == 1);
+                           ctx->hit, ctx->partials, tu.sphere_hull_per_wave == 1,
+                           ctx->epa_records, ctx->epa_record_cap, ctx->epa_unresolved_pairs);
+        time_end(ctx, t);
+        // Runs past the LDS polytope bounds (none on most steps: every
+        // workgroup then exits at once).
+        time_begin(ctx, "np_epa_continue", t);
+        hipLaunchKernelGGL(k_epa_continue, dim3(std::min(ctx->epa_record_cap, kEpaContinueGroups)),
+                           dim3(kContThreads), 0, ctx->stream, B, W, ctx->counters, ctx->epa_records,
+                           ctx->epa_record_cap, opt, ctx->slots, ctx->hit, ctx->partials,
+                           ctx->epa_unresolved_pairs);
         time_end(ctx, t);
     }
     time_begin(ctx, "np_sphere_sphere", t, cs);

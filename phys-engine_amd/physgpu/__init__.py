@@ -13,6 +13,7 @@ All work runs in the HIP library; when it cannot be loaded every call raises.
 import ctypes as C
 import enum
 import os
+import warnings
 
 import numpy as np
 
@@ -69,6 +70,8 @@ def library():
         "physgpu_download_manifolds": [vp, vp, u64],
         "physgpu_narrowphase_stats_get": [vp, C.POINTER(abi.NarrowPhaseStats)],
         "physgpu_step_counts": [vp, C.POINTER(u64), C.POINTER(u64)],
+        "physgpu_epa_unresolved": [vp, vp, u64, C.POINTER(u64)],
+        "physgpu_step_stats_get": [vp, C.POINTER(abi.StepStats)],
         "physgpu_kernel_times": [vp, C.POINTER(C.c_char_p), C.POINTER(dbl), C.POINTER(u64), u32,
                                  C.POINTER(u32)],
         "physgpu_set_worlds": [vp, u32, vp, i32],
@@ -96,10 +99,18 @@ def library():
     return L
 
 
+class EpaUnresolvedWarning(RuntimeWarning):
+    """Some hull pairs of a step got no EPA result (PHYSGPU_EPA_UNRESOLVED);
+    CollisionPipeline.unresolved_pairs() lists them."""
+
+
 def _check(status, allow=()):
     if status == abi.OK or status in allow:
         return status
     msg = library().physgpu_last_error().decode(errors="replace")
+    if status == abi.EPA_UNRESOLVED:  # every result is written
+        warnings.warn(msg, EpaUnresolvedWarning, stacklevel=3)
+        return status
     if status == abi.CAPACITY_EXCEEDED:
         raise CapacityError(status, msg)
     if status == abi.INVALID_ARGUMENT:
@@ -391,6 +402,21 @@ class CollisionPipeline:
         out = np.zeros(max(m, 1), MANIFOLD_DTYPE)
         _check(library().physgpu_download_manifolds(self._ctx, out.ctypes.data, m))
         return out[:m]
+
+    def unresolved_pairs(self):
+        """Hull pairs of the last narrow phase without an EPA result, uint32 [U, 2]."""
+        L = library()
+        count = C.c_uint64(0)
+        _check(L.physgpu_epa_unresolved(self._ctx, None, 0, C.byref(count)))
+        out = np.zeros((max(count.value, 1), 2), np.uint32)
+        _check(L.physgpu_epa_unresolved(self._ctx, out.ctypes.data, count.value, C.byref(count)))
+        return out[:count.value]
+
+    def step_stats(self):
+        """StepStats collision fields of the last step (dict)."""
+        stats = abi.StepStats()
+        _check(library().physgpu_step_stats_get(self._ctx, C.byref(stats)))
+        return stats.as_dict()
 
     def narrowphase_stats(self):
         stats = abi.NarrowPhaseStats()

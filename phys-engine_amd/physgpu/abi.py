@@ -12,6 +12,7 @@ INVALID_ARGUMENT = 1
 CAPACITY_EXCEEDED = 2
 DEVICE_ERROR = 3
 UNSUPPORTED = 4
+EPA_UNRESOLVED = 5
 
 SWEEP_AND_PRUNE = 0
 UNIFORM_GRID = 1
@@ -31,9 +32,12 @@ PAIR_SPHERE_HULL = 3
 PAIR_BOX_HULL = 4
 PAIR_HULL_HULL = 5
 
-# Bounded EPA polytope of the GPU narrow phase (include/physgpu.h).
+# EPA polytope of the GPU narrow phase (include/physgpu.h): exact in LDS up
+# to these bounds, then continued over at most EPA_MAX_DISTINCT_FACES
+# distinct faces with copy counts.
 EPA_MAX_FACES = 256
 EPA_MAX_HORIZON = 128
+EPA_MAX_DISTINCT_FACES = 1024
 EPA_MAX_ITERATIONS = 64
 
 
@@ -90,10 +94,32 @@ class NarrowPhaseStats(C.Structure):
         ("hull_contacts", C.c_uint64),
         ("gjk_epa_failures", C.c_uint64),
         ("epa_overflows", C.c_uint64),
+        ("epa_unresolved", C.c_uint64),
     ]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class StepStats(C.Structure):
+    """Collision fields of phys::StepStats (physgpu_step_stats)."""
+    _fields_ = [
+        ("broad_phase_ms", C.c_double),
+        ("narrow_phase_ms", C.c_double),
+        ("broad_phase_collect_ms", C.c_double),
+        ("broad_phase_filter_ms", C.c_double),
+        ("possible_pairs", C.c_uint64),
+        ("candidate_pairs", C.c_uint64),
+        ("contact_count", C.c_uint64),
+        ("broad_phase", BroadPhaseStats),
+        ("narrow_phase", NarrowPhaseStats),
+    ]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_[:7]}
+        d["broad_phase"] = self.broad_phase.as_dict()
+        d["narrow_phase"] = self.narrow_phase.as_dict()
+        return d
 
 
 class Manifold(C.Structure):

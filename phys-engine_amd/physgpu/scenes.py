@@ -44,6 +44,7 @@ class Scene:
     local_pose: np.ndarray = None
     friction: np.ndarray = None
     restitution: np.ndarray = None
+    body: np.ndarray = None  # owning body per collider (world level); None: one each
     meta: dict = field(default_factory=dict)
 
     @property
@@ -71,7 +72,7 @@ class Scene:
                      shape_data=take(self.shape_data), hull_range=hr, hull_points=hp,
                      body_position=take(self.body_position), local_pose=take(self.local_pose),
                      friction=take(self.friction), restitution=take(self.restitution),
-                     meta=dict(self.meta, subset=len(index)))
+                     body=take(self.body), meta=dict(self.meta, subset=len(index)))
 
 
 def random_quaternions(rng, n):
@@ -400,3 +401,39 @@ def with_world_extras(scene, seed=7, local_rotation=True):
     scene.friction = rng.uniform(-0.1, 1.0, n).astype(np.float32)
     scene.restitution = rng.uniform(0.0, 0.8, n).astype(np.float32)
     return scene
+
+
+def compound_bodies(bodies=2000, seed=20261207, with_hulls=True, pairs_per_body=1.5):
+    """World-level scene of compound bodies: body b carries 1 to 4 colliders
+    (the mixed shape mix) at local offsets U[-0.6, 0.6]^3 with random local
+    orientations; bodies sit at identity rotation (the reference has no
+    body-rotation setter) in a cube sized for about `pairs_per_body` candidate
+    pairs per collider, with per-body friction U[-0.1, 1] and restitution
+    U[0, 0.8].  Colliders are listed body by body, so body ids appear in
+    increasing order (the reference's body handle order)."""
+    rng = np.random.default_rng(seed)
+    per = rng.integers(1, 5, bodies)
+    n = int(per.sum())
+    body = np.repeat(np.arange(bodies, dtype=np.uint32), per)
+    shapes = mixed(n, seed=seed + 1, side=1.0, with_hulls=with_hulls)
+    side = mixed_side_for(n, pairs_per_body)
+    bpos = rng.uniform(-side / 2, side / 2, (3, bodies)).astype(np.float32)
+    lp = np.zeros((7, n), np.float32)
+    lp[:3] = rng.uniform(-0.6, 0.6, (3, n))
+    lp[3:7] = random_quaternions(rng, n)
+    sc = Scene(pose=np.zeros((7, n), np.float32), shape_type=shapes.shape_type,
+               shape_data=shapes.shape_data, hull_range=shapes.hull_range,
+               hull_points=shapes.hull_points,
+               meta=dict(workload="compound_bodies", bodies=bodies, n=n, seed=seed))
+    sc.body = body
+    sc.body_position = np.ascontiguousarray(bpos[:, body])
+    sc.local_pose = lp
+    # Collider world pose = body pose (identity rotation) composed with the
+    # local transform: position body + local (the rotation is the local one).
+    sc.pose = np.ascontiguousarray(np.concatenate([sc.body_position + lp[:3], lp[3:7]]),
+                                   np.float32)
+    fr = rng.uniform(-0.1, 1.0, bodies).astype(np.float32)
+    rs = rng.uniform(0.0, 0.8, bodies).astype(np.float32)
+    sc.friction = np.ascontiguousarray(fr[body])
+    sc.restitution = np.ascontiguousarray(rs[body])
+    return sc

@@ -17,6 +17,9 @@ returns.  The fixtures hold inputs and expected outputs only.
   gjk.npz       GJK + EPA (status, normal, depth, final simplex) over hull,
                 box and sphere pairs with this project's support maps
   world.npz     world-level collision section (body-local anchors, materials)
+  world_bodies.npz  world-level collision section of compound bodies (1 to 4
+                colliders per body, spheres and boxes): same-body filter,
+                StepStats candidatePairs / possiblePairs, per algorithm
   large.json    pair counts and SHA-256 of the pair lists at 1,048,576 mixed
                 bodies and the 262,145-box stack
 """
@@ -160,11 +163,30 @@ def pair_hash(pairs):
     return hashlib.sha256(np.ascontiguousarray(pairs, np.uint32).tobytes()).hexdigest()
 
 
-def main(large=True):
+def world_bodies(ref):
+    """Compound bodies through the reference's PhysicsWorld::step."""
+    sc = scenes.compound_bodies(2000, seed=20261207, with_hulls=False)
+    out = scene_arrays(sc)
+    out["body"] = sc.body
+    for algo, name in ((0, "sap"), (1, "grid"), (2, "tree")):
+        m, t, st = ref.world_collide(sc, algorithm=algo)
+        out[f"contacts_{name}"] = m
+        out[f"np_stats_{name}"] = st
+        out[f"counts_{name}"] = np.array([t["candidate_pairs"], t["possible_pairs"], len(m)],
+                                         np.int64)
+    return out
+
+
+def main(large=True, only=None):
     O.build(ref=True)
     ref = O.CpuCollision("ref")
-    orc = O.CpuCollision("oracle")
     save = lambda name, d: np.savez_compressed(os.path.join(HERE, name), **d)  # noqa: E731
+    if only == "world_bodies":
+        save("world_bodies.npz", world_bodies(ref))
+        print("world_bodies.npz written")
+        return
+    orc = O.CpuCollision("oracle")
+    save("world_bodies.npz", world_bodies(ref))
 
     save("spheres.npz", broad_and_narrow(ref, scenes.spheres(4096, seed=20261203),
                                          with_world=True))
@@ -222,4 +244,5 @@ def main(large=True):
 
 
 if __name__ == "__main__":
-    main(large="--no-large" not in sys.argv)
+    only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
+    main(large="--no-large" not in sys.argv, only=only)

@@ -32,6 +32,24 @@ def _seeds(man, seed=1, points=True):
     return w.astype(np.float32), gd, gp
 
 
+def _sphere_separation(sc, man):
+    """|D| of sphere-sphere (centre distance) and outside-branch sphere-box
+    (radius - depth, the centre's distance to the box) manifolds, in float64;
+    inf for the others."""
+    a, b = man["first"].astype(np.int64), man["second"].astype(np.int64)
+    ta, tb = sc.shape_type[a], sc.shape_type[b]
+    out = np.full(len(man), np.inf)
+    ss = (ta == abi.SPHERE) & (tb == abi.SPHERE)
+    d = sc.pose[:3, b].astype(np.float64) - sc.pose[:3, a].astype(np.float64)
+    out[ss] = np.linalg.norm(d[:, ss], axis=0)
+    sb = (ta == abi.SPHERE) ^ (tb == abi.SPHERE)
+    sb &= (ta != abi.HULL) & (tb != abi.HULL)
+    r = np.where(ta == abi.SPHERE, sc.shape_data[0, a], sc.shape_data[0, b]).astype(np.float64)
+    gap = r - man["penetration"][:, 0].astype(np.float64)
+    out[sb & (gap > 0)] = gap[sb & (gap > 0)]
+    return np.maximum(out, 1e-12)
+
+
 def _rel(got, want):
     scale = np.maximum(np.abs(want).max(0), 1e-3)
     return np.abs(got - want).max(0) / scale
@@ -143,23 +161,34 @@ def test_batched_mixed_worlds(physgpu, oracle):
     assert skipped == int(sphere_hull.sum()) > 0
     assert np.all(got[:, sphere_hull] == 0)
     pairs = _pairs_of(man)
-    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    # FP64 runs stop at the LDS bounds (no continuation): the GPU adjoint
+    # differentiates only runs inside them, and past them FP64 can settle on a
+    # neighbouring near-coplanar face within the same-face normal tolerance.
+    oracle.mirror_gpu_epa(continuation=False)
     try:
         want, status = oracle.contact_adjoint_f64(sc, pairs, gn, gd, gp, workers=16)
         fd, stable = oracle.contact_fd_f64(sc, pairs, gn, gd, gp, workers=16)
         n64, _, _, _ = oracle.contact_geometry_f64(sc, pairs)
     finally:
-        oracle.set_epa_bounds(0, 0)
+        oracle.mirror_gpu_epa(False)
     assert np.array_equal(status == -1, sphere_hull)
     same_face = np.abs(n64.T - man["normal"]).max(1) < 1e-5
     ok = (status == 1) & (~hull | same_face)
     rel = _rel(got[:, ok].astype(np.float64), want[:, ok])
+    # Random scenes are not kept away from ill-conditioned contacts either: a
+    # sphere's normal is a unit vector along a separation |D| (centre to
+    # centre, or centre to the box's closest point), whose derivative scales
+    # as 1/|D|, so FP32 rounding of the ~10 m coordinates (ulp ~1e-6) gives
+    # relative errors ~1e-7 * 16 m / |D| there (a centre 5 um off a box face
+    # is 3e-3).  The tolerance takes that bound on top of TOL.
+    delta = _sphere_separation(sc, man)[ok]
+    tol = TOL + 1e-7 * 16.0 / delta
     print(f"batched: {ok.sum()} differentiated manifolds ({(ok & hull).sum()} through EPA), "
-          f"max rel {rel.max():.2e}, {(rel > TOL).sum()} above {TOL}; "
-          f"FD-stable {stable[ok].mean():.4f}")
-    # Random scenes are not kept away from ties: compare where FP64 differences
-    # see one smooth branch.
-    assert (rel[stable[ok]] <= TOL).all()
+          f"max rel {rel.max():.2e}, {(rel > TOL).sum()} above {TOL} "
+          f"({(delta < 1e-3).sum()} with |D| < 1 mm); FD-stable {stable[ok].mean():.4f}")
+    # Compare where FP64 differences see one smooth branch.
+    assert (rel[stable[ok]] <= tol[stable[ok]]).all()
+    assert (rel[stable[ok] & (delta >= 1e-3)] <= TOL).all()
     assert ok[hull & ~sphere_hull].mean() > 0.8
 
 
@@ -219,13 +248,16 @@ def test_hull_pairs_through_the_epa_face(physgpu, oracle):
     assert np.array_equal(depth[0][poly].view(np.uint32),
                           man["penetration"][poly, 0].view(np.uint32))
     pairs = _pairs_of(man)[poly]
-    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    # FP64 runs stop at the LDS bounds (no continuation): the GPU adjoint
+    # differentiates only runs inside them, and past them FP64 can settle on a
+    # neighbouring near-coplanar face within the same-face normal tolerance.
+    oracle.mirror_gpu_epa(continuation=False)
     try:
         want, status = oracle.contact_adjoint_f64(sc, pairs, w[:, poly], gd[:, poly], workers=16)
         fd, stable = oracle.contact_fd_f64(sc, pairs, w[:, poly], gd[:, poly], workers=16)
         n64, _, _, _ = oracle.contact_geometry_f64(sc, pairs)
     finally:
-        oracle.set_epa_bounds(0, 0)
+        oracle.mirror_gpu_epa(False)
     same_face = np.abs(n64.T - man["normal"][poly]).max(1) < 1e-5
     ok = (status == 1) & stable & same_face
     print(f"hull manifolds {poly.sum()}: compared {ok.sum()} (same FP64 face and smooth)")

@@ -14,9 +14,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module", autouse=True)
 def bounded_oracle(oracle):
-    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    oracle.mirror_gpu_epa()
     yield
-    oracle.set_epa_bounds(0, 0)
+    oracle.mirror_gpu_epa(False)
 
 
 @pytest.fixture(scope="module")

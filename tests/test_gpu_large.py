@@ -40,9 +40,9 @@ def big_pipe(physgpu):
 
 @pytest.fixture(scope="module", autouse=True)
 def bounded_oracle(oracle):
-    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    oracle.mirror_gpu_epa()
     yield
-    oracle.set_epa_bounds(0, 0)
+    oracle.mirror_gpu_epa(False)
 
 
 @pytest.mark.parametrize("algo", ALGOS)
@@ -65,12 +65,38 @@ def test_mixed_1m_manifolds_bitwise(big_pipe, mixed_1m, oracle):
     stats = big_pipe.narrowphase_stats()
     rec, hit = oracle.generate_contacts(mixed_1m, pairs, workers=min(os.cpu_count() or 1, 16))
     assert_manifolds_equal(man, compact(rec, hit), "mixed 1M")
-    st, _, _, _ = oracle.gjk_epa(mixed_1m, pairs)
     hull = ((mixed_1m.shape_type[pairs[:, 0]] == abi.HULL) |
             (mixed_1m.shape_type[pairs[:, 1]] == abi.HULL))
-    assert stats["epa_overflows"] == int(((st == 3) & hull).sum())
-    # The bound is hit by a small minority of the hull pairs GJK reports as intersecting.
-    assert stats["epa_overflows"] < 0.05 * max(1, int(((st > 0) & hull).sum()))
+    st, _, _, _ = oracle.gjk_epa(mixed_1m, pairs[hull])
+    assert stats["epa_unresolved"] == int((st == 3).sum()) == 0
+    assert stats["gjk_epa_failures"] == int((st == 2).sum())
+    # Every pair whose polytope outgrows the LDS bounds is in the fixture of
+    # the reference's own unbounded results, and was continued on the GPU.
+    ovf = np.load(os.path.join(GOLDEN, "epa_overflow.npz"), allow_pickle=False)
+    assert stats["epa_overflows"] == len(ovf["pairs"])
+    check_against_reference(man, ovf)
+
+
+def check_against_reference(man, ovf):
+    """GPU manifolds of the fixture's pairs against the reference's unbounded
+    computeEpa (tests/golden/make_epa_overflow.py): a contact with the
+    reference's normal and depth bits where it converged, none where it
+    failed (pairs the reference did not finish within its limits are only
+    counted)."""
+    got = {(int(m["first"]), int(m["second"])): m for m in man
+           if m["shape_pair"] >= abi.PAIR_SPHERE_HULL}
+    converged = 0
+    for (a, b), rs, rn, rd in zip(ovf["pairs"], ovf["ref_status"], ovf["ref_normal"],
+                                  ovf["ref_depth"]):
+        m = got.get((int(a), int(b)))
+        if rs == 1:
+            assert m is not None, (a, b)
+            assert np.array_equal(m["normal"].view(np.uint32), rn.view(np.uint32)), (a, b)
+            assert m["penetration"][0].view(np.uint32) == rd.view(np.uint32), (a, b)
+            converged += 1
+        elif rs == 2:
+            assert m is None, (a, b)
+    assert converged == int((ovf["ref_status"] == 1).sum()) > 0
 
 
 def test_stack_262145(big_pipe, oracle):

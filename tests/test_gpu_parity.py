@@ -28,9 +28,9 @@ CASES = _cases()
 @pytest.fixture(scope="module", autouse=True)
 def bounded_oracle(oracle):
     """Expected results use the GPU's EPA polytope bounds."""
-    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    oracle.mirror_gpu_epa()
     yield
-    oracle.set_epa_bounds(0, 0)
+    oracle.mirror_gpu_epa(False)
 
 
 @pytest.fixture(scope="module")
@@ -79,8 +79,14 @@ def test_contacts_bitwise(pipe, oracle, name):
     assert np.array_equal(got["pair_index"], np.nonzero(hit)[0].astype(np.uint32))
     st, _, _, _ = oracle.gjk_epa(sc, pairs)
     hull = (sc.shape_type[pairs[:, 0]] == abi.HULL) | (sc.shape_type[pairs[:, 1]] == abi.HULL)
-    assert stats["epa_overflows"] == int(((st == 3) & hull).sum())
+    assert stats["epa_unresolved"] == int(((st == 3) & hull).sum())
     assert stats["gjk_epa_failures"] == int(((st == 2) & hull).sum())
+    oracle.mirror_gpu_epa(continuation=False)
+    try:
+        sb, _, _, _ = oracle.gjk_epa(sc, pairs)
+    finally:
+        oracle.mirror_gpu_epa()
+    assert stats["epa_overflows"] == int(((sb == 3) & hull).sum())
     for cls, key in ((0, "sphere_sphere"), (1, "sphere_box"), (2, "box_box")):
         assert stats[key + "_contacts"] == int((hit & (rec["shape_pair"] == cls)).sum())
 

@@ -17,7 +17,7 @@ def scene_from(g, prefix=""):
                         shape_data=g[prefix + "shape_data"], hull_range=get("hull_range"),
                         hull_points=get("hull_points"), body_position=get("body_position"),
                         local_pose=get("local_pose"), friction=get("friction"),
-                        restitution=get("restitution"))
+                        restitution=get("restitution"), body=get("body"))
 
 
 def stats_match(got, want_row, algo):
@@ -101,18 +101,65 @@ def test_gjk_epa_fixture(oracle, defaults):
 
 
 def test_bounded_epa_equals_reference_within_bounds(oracle):
-    """With the GPU's polytope bounds the oracle's EPA is the reference's on
-    every pair that stays inside them; the others report overflow (status 3)."""
+    """With the GPU's polytope bounds and no continuation the oracle's EPA is
+    the reference's on every pair that stays inside them; the others report
+    overflow (status 3)."""
     g = golden("gjk.npz")
     sc = scene_from(g)
-    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    oracle.mirror_gpu_epa(continuation=False)
     try:
         st, nrm, dep, _ = oracle.gjk_epa(sc, g["pairs"], 64, 64, 1e-6)
     finally:
-        oracle.set_epa_bounds(0, 0)
+        oracle.mirror_gpu_epa(False)
     inside = st != 3
-    assert inside.mean() > 0.97
+    assert inside.mean() > 0.97 and (~inside).sum() > 0
     assert np.array_equal(st[inside], g["status"][inside])
     assert np.array_equal(nrm[inside].view(np.uint32), g["normal"][inside].view(np.uint32))
     assert np.array_equal(dep[inside].view(np.uint32), g["depth"][inside].view(np.uint32))
     assert np.all(g["status"][~inside] >= 1)  # GJK hits the reference kept expanding
+
+
+def test_gpu_epa_equals_reference(oracle):
+    """The GPU's EPA (the reference's list within the LDS bounds, continued
+    over distinct faces with copy counts past them) gives the reference's
+    status, normal and depth bits on every pair of the fixture, the pairs
+    that outgrow the bounds included."""
+    g = golden("gjk.npz")
+    sc = scene_from(g)
+    oracle.mirror_gpu_epa()
+    try:
+        st, nrm, dep, _ = oracle.gjk_epa(sc, g["pairs"], 64, 64, 1e-6)
+    finally:
+        oracle.mirror_gpu_epa(False)
+    assert not np.any(st == 3)
+    assert np.array_equal(st, g["status"])
+    assert np.array_equal(nrm.view(np.uint32), g["normal"].view(np.uint32))
+    assert np.array_equal(dep.view(np.uint32), g["depth"].view(np.uint32))
+
+
+@pytest.mark.parametrize("algo", [0, 1, 2])
+def test_world_compound_bodies_match_reference(oracle, algo):
+    """PhysicsWorld::step with 1 to 4 colliders per body (world_bodies.npz,
+    recorded from the reference): same-body pairs erased in order
+    (physics_world.cpp:630-633), body-local anchors, combined materials,
+    StepStats candidatePairs / possiblePairs / contactCount and the narrow
+    phase counters, bit for bit (the oracle's first / second are collider
+    indices, the reference's body indices)."""
+    g = golden("world_bodies.npz")
+    sc = scene_from(g)
+    name = ("sap", "grid", "tree")[algo]
+    m, t, st = oracle.world_collide(sc, algorithm=algo)
+    want = g["contacts_" + name]
+    got = m.copy()
+    got["first"] = sc.body[m["first"]]
+    got["second"] = sc.body[m["second"]]
+    w = lambda a: np.ascontiguousarray(a).view(np.uint32).reshape(len(a), 40)[:, :36]  # noqa
+    assert w(got).shape == w(want).shape and np.array_equal(w(got), w(want))
+    cand, possible, contacts = (int(v) for v in g["counts_" + name])
+    assert (t["candidate_pairs"], t["possible_pairs"], len(m)) == (cand, possible, contacts)
+    assert np.array_equal(st, g["np_stats_" + name])
+    # possiblePairs in closed form: C(n, 2) minus the pairs inside each body.
+    k = np.bincount(sc.body)
+    assert possible == sc.count * (sc.count - 1) // 2 - int((k * (k - 1) // 2).sum())
+    # The filter removed something, so the test exercises it.
+    assert cand < oracle.find_pairs(oracle.compute_aabbs(sc), algo)[1]["aabb_pairs"]

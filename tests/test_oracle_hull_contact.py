@@ -19,9 +19,9 @@ def bounded_epa(oracle):
     # The GPU's polytope bounds keep every EPA call short (a few FP32 pairs
     # grow the reference's unbounded polytope for minutes); the contact point
     # of a converged pair does not depend on them.
-    oracle.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
+    oracle.mirror_gpu_epa()
     yield
-    oracle.set_epa_bounds(0, 0)
+    oracle.mirror_gpu_epa(False)
 
 
 @pytest.fixture(scope="module")
