@@ -1,11 +1,13 @@
 #!/usr/bin/env python3
 """Collision-step benchmark on one MI355X: 1,048,576 mixed bodies.
 
-A step is one pass of the collision section of PhysicsWorld::step over
-device-resident colliders: AABB refresh, broad phase (SAP, uniform grid or
-LBVH), canonical pair order, narrow phase (SS, SB, BB SAT + manifold,
-GJK/EPA for hulls) and ordered manifold compaction.  Inputs are generated
-once (seeded) and uploaded before timing.
+A step is the pose refresh (the integrated poses a stepping loop hands over,
+read from device memory by physgpu_update_poses) plus one pass of the
+collision section of PhysicsWorld::step over device-resident colliders: AABB
+refresh, broad phase (SAP, uniform grid or LBVH), canonical pair order,
+narrow phase (SS, SB, BB SAT + manifold, GJK/EPA for hulls) and ordered
+manifold compaction.  Inputs are generated once (seeded) and uploaded before
+timing.
 
 Prints one JSON line (see the "contract" fields) to stdout; details go to
 stderr.  `value` is the ms/step of the fastest broad phase (all three return
@@ -40,6 +42,14 @@ MEASURED_TRAFFIC_1M = {
 }
 
 
+# Kernels of the broad phase (timer names of the library's launches).
+BROADPHASE_PREFIXES = ("aabb_", "grid_", "radix_", "sap_", "lbvh_", "gather_")
+
+
+def kernel_launches(per_algo, name):
+    return per_algo.get("launches", {}).get(name, 1)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -65,24 +75,33 @@ def kernel_bytes(name, n, pairs, hull_bodies, hull_points, table_entries):
     }.get(name)
 
 
-def run_algorithm(pipe, algo, steps, warmup):
-    """Eager steps, then the same step replayed from a HIP graph (the form a
-    stepping loop uses: one launch per step), then per-kernel timings."""
+def run_algorithm(pipe, algo, steps, warmup, pose_dev):
+    """A step = the pose refresh from device memory (physgpu_update_poses,
+    the integrated poses a stepping loop hands over) + the collision section.
+    Eager launches, then the collision section replayed from a HIP graph (one
+    launch per step), then per-kernel HIP-event timings of eager steps."""
+    def step(graph):
+        pipe.update_poses_device(pose_dev)
+        if graph:
+            pipe.collide_graph(algorithm=algo, sync=False)
+        else:
+            pipe.collide(algorithm=algo, sync=False)
+
     for _ in range(warmup):
-        pipe.collide(algorithm=algo, sync=False)
+        step(False)
     pipe.step_counts()
     t0 = time.perf_counter()
     for _ in range(steps):
-        pipe.collide(algorithm=algo, sync=False)
+        step(False)
     npairs, nman = pipe.step_counts()  # synchronises the stream
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / steps
     for _ in range(warmup):
-        pipe.collide_graph(algorithm=algo, sync=False)
+        step(True)
     pipe.step_counts()
     t0 = time.perf_counter()
     for _ in range(steps):
-        pipe.collide_graph(algorithm=algo, sync=False)
+        step(True)
     gpairs, gman = pipe.step_counts()
     graph_ms = (time.perf_counter() - t0) * 1e3 / steps
     if (gpairs, gman) != (npairs, nman):
@@ -90,24 +109,41 @@ def run_algorithm(pipe, algo, steps, warmup):
     # Second pass with per-kernel HIP events on the library's stream.
     pipe.set_timing(True)
     for _ in range(steps):
-        pipe.collide(algorithm=algo, sync=False)
+        step(False)
     pipe.step_counts()
     kt = pipe.kernel_times()
     pipe.set_timing(False)
     return ms, graph_ms, npairs, nman, kt
 
 
-def cpu_baseline(scene, seconds_budget=30.0):
-    """The oracle's CPU restatement of the same collision step (port of the
-    reference: its grid broad phase single-threaded as in the reference, the
-    narrow phase on a worker pool), one step of the full workload."""
+def host_info():
+    """CPU of the host the baseline runs on (nproc, affinity, model)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu": model}
+
+
+def cpu_baseline(scene, steps=3):
+    """The oracle's CPU restatement of the same collision step (bit-exact port
+    of the reference's collision section): grid broad phase single-threaded as
+    in the reference, the narrow phase on a worker pool of the host's share of
+    cores (at most 16 per GPU); median of `steps` steps after one warm-up."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     orc = O.CpuCollision("oracle")
-    # The GPU's EPA polytope bounds: the reference's unbounded FP32 EPA can
-    # duplicate faces without end on a few hull pairs (minutes per pair).
-    orc.set_epa_bounds(abi.EPA_MAX_FACES, abi.EPA_MAX_HORIZON)
-    cores = min(os.cpu_count() or 1, 16)
+    # EPA as the GPU runs it (the reference's list within the LDS bounds,
+    # continued past them over distinct faces with copy counts): the
+    # reference's own list takes minutes on a few hull pairs.
+    orc.mirror_gpu_epa()
+    host = host_info()
+    cores = max(1, min(host["affinity"], 16))
     w = scenes.with_world_extras(scenes.Scene(pose=scene.pose, shape_type=scene.shape_type,
                                               shape_data=scene.shape_data,
                                               hull_range=scene.hull_range,
@@ -115,14 +151,56 @@ def cpu_baseline(scene, seconds_budget=30.0):
                                  local_rotation=True)
     t0 = time.perf_counter()
     m, t, s = orc.world_collide(w, algorithm=abi.UNIFORM_GRID, bp_workers=1, np_workers=cores,
-                                warmup=0, steps=1, capacity=4 * scene.count)
+                                warmup=1, steps=steps, capacity=4 * scene.count)
     wall = time.perf_counter() - t0
     step_ms = t["broadphase_ms"] + t["narrowphase_ms"] + t["aabb_ms"]
     return {"value": round(step_ms, 3), "unit": "ms/step", "cores": cores, "kind": "port",
-            "sample": f"1 step of the full {scene.count}-body workload (grid broad phase, "
-                      f"1 thread; narrow phase on {cores} threads, EPA bounded as on the GPU); broadphase "
-                      f"{t['broadphase_ms']:.1f} ms, narrowphase {t['narrowphase_ms']:.1f} ms, "
-                      f"AABB {t['aabb_ms']:.1f} ms; {len(m)} manifolds; wall {wall:.1f} s"}
+            "host": host,
+            "sample": f"median of {steps} steps (after 1 warm-up) of the full {scene.count}-body "
+                      f"workload: AABB refresh + grid broad phase (1 thread, as the reference) + "
+                      f"narrow phase ({cores} threads); broadphase {t['broadphase_ms']:.1f} ms, "
+                      f"narrowphase {t['narrowphase_ms']:.1f} ms, AABB {t['aabb_ms']:.1f} ms; "
+                      f"{len(m)} manifolds; wall {wall:.1f} s"}
+
+
+def fp64_error(scene, pairs, man):
+    """|FP32 - FP64| of the step's manifolds, per shape-pair class: the GPU's
+    records against the oracle's FP64 re-evaluation of the same pairs (the
+    same algorithm in double), over the manifolds where FP64 reaches the same
+    outcome (a contact with the same point count).  Normals, depths and the
+    collider-local contact anchors (A's and B's).  EPA against a sphere's
+    curved support stops on whichever polytope face meets the 1e-6 test first,
+    so FP64 can settle on another face there: the sphere-hull class reports
+    the share of manifolds whose normals agree to 1e-4 besides its maxima."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    orc = O.CpuCollision("oracle")
+    orc.mirror_gpu_epa()
+    t0 = time.perf_counter()
+    rec, hit = orc.generate_contacts_f64(scene, pairs)
+    k = man["pair_index"].astype(np.int64)
+    ok = hit[k] & (rec[k, 0] == man["point_count"])
+    names = ("sphere_sphere", "sphere_box", "box_box", "sphere_hull", "box_hull", "hull_hull")
+    out = {"manifolds": int(len(man)), "compared": int(ok.sum()),
+           "note": "max |FP32 - FP64| over manifolds with the same FP64 outcome; anchors are "
+                   "collider-local contact points (m)"}
+    for cls, name in enumerate(names):
+        sel = ok & (man["shape_pair"] == cls)
+        if not sel.any():
+            continue
+        r, g = rec[k[sel]], man[sel]
+        used = np.arange(4)[None, :] < g["point_count"][:, None]
+        d64 = r[:, 4:32].reshape(-1, 4, 7)
+        n_err = np.abs(r[:, 1:4] - g["normal"]).max(1)
+        a_err = max(np.abs(d64[:, :, 0:3] - g["anchor_a"])[used].max(),
+                    np.abs(d64[:, :, 3:6] - g["anchor_b"])[used].max())
+        out[name] = {"compared": int(sel.sum()), "normal": float(n_err.max()),
+                     "depth": float(np.abs(d64[:, :, 6] - g["penetration"])[used].max()),
+                     "anchor": float(a_err),
+                     "normal_within_1e-4": round(float((n_err <= 1e-4).mean()), 4)}
+    log(f"fp64: {int(ok.sum())} of {len(man)} manifolds compared in "
+        f"{time.perf_counter() - t0:.1f} s")
+    return out
 
 
 def _hip():
@@ -345,6 +423,13 @@ def main():
     pipe = physgpu.CollisionPipeline(n, max_pairs=max(4 * n // 8, 1 << 16),
                                      max_hull_points=max(hull_points, 1))
     pipe.set_colliders(scene)
+    # The poses a stepping loop hands over every step, resident on the device.
+    import ctypes as C
+    hip = _hip()
+    pose_dev = C.c_void_p()
+    assert hip.hipMalloc(C.byref(pose_dev), C.c_size_t(scene.pose.nbytes)) == 0
+    assert hip.hipMemcpy(pose_dev, scene.pose.ctypes.data_as(C.c_void_p),
+                         C.c_size_t(scene.pose.nbytes), 1) == 0  # host to device
     table_entries = 0
     k = 2
     while k < 9 and (1 << (3 * k)) < 2 * n:
@@ -354,11 +439,13 @@ def main():
     per = {}
     for name in args.algorithms.split(","):
         algo = ALGOS[name]
-        eager_ms, graph_ms, npairs, nman, kt = run_algorithm(pipe, algo, args.steps, args.warmup)
+        eager_ms, graph_ms, npairs, nman, kt = run_algorithm(pipe, algo, args.steps, args.warmup,
+                                                             pose_dev.value)
         ms = min(eager_ms, graph_ms)
         per[name] = {"ms_per_step": round(ms, 4), "eager_ms_per_step": round(eager_ms, 4),
                      "graph_ms_per_step": round(graph_ms, 4), "pairs": npairs, "manifolds": nman,
-                     "kernels": {k: round(v[0] * 1e3, 2) for k, v in kt.items()}}
+                     "kernels": {k: round(v[0] * 1e3, 2) for k, v in kt.items()},
+                     "launches": {k: v[1] // args.steps for k, v in kt.items()}}
         log(f"{name}: {ms:.3f} ms/step (eager {eager_ms:.3f}, graph replay {graph_ms:.3f}), "
             f"{npairs} pairs, {nman} manifolds")
         for kn, (avg, launches) in sorted(kt.items(), key=lambda x: -x[1][0] * x[1][1]):
@@ -381,6 +468,19 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "bytes_per_launch": b, "avg_us": us}
         break
+    # The broad phase as a whole against its compulsory bytes B_bp = 108 N + 8 P
+    # (pose + shape read 44 B, AABB write 24 B, sorted AABB read 24 B, one
+    # key/index sort round trip 16 B per body; 8 B per pair written), over the
+    # summed device time of its kernels (AABB refresh through pair emission;
+    # the canonical pair order runs on the side stream during the narrow phase).
+    bp_names = [kn for kn in kt if kn.startswith(BROADPHASE_PREFIXES)]
+    bp_us = sum(kt[kn] * kernel_launches(per[best], kn) for kn in bp_names)
+    bp_bytes = 108 * n + 8 * npairs
+    roof_bp = {"bound": "hbm", "kernels": sorted(bp_names), "avg_us": round(bp_us, 2),
+               "bytes": bp_bytes,
+               "achieved": round(bp_bytes / (bp_us * 1e-6) / 1e9, 1) if bp_us else None,
+               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(bp_bytes / (bp_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if bp_us else None}
 
     line = {
         "metric": "collision ms/step at 1M bodies",
@@ -404,11 +504,17 @@ def main():
                                  else "eager launches")},
         "pairs_per_s": round(npairs / (ms * 1e-3), 1),
         "manifolds_per_s": round(per[best]["manifolds"] / (ms * 1e-3), 1),
-        "per_algorithm": {k: {kk: vv for kk, vv in v.items() if kk != "kernels"}
+        "per_algorithm": {k: {kk: vv for kk, vv in v.items() if kk not in ("kernels", "launches")}
                           for k, v in per.items()},
         "roofline": roof,
+        "roofline_broadphase": roof_bp,
         "cpu_baseline": None,
     }
+    # Error against an FP64 re-evaluation of the headline step's manifolds.
+    try:
+        line["fp64_max_error"] = fp64_error(scene, pipe.pairs(), pipe.manifolds())
+    except Exception as e:  # reported, never fatal for the GPU number
+        line["fp64_max_error"] = {"error": repr(e)}
     if args.batched_worlds > 0:
         line["batched"] = batched_bench(args.batched_worlds, args.steps, args.warmup)
     if args.adjoint_pairs > 0:
@@ -424,6 +530,7 @@ def main():
     log(json.dumps(per, indent=1))
     print(json.dumps(line), flush=True)
     pipe.close()
+    hip.hipFree(pose_dev)
     return 0
 
 

@@ -273,7 +273,8 @@ physgpu_status physgpu_set_colliders(physgpu_context* ctx, const physgpu_collide
  * resident collider) and keep their shapes: the per-step input of the AABB
  * refresh (physics_world.cpp:539-551 reads the integrated poses).  A captured
  * step graph stays valid.  Host memory is copied before the call returns;
- * device memory is copied in stream order. */
+ * device memory is read in stream order (it must stay unchanged until the
+ * context's stream has passed this call). */
 physgpu_status physgpu_update_poses(physgpu_context* ctx, const float* pose,
                                     physgpu_memory memory);
 

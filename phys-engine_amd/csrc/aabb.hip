@@ -510,11 +510,11 @@ void launch_compute_aabbs(physgpu_context* ctx, bool with_stats)
     time_end(ctx, t);
 }
 
-void launch_pose_aos(physgpu_context* ctx)
+void launch_pose_aos(physgpu_context* ctx, const float* src)
 {
     if (ctx->n)
         hipLaunchKernelGGL(k_pose_aos, dim3(div_up(ctx->n, kBlock)), dim3(kBlock), 0, ctx->stream,
-                           ctx->n, ctx->pose, ctx->pose4);
+                           ctx->n, src ? src : ctx->pose, ctx->pose4);
 }
 
 void launch_shape_aos(physgpu_context* ctx)

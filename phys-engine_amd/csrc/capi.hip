@@ -618,14 +618,17 @@ physgpu_status physgpu_update_poses(physgpu_context* ctx, const float* pose,
     if (memory != PHYSGPU_HOST && memory != PHYSGPU_DEVICE)
         return fail(PHYSGPU_INVALID_ARGUMENT, "unknown memory kind");
     PG_TRY_HIP(hipSetDevice(ctx->device));
-    if (ctx->n)
-        PG_TRY_HIP(hipMemcpyAsync(ctx->pose, pose, 7ull * ctx->n * 4,
-                                  memory == PHYSGPU_HOST ? hipMemcpyHostToDevice
-                                                         : hipMemcpyDeviceToDevice,
-                                  ctx->stream));
-    launch_pose_aos(ctx);
-    if (memory == PHYSGPU_HOST)
+    if (memory == PHYSGPU_HOST) {
+        if (ctx->n)
+            PG_TRY_HIP(hipMemcpyAsync(ctx->pose, pose, 7ull * ctx->n * 4, hipMemcpyHostToDevice,
+                                      ctx->stream));
+        launch_pose_aos(ctx);
         PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        // Device planes are read in place by the layout kernel (stream order).
+        launch_pose_aos(ctx, pose);
+    }
+    PG_TRY_HIP(hipGetLastError());
     return PHYSGPU_OK;
 }
 

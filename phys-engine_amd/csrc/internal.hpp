@@ -335,7 +335,8 @@ void launch_step_begin(physgpu_context* ctx);
 // width statistics behind the automatic grid cell (what
 // launch_aabb_reductions computes), in the same launch.
 void launch_compute_aabbs(physgpu_context* ctx, bool with_stats = false);
-void launch_pose_aos(physgpu_context* ctx);   // ctx->pose -> ctx->pose4
+// Pose planes (ctx->pose, or a caller's device planes `src`) -> ctx->pose4.
+void launch_pose_aos(physgpu_context* ctx, const float* src = nullptr);
 void launch_shape_aos(physgpu_context* ctx);  // shape planes -> ctx->shape4
 void launch_hull_list(physgpu_context* ctx);
 void launch_aabb_reductions(physgpu_context* ctx);
