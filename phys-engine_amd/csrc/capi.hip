@@ -113,7 +113,7 @@ void free_all(physgpu_context* c)
                     c->adjoint_skipped, c->partials, c->tree_tight, c->tree_fat, c->tree_rank,
                     c->tree_rank_user, c->tree_stage_ids, c->tree_stage_bounds,
                     c->tree_stage_flags, c->tree_stage_pairs, c->epa_records,
-                    c->epa_unresolved_pairs};
+                    c->epa_unresolved_pairs, c->epa_flags, c->epa_epoch};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -137,6 +137,10 @@ void free_all(physgpu_context* c)
         hipEventDestroy(c->join_event);
     if (c->side_stream)
         hipStreamDestroy(c->side_stream);
+    if (c->early_join_event)
+        hipEventDestroy(c->early_join_event);
+    if (c->early_stream)
+        hipStreamDestroy(c->early_stream);
     if (c->stream && c->owns_stream)
         hipStreamDestroy(c->stream);
 }
@@ -443,6 +447,8 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(hipEventCreateWithFlags(&c->sort_event, hipEventDisableTiming));
     A(hipEventCreateWithFlags(&c->fork_event, hipEventDisableTiming));
     A(hipEventCreateWithFlags(&c->join_event, hipEventDisableTiming));
+    A(hipStreamCreateWithFlags(&c->early_stream, hipStreamNonBlocking));
+    A(hipEventCreateWithFlags(&c->early_join_event, hipEventDisableTiming));
     A(alloc(c->pose, 7 * N));
     A(alloc(c->pose4, 2 * N));
     A(alloc(c->shape4, 2 * N));
@@ -495,6 +501,13 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     c->epa_record_cap = uint32_t(std::min<uint64_t>(std::max<uint64_t>(P / 1024, 64), 8192));
     A(alloc(c->epa_records, size_t(c->epa_record_cap) * kEpaRecordWords));
     A(alloc(c->epa_unresolved_pairs, c->epa_record_cap));
+    A(alloc(c->epa_flags, 2 * size_t(c->epa_record_cap) + 2));
+    A(alloc(c->epa_epoch, 1));
+    // Hand-over flags compare with the epoch of each narrow phase (from 1).
+    if (e == hipSuccess)
+        A(hipMemsetAsync(c->epa_flags, 0, (2 * size_t(c->epa_record_cap) + 2) * 4, c->stream));
+    if (e == hipSuccess)
+        A(hipMemsetAsync(c->epa_epoch, 0, 4, c->stream));
     A(alloc(c->counters, 1));
     A(hipHostMalloc(reinterpret_cast<void**>(&c->host_counters), sizeof(Counters)));
     if (e != hipSuccess) {

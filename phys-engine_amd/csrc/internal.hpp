@@ -38,6 +38,12 @@ constexpr uint32_t kMaxEpaGroups = 1u << 20;
 // record (iteration, vertices, faces), workgroups of the continuation kernel.
 constexpr uint32_t kEpaRecordWords = 1536;
 constexpr uint32_t kEpaContinueGroups = 256;
+// Workgroups of the consumer that continues handed-over runs while GJK + EPA
+// still runs, and how long it waits for them at most (s_memrealtime ticks of
+// 10 ns).
+constexpr uint32_t kEpaEarlyGroups = 64;
+constexpr uint64_t kEpaEarlyPollTicks = 500000;
+constexpr uint64_t kEpaEarlyStartTicks = 5000;
 enum PartialSlot : uint32_t {
     kSlotGridTests = 0,
     kSlotGridOverflowTests,
@@ -52,6 +58,7 @@ enum PartialSlot : uint32_t {
     kSlotContacts = kSlotCentreMax + 3,   // one per shape-pair class (6 slots)
     kSlotGjkFailures = kSlotContacts + 6,
     kSlotEpaOverflows,
+    kSlotEpaItemsDone,  // GJK + EPA work items finished (the hand-over consumer's stop test)
     kPartialSlots
 };
 enum FoldOp : uint32_t { kFoldSum64 = 0, kFoldSum32, kFoldMax32, kFoldMin32 };
@@ -126,6 +133,10 @@ struct physgpu_context {
     hipEvent_t sort_event = nullptr;  // pairs emitted: the canonical sort may start
     hipEvent_t fork_event = nullptr;
     hipEvent_t join_event = nullptr;
+    // Third stream: the consumer of EPA runs handed over past the LDS bounds
+    // (k_epa_early) runs on it beside GJK + EPA; joined before compaction.
+    hipStream_t early_stream = nullptr;
+    hipEvent_t early_join_event = nullptr;
     uint32_t cap_n = 0;
     uint64_t cap_p = 0;
     uint32_t cap_h = 0;
@@ -236,6 +247,11 @@ struct physgpu_context {
     uint32_t* epa_records = nullptr;
     uint64_t* epa_unresolved_pairs = nullptr;
     uint32_t epa_record_cap = 0;
+    // Per record: ready and claim words (epoch-tagged, 2 x epa_record_cap), then
+    // "k_epa_continue has started" and "k_gjk_epa has started" (zeroed once
+    // at create), and the epoch (one per narrow phase).
+    uint32_t* epa_flags = nullptr;
+    uint32_t* epa_epoch = nullptr;
 
     // World-level extras (optional): per collider, the pose of its body, its
     // body index, its local transform and its body's materials.

@@ -35,9 +35,15 @@ constexpr uint32_t kMaxEdges = PHYSGPU_EPA_MAX_HORIZON;
 // EPA's partial slots: contacts of classes 3..5, failures, overflows, written
 // by atomic adds spread over kEpaPartials addresses each.
 constexpr uint32_t kEpaPartialFirst = kSlotContacts + 3;
-constexpr uint32_t kEpaPartialSlots = 5;
+constexpr uint32_t kEpaPartialSlots = 6;
 constexpr uint32_t kEpaPartials = 1024;
-static_assert(kSlotGjkFailures == kEpaPartialFirst + 3 && kSlotEpaOverflows == kEpaPartialFirst + 4,
+// Work items finished by GJK + EPA (kSlotEpaItemsDone), counted on
+// kEpaDoneSlots addresses one cache line apart.
+constexpr uint32_t kEpaDoneSlots = 64;
+constexpr uint32_t kEpaDoneStride = 16;
+static_assert(kEpaDoneSlots * kEpaDoneStride <= kEpaPartials, "done counts lie in the zeroed slot");
+static_assert(kSlotGjkFailures == kEpaPartialFirst + 3 && kSlotEpaOverflows == kEpaPartialFirst + 4 &&
+                  kSlotEpaItemsDone == kEpaPartialFirst + 5,
               "EPA partial slots are contiguous");
 
 struct Shape {
@@ -730,19 +736,20 @@ static_assert(kMaxFaces <= kContThreads, "a handed-over list is staged one face 
 
 struct ContShared {
     float vx[kMaxVertices], vy[kMaxVertices], vz[kMaxVertices];
-    uint32_t fabc[kContFaces];  // a | b << 8 | c << 16
-    float fnx[kContFaces], fny[kContFaces], fnz[kContFaces];
+    alignas(16) uint32_t fabc[kContFaces];  // a | b << 8 | c << 16
+    alignas(16) float fnx[kContFaces];
+    alignas(16) float fny[kContFaces];
+    alignas(16) float fnz[kContFaces];
     alignas(16) float fd[kContFaces];
-    uint32_t mult[kContFaces];
-    uint8_t gen[kContFaces];       // 0: the handed-over faces; 1, 2, ...: later iterations
+    alignas(16) uint32_t mult[kContFaces];
+    alignas(16) uint8_t gen[kContFaces];  // 0: the handed-over faces; 1, 2, ...: later iterations
     uint32_t vis_abc[kContFaces];  // visible faces in order (first copies while staging)
     unsigned long long sum[kEdgeKeys];  // signed running sums per unordered vertex pair
     uint32_t first[kEdgeKeys];          // first call index per unordered vertex pair
     uint32_t wsum[2][kContWaves];       // per-wavefront partials of the block scans (2 buffers)
     float wbest_v[kContWaves];
     uint32_t wbest_i[kContWaves];
-    float sup[3];
-    uint32_t best;
+    uint32_t wnan[kContWaves];
 };
 
 // Edge e (0, 1, 2 = (a,b), (b,c), (c,a)) of a packed face.
@@ -795,7 +802,8 @@ __device__ __forceinline__ uint32_t block_prefix(ContShared& S, uint32_t c, uint
 
 // Closest face: the first minimum of S.fd[0, nf) in index order, as the
 // reference's sequential scan (std::min_element) picks it.  Every thread
-// calls it and gets the index.
+// calls it and gets the index (one barrier: each wavefront combines the
+// wavefronts' results itself).
 __device__ uint32_t block_closest(ContShared& S, uint32_t nf)
 {
     const uint32_t t = threadIdx.x, lane = lane_id(), w = t >> 6;
@@ -815,47 +823,45 @@ __device__ uint32_t block_closest(ContShared& S, uint32_t nf)
             }
         }
     }
-    if (__syncthreads_or(nan)) {
-        // A NaN distance: the sequential scan itself (strict <).
-        if (t == 0) {
-            uint32_t best = 0;
-            for (uint32_t f = 1; f < nf; ++f)
-                if (S.fd[f] < S.fd[best])
-                    best = f;
-            S.best = best;
-        }
-        __syncthreads();
-        return S.best;
-    }
+    const bool wave_nan = __ballot(nan) != 0;
     wave_arg_best<false>(v, i);
     if (lane == 0) {
         S.wbest_v[w] = v;
         S.wbest_i[w] = i;
+        S.wnan[w] = wave_nan ? 1u : 0u;
     }
     __syncthreads();
-    if (w == 0) {
-        float bv = lane < kContWaves ? S.wbest_v[lane] : 0.0f;
-        uint32_t bi = lane < kContWaves ? S.wbest_i[lane] : kNoIndex;
-        wave_arg_best<false>(bv, bi);
-        if (lane == 0)
-            S.best = bi;
+    bool any_nan = false;
+#pragma unroll
+    for (uint32_t q = 0; q < kContWaves; ++q)
+        any_nan = any_nan || S.wnan[q] != 0u;
+    if (any_nan) {
+        // A NaN distance: the sequential scan itself (strict <), by every
+        // thread alike.
+        uint32_t best = 0;
+        for (uint32_t f = 1; f < nf; ++f)
+            if (S.fd[f] < S.fd[best])
+                best = f;
+        return best;
     }
-    __syncthreads();
-    return S.best;
+    float bv = lane < kContWaves ? S.wbest_v[lane] : 0.0f;
+    uint32_t bi = lane < kContWaves ? S.wbest_i[lane] : kNoIndex;
+    wave_arg_best<false>(bv, bi);
+    return bi;
 }
 
 // Iterations [first, iterations) over the nf distinct faces in S (first-copy
 // order, multiplicities in S.mult, generation 0) and the vertices up to
 // 4 + first.  Returns 1 converged, 0 the reference's failure, -1 storage
 // bound, -2 undecided closest face; normal / depth uniform across the block.
-// Every thread of the workgroup calls it; wavefront 0 evaluates the support
-// maps (ha, hb hold both shapes' points per lane).
+// Every thread of the workgroup calls it (ha, hb hold both shapes' points per
+// lane).
 __device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, const pose& ta,
                                const WaveHull& ha, const Shape& sb, const pose& tb,
                                const WaveHull& hb, uint32_t first, uint32_t nf,
                                uint32_t iterations, float tol, f3& normal, float& depth)
 {
-    const uint32_t t = threadIdx.x, w = t >> 6, f0 = kFacesPerThread * t;
+    const uint32_t t = threadIdx.x, f0 = kFacesPerThread * t;
     normal = {0.0f, 0.0f, 0.0f};
     depth = 0.0f;
     uint32_t nv = 4 + first;
@@ -869,45 +875,46 @@ __device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, 
         const float cd = S.fd[bi];
         const uint32_t g0 = S.gen[bi];
         const uint32_t si = nv;
-        if (w == 0) {
-            const f3 sp =
-                sub(support_wave(B, sa, ta, cn, ha), support_wave(B, sb, tb, neg(cn), hb));
-            if (t == 0) {
-                S.sup[0] = sp.x;
-                S.sup[1] = sp.y;
-                S.sup[2] = sp.z;
-                S.vx[si] = sp.x;
-                S.vy[si] = sp.y;
-                S.vz[si] = sp.z;
-            }
+        // Every wavefront evaluates the support maps (same bits), so none
+        // waits for another's.
+        const f3 sup = sub(support_wave(B, sa, ta, cn, ha), support_wave(B, sb, tb, neg(cn), hb));
+        if (t == 0) {
+            S.vx[si] = sup.x;
+            S.vy[si] = sup.y;
+            S.vz[si] = sup.z;
         }
         ++nv;
         for (uint32_t q = t; q < kEdgeKeys; q += kContThreads) {
             S.sum[q] = 0ull;
             S.first[q] = kNoIndex;
         }
-        // This thread's faces, read before anything moves, and whether one
-        // of them ties the closest face within its iteration (undecided).
-        uint32_t abc[kFacesPerThread], m[kFacesPerThread];
-        uint8_t g[kFacesPerThread];
-        float nx[kFacesPerThread], ny[kFacesPerThread], nz[kFacesPerThread], d[kFacesPerThread];
+        // This thread's faces (one 16-byte read per array), read before
+        // anything moves, and whether one of them ties the closest face
+        // within its iteration (undecided).
+        const uint4 abc4 = reinterpret_cast<const uint4*>(S.fabc)[t];
+        const float4 nx4 = reinterpret_cast<const float4*>(S.fnx)[t];
+        const float4 ny4 = reinterpret_cast<const float4*>(S.fny)[t];
+        const float4 nz4 = reinterpret_cast<const float4*>(S.fnz)[t];
+        const float4 d4 = reinterpret_cast<const float4*>(S.fd)[t];
+        const uint4 m4 = reinterpret_cast<const uint4*>(S.mult)[t];
+        const uint32_t g4 = reinterpret_cast<const uint32_t*>(S.gen)[t];
+        const uint32_t abc[kFacesPerThread] = {abc4.x, abc4.y, abc4.z, abc4.w};
+        const uint32_t m[kFacesPerThread] = {m4.x, m4.y, m4.z, m4.w};
+        const uint8_t g[kFacesPerThread] = {uint8_t(g4), uint8_t(g4 >> 8), uint8_t(g4 >> 16),
+                                            uint8_t(g4 >> 24)};
+        const float nx[kFacesPerThread] = {nx4.x, nx4.y, nx4.z, nx4.w};
+        const float ny[kFacesPerThread] = {ny4.x, ny4.y, ny4.z, ny4.w};
+        const float nz[kFacesPerThread] = {nz4.x, nz4.y, nz4.z, nz4.w};
+        const float d[kFacesPerThread] = {d4.x, d4.y, d4.z, d4.w};
         bool tie = false;
 #pragma unroll
         for (uint32_t j = 0; j < kFacesPerThread; ++j) {
             const uint32_t f = f0 + j;
-            abc[j] = S.fabc[f];
-            nx[j] = S.fnx[f];
-            ny[j] = S.fny[f];
-            nz[j] = S.fnz[f];
-            d[j] = S.fd[f];
-            m[j] = S.mult[f];
-            g[j] = S.gen[f];
             tie = tie || (g0 != 0 && f < nf && f > bi && g[j] == g0 && d[j] == cd &&
                           (nx[j] != cn.x || ny[j] != cn.y || nz[j] != cn.z));
         }
         if (__syncthreads_or(tie))
             return -2;
-        const f3 sup{S.sup[0], S.sup[1], S.sup[2]};
         if (dot(sup, cn) - cd <= tol) {
             normal = cn;
             depth = max_ref(cd, 0.0f);
@@ -1021,8 +1028,12 @@ __global__ __launch_bounds__(kClassifyThreads) void k_classify(Bodies B, World W
                                                                uint32_t* __restrict__ buckets,
                                                                uint64_t cap,
                                                                uint32_t* __restrict__ hit,
-                                                               uint64_t* __restrict__ epa_partials)
+                                                               uint64_t* __restrict__ epa_partials,
+                                                               uint32_t* __restrict__ epa_epoch)
 {
+    // A new epoch for the EPA hand-over flags of this narrow phase.
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *epa_epoch += 1u;
     // The EPA kernels that follow add their counts into these partial slots.
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kEpaPartialSlots * kEpaPartials;
          i += gridDim.x * blockDim.x)
@@ -1359,8 +1370,14 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
                physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
                uint32_t* __restrict__ hit, uint64_t* __restrict__ partials, bool solo_sphere_hull,
                uint32_t* __restrict__ records, uint32_t record_cap,
-               uint64_t* __restrict__ unresolved)
+               uint64_t* __restrict__ unresolved, uint32_t* __restrict__ ready,
+               const uint32_t* __restrict__ epa_epoch)
 {
+    const uint32_t epoch = *epa_epoch;
+    // Tells k_epa_early that this kernel runs beside it.
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(&ready[2 * record_cap + 1], epoch, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     __shared__ EpaShared S;
     // GJK simplices (4 points x 3) and candidate positions of the rows that
     // found an intersection, for the EPA runs that follow.
@@ -1375,7 +1392,8 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
     uint32_t contacts3 = 0, contacts4 = 0, contacts5 = 0;
     uint32_t failures = 0, overflows = 0;
     uint32_t work = 0;  // EPA iterations of this wavefront so far
-    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+    uint32_t items_done = 0;
+    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x, ++items_done) {
         // Sphere-hull items are one candidate (all four rows on it) when
         // solo_sphere_hull is set; the other candidates go four to an item,
         // one per row, and run their EPAs in turn.
@@ -1482,6 +1500,13 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
                 if (slot < record_cap) {
                     write_handover(S, records + size_t(slot) * kEpaRecordWords, k, cls, a, b,
                                    stop);
+                    // Publish the record to the consumers (every lane's
+                    // writes first, then the flag with release semantics).
+                    __threadfence();
+                    wave_sync();
+                    if (lane == 0)
+                        __hip_atomic_store(&ready[slot], epoch, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_AGENT);
                 } else if (lane == 0) {
                     const uint32_t u = atomicAdd(&c->epa_unresolved, 1u);
                     if (u < record_cap)
@@ -1499,110 +1524,230 @@ void k_gjk_epa(Bodies B, World W, const uint64_t* __restrict__ pairs, Counters* 
         epa_partial_add(partials + size_t(kSlotContacts + 5) * kMaxPartialBlocks, contacts5);
         epa_partial_add(partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks, failures);
         epa_partial_add(partials + size_t(kSlotEpaOverflows) * kMaxPartialBlocks, overflows);
+        // Items finished (relaxed: k_epa_early only uses the count to stop
+        // looking; a record it misses is k_epa_continue's).
+        if (items_done)
+            __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(
+                                       partials + size_t(kSlotEpaItemsDone) * kMaxPartialBlocks +
+                                       (blockIdx.x % kEpaDoneSlots) * kEpaDoneStride),
+                                   (unsigned long long)items_done, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// EPA runs handed over by k_gjk_epa (one record each), continued over
-// distinct faces with multiplicities (epa_multi_block): one workgroup of
-// kContThreads threads per run, four faces per thread; workgroup g takes
-// records g, g + gridDim.x, ...  The exact list handed over becomes distinct
-// faces in first-copy order with their copy counts.  Results are stored like
-// k_gjk_epa's; undecided runs are listed as unresolved.
+// One handed-over run (record R), continued over distinct faces with
+// multiplicities (epa_multi_block) by the whole workgroup: the exact list
+// handed over becomes distinct faces in first-copy order with their copy
+// counts.  Results are stored like k_gjk_epa's; undecided runs are listed as
+// unresolved.
+__device__ void continue_record(ContShared& S, const Bodies& B, const World& W, Counters* c,
+                                const uint32_t* __restrict__ R, uint32_t record_cap,
+                                const physgpu_narrowphase_options& opt,
+                                physgpu_manifold* __restrict__ slots, uint32_t* __restrict__ hit,
+                                uint64_t* __restrict__ partials,
+                                uint64_t* __restrict__ unresolved)
+{
+    const uint32_t t = threadIdx.x;
+    const uint32_t k = R[kRecK], cls = R[kRecCls], a = R[kRecA], b = R[kRecB];
+    const uint32_t it0 = R[kRecIt], nf0 = R[kRecNf];
+    const Shape sa = load_shape(B, a), sb = load_shape(B, b);
+    const pose ta = load_pose(B, a), tb = load_pose(B, b);
+    const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
+    if (t < 4 + it0) {
+        S.vx[t] = __uint_as_float(R[kRecVx + t]);
+        S.vy[t] = __uint_as_float(R[kRecVy + t]);
+        S.vz[t] = __uint_as_float(R[kRecVz + t]);
+    }
+    if (t < nf0) {  // nf0 <= kMaxFaces = kContThreads
+        S.fabc[t] = R[kRecFabc + t];
+        S.mult[t] = 0;
+    }
+    __syncthreads();
+    // First copy of every face (a face's copies have equal a, b, c and hence
+    // equal normal and distance bits); copies counted on it.
+    bool lead = false;
+    if (t < nf0) {
+        const uint32_t abc = S.fabc[t];
+        uint32_t firstc = t;
+        for (uint32_t g = 0; g < t; ++g)
+            if (S.fabc[g] == abc) {
+                firstc = g;
+                break;
+            }
+        lead = firstc == t;
+        atomicAdd(&S.mult[firstc], 1u);
+    }
+    __syncthreads();
+    uint32_t abc = 0, m = 0;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f, d = 0.0f;
+    if (lead) {
+        abc = R[kRecFabc + t];
+        nx = __uint_as_float(R[kRecFnx + t]);
+        ny = __uint_as_float(R[kRecFny + t]);
+        nz = __uint_as_float(R[kRecFnz + t]);
+        d = __uint_as_float(R[kRecFd + t]);
+        m = S.mult[t];
+    }
+    uint32_t nf = 0;
+    const uint32_t pos = block_prefix<1>(S, lead ? 1u : 0u, nf, 0);
+    if (lead) {
+        S.fabc[pos] = abc;
+        S.fnx[pos] = nx;
+        S.fny[pos] = ny;
+        S.fnz[pos] = nz;
+        S.fd[pos] = d;
+        S.mult[pos] = m;
+        S.gen[pos] = 0;
+    }
+    __syncthreads();
+    f3 n;
+    float depth;
+    const int st = epa_multi_block(S, B, sa, ta, ha, sb, tb, hb, it0, nf, opt.epa_max_iterations,
+                                   opt.epa_tolerance, n, depth);
+    if (t < 64) {  // wavefront 0 stores the result
+        if (st == 1) {
+            const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
+                                   support_wave(B, sb, tb, neg(n), hb)),
+                               0.5f);
+            if (t == 0) {
+                Contact ct;
+                ct.normal = n;
+                ct.count = 1;
+                ct.depth[0] = depth;
+                set_anchors(ct, 0, w, ta, tb);
+                store_contact(ct, k, a, b, cls, W, B.n, slots);
+                hit[k] = 1;
+                atomicAdd(reinterpret_cast<unsigned long long*>(
+                              partials + size_t(kSlotContacts + cls) * kMaxPartialBlocks),
+                          1ull);
+            }
+        } else if (st == 0) {
+            if (t == 0)
+                atomicAdd(reinterpret_cast<unsigned long long*>(
+                              partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks),
+                          1ull);
+        } else if (t == 0) {
+            const uint32_t u = atomicAdd(&c->epa_unresolved, 1u);
+            if (u < record_cap)
+                unresolved[u] = (uint64_t(a) << 32) | b;
+        }
+    }
+    __syncthreads();
+}
+
+// After k_gjk_epa: every handed-over run that k_epa_early has not claimed
+// (workgroup g takes records g, g + gridDim.x, ...).  claim[record_cap] is
+// the word that tells k_epa_early this kernel has started.
 __global__ __launch_bounds__(kContThreads) void k_epa_continue(
     Bodies B, World W, Counters* c, const uint32_t* __restrict__ records, uint32_t record_cap,
     physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
-    uint32_t* __restrict__ hit, uint64_t* __restrict__ partials, uint64_t* __restrict__ unresolved)
+    uint32_t* __restrict__ hit, uint64_t* __restrict__ partials, uint64_t* __restrict__ unresolved,
+    uint32_t* __restrict__ claim, const uint32_t* __restrict__ epa_epoch)
 {
     __shared__ ContShared S;
-    const uint32_t t = threadIdx.x;
+    __shared__ uint32_t won;
+    const uint32_t epoch = *epa_epoch;
+    // k_gjk_epa has finished: k_epa_early stops taking runs.
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(&claim[record_cap], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t count = min(c->epa_continued, record_cap);
     for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
-        const uint32_t* R = records + size_t(r) * kEpaRecordWords;
-        const uint32_t k = R[kRecK], cls = R[kRecCls], a = R[kRecA], b = R[kRecB];
-        const uint32_t it0 = R[kRecIt], nf0 = R[kRecNf];
-        const Shape sa = load_shape(B, a), sb = load_shape(B, b);
-        const pose ta = load_pose(B, a), tb = load_pose(B, b);
-        const WaveHull ha = wave_hull(B, sa, ta), hb = wave_hull(B, sb, tb);
-        if (t < 4 + it0) {
-            S.vx[t] = __uint_as_float(R[kRecVx + t]);
-            S.vy[t] = __uint_as_float(R[kRecVy + t]);
-            S.vz[t] = __uint_as_float(R[kRecVz + t]);
-        }
-        if (t < nf0) {  // nf0 <= kMaxFaces = kContThreads
-            S.fabc[t] = R[kRecFabc + t];
-            S.mult[t] = 0;
+        if (threadIdx.x == 0)
+            won = atomicExch(&claim[r], epoch) != epoch ? 1u : 0u;
+        __syncthreads();
+        if (won)
+            continue_record(S, B, W, c, records + size_t(r) * kEpaRecordWords, record_cap, opt,
+                            slots, hit, partials, unresolved);
+        __syncthreads();
+    }
+}
+
+// While k_gjk_epa still runs (on another stream): take handed-over runs as
+// they are published (ready flag of this epoch, claim word).  Stops when
+// every GJK + EPA work item has finished and no published run is left to
+// take, when k_epa_continue has started (it takes every run not taken here),
+// when the wait passes kEpaEarlyPollTicks, or when k_gjk_epa has not started
+// within kEpaEarlyStartTicks (the two streams then run one after the other,
+// this one first: waiting longer would only delay k_gjk_epa).  The result
+// never depends on which kernel takes a run, and every wait is bounded.
+__global__ __launch_bounds__(kContThreads) void k_epa_early(
+    Bodies B, World W, Counters* c, const uint32_t* __restrict__ records, uint32_t record_cap,
+    physgpu_narrowphase_options opt, physgpu_manifold* __restrict__ slots,
+    uint32_t* __restrict__ hit, uint64_t* __restrict__ partials, uint64_t* __restrict__ unresolved,
+    uint32_t* __restrict__ flags, const uint32_t* __restrict__ epa_epoch, bool solo_sphere_hull)
+{
+    __shared__ ContShared S;
+    __shared__ uint32_t cand, take, stop;
+    __shared__ unsigned long long finished;
+    const uint32_t t = threadIdx.x;
+    const uint32_t epoch = *epa_epoch;
+    uint32_t* ready = flags;
+    uint32_t* claim = flags + record_cap;
+    // Work items of k_gjk_epa (its own count).
+    const uint32_t c3 = c->class_count[3], c4 = c->class_count[4];
+    const uint32_t total = c3 + c4 + c->class_count[5];
+    const uint32_t solo_items = solo_sphere_hull ? c3 : 0;
+    const uint64_t items = solo_items + (total - solo_items + kRowsPerWave - 1) / kRowsPerWave;
+    unsigned long long* done = reinterpret_cast<unsigned long long*>(
+        partials + size_t(kSlotEpaItemsDone) * kMaxPartialBlocks);
+    const uint64_t start = __builtin_amdgcn_s_memrealtime();
+    for (;;) {
+        // Thread 0 decides for the workgroup: 1 = k_epa_continue has started,
+        // 2 = the wait is over.
+        if (t == 0) {
+            cand = kNoIndex;
+            finished = 0;
+            const uint64_t waited = __builtin_amdgcn_s_memrealtime() - start;
+            const bool gjk_started = __hip_atomic_load(&ready[2 * record_cap + 1],
+                                                       __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT) == epoch;
+            stop = __hip_atomic_load(&claim[record_cap], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) == epoch
+                       ? 1u
+                       : (waited > kEpaEarlyPollTicks ||
+                                  (!gjk_started && waited > kEpaEarlyStartTicks)
+                              ? 2u
+                              : 0u);
         }
         __syncthreads();
-        // First copy of every face (a face's copies have equal a, b, c and
-        // hence equal normal and distance bits); copies counted on it.
-        bool lead = false;
-        if (t < nf0) {
-            const uint32_t abc = S.fabc[t];
-            uint32_t firstc = t;
-            for (uint32_t g = 0; g < t; ++g)
-                if (S.fabc[g] == abc) {
-                    firstc = g;
+        const uint32_t stopping = stop;
+        if (t < kEpaDoneSlots)
+            atomicAdd(&finished, __hip_atomic_load(&done[t * kEpaDoneStride], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT));
+        const uint32_t n =
+            min(__hip_atomic_load(&c->epa_continued, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                record_cap);
+        if (!stopping)
+            for (uint32_t r = t; r < n; r += kContThreads)
+                if (__hip_atomic_load(&ready[r], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+                        epoch &&
+                    __hip_atomic_load(&claim[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                        epoch) {
+                    atomicMin(&cand, r);
                     break;
                 }
-            lead = firstc == t;
-            atomicAdd(&S.mult[firstc], 1u);
-        }
         __syncthreads();
-        uint32_t abc = 0, m = 0;
-        float nx = 0.0f, ny = 0.0f, nz = 0.0f, d = 0.0f;
-        if (lead) {
-            abc = R[kRecFabc + t];
-            nx = __uint_as_float(R[kRecFnx + t]);
-            ny = __uint_as_float(R[kRecFny + t]);
-            nz = __uint_as_float(R[kRecFnz + t]);
-            d = __uint_as_float(R[kRecFd + t]);
-            m = S.mult[t];
-        }
-        uint32_t nf = 0;
-        const uint32_t pos = block_prefix<1>(S, lead ? 1u : 0u, nf, 0);
-        if (lead) {
-            S.fabc[pos] = abc;
-            S.fnx[pos] = nx;
-            S.fny[pos] = ny;
-            S.fnz[pos] = nz;
-            S.fd[pos] = d;
-            S.mult[pos] = m;
-            S.gen[pos] = 0;
-        }
+        if (t == 0)
+            take = cand != kNoIndex && atomicExch(&claim[cand], epoch) != epoch ? cand : kNoIndex;
         __syncthreads();
-        f3 n;
-        float depth;
-        const int st = epa_multi_block(S, B, sa, ta, ha, sb, tb, hb, it0, nf,
-                                       opt.epa_max_iterations, opt.epa_tolerance, n, depth);
-        if (t < 64) {  // wavefront 0 stores the result
-            if (st == 1) {
-                const f3 w = scale(add(support_wave(B, sa, ta, n, ha),
-                                       support_wave(B, sb, tb, neg(n), hb)),
-                                   0.5f);
-                if (t == 0) {
-                    Contact ct;
-                    ct.normal = n;
-                    ct.count = 1;
-                    ct.depth[0] = depth;
-                    set_anchors(ct, 0, w, ta, tb);
-                    store_contact(ct, k, a, b, cls, W, B.n, slots);
-                    hit[k] = 1;
-                    atomicAdd(reinterpret_cast<unsigned long long*>(
-                                  partials + size_t(kSlotContacts + cls) * kMaxPartialBlocks),
-                              1ull);
-                }
-            } else if (st == 0) {
-                if (t == 0)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(
-                                  partials + size_t(kSlotGjkFailures) * kMaxPartialBlocks),
-                              1ull);
-            } else if (t == 0) {
-                const uint32_t u = atomicAdd(&c->epa_unresolved, 1u);
-                if (u < record_cap)
-                    unresolved[u] = (uint64_t(a) << 32) | b;
-            }
+        const uint32_t got = take, seen = cand;
+        const bool all_items = finished >= items;
+        __syncthreads();  // every thread has read them before thread 0 resets them
+        if (got != kNoIndex) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_s_setprio(3);
+            continue_record(S, B, W, c, records + size_t(got) * kEpaRecordWords, record_cap, opt,
+                            slots, hit, partials, unresolved);
+            __builtin_amdgcn_s_setprio(0);
+            continue;
         }
-        __syncthreads();
+        if (stopping)
+            break;
+        if (seen != kNoIndex)
+            continue;  // another workgroup claimed it first: look again
+        if (all_items)
+            break;
+        __builtin_amdgcn_s_sleep(127);
     }
 }
 
@@ -1984,7 +2129,8 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
     time_begin(ctx, "np_classify", t);
     hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0, ctx->stream, B,
                        W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit,
-                       ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks);
+                       ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks,
+                       ctx->epa_epoch);
     time_end(ctx, t);
     // With hulls, GJK + EPA run here while the SS / SB / BB kernels run on the
     // side stream.
@@ -1994,19 +2140,33 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
         hipEventRecord(ctx->fork_event, ctx->stream);
         hipStreamWaitEvent(ctx->side_stream, ctx->fork_event, 0);
         cs = ctx->side_stream;
+        // Runs handed over past the LDS bounds are continued while GJK + EPA
+        // still runs, by a consumer on a third stream, submitted first so its
+        // workgroups hold CUs before GJK + EPA fills them.
+        hipStreamWaitEvent(ctx->early_stream, ctx->fork_event, 0);
+        time_begin(ctx, "np_epa_early", t, ctx->early_stream);
+        hipLaunchKernelGGL(k_epa_early, dim3(std::min(ctx->epa_record_cap, kEpaEarlyGroups)),
+                           dim3(kContThreads), 0, ctx->early_stream, B, W, ctx->counters,
+                           ctx->epa_records, ctx->epa_record_cap, opt, ctx->slots, ctx->hit,
+                           ctx->partials, ctx->epa_unresolved_pairs, ctx->epa_flags,
+                           ctx->epa_epoch, tu.sphere_hull_per_wave == 1);
+        time_end(ctx, t);
+        hipEventRecord(ctx->early_join_event, ctx->early_stream);
         time_begin(ctx, "np_gjk_epa", t);
         hipLaunchKernelGGL(k_gjk_epa, dim3(tu.epa_groups), dim3(kEpaBlock), 0, ctx->stream, B,
                            W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
                            ctx->hit, ctx->partials, tu.sphere_hull_per_wave == 1,
-                           ctx->epa_records, ctx->epa_record_cap, ctx->epa_unresolved_pairs);
+                           ctx->epa_records, ctx->epa_record_cap, ctx->epa_unresolved_pairs,
+                           ctx->epa_flags, ctx->epa_epoch);
         time_end(ctx, t);
-        // Runs past the LDS polytope bounds (none on most steps: every
-        // workgroup then exits at once).
+        // Runs past the LDS polytope bounds that k_epa_early (side stream)
+        // has not taken (none on most steps: every workgroup exits at once).
         time_begin(ctx, "np_epa_continue", t);
         hipLaunchKernelGGL(k_epa_continue, dim3(std::min(ctx->epa_record_cap, kEpaContinueGroups)),
                            dim3(kContThreads), 0, ctx->stream, B, W, ctx->counters, ctx->epa_records,
                            ctx->epa_record_cap, opt, ctx->slots, ctx->hit, ctx->partials,
-                           ctx->epa_unresolved_pairs);
+                           ctx->epa_unresolved_pairs, ctx->epa_flags + ctx->epa_record_cap,
+                           ctx->epa_epoch);
         time_end(ctx, t);
     }
     time_begin(ctx, "np_sphere_sphere", t, cs);
@@ -2024,10 +2184,13 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
                        W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
                        ctx->hit, ctx->partials + size_t(kSlotContacts + 2) * kMaxPartialBlocks);
     time_end(ctx, t);
+
     if (fork || sort_on_side) {
         hipEventRecord(ctx->join_event, ctx->side_stream);
         hipStreamWaitEvent(ctx->stream, ctx->join_event, 0);
     }
+    if (fork)
+        hipStreamWaitEvent(ctx->stream, ctx->early_join_event, 0);
     // Contact and failure counts from the per-workgroup partials.
     FoldList fold{};
     for (uint32_t q = 0; q < 3; ++q)
