@@ -478,6 +478,34 @@ physgpu_status physgpu_tree_find_pairs(physgpu_context* ctx, int policy, physgpu
                                        uint64_t capacity, uint64_t* count,
                                        physgpu_broadphase_stats* stats);
 
+/* ---- Allocation-free broad phase -------------------------------------
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  physgpu_find_candidate_pairs over device arrays,
+ * with no context: the working memory is one fixed region per device inside
+ * the library (room for PHYSGPU_NOALLOC_MAX_BODIES bodies and
+ * PHYSGPU_NOALLOC_MAX_PAIRS pairs, about 18 MiB, held on every device the
+ * call runs on until the process ends).  Nothing is allocated and nothing
+ * synchronises: the work is enqueued on `hip_stream` (a hipStream_t of the
+ * current device; NULL = the null stream).
+ *   bounds    n device records, finite and ordered (not checked here)
+ *   out       device, receives min(count, capacity) pairs in the canonical
+ *             order (may be NULL when capacity is 0)
+ *   count     device uint64: the full pair count; pairs past
+ *             PHYSGPU_NOALLOC_MAX_PAIRS or capacity are not written; ~0 when
+ *             the LBVH traversal overflowed (never for valid bounds)
+ * Returns PHYSGPU_INVALID_ARGUMENT for n above PHYSGPU_NOALLOC_MAX_BODIES,
+ * an unknown algorithm or missing arrays.  All calls on one device share the
+ * region and it keeps the last call's contents: calls that overlap in time
+ * on one device (other streams, other host threads) race and must be
+ * serialised by the caller. */
+#define PHYSGPU_NOALLOC_MAX_BODIES 65536
+#define PHYSGPU_NOALLOC_MAX_PAIRS 131072
+physgpu_status physgpu_find_candidate_pairs_noalloc(const physgpu_aabb* bounds, uint32_t n,
+                                                    int algorithm, double grid_cell_size,
+                                                    physgpu_pair* out, uint64_t capacity,
+                                                    uint64_t* count, void* hip_stream);
+
 /* Average device time (ms) per launch of each kernel since timing was
  * enabled.  Writes up to `capacity` entries; names are static strings. */
 physgpu_status physgpu_kernel_times(physgpu_context* ctx, const char** names, double* avg_ms,

@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -1655,6 +1657,189 @@ physgpu_status physgpu_kernel_times(physgpu_context* ctx, const char** names, do
         if (launches)
             launches[i] = k.launches;
     }
+    return PHYSGPU_OK;
+}
+
+}  // extern "C"
+
+// ---- Allocation-free broad phase (physgpu_find_candidate_pairs_noalloc) ----
+// The working memory of one broad phase of at most PHYSGPU_NOALLOC_MAX_BODIES
+// bodies and PHYSGPU_NOALLOC_MAX_PAIRS pairs, carved from one fixed array per
+// device (the library holds it on every device it runs on until the process
+// ends).  Host code never passes `scratch` itself (a host-side stand-in):
+// it uses the per-device address from hipGetSymbolAddress.
+
+namespace pg {
+namespace {
+
+constexpr uint32_t kNoAllocBodies = PHYSGPU_NOALLOC_MAX_BODIES;
+constexpr uint64_t kNoAllocPairs = PHYSGPU_NOALLOC_MAX_PAIRS;
+constexpr size_t kScratchAlign = 256;
+
+uint32_t noalloc_table_bits()
+{
+    uint32_t k = 2;
+    while (k < 9 && (1ull << (3 * k)) < 2ull * kNoAllocBodies)
+        ++k;
+    return 3 * k;
+}
+
+// Assigns the broad-phase buffers of `c` (sized for the maximum) from `base`
+// in the order of physgpu_create; returns the bytes used.  With base == NULL
+// only the size is computed.
+size_t carve_noalloc(physgpu_context& c, char* base)
+{
+    size_t off = 0;
+    auto take = [&](auto*& p, size_t count) {
+        using T = std::remove_reference_t<decltype(*p)>;
+        off = (off + kScratchAlign - 1) & ~(kScratchAlign - 1);
+        p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += std::max<size_t>(count, 1) * sizeof(T);
+    };
+    const size_t N = kNoAllocBodies, P = kNoAllocPairs;
+    const size_t tiles = std::max<size_t>(div_up(N, kSortMinTile), div_up(P, kSortMinTile));
+    take(c.aabb, 2 * N);
+    take(c.sorted_aabb, 2 * N);
+    take(c.items_a, N);
+    take(c.items_b, N);
+    take(c.histogram, 256 * tiles);
+    take(c.digit_total, 256);
+    take(c.cell_table, 2 * (size_t(1) << noalloc_table_bits()));
+    take(c.overflow_list, N);
+    take(c.grid_flag, N);
+    take(c.node_left, N);
+    take(c.node_right, N);
+    take(c.node_parent, 2 * N);
+    take(c.node_last, N);
+    take(c.node_box, 2 * N);
+    take(c.node_first, N);
+    take(c.level_box, 2 * (N / 32 + N / 1024 + N / 32768 + 3));
+    take(c.pairs_raw, P);
+    take(c.pairs_tmp, P);
+    take(c.seg_off, N + 1);
+    take(c.seg_tiles, div_up(N + 1, 2048));
+    take(c.pair_rank, P);
+    take(c.pair_perm, P);
+    take(c.sort_scratch, P);
+    take(c.sort_scratch_perm, P);
+    take(c.seg_lists, 2 * N);
+    take(c.partials, size_t(kPartialSlots) * kMaxPartialBlocks);
+    take(c.counters, 1);
+    c.hist_capacity = uint32_t(256 * tiles);
+    return off;
+}
+
+// Upper bound of carve_noalloc's size (each buffer plus its alignment).
+constexpr size_t noalloc_bytes()
+{
+    const size_t N = kNoAllocBodies, P = kNoAllocPairs, a = kScratchAlign;
+    const size_t tiles = (std::max(N, size_t(P)) + kSortMinTile - 1) / kSortMinTile;
+    size_t k = 2;
+    while (k < 9 && (size_t(1) << (3 * k)) < 2 * N)
+        ++k;
+    return 27 * a + 2 * N * 16 * 2 + N * 8 * 2 + 256 * tiles * 4 + 256 * 4 +
+           2 * (size_t(1) << (3 * k)) * 4 + N * 4 * 2 + N * 4 * 2 + 2 * N * 4 + N * 4 +
+           2 * N * 16 + N * 4 + 2 * (N / 32 + N / 1024 + N / 32768 + 3) * 16 + P * 8 * 2 +
+           (N + 1) * 4 + ((N + 1 + 2047) / 2048) * 4 + P * 4 * 2 + P * 8 + P * 4 + 2 * N * 4 +
+           size_t(kPartialSlots) * kMaxPartialBlocks * 8 + sizeof(Counters);
+}
+
+}  // namespace
+}  // namespace pg
+
+// The fixed working memory (one copy per device).
+static __device__ __align__(256) unsigned char scratch[pg::noalloc_bytes()];
+
+namespace pg {
+namespace {
+
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxNoAllocDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current).
+physgpu_status scratch_on(int device, void** out)
+{
+    if (device < 0 || device >= kMaxNoAllocDevices)
+        return fail(PHYSGPU_UNSUPPORTED, "device index beyond the allocation-free table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return check_hip(e, "hipGetSymbolAddress(scratch)");
+        if (!p)
+            return fail(PHYSGPU_DEVICE_ERROR, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return PHYSGPU_OK;
+}
+
+// Canonical pairs (first << 32 | second) to physgpu_pair records, at most
+// `capacity`, and the full pair count (~0 when the LBVH traversal stack
+// overflowed).
+__global__ __launch_bounds__(kBlock) void k_noalloc_out(const uint64_t* __restrict__ pairs,
+                                                        const Counters* __restrict__ c,
+                                                        physgpu_pair* __restrict__ out,
+                                                        uint64_t capacity,
+                                                        uint64_t* __restrict__ count)
+{
+    const uint64_t total = c->pair_count;
+    const uint64_t held = min(min(total, uint64_t(c->sort_count)), capacity);
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < held; i += stride) {
+        const uint64_t p = pairs[i];
+        out[i] = physgpu_pair{uint32_t(p >> 32), uint32_t(p)};
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *count = c->tree_overflow ? ~0ull : total;
+}
+
+}  // namespace
+}  // namespace pg
+
+extern "C" {
+
+physgpu_status physgpu_find_candidate_pairs_noalloc(const physgpu_aabb* bounds, uint32_t n,
+                                                    int algorithm, double grid_cell_size,
+                                                    physgpu_pair* out, uint64_t capacity,
+                                                    uint64_t* count, void* hip_stream)
+{
+    if (!count || (n && !bounds) || (capacity && !out))
+        return fail(PHYSGPU_INVALID_ARGUMENT, "count is NULL, or bounds / out missing");
+    if (algorithm < PHYSGPU_SWEEP_AND_PRUNE || algorithm > PHYSGPU_DYNAMIC_TREE)
+        return fail(PHYSGPU_INVALID_ARGUMENT, "unknown broadphase algorithm");
+    if (n > kNoAllocBodies)
+        return fail(PHYSGPU_INVALID_ARGUMENT,
+                    "n = " + std::to_string(n) + " is above PHYSGPU_NOALLOC_MAX_BODIES (" +
+                        std::to_string(kNoAllocBodies) + ")");
+    int device = 0;
+    PG_TRY_HIP(hipGetDevice(&device));
+    void* base = nullptr;
+    physgpu_status st = scratch_on(device, &base);
+    if (st != PHYSGPU_OK)
+        return st;
+    // A context on the stack over the fixed memory (nothing is allocated).
+    physgpu_context c;
+    c.device = device;
+    c.tuning = default_tuning();
+    c.stream = static_cast<hipStream_t>(hip_stream);
+    c.cap_n = kNoAllocBodies;
+    c.cap_p = kNoAllocPairs;
+    c.table_bits = noalloc_table_bits();
+    if (carve_noalloc(c, static_cast<char*>(base)) > sizeof(scratch))
+        return fail(PHYSGPU_DEVICE_ERROR, "allocation-free layout exceeds its fixed memory");
+    c.n = n;
+    c.pairs = c.pairs_raw;
+    launch_step_begin(&c);
+    if (n)
+        launch_convert_aabbs_in(&c, bounds);
+    run_broadphase(&c, algorithm, grid_cell_size, false);
+    hipLaunchKernelGGL(k_noalloc_out, dim3(256), dim3(kBlock), 0, c.stream, c.pairs, c.counters, out,
+                       capacity, count);
+    PG_TRY_HIP(hipGetLastError());
     return PHYSGPU_OK;
 }
 
