@@ -238,10 +238,11 @@ __device__ __forceinline__ f3 lds_vertex(const Shared& S, uint32_t i)
     return {S.vx[i], S.vy[i], S.vz[i]};
 }
 
-// makeFace, epa.cpp:29-47, over the LDS vertices; writes face slot f.
+// makeFace, epa.cpp:29-47, over the LDS vertices; writes face slot f and
+// returns its distance.
 template <class Shared>
-__device__ __forceinline__ void make_face_lds(Shared& S, uint32_t f, uint32_t a, uint32_t b,
-                                              uint32_t c)
+__device__ __forceinline__ float make_face_lds(Shared& S, uint32_t f, uint32_t a, uint32_t b,
+                                               uint32_t c)
 {
     const f3 va = lds_vertex(S, a);
     f3 n = cross(sub(lds_vertex(S, b), va), sub(lds_vertex(S, c), va));
@@ -261,6 +262,7 @@ __device__ __forceinline__ void make_face_lds(Shared& S, uint32_t f, uint32_t a,
     S.fny[f] = n.y;
     S.fnz[f] = n.z;
     S.fd[f] = d;
+    return d;
 }
 
 // A hull's points (at most 64) or a box's 8 corners held one per lane for the
@@ -743,13 +745,18 @@ struct ContShared {
     alignas(16) float fd[kContFaces];
     alignas(16) uint32_t mult[kContFaces];
     alignas(16) uint8_t gen[kContFaces];  // 0: the handed-over faces; 1, 2, ...: later iterations
-    uint32_t vis_abc[kContFaces];  // visible faces in order (first copies while staging)
-    unsigned long long sum[kEdgeKeys];  // signed running sums per unordered vertex pair
-    uint32_t first[kEdgeKeys];          // first call index per unordered vertex pair
-    uint32_t wsum[2][kContWaves];       // per-wavefront partials of the block scans (2 buffers)
+    // Signed running sum and first call index per unordered vertex pair:
+    // zero / kNoIndex between iterations (each iteration's first calls reset
+    // the pairs that iteration used).
+    unsigned long long sum[kEdgeKeys];
+    uint32_t first[kEdgeKeys];
+    uint32_t vis_abc[kContFaces];  // visible faces in order
+    uint32_t wsum[2][kContWaves];  // per-wavefront partials of the block scans (2 buffers)
+    // Per wavefront: first minimum of the face distances its threads wrote
+    // last, and flags (bit 0 a NaN distance, bit 1 past the storage bounds).
     float wbest_v[kContWaves];
     uint32_t wbest_i[kContWaves];
-    uint32_t wnan[kContWaves];
+    uint32_t wflags[kContWaves];
 };
 
 // Edge e (0, 1, 2 = (a,b), (b,c), (c,a)) of a packed face.
@@ -760,22 +767,15 @@ __device__ __forceinline__ void face_edge(uint32_t abc, uint32_t e, uint32_t& x,
     y = e == 0 ? v1 : (e == 1 ? v2 : v0);
 }
 
-__device__ __forceinline__ void horizon_call(ContShared& S, uint32_t x, uint32_t y, uint32_t m,
-                                             uint32_t call)
-{
-    const uint32_t key = edge_key(x, y);
-    const long long delta = x < y ? (long long)m : -(long long)m;
-    atomicAdd(&S.sum[key], (unsigned long long)delta);
-    atomicMin(&S.first[key], call);
-}
-
 // Exclusive prefix sum, in thread order, of per-thread counts below 2^Bits
-// over the workgroup, and its total.  Every thread calls it; one barrier.
-// Consecutive calls alternate between the two partial buffers (`buf`), and
-// any two calls on one buffer are separated by a barrier of the call between.
+// over the workgroup, and its total; `any` (optional) receives the OR of a
+// per-thread flag.  Every thread calls it; one barrier.  Consecutive calls
+// alternate between the two partial buffers (`buf`), and any two calls on
+// one buffer are separated by a barrier of the call between.
 template <uint32_t Bits>
 __device__ __forceinline__ uint32_t block_prefix(ContShared& S, uint32_t c, uint32_t& total,
-                                                 uint32_t buf)
+                                                 uint32_t buf, bool flag = false,
+                                                 bool* any = nullptr)
 {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t below = lanes_below(lane);
@@ -786,76 +786,82 @@ __device__ __forceinline__ uint32_t block_prefix(ContShared& S, uint32_t c, uint
         in_wave += (uint32_t)__popcll(m & below) << b;
         wave_total += (uint32_t)__popcll(m) << b;
     }
+    const bool wave_flag = __ballot(flag) != 0;
     if (lane == 0)
-        S.wsum[buf][w] = wave_total;
+        S.wsum[buf][w] = wave_total | (wave_flag ? 0x80000000u : 0u);
     __syncthreads();
-    uint32_t off = 0, tot = 0;
+    uint32_t off = 0, tot = 0, flags = 0;
 #pragma unroll
     for (uint32_t q = 0; q < kContWaves; ++q) {
         const uint32_t x = S.wsum[buf][q];
-        off += q < w ? x : 0u;
-        tot += x;
+        flags |= x;
+        off += q < w ? x & 0x7FFFFFFFu : 0u;
+        tot += x & 0x7FFFFFFFu;
     }
     total = tot;
+    if (any)
+        *any = (flags & 0x80000000u) != 0;
     return off + in_wave;
 }
 
-// Closest face: the first minimum of S.fd[0, nf) in index order, as the
-// reference's sequential scan (std::min_element) picks it.  Every thread
-// calls it and gets the index (one barrier: each wavefront combines the
-// wavefronts' results itself).
-__device__ uint32_t block_closest(ContShared& S, uint32_t nf)
+// Publishes this wavefront's first minimum (v, i) of the face distances its
+// threads wrote (i == kNoIndex: none) and its NaN / storage-bound flags; the
+// caller's next barrier makes them visible to closest_face.
+__device__ __forceinline__ void publish_best(ContShared& S, float v, uint32_t i, bool nan,
+                                             bool bound)
 {
-    const uint32_t t = threadIdx.x, lane = lane_id(), w = t >> 6;
-    const float4 d4 = reinterpret_cast<const float4*>(S.fd)[t];
-    const float d[4] = {d4.x, d4.y, d4.z, d4.w};
-    const uint32_t f0 = kFacesPerThread * t;
-    float v = 0.0f;
-    uint32_t i = kNoIndex;
-    bool nan = false;
-#pragma unroll
-    for (uint32_t j = 0; j < kFacesPerThread; ++j) {
-        if (f0 + j < nf) {
-            nan = nan || d[j] != d[j];
-            if (i == kNoIndex || d[j] < v) {
-                v = d[j];
-                i = f0 + j;
-            }
-        }
-    }
-    const bool wave_nan = __ballot(nan) != 0;
-    wave_arg_best<false>(v, i);
-    if (lane == 0) {
+    const uint32_t w = threadIdx.x >> 6;
+    const bool wave_nan = __ballot(nan) != 0, wave_bound = __ballot(bound) != 0;
+    wave_min_first(v, i);
+    if (lane_id() == 0) {
         S.wbest_v[w] = v;
         S.wbest_i[w] = i;
-        S.wnan[w] = wave_nan ? 1u : 0u;
+        S.wflags[w] = (wave_nan ? 1u : 0u) | (wave_bound ? 2u : 0u);
     }
-    __syncthreads();
-    bool any_nan = false;
+}
+
+__device__ __forceinline__ bool published_bound(const ContShared& S)
+{
+    uint32_t flags = 0;
 #pragma unroll
     for (uint32_t q = 0; q < kContWaves; ++q)
-        any_nan = any_nan || S.wnan[q] != 0u;
-    if (any_nan) {
-        // A NaN distance: the sequential scan itself (strict <), by every
-        // thread alike.
+        flags |= S.wflags[q];
+    return (flags & 2u) != 0;
+}
+
+// Closest face from the published minima: the first minimum of S.fd[0, nf)
+// in index order, as the reference's sequential scan (std::min_element)
+// picks it; with a NaN distance among the faces, that scan itself.  Every
+// thread gets the index.
+__device__ __forceinline__ uint32_t closest_face(const ContShared& S, uint32_t nf)
+{
+    float bv = 0.0f;
+    uint32_t bi = kNoIndex, flags = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kContWaves; ++q) {
+        flags |= S.wflags[q];
+        arg_take<false>(bv, bi, S.wbest_v[q], S.wbest_i[q]);
+    }
+    if (flags & 1u) {
         uint32_t best = 0;
         for (uint32_t f = 1; f < nf; ++f)
             if (S.fd[f] < S.fd[best])
                 best = f;
-        return best;
+        bi = best;
     }
-    float bv = lane < kContWaves ? S.wbest_v[lane] : 0.0f;
-    uint32_t bi = lane < kContWaves ? S.wbest_i[lane] : kNoIndex;
-    wave_arg_best<false>(bv, bi);
     return bi;
 }
 
 // Iterations [first, iterations) over the nf distinct faces in S (first-copy
 // order, multiplicities in S.mult, generation 0) and the vertices up to
-// 4 + first.  Returns 1 converged, 0 the reference's failure, -1 storage
-// bound, -2 undecided closest face; normal / depth uniform across the block.
-// Every thread of the workgroup calls it (ha, hb hold both shapes' points per
-// lane).
+// 4 + first, with the pair sums cleared and the closest-face minima of those
+// faces published (publish_best, then a barrier).  Returns 1 converged, 0 the
+// reference's failure, -1 storage bound, -2 undecided closest face; normal /
+// depth uniform across the block.  Every thread of the workgroup calls it
+// (ha, hb hold both shapes' points per lane).  Barriers per iteration: the
+// scan of the visible faces (which also ORs the undecided test), the end of
+// the horizon, one scan per kContThreads horizon calls, and the end of the
+// writes (which also publishes the next closest face).
 __device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, const pose& ta,
                                const WaveHull& ha, const Shape& sb, const pose& tb,
                                const WaveHull& hb, uint32_t first, uint32_t nf,
@@ -870,27 +876,12 @@ __device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, 
     for (uint32_t it = first; it < iterations; ++it) {
         if (nf == 0)
             return 0;
-        const uint32_t bi = block_closest(S, nf);
+        const uint32_t bi = closest_face(S, nf);
         const f3 cn{S.fnx[bi], S.fny[bi], S.fnz[bi]};
         const float cd = S.fd[bi];
         const uint32_t g0 = S.gen[bi];
         const uint32_t si = nv;
-        // Every wavefront evaluates the support maps (same bits), so none
-        // waits for another's.
-        const f3 sup = sub(support_wave(B, sa, ta, cn, ha), support_wave(B, sb, tb, neg(cn), hb));
-        if (t == 0) {
-            S.vx[si] = sup.x;
-            S.vy[si] = sup.y;
-            S.vz[si] = sup.z;
-        }
-        ++nv;
-        for (uint32_t q = t; q < kEdgeKeys; q += kContThreads) {
-            S.sum[q] = 0ull;
-            S.first[q] = kNoIndex;
-        }
-        // This thread's faces (one 16-byte read per array), read before
-        // anything moves, and whether one of them ties the closest face
-        // within its iteration (undecided).
+        // This thread's faces (one 16-byte read per array).
         const uint4 abc4 = reinterpret_cast<const uint4*>(S.fabc)[t];
         const float4 nx4 = reinterpret_cast<const float4*>(S.fnx)[t];
         const float4 ny4 = reinterpret_cast<const float4*>(S.fny)[t];
@@ -906,46 +897,77 @@ __device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, 
         const float ny[kFacesPerThread] = {ny4.x, ny4.y, ny4.z, ny4.w};
         const float nz[kFacesPerThread] = {nz4.x, nz4.y, nz4.z, nz4.w};
         const float d[kFacesPerThread] = {d4.x, d4.y, d4.z, d4.w};
+        // Every wavefront evaluates the support maps (same bits), so none
+        // waits for another's.
+        const f3 sup = sub(support_wave(B, sa, ta, cn, ha), support_wave(B, sb, tb, neg(cn), hb));
+        if (t == 0) {
+            S.vx[si] = sup.x;
+            S.vy[si] = sup.y;
+            S.vz[si] = sup.z;
+        }
+        ++nv;
+        // Whether one of this thread's faces ties the closest face within its
+        // iteration (undecided), and which are visible.
         bool tie = false;
+        bool vis[kFacesPerThread];
+        uint32_t nvis = 0;
 #pragma unroll
         for (uint32_t j = 0; j < kFacesPerThread; ++j) {
             const uint32_t f = f0 + j;
             tie = tie || (g0 != 0 && f < nf && f > bi && g[j] == g0 && d[j] == cd &&
                           (nx[j] != cn.x || ny[j] != cn.y || nz[j] != cn.z));
+            vis[j] = f < nf &&
+                     dot(f3{nx[j], ny[j], nz[j]}, sub(sup, lds_vertex(S, abc[j] & 0xFFu))) > tol;
+            nvis += vis[j] ? 1u : 0u;
         }
-        if (__syncthreads_or(tie))
+        const bool converged = dot(sup, cn) - cd <= tol;
+        // The pair sums do not depend on the order of the calls: they go in
+        // before the scan that orders the visible faces.
+        if (!converged) {
+#pragma unroll
+            for (uint32_t j = 0; j < kFacesPerThread; ++j)
+                if (vis[j])
+#pragma unroll
+                    for (uint32_t e = 0; e < 3; ++e) {
+                        uint32_t x, y;
+                        face_edge(abc[j], e, x, y);
+                        const long long delta = x < y ? (long long)m[j] : -(long long)m[j];
+                        atomicAdd(&S.sum[edge_key(x, y)], (unsigned long long)delta);
+                    }
+        }
+        uint32_t visible = 0;
+        bool undecided = false;
+        const uint32_t before = block_prefix<3>(S, nvis, visible, buf, tie, &undecided);
+        buf ^= 1u;
+        if (undecided)
             return -2;
-        if (dot(sup, cn) - cd <= tol) {
+        if (converged) {
             normal = cn;
             depth = max_ref(cd, 0.0f);
             return 1;
         }
-        bool vis[kFacesPerThread];
-        uint32_t nvis = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < kFacesPerThread; ++j) {
-            vis[j] = f0 + j < nf &&
-                     dot(f3{nx[j], ny[j], nz[j]}, sub(sup, lds_vertex(S, abc[j] & 0xFFu))) > tol;
-            nvis += vis[j] ? 1u : 0u;
-        }
-        uint32_t visible = 0;
-        uint32_t before = block_prefix<3>(S, nvis, visible, buf);
-        buf ^= 1u;
+        // First call of every pair (call 3 r + e is edge e of visible face
+        // r; this thread's calls are 3 before ... 3 (before + nvis) - 1, in
+        // order); kept faces move down in order, their first minimum taken
+        // on the way.
+        float bv = 0.0f;
+        uint32_t bidx = kNoIndex;
+        bool nan = false;
+        uint32_t r = 0;
 #pragma unroll
         for (uint32_t j = 0; j < kFacesPerThread; ++j) {
             const uint32_t f = f0 + j;
             if (vis[j]) {
-                S.vis_abc[before] = abc[j];
+                S.vis_abc[before + r] = abc[j];
 #pragma unroll
                 for (uint32_t e = 0; e < 3; ++e) {
                     uint32_t x, y;
                     face_edge(abc[j], e, x, y);
-                    horizon_call(S, x, y, m[j], 3 * before + e);
+                    atomicMin(&S.first[edge_key(x, y)], 3 * (before + r) + e);
                 }
-                ++before;
+                ++r;
             } else if (f < nf) {
-                // Kept faces move down in order.
-                const uint32_t pos = f - before;
+                const uint32_t pos = f - before - r;
                 S.fabc[pos] = abc[j];
                 S.fnx[pos] = nx[j];
                 S.fny[pos] = ny[j];
@@ -953,12 +975,18 @@ __device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, 
                 S.fd[pos] = d[j];
                 S.mult[pos] = m[j];
                 S.gen[pos] = g[j];
+                nan = nan || d[j] != d[j];
+                if (bidx == kNoIndex || d[j] < bv) {
+                    bv = d[j];
+                    bidx = pos;
+                }
             }
         }
         __syncthreads();
-        // New faces: one per unordered vertex pair with a nonzero sum, in the
-        // order of each pair's first call (call 3 r + e is edge e of visible
-        // face r), calls taken kContThreads at a time.
+        // New faces: one per unordered vertex pair with a nonzero sum, made
+        // at the pair's first call, in call order; calls taken kContThreads at
+        // a time.  The thread of a pair's first call also clears the pair
+        // (the pair's other calls only compare its first call with theirs).
         const uint32_t keep = nf - visible, calls = 3 * visible;
         uint32_t ne = 0;
         bool bound = false;
@@ -970,8 +998,12 @@ __device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, 
             if (call < calls) {
                 face_edge(S.vis_abc[call / 3], call % 3, x, y);
                 const uint32_t key = edge_key(x, y);
-                sk = (long long)S.sum[key];
-                emit = sk != 0 && S.first[key] == call;
+                if (S.first[key] == call) {
+                    sk = (long long)S.sum[key];
+                    emit = sk != 0;
+                    S.sum[key] = 0ull;
+                    S.first[key] = kNoIndex;
+                }
             }
             uint32_t round = 0;
             const uint32_t pos = keep + ne + block_prefix<1>(S, emit ? 1u : 0u, round, buf);
@@ -980,19 +1012,26 @@ __device__ int epa_multi_block(ContShared& S, const Bodies& B, const Shape& sa, 
                 const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
                 const unsigned long long mm = (unsigned long long)(sk > 0 ? sk : -sk);
                 if (pos < kContFaces && mm <= 0xFFFFFFFFull) {
-                    make_face_lds(S, pos, sk > 0 ? lo : hi, sk > 0 ? hi : lo, si);
+                    const float dn = make_face_lds(S, pos, sk > 0 ? lo : hi, sk > 0 ? hi : lo, si);
                     S.mult[pos] = uint32_t(mm);
                     S.gen[pos] = uint8_t(gen);
+                    nan = nan || dn != dn;
+                    if (bidx == kNoIndex || dn < bv) {
+                        bv = dn;
+                        bidx = pos;
+                    }
                 } else {
                     bound = true;
                 }
             }
             ne += round;
         }
-        if (__syncthreads_or(bound))
-            return -1;
         if (ne == 0)
             return 0;
+        publish_best(S, bv, bidx, nan, bound);
+        __syncthreads();
+        if (published_bound(S))
+            return -1;
         nf = keep + ne;
         ++gen;
     }
@@ -1599,6 +1638,12 @@ __device__ void continue_record(ContShared& S, const Bodies& B, const World& W, 
         S.mult[pos] = m;
         S.gen[pos] = 0;
     }
+    for (uint32_t q = t; q < kEdgeKeys; q += kContThreads) {
+        S.sum[q] = 0ull;
+        S.first[q] = kNoIndex;
+    }
+    // Each lead is its own first minimum (one face per thread).
+    publish_best(S, d, lead ? pos : kNoIndex, lead && d != d, false);
     __syncthreads();
     f3 n;
     float depth;

@@ -372,6 +372,37 @@ __device__ __forceinline__ bool wave_arg_best_lanes(float& v, uint32_t& i, uint3
     return false;
 }
 
+// First minimum over the wavefront of (v, i) pairs (lanes with i == kNoIndex
+// hold none): the lowest v by DPP min, then the lowest index holding it by
+// DPP min, so ties go to the lower index as in a sequential strict-< scan.
+// Every lane ends with the result.  A NaN v is passed over (callers that can
+// meet one test for it themselves).
+__device__ __forceinline__ void wave_min_first(float& v, uint32_t& i)
+{
+    float x = i != kNoIndex && v == v ? v : __builtin_inff();
+    x = fminf(x, dpp_f32<0xB1>(x));
+    x = fminf(x, dpp_f32<0x4E>(x));
+    x = fminf(x, dpp_f32<0x141>(x));
+    x = fminf(x, dpp_f32<0x140>(x));
+    const auto x16 = __builtin_amdgcn_permlane16_swap(__float_as_int(x), __float_as_int(x), false,
+                                                      false);
+    x = fminf(__int_as_float(x16[0]), __int_as_float(x16[1]));
+    const auto x32 = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(x), false,
+                                                      false);
+    const float m = fminf(__int_as_float(x32[0]), __int_as_float(x32[1]));
+    uint32_t k = i != kNoIndex && v == m ? i : kNoIndex;
+    k = min(k, dpp_u32<0xB1>(k));
+    k = min(k, dpp_u32<0x4E>(k));
+    k = min(k, dpp_u32<0x141>(k));
+    k = min(k, dpp_u32<0x140>(k));
+    const auto k16 = __builtin_amdgcn_permlane16_swap((int)k, (int)k, false, false);
+    k = min((uint32_t)k16[0], (uint32_t)k16[1]);
+    const auto k32 = __builtin_amdgcn_permlane32_swap((int)k, (int)k, false, false);
+    k = min((uint32_t)k32[0], (uint32_t)k32[1]);
+    v = m;
+    i = k;
+}
+
 // row_arg_best for candidates whose index is chunk * 16 + (lane within the
 // 16-lane row), each lane holding the first extreme of its own chunks: the
 // row's extreme by DPP min / max, then its lowest chunk and lane by ballots.
