@@ -20,7 +20,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "physgpu", "libphysgpu.so")
-ARCH = os.environ.get("PHYSGPU_ARCH", "gfx950")
+# The kernels use gfx950-only instructions (v_permlane16/32_swap, MI355X's
+# LDS size): the build is for gfx950 alone.
+ARCH = "gfx950"
+if os.environ.get("PHYSGPU_ARCH", ARCH) != ARCH:
+    raise RuntimeError(f"PHYSGPU_ARCH={os.environ['PHYSGPU_ARCH']}: this library builds for "
+                       f"{ARCH} (MI355X) only")
 
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -102,19 +102,45 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* wave_t
     return run;
 }
 
-// In-place exclusive scan of each tile plus its tile offset.
-// Exclusive scan of tile blockIdx.x; its offset is the sum of the earlier
-// tiles' sums (at most a few hundred), reduced by the workgroup itself.
+// Up to this many tiles each workgroup of k_tile_apply sums the earlier
+// tiles' totals itself (O(tiles^2) loads over the grid, cheaper than one more
+// launch at these sizes); past it k_tile_offsets scans the totals first.
+constexpr uint32_t kDirectTiles = 1024;
+
+// Exclusive scan, in place, of `count` tile totals by one workgroup.
+__global__ __launch_bounds__(256) void k_tile_offsets(uint32_t* __restrict__ tiles, uint32_t count)
+{
+    __shared__ uint32_t wave_tot[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < count; base += 256) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t x = i < count ? tiles[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive(x, wave_tot, total);
+        if (i < count)
+            tiles[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// In-place exclusive scan of tile blockIdx.x plus its offset: the sum of the
+// earlier tiles' totals, reduced by the workgroup itself (direct), or read
+// from the scanned totals.
 __global__ __launch_bounds__(256) void k_tile_apply(uint32_t* __restrict__ v, uint32_t n,
-                                                    const uint32_t* __restrict__ tiles)
+                                                    const uint32_t* __restrict__ tiles,
+                                                    bool direct)
 {
     __shared__ uint32_t wave_tot[4];
     constexpr uint32_t kPer = kTile / 256;
-    uint32_t before = 0;
-    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256)
-        before += tiles[t];
-    uint32_t all;
-    block_exclusive(before, wave_tot, all);
+    uint32_t all = 0;
+    if (direct) {
+        uint32_t before = 0;
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256)
+            before += tiles[t];
+        block_exclusive(before, wave_tot, all);
+    } else {
+        all = tiles[blockIdx.x];
+    }
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
     uint32_t x[kPer], mine = 0;
 #pragma unroll
@@ -408,7 +434,11 @@ void scan_u32(physgpu_context* ctx, uint32_t* v, uint32_t n, hipStream_t stream)
 {
     const uint32_t tiles = div_up(n, kTile);
     hipLaunchKernelGGL(k_tile_sum, dim3(tiles), dim3(256), 0, stream, v, n, ctx->seg_tiles);
-    hipLaunchKernelGGL(k_tile_apply, dim3(tiles), dim3(256), 0, stream, v, n, ctx->seg_tiles);
+    const bool direct = tiles <= kDirectTiles;
+    if (!direct)
+        hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(256), 0, stream, ctx->seg_tiles, tiles);
+    hipLaunchKernelGGL(k_tile_apply, dim3(tiles), dim3(256), 0, stream, v, n, ctx->seg_tiles,
+                       direct);
 }
 
 }  // namespace

@@ -108,6 +108,32 @@ def test_collide_pipeline(pipe, oracle, name, algo):
     assert npairs == len(want_pairs) and nman == int(hit.sum())
 
 
+@pytest.mark.parametrize("per_wave", [1, 4])
+def test_sphere_hull_per_wave_bitwise(pipe, oracle, per_wave):
+    """Both GJK layouts of sphere-hull candidates (one per wavefront, or one
+    per 16-lane row beside box-hull and hull-hull candidates) give the
+    oracle's manifolds bit for bit, on a scene whose hull candidate count is
+    not a multiple of four (so the last item has empty rows)."""
+    sc = CASES["hulls4096"]
+    print(f"sphere_hull_per_wave {per_wave}", flush=True)
+    aabbs = oracle.compute_aabbs(sc)
+    want_pairs, _ = oracle.find_pairs(aabbs, abi.UNIFORM_GRID)
+    t = sc.shape_type
+    hull = (t[want_pairs[:, 0]] == abi.HULL) | (t[want_pairs[:, 1]] == abi.HULL)
+    sphere_hull = hull & ((t[want_pairs[:, 0]] == abi.SPHERE) | (t[want_pairs[:, 1]] == abi.SPHERE))
+    assert hull.sum() % 4 != 0 and (hull & ~sphere_hull).sum() % 4 != 0
+    pipe.set_colliders(sc)
+    pipe.set_tuning(dict(sphere_hull_per_wave=per_wave))
+    try:
+        pipe.collide(algorithm=abi.UNIFORM_GRID)
+        man = pipe.manifolds()
+    finally:
+        pipe.set_tuning(None)
+    rec, hit = oracle.generate_contacts(sc, want_pairs)
+    assert (hit & sphere_hull).sum() > 20
+    assert_manifolds_equal(man, compact(rec, hit), f"sphere_hull_per_wave {per_wave}")
+
+
 def test_boundary_bounds(pipe, oracle):
     """The reference's boundary AABB sets (tests/collision/broadphase_tests.cpp:89-105)."""
     F = np.float32
