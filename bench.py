@@ -203,6 +203,73 @@ def fp64_error(scene, pairs, man):
     return out
 
 
+# Narrow-phase kernels on the step's main stream (the hand-over consumer,
+# np_epa_early, runs beside np_gjk_epa on a stream of its own).
+NP_KERNELS = ("np_classify", "np_gjk_epa", "np_epa_continue", "np_sphere_sphere",
+              "np_sphere_box", "np_box_box", "np_compact")
+CLASS_NAMES = ("sphere_sphere", "sphere_box", "box_box", "sphere_hull", "box_hull", "hull_hull")
+
+
+def flop_count(scene, pairs, cores):
+    """FP32 operations of the step's narrow phase: the oracle's op-counting
+    scalar (add/subtract, multiply, divide, square root of the reference
+    algorithm; EPA as the GPU runs it past the LDS bounds) over every
+    candidate pair, per shape-pair class, with GJK / EPA iteration counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    orc = O.CpuCollision("oracle")
+    orc.mirror_gpu_epa()
+    t0 = time.perf_counter()
+    f = orc.count_flops(scene, pairs, workers=cores).astype(np.int64)
+    t = scene.shape_type
+    ta, tb = t[pairs[:, 0]].astype(np.int64), t[pairs[:, 1]].astype(np.int64)
+    lo, hi = np.minimum(ta, tb), np.maximum(ta, tb)
+    cls = np.where(hi < 2, lo + hi, 3 + lo)
+    total = f[:, :4].sum(axis=1)
+    out = {}
+    for c, name in enumerate(CLASS_NAMES):
+        m = cls == c
+        if not m.any():
+            continue
+        e = {"pairs": int(m.sum()), "flops": int(total[m].sum()),
+             "flops_per_pair": round(float(total[m].mean()), 1)}
+        if c >= 3:
+            gi, ei = int(f[m, 5].sum()), int(f[m, 6].sum())
+            gf = int(f[m, 4].sum())
+            e.update({"gjk_iterations": gi, "epa_iterations": ei,
+                      "gjk_flops_per_iteration": round(gf / max(gi, 1), 1),
+                      "epa_flops_per_iteration": round((int(total[m].sum()) - gf) / max(ei, 1), 1)})
+        out[name] = e
+    log(f"flop count: {len(pairs)} pairs in {time.perf_counter() - t0:.1f} s")
+    return out
+
+
+def roofline_narrowphase(per_class, kernels):
+    """FP32 VALU roofline of the narrow phase: the dominant kernel (GJK + EPA
+    of the hull pairs; its flops include the iterations continued past the LDS
+    bounds by the consumer kernels) and the whole narrow phase (every class
+    over the summed time of its main-stream kernels), against MI355X's FP32
+    vector peak."""
+    hull = sum(v["flops"] for k, v in per_class.items() if k.endswith("hull"))
+    allf = sum(v["flops"] for v in per_class.values())
+    us_gjk = kernels.get("np_gjk_epa")
+    us_np = sum(kernels.get(k, 0.0) for k in NP_KERNELS)
+    peak = VALU_FP32_PEAK_TFLOPS
+
+    def rate(fl, us):
+        return round(fl / (us * 1e-6) / 1e12, 4) if us else None
+
+    return {"bound": "valu", "kernel": "np_gjk_epa", "flops": hull, "avg_us": us_gjk,
+            "achieved": rate(hull, us_gjk), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(rate(hull, us_gjk) / peak, 6) if us_gjk else None,
+            "step": {"flops": allf, "avg_us": round(us_np, 2), "achieved": rate(allf, us_np),
+                     "frac": round(rate(allf, us_np) / peak, 6) if us_np else None},
+            "per_class": per_class,
+            "note": "FP32 add/sub/mul/div/sqrt of the reference algorithm counted by the oracle's "
+                    "op-counting scalar; no FMA (the reference's operation order) and no packed "
+                    "FP32, so a quarter of the peak is this code's ceiling"}
+
+
 def _hip():
     import ctypes as C
     return C.CDLL("libamdhip64.so")
@@ -452,6 +519,7 @@ def main():
             log(f"   {kn:24s} {avg * 1e3:9.2f} us x {launches // args.steps}")
     best = min(per, key=lambda k: per[k]["ms_per_step"])
     ms = per[best]["ms_per_step"]
+    step_pairs, step_man = pipe.pairs(), pipe.manifolds()  # the same for every broad phase
     npairs = per[best]["pairs"]
 
     # Roofline of the dominant bandwidth kernel of the headline step.
@@ -510,11 +578,6 @@ def main():
         "roofline_broadphase": roof_bp,
         "cpu_baseline": None,
     }
-    # Error against an FP64 re-evaluation of the headline step's manifolds.
-    try:
-        line["fp64_max_error"] = fp64_error(scene, pipe.pairs(), pipe.manifolds())
-    except Exception as e:  # reported, never fatal for the GPU number
-        line["fp64_max_error"] = {"error": repr(e)}
     if args.batched_worlds > 0:
         line["batched"] = batched_bench(args.batched_worlds, args.steps, args.warmup)
     if args.adjoint_pairs > 0:
@@ -523,10 +586,23 @@ def main():
         aabbs = pipe.compute_aabbs()[:args.tree_proxies]
         line["proxy_tree"] = tree_bench(np.ascontiguousarray(aabbs, np.float32), min(args.steps, 5))
     if not args.no_cpu_baseline:
+        # The CPU leg (the oracle, as checker and baseline only): the CPU step
+        # time, the error against an FP64 re-evaluation of the headline
+        # step's manifolds, and the operation count of its narrow phase.
         try:
             line["cpu_baseline"] = cpu_baseline(scene)
         except Exception as e:  # reported, never fatal for the GPU number
             line["cpu_baseline"] = {"error": repr(e)}
+        try:
+            line["fp64_max_error"] = fp64_error(scene, step_pairs, step_man)
+        except Exception as e:
+            line["fp64_max_error"] = {"error": repr(e)}
+        try:
+            cores = max(1, min(host_info()["affinity"], 16))
+            line["roofline_narrowphase"] = roofline_narrowphase(
+                flop_count(scene, step_pairs, cores), per[best]["kernels"])
+        except Exception as e:
+            line["roofline_narrowphase"] = {"error": repr(e)}
     log(json.dumps(per, indent=1))
     print(json.dumps(line), flush=True)
     pipe.close()

@@ -63,6 +63,9 @@ class CpuCollision:
             L.orc_generate_contacts_f64.argtypes = [
                 C.POINTER(abi.Colliders), vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
                 vp, vp]
+            L.orc_count_flops.argtypes = [
+                C.POINTER(abi.Colliders), vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
+                C.c_uint32, vp]
         else:
             L.ref_generate_contacts.argtypes = [C.POINTER(abi.Colliders), vp, C.c_uint64, vp, vp]
         getattr(L, p + "gjk_epa").argtypes = [
@@ -183,6 +186,21 @@ class CpuCollision:
                                                 hit.ctypes.data)
         self._check(rc, "generate_contacts_f64")
         return out[:n], hit[:n].astype(bool)
+
+    def count_flops(self, scene, pairs, gjk_iterations=64, epa_iterations=64, tolerance=1e-6,
+                    workers=1):
+        """Floating-point operations of each pair's contact in FP32, uint64 [n, 8]:
+        add/subtract, multiply, divide, square root, of which in GJK, GJK
+        iterations, EPA iterations, touching (physcpu.cpp orc_count_flops)."""
+        assert self.which == "oracle"
+        c = abi.colliders_struct(scene)
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        n = pairs.shape[0]
+        out = np.zeros((max(n, 1), 8), np.uint64)
+        rc = self.lib.orc_count_flops(C.byref(c), pairs.ctypes.data, n, gjk_iterations,
+                                      epa_iterations, tolerance, workers, out.ctypes.data)
+        self._check(rc, "count_flops")
+        return out[:n]
 
     def gjk_epa(self, scene, pairs, gjk_iterations=64, epa_iterations=64, tolerance=1e-6):
         c = abi.colliders_struct(scene)

@@ -32,6 +32,16 @@
 
 namespace orc {
 
+// Floating-point operation counts of the calling thread (only the Counted
+// scalar below adds to them) and GJK / EPA iteration counts (every scalar
+// type), for the narrow phase's operation count (orc_count_flops).
+struct FlopCount {
+    uint64_t add = 0, mul = 0, div = 0, sqrt = 0;
+    uint64_t total() const { return add + mul + div + sqrt; }
+};
+thread_local FlopCount t_flops;
+thread_local uint64_t t_gjk_iterations = 0, t_epa_iterations = 0, t_gjk_flops = 0;
+
 // ---------------------------------------------------------------- AABB
 template <typename T>
 struct Box6 {
@@ -748,6 +758,7 @@ bool gjk(const Support<T>& sa, const Support<T>& sb, Simplex<T>& s, uint32_t ite
     s = Simplex<T>{};
     V3<T> dir{T(1), T(0), T(0)};
     for (uint32_t it = 0; it < iterations; ++it) {
+        ++t_gjk_iterations;
         if (lsq(dir) < T(1e-12f))
             break;
         V3<T> p = sub(sa(dir), sb(neg(dir)));
@@ -1036,6 +1047,7 @@ int epa(const Support<T>& sa, const Support<T>& sb, const Simplex<T>& s, const N
         ~IterNote() { g_epa_iters[it < 65 ? it : 65]++; }
     } note;
     for (uint32_t it = 0; it < iterations; ++it, ++note.it) {
+        ++t_epa_iterations;
         if (faces.empty())
             return 0;
         size_t best = 0;
@@ -1110,7 +1122,10 @@ bool convexPair(const Shape<T>& a, const Pose<T>& ta, const Shape<T>& b, const P
 {
     Support<T> sa{&a, &ta}, sb{&b, &tb};
     Simplex<T> s;
-    if (!gjk(sa, sb, s, opt.gjkIterations)) {
+    const uint64_t flops0 = t_flops.total();
+    const bool intersect = gjk(sa, sb, s, opt.gjkIterations);
+    t_gjk_flops += t_flops.total() - flops0;
+    if (!intersect) {
         if (status)
             *status = 0;
         return false;
@@ -1144,7 +1159,7 @@ template <typename T>
 bool generateContact(const Shape<T>& a, const Pose<T>& ta, const Shape<T>& b, const Pose<T>& tb,
                      const NarrowOptions& opt, Contact<T>& c, int* gjkStatus)
 {
-    std::memset(&c, 0, sizeof(c));
+    c = Contact<T>{};
     if (a.type == kHull || b.type == kHull)
         return np::convexPair(a, ta, b, tb, opt, c, gjkStatus);
     if (a.type == kSphere && b.type == kSphere)
@@ -1162,6 +1177,60 @@ bool generateContact(const Shape<T>& a, const Pose<T>& ta, const Shape<T>& b, co
     return np::boxBox(a.data, ta, b.data, tb, c);
 }
 
+// FP32 with every arithmetic operation counted (add / subtract, multiply,
+// divide, square root; negation, comparison and absolute value are not
+// floating-point work): each operation is the float operation, so values and
+// branches are those of generateContact<float>.
+struct Counted {
+    float v;
+    Counted() : v(0.0f) {}
+    template <typename U, typename = std::enable_if_t<std::is_arithmetic_v<U>>>
+    Counted(U x) : v(float(x))
+    {
+    }
+    explicit operator float() const { return v; }
+};
+inline Counted operator+(Counted a, Counted b)
+{
+    ++t_flops.add;
+    return Counted(a.v + b.v);
+}
+inline Counted operator-(Counted a, Counted b)
+{
+    ++t_flops.add;
+    return Counted(a.v - b.v);
+}
+inline Counted operator-(Counted a) { return Counted(-a.v); }
+inline Counted operator*(Counted a, Counted b)
+{
+    ++t_flops.mul;
+    return Counted(a.v * b.v);
+}
+inline Counted operator/(Counted a, Counted b)
+{
+    ++t_flops.div;
+    return Counted(a.v / b.v);
+}
+inline bool operator<(Counted a, Counted b) { return a.v < b.v; }
+inline bool operator>(Counted a, Counted b) { return a.v > b.v; }
+inline bool operator<=(Counted a, Counted b) { return a.v <= b.v; }
+inline bool operator>=(Counted a, Counted b) { return a.v >= b.v; }
+inline bool operator==(Counted a, Counted b) { return a.v == b.v; }
+inline bool operator!=(Counted a, Counted b) { return a.v != b.v; }
+template <>
+inline Counted absOf<Counted>(Counted x) { return Counted(std::abs(x.v)); }
+template <>
+inline Counted sqrtOf<Counted>(Counted x)
+{
+    ++t_flops.sqrt;
+    return Counted(std::sqrt(x.v));
+}
+template <>
+inline Counted bigOf<Counted>() { return Counted(std::numeric_limits<float>::max()); }
+
+template bool generateContact<Counted>(const Shape<Counted>&, const Pose<Counted>&,
+                                       const Shape<Counted>&, const Pose<Counted>&,
+                                       const NarrowOptions&, Contact<Counted>&, int*);
 template bool generateContact<float>(const Shape<float>&, const Pose<float>&, const Shape<float>&,
                                      const Pose<float>&, const NarrowOptions&, Contact<float>&,
                                      int*);
@@ -1597,6 +1666,41 @@ int orc_generate_contacts(const physgpu_colliders* c, const physgpu_pair* pairs,
             hit[k] = h ? 1 : 0;
             if (gjkStatus)
                 gjkStatus[k] = status;
+        };
+        Pool::run(n >= 4096 ? workers : 1, n, one);
+        return PHYSGPU_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// Floating-point operations of generateContact in FP32 (the operation
+// counts of the Counted scalar), per pair k: out[k*8 + 0..7] = add/subtract,
+// multiply, divide, square root, of which in GJK, then GJK iterations, EPA
+// iterations, and whether the pair touches (0 / 1).
+int orc_count_flops(const physgpu_colliders* c, const physgpu_pair* pairs, uint64_t n,
+                    uint32_t gjkIterations, uint32_t epaIterations, float tolerance,
+                    uint32_t workers, uint64_t* out)
+{
+    try {
+        View<Counted> v(c);
+        NarrowOptions opt = makeOptions(gjkIterations, epaIterations, tolerance);
+        auto one = [&](size_t k) {
+            const uint32_t a = pairs[k].first, b = pairs[k].second;
+            t_flops = FlopCount{};
+            t_gjk_iterations = t_epa_iterations = t_gjk_flops = 0;
+            Contact<Counted> ct;
+            const bool h = generateContact(v.shape(a), v.pose(a), v.shape(b), v.pose(b), opt, ct,
+                                           nullptr);
+            uint64_t* o = out + k * 8;
+            o[0] = t_flops.add;
+            o[1] = t_flops.mul;
+            o[2] = t_flops.div;
+            o[3] = t_flops.sqrt;
+            o[4] = t_gjk_flops;
+            o[5] = t_gjk_iterations;
+            o[6] = t_epa_iterations;
+            o[7] = h ? 1 : 0;
         };
         Pool::run(n >= 4096 ? workers : 1, n, one);
         return PHYSGPU_OK;
