@@ -476,7 +476,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->node_last, N));
     A(alloc(c->node_box, 2 * N));
     A(alloc(c->node_first, N));
-    A(alloc(c->level_box, 2 * (N / 32 + N / 1024 + N / 32768 + 3)));
+    A(alloc(c->level_box, 2 * lbvh_level_boxes(N)));
     A(alloc(c->pairs_raw, P));
     A(alloc(c->pairs_tmp, P));
     A(alloc(c->seg_off, N + 1));
@@ -1713,7 +1713,7 @@ size_t carve_noalloc(physgpu_context& c, char* base)
     take(c.node_last, N);
     take(c.node_box, 2 * N);
     take(c.node_first, N);
-    take(c.level_box, 2 * (N / 32 + N / 1024 + N / 32768 + 3));
+    take(c.level_box, 2 * lbvh_level_boxes(N));
     take(c.pairs_raw, P);
     take(c.pairs_tmp, P);
     take(c.seg_off, N + 1);
@@ -1739,7 +1739,7 @@ constexpr size_t noalloc_bytes()
         ++k;
     return 27 * a + 2 * N * 16 * 2 + N * 8 * 2 + 256 * tiles * 4 + 256 * 4 +
            2 * (size_t(1) << (3 * k)) * 4 + N * 4 * 2 + N * 4 * 2 + 2 * N * 4 + N * 4 +
-           2 * N * 16 + N * 4 + 2 * (N / 32 + N / 1024 + N / 32768 + 3) * 16 + P * 8 * 2 +
+           2 * N * 16 + N * 4 + 2 * lbvh_level_boxes(N) * 16 + P * 8 * 2 +
            (N + 1) * 4 + ((N + 1 + 2047) / 2048) * 4 + P * 4 * 2 + P * 8 + P * 4 + 2 * N * 4 +
            size_t(kPartialSlots) * kMaxPartialBlocks * 8 + sizeof(Counters);
 }

@@ -22,6 +22,10 @@ constexpr uint32_t kStrideGrid = 2048;  // workgroups of grid-stride kernels (8 
 // tuning's sort_items); buffers are sized for the smallest tile.
 constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kSortMinTile = kSortThreads * 4;
+// LBVH level boxes: aligned blocks of 4^k leaves, k = 1 .. at most this.
+constexpr uint32_t kLbvhMaxLevels = 16;
+// Level boxes for n leaves: sum of ceil(n / 4^k) <= n / 3 + kLbvhMaxLevels.
+constexpr size_t lbvh_level_boxes(size_t n) { return n / 3 + kLbvhMaxLevels + 1; }
 
 // Compaction tiling.
 constexpr uint32_t kScanTile = 256;  // candidates per compaction tile
@@ -179,12 +183,13 @@ struct physgpu_context {
 
     // LBVH: internal nodes [0, n-1), leaves by sorted position.
     int32_t* node_left = nullptr;
-    int32_t* node_right = nullptr;
+    int32_t* node_right = nullptr;    // per split position s: the right child of the node
+                                      // splitting s | s + 1 (the rope of subtrees ending at s)
     int32_t* node_parent = nullptr;   // internal parents (n-1) then leaf parents (n)
     uint32_t* node_first = nullptr;   // first leaf position covered by an internal node
     uint32_t* node_last = nullptr;    // last leaf position covered by an internal node
-    float4* node_box = nullptr;       // internal node bounds, 2 float4 per node
-    float4* level_box = nullptr;      // boxes of 32 / 1024 / 32768-leaf groups
+    float4* node_box = nullptr;       // internal node records: bounds, left child, rope
+    float4* level_box = nullptr;      // boxes of aligned 4^k-leaf blocks, k = 1, 2, ...
     uint64_t* lbvh_sorted = nullptr;  // leaf order of the last build (items_a or items_b)
 
     // Pairs as first << 32 | second: raw emission buffer, sort scratch, and

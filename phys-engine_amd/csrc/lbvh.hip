@@ -13,11 +13,16 @@
 //      leaves, so its box is a range union built from precomputed boxes of
 //      aligned 32 / 1024 / 32768-leaf groups; no atomics or fences (a
 //      bottom-up refit would need device-scope fences between XCDs).
-// Query: one thread per leaf, stack traversal; a subtree is skipped when all
-// of its leaves sit at or before the querying leaf in sorted order, so each
-// pair is found once.  Children are encoded with bit 31 set for leaves; a
-// stack entry with bit 31 is an overlapping leaf, emitted when popped (it is
-// never used to index the internal-node arrays).
+// Query: one thread per leaf, a stackless depth-first walk by ropes.  Node
+// k's children are the split's two sides, so the subtree after one whose
+// leaves end at l (l < n - 1) in depth-first order is the right child of the
+// node that splits l | l + 1: the build records that child per split
+// position, and the walk goes there after a subtree it skips or finishes
+// (the last leaf's rope ends the walk).  Depth-first order is leaf order, so
+// leaf p's walk starts at its own rope: it covers exactly the leaves after p
+// (each pair is found once, by its lower sorted position), descending into a
+// subtree only when its box meets p's.  Children and ropes are encoded with
+// bit 31 set for leaves; kEnd ends the walk.
 #include <stddef.h>
 
 #include <algorithm>
@@ -30,7 +35,7 @@ namespace pg {
 namespace {
 
 constexpr uint32_t kLeaf = 0x80000000u;
-constexpr int kStack = 96;  // > 62-bit key depth + pending siblings
+constexpr uint32_t kEnd = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v)
 {
@@ -76,7 +81,7 @@ __device__ __forceinline__ int delta(const uint64_t* __restrict__ items, int n, 
 
 __global__ __launch_bounds__(kBlock) void k_build(uint32_t n, const uint64_t* __restrict__ items,
                                                   int32_t* __restrict__ left,
-                                                  int32_t* __restrict__ right,
+                                                  uint32_t* __restrict__ rope_of_split,
                                                   int32_t* __restrict__ parent,
                                                   uint32_t* __restrict__ first_out,
                                                   uint32_t* __restrict__ last)
@@ -108,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_build(uint32_t n, const uint64_t* __
     const int32_t lc = (first == split) ? int32_t(uint32_t(split) | kLeaf) : split;
     const int32_t rc = (lastpos == split + 1) ? int32_t(uint32_t(split + 1) | kLeaf) : split + 1;
     left[i] = lc;
-    right[i] = rc;
+    rope_of_split[split] = uint32_t(rc);
     first_out[i] = uint32_t(first);
     last[i] = uint32_t(lastpos);
     // Parents: internal nodes in [0, n-1), leaves in [n-1, 2n-1).
@@ -118,9 +123,24 @@ __global__ __launch_bounds__(kBlock) void k_build(uint32_t n, const uint64_t* __
 
 // Node bounds without inter-thread synchronisation: an internal node covers
 // the sorted leaves [first, last], so its box is the union of that range,
-// assembled from aligned blocks of 32^3, 32^2, 32 leaves and single leaves.
-// (min/max are exact, so the order of the unions does not change the box.)
-constexpr uint32_t kSpan = 32;
+// assembled greedily from the boxes of aligned blocks of 4^k leaves (at most
+// three blocks per level on each side, a handful for the common short
+// ranges).  min/max are exact, so the order of the unions does not change the
+// box.  Level k (1 <= k <= levels) holds count[k] boxes from level_box +
+// 2 offset[k].
+struct LevelTable {
+    uint32_t offset[kLbvhMaxLevels + 1];
+    uint32_t count[kLbvhMaxLevels + 1];
+    uint32_t levels;
+};
+
+__device__ __forceinline__ void empty_box(float u[6])
+{
+    for (int k = 0; k < 3; ++k) {
+        u[k] = __int_as_float(0x7f800000);
+        u[3 + k] = -__int_as_float(0x7f800000);
+    }
+}
 
 __device__ __forceinline__ void unite(float u[6], const float4* __restrict__ r, size_t i)
 {
@@ -131,73 +151,81 @@ __device__ __forceinline__ void unite(float u[6], const float4* __restrict__ r, 
     }
 }
 
-// Boxes of consecutive groups of 32 records (one per half wave).
-__global__ __launch_bounds__(kBlock) void k_group_boxes(uint32_t count, const float4* __restrict__ in,
-                                                        float4* __restrict__ out)
+// Up to five levels above `in` (boxes of level first_level - 1, count_in of
+// them): each workgroup reduces 1024 consecutive input boxes to 256, 64, 16,
+// 4 and 1 boxes of levels first_level .. first_level + 4.
+__global__ __launch_bounds__(256) void k_levels(const float4* __restrict__ in, uint32_t count_in,
+                                                float4* __restrict__ base, LevelTable lt,
+                                                uint32_t first_level)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool ok = i < count;
+    __shared__ float box[256][6];
+    const uint32_t t = threadIdx.x;
+    const uint64_t i0 = (uint64_t(blockIdx.x) * 256 + t) * 4;
     float b[6];
-    if (ok) {
-        const Rec r = load_rec(in, i);
+    empty_box(b);
+    for (uint32_t j = 0; j < 4; ++j)
+        if (i0 + j < count_in)
+            unite(b, in, i0 + j);
+    uint32_t lev = first_level;
+    uint64_t idx = uint64_t(blockIdx.x) * 256 + t;
+    if (lev <= lt.levels && idx < lt.count[lev])
+        store_rec(base + 2 * size_t(lt.offset[lev]), idx, b, 0);
+    for (uint32_t width = 64; width >= 1; width /= 4) {
         for (int k = 0; k < 6; ++k)
-            b[k] = r.b[k];
-    } else {
-        for (int k = 0; k < 3; ++k) {
-            b[k] = __int_as_float(0x7f800000);
-            b[3 + k] = -__int_as_float(0x7f800000);
+            box[t][k] = b[k];
+        __syncthreads();
+        if (t < width) {
+            empty_box(b);
+            for (uint32_t j = 0; j < 4; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    b[k] = min_ref(b[k], box[4 * t + j][k]);
+                    b[3 + k] = max_ref(b[3 + k], box[4 * t + j][3 + k]);
+                }
         }
+        __syncthreads();
+        ++lev;
+        idx = uint64_t(blockIdx.x) * width + t;
+        if (t < width && lev <= lt.levels && idx < lt.count[lev])
+            store_rec(base + 2 * size_t(lt.offset[lev]), idx, b, 0);
     }
-    for (uint32_t off = 1; off < kSpan; off <<= 1)
-        for (int k = 0; k < 3; ++k) {
-            b[k] = fminf(b[k], __shfl_xor(b[k], off));
-            b[3 + k] = fmaxf(b[3 + k], __shfl_xor(b[3 + k], off));
-        }
-    if ((i % kSpan) == 0 && ok)
-        store_rec(out, i / kSpan, b, 0);
 }
 
+// Node records: the box, the left child (index word) and the rope (world
+// word), so one 32-byte read serves a visit.
 __global__ __launch_bounds__(kBlock) void k_node_boxes(uint32_t n, const float4* __restrict__ leaves,
-                                                       const float4* __restrict__ l1,
-                                                       const float4* __restrict__ l2,
-                                                       const float4* __restrict__ l3,
+                                                       const float4* __restrict__ levels,
+                                                       LevelTable lt,
                                                        const uint32_t* __restrict__ first,
                                                        const uint32_t* __restrict__ last,
+                                                       const int32_t* __restrict__ left,
+                                                       const uint32_t* __restrict__ rope_of_split,
                                                        float4* __restrict__ nodes)
 {
     const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node + 1 >= n)
         return;
-    uint32_t i = first[node];
-    const uint32_t l = last[node];
+    uint64_t i = first[node];
+    const uint64_t l = last[node];
     float u[6];
-    for (int k = 0; k < 3; ++k) {
-        u[k] = __int_as_float(0x7f800000);
-        u[3 + k] = -__int_as_float(0x7f800000);
-    }
-    constexpr uint32_t s1 = kSpan, s2 = kSpan * kSpan, s3 = kSpan * kSpan * kSpan;
+    empty_box(u);
     while (i <= l) {
-        if (i % s3 == 0 && uint64_t(i) + s3 - 1 <= l) {
-            unite(u, l3, i / s3);
-            i += s3;
-        } else if (i % s2 == 0 && uint64_t(i) + s2 - 1 <= l) {
-            unite(u, l2, i / s2);
-            i += s2;
-        } else if (i % s1 == 0 && uint64_t(i) + s1 - 1 <= l) {
-            unite(u, l1, i / s1);
-            i += s1;
-        } else {
+        // The largest aligned block starting at i that ends within the range.
+        uint32_t k = 0;
+        while (k < lt.levels && (i & ((uint64_t(4) << (2 * k)) - 1)) == 0 &&
+               i + (uint64_t(4) << (2 * k)) - 1 <= l)
+            ++k;
+        if (k == 0)
             unite(u, leaves, i);
-            i += 1;
-        }
+        else
+            unite(u, levels + 2 * size_t(lt.offset[k]), i >> (2 * k));
+        i += uint64_t(1) << (2 * k);
     }
-    store_rec(nodes, node, u, 0);
+    const uint32_t end = last[node];
+    store_rec(nodes, node, u, uint32_t(left[node]), end + 1 < n ? rope_of_split[end] : kEnd);
 }
 
 __global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float4* __restrict__ s,
-                                                  const int32_t* __restrict__ left,
-                                                  const int32_t* __restrict__ right,
-                                                  const uint32_t* __restrict__ last,
+                                                  const uint32_t* __restrict__ rope_of_split,
                                                   const float4* __restrict__ nodes,
                                                   Counters* counters, uint64_t* __restrict__ out,
                                                   uint64_t cap, uint64_t* __restrict__ visits_part,
@@ -206,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float4* __re
     __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
     WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
     unsigned long long visits = 0, leaves = 0;
-    const unsigned long long max_visits = 2ull * n + 64;
+    const unsigned long long max_steps = 2ull * n + 64;
     // Grid-stride over leaves; every lane stays in the loops so the
     // wave-wide appends run converged.
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -214,63 +242,45 @@ __global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float4* __re
     const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
     for (uint32_t round = 0; round < rounds; ++round) {
         const uint32_t p = round * stride + block * blockDim.x + threadIdx.x;
-        unsigned long long my_visits = 0;
-        int32_t stack[kStack];
-        int sp = 0;
-        Rec me{{0, 0, 0, 0, 0, 0}, 0, 0};
         const bool active = p < n && n >= 2;
+        Rec me{{0, 0, 0, 0, 0, 0}, 0, 0};
+        uint32_t node = kEnd;
         if (active) {
             me = load_rec(s, p);
-            stack[sp++] = 0;  // root
+            node = p + 1 < n ? rope_of_split[p] : kEnd;
         }
-        // Converged loop: every lane pops until all stacks are empty.  A lane
-        // never visits more nodes than the tree has; the cap only guards
-        // against a malformed tree.
+        // Converged loop: every lane steps until all walks have ended.  A walk
+        // enters each node at most once; the cap only guards against a
+        // malformed tree.
+        unsigned long long steps = 0;
         for (;;) {
-            if (my_visits > max_visits) {
+            if (node != kEnd && steps > max_steps) {
                 atomicAdd(&counters->tree_overflow, 1u);
-                sp = 0;
+                node = kEnd;
             }
-            const bool more = sp > 0;
+            const bool more = node != kEnd;
             if (__ballot(more) == 0)
                 break;
             bool hit = false;
             uint32_t b = 0;
             if (more) {
-                const int32_t entry = stack[--sp];
-                if (entry & int32_t(kLeaf)) {
-                    // A leaf pushed below already passed the order and overlap tests.
-                    hit = true;
-                    b = load_rec(s, uint32_t(entry) & 0x7FFFFFFFu).index;
+                ++steps;
+                if (node & kLeaf) {
+                    const uint32_t q = node & ~kLeaf;
+                    const Rec o = load_rec(s, q);
+                    const uint32_t rope = q + 1 < n ? rope_of_split[q] : kEnd;
+                    ++leaves;
+                    hit = rec_overlap(me.b, o.b);
+                    b = o.index;
+                    node = rope;
                 } else {
-                    ++my_visits;
-                    const int32_t kids[2] = {left[entry], right[entry]};
-                    for (int c = 0; c < 2; ++c) {
-                        const int32_t child = kids[c];
-                        bool take = false;
-                        if (child & int32_t(kLeaf)) {
-                            const uint32_t q = uint32_t(child) & 0x7FFFFFFFu;
-                            if (q <= p)
-                                continue;
-                            ++leaves;
-                            take = rec_overlap(me.b, load_rec(s, q).b);
-                        } else {
-                            if (last[child] <= p)
-                                continue;
-                            take = rec_overlap(me.b, load_rec(nodes, uint32_t(child)).b);
-                        }
-                        if (take) {
-                            if (sp < kStack)
-                                stack[sp++] = child;
-                            else
-                                atomicAdd(&counters->tree_overflow, 1u);
-                        }
-                    }
+                    ++visits;
+                    const Rec nd = load_rec(nodes, node);
+                    node = rec_overlap(me.b, nd.b) ? nd.index : nd.world;
                 }
             }
             wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count, cap);
         }
-        visits += my_visits;
     }
     wq_flush_block(queue, out, &counters->pair_count, cap);
     block_partial(visits, visits_part);
@@ -295,8 +305,8 @@ void launch_lbvh(physgpu_context* ctx)
         return;
     time_begin(ctx, "lbvh_build", t);
     hipLaunchKernelGGL(k_build, dim3(div_up(n - 1, kBlock)), dim3(kBlock), 0, ctx->stream, n,
-                       sorted, ctx->node_left, ctx->node_right, ctx->node_parent, ctx->node_first,
-                       ctx->node_last);
+                       sorted, ctx->node_left, reinterpret_cast<uint32_t*>(ctx->node_right),
+                       ctx->node_parent, ctx->node_first, ctx->node_last);
     time_end(ctx, t);
     launch_lbvh_boxes_and_query(ctx);
 }
@@ -316,27 +326,33 @@ void launch_lbvh_boxes_and_query(physgpu_context* ctx)
 {
     const uint32_t n = ctx->n;
     Timer t;
-    // Group boxes: 32, 1024 and 32768 leaves.
-    const uint32_t c1 = div_up(n, kSpan), c2 = div_up(c1, kSpan), c3 = div_up(c2, kSpan);
-    float4* l1 = ctx->level_box;
-    float4* l2 = l1 + 2 * size_t(c1);
-    float4* l3 = l2 + 2 * size_t(c2);
+    // Level boxes of aligned 4^k-leaf blocks, five levels per launch.
+    LevelTable lt{};
+    uint32_t total = 0, count = n;
+    while (count > 1 && lt.levels < kLbvhMaxLevels) {
+        count = div_up(count, 4u);
+        ++lt.levels;
+        lt.offset[lt.levels] = total;
+        lt.count[lt.levels] = count;
+        total += count;
+    }
     time_begin(ctx, "lbvh_node_boxes", t);
-    hipLaunchKernelGGL(k_group_boxes, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->sorted_aabb, l1);
-    hipLaunchKernelGGL(k_group_boxes, dim3(div_up(c1, kBlock)), dim3(kBlock), 0, ctx->stream, c1,
-                       l1, l2);
-    hipLaunchKernelGGL(k_group_boxes, dim3(div_up(c2, kBlock)), dim3(kBlock), 0, ctx->stream, c2,
-                       l2, l3);
+    for (uint32_t from = 1; from <= lt.levels; from += 5) {
+        const float4* in = from == 1 ? ctx->sorted_aabb
+                                     : ctx->level_box + 2 * size_t(lt.offset[from - 1]);
+        const uint32_t count_in = from == 1 ? n : lt.count[from - 1];
+        hipLaunchKernelGGL(k_levels, dim3(div_up(count_in, 1024u)), dim3(256), 0, ctx->stream, in,
+                           count_in, ctx->level_box, lt, from);
+    }
     hipLaunchKernelGGL(k_node_boxes, dim3(div_up(n - 1, kBlock)), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->sorted_aabb, l1, l2, l3, ctx->node_first, ctx->node_last,
+                       ctx->sorted_aabb, ctx->level_box, lt, ctx->node_first, ctx->node_last,
+                       ctx->node_left, reinterpret_cast<const uint32_t*>(ctx->node_right),
                        ctx->node_box);
     time_end(ctx, t);
-    (void)c3;
     time_begin(ctx, "lbvh_query", t);
     const uint32_t qgrid = std::min<uint32_t>(div_up(n, kBlock), 4096);
     hipLaunchKernelGGL(k_query, dim3(qgrid), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->sorted_aabb, ctx->node_left, ctx->node_right, ctx->node_last,
+                       ctx->sorted_aabb, reinterpret_cast<const uint32_t*>(ctx->node_right),
                        ctx->node_box, ctx->counters, ctx->pairs_raw, ctx->cap_p,
                        ctx->partials + size_t(kSlotTreeVisits) * kMaxPartialBlocks,
                        ctx->partials + size_t(kSlotTreeLeaves) * kMaxPartialBlocks);
