@@ -144,13 +144,21 @@ __global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float4* 
                 const float rx = ra[u].x, ry0 = ra[u].y, ry1 = ra[u].z, rz0 = ra[u].w;
                 const float rz1 = rb[u].x;
                 const uint32_t ri = __float_as_uint(rb[u].y);
+                bool hit[kColsPerLane];
+                bool any = false;
 #pragma unroll
                 for (uint32_t k = 0; k < kColsPerLane; ++k) {
-                    const bool hit = c_j[k] > i && c_minx[k] <= rx && ry0 <= c_maxy[k] &&
-                                     ry1 >= c_miny[k] && rz0 <= c_maxz[k] && rz1 >= c_minz[k];
-                    wq_push(queue, hit, pair_value(ri, c_idx[k]), out, &counters->pair_count,
-                            cap);
+                    hit[k] = c_j[k] > i && c_minx[k] <= rx && ry0 <= c_maxy[k] &&
+                             ry1 >= c_miny[k] && rz0 <= c_maxz[k] && rz1 >= c_minz[k];
+                    any = any || hit[k];
                 }
+                // One ballot per row; the queue is touched only on a hit.
+                if (__ballot(any) == 0)
+                    continue;
+#pragma unroll
+                for (uint32_t k = 0; k < kColsPerLane; ++k)
+                    wq_push(queue, hit[k], pair_value(ri, c_idx[k]), out, &counters->pair_count,
+                            cap);
             }
         }
     }
