@@ -314,13 +314,17 @@ def batched_bench(worlds, steps, warmup, check_every=64):
     log(f"batched: graph {graph_ms:.3f} ms/step, eager {eager_ms:.3f} ms/step, "
         f"{npairs} pairs, {nman} manifolds")
 
-    # Per-world contexts on one shared stream.
+    # Per-world contexts on one shared stream, for a sample of every
+    # `sample_every`-th world (at most 1,024 contexts alive); the loop time is
+    # scaled to all worlds by the sample's share of the batch's bodies.
     hip = _hip()
     stream = C.c_void_p()
     assert hip.hipStreamCreateWithFlags(C.byref(stream), C.c_uint(1)) == 0
     t0 = time.perf_counter()
+    sample_every = max(1, -(-worlds // 1024))
+    sample = list(range(0, worlds, sample_every))
     ctxs = []
-    for w in range(worlds):
+    for w in sample:
         sub = sc.subset(np.arange(off[w], off[w + 1]))
         nw = sub.count
         p = physgpu.CollisionPipeline(nw, max_pairs=max(4 * nw, 1024),
@@ -328,7 +332,7 @@ def batched_bench(worlds, steps, warmup, check_every=64):
         p.set_stream(stream.value)
         p.set_colliders(sub)
         ctxs.append(p)
-    log(f"per-world: {worlds} contexts set up in {time.perf_counter() - t0:.1f} s")
+    log(f"per-world: {len(ctxs)} contexts set up in {time.perf_counter() - t0:.1f} s")
     for p in ctxs:
         p.collide(algorithm=grid, sync=False)
     assert hip.hipStreamSynchronize(stream) == 0
@@ -338,10 +342,13 @@ def batched_bench(worlds, steps, warmup, check_every=64):
         for p in ctxs:
             p.collide(algorithm=grid, sync=False)
     assert hip.hipStreamSynchronize(stream) == 0
-    loop_ms = (time.perf_counter() - t0) * 1e3 / loops
-    # Identical results: counts of every world, all words of every check_every-th world.
+    sample_ms = (time.perf_counter() - t0) * 1e3 / loops
+    sample_bodies = int(sum(counts[w] for w in sample))
+    loop_ms = sample_ms * worlds / len(sample)
+    # Identical results: counts of every sampled world, all words of every
+    # check_every-th world.
     same = True
-    for w, p in enumerate(ctxs):
+    for w, p in zip(sample, ctxs):
         pw, mw = p.step_counts()
         if pw != po[w + 1] - po[w] or mw != mo[w + 1] - mo[w]:
             same = False
@@ -355,7 +362,8 @@ def batched_bench(worlds, steps, warmup, check_every=64):
         p.close()
     hip.hipStreamDestroy(stream)
     pipe.close()
-    log(f"per-world loop: {loop_ms:.1f} ms per pass over {worlds} worlds; identical: {same}")
+    log(f"per-world loop: {sample_ms:.1f} ms per pass over {len(sample)} sampled worlds "
+        f"({sample_bodies} bodies), {loop_ms:.1f} ms scaled to {worlds}; identical: {same}")
     return {
         "workload": f"{worlds} worlds, bodies log-uniform in [16, 4096] ({sc.count} total), "
                     "mixed shapes, ~1 pair per body",
@@ -363,6 +371,9 @@ def batched_bench(worlds, steps, warmup, check_every=64):
         "graph_ms_per_step": round(graph_ms, 4), "eager_ms_per_step": round(eager_ms, 4),
         "worlds_per_s": round(worlds / (graph_ms * 1e-3), 1),
         "per_world_loop_ms": round(loop_ms, 3),
+        "per_world_sample": f"every {sample_every}th world ({len(sample)} contexts, "
+                            f"{sample_bodies} bodies): {sample_ms:.3f} ms, scaled by "
+                            f"{worlds}/{len(sample)} worlds",
         "per_world_worlds_per_s": round(worlds / (loop_ms * 1e-3), 1),
         "speedup_vs_per_world": round(loop_ms / graph_ms, 2),
         "identical_to_per_world": same,

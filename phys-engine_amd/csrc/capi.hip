@@ -137,14 +137,37 @@ void free_all(physgpu_context* c)
         hipEventDestroy(c->fork_event);
     if (c->join_event)
         hipEventDestroy(c->join_event);
-    if (c->side_stream)
-        hipStreamDestroy(c->side_stream);
     if (c->early_join_event)
         hipEventDestroy(c->early_join_event);
-    if (c->early_stream)
-        hipStreamDestroy(c->early_stream);
+    // side_stream and early_stream belong to the device (helper_streams).
     if (c->stream && c->owns_stream)
         hipStreamDestroy(c->stream);
+}
+
+// The narrow phase's two helper streams (SS / SB / BB beside GJK + EPA, and
+// the EPA hand-over consumer), one pair per device for every context of the
+// process: thousands of contexts (a world per context) would otherwise hold
+// thousands of streams.  Contexts that share them only queue their helper
+// work behind each other; every step still joins its own work by events.
+// Created at a device's first context and kept until the process ends.
+constexpr int kMaxHelperDevices = 64;
+std::mutex g_helper_mutex;
+hipStream_t g_helper_side[kMaxHelperDevices] = {};
+hipStream_t g_helper_early[kMaxHelperDevices] = {};
+
+hipError_t helper_streams(int device, hipStream_t& side, hipStream_t& early)
+{
+    if (device < 0 || device >= kMaxHelperDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_helper_mutex);
+    hipError_t e = hipSuccess;
+    if (!g_helper_side[device])
+        e = hipStreamCreateWithFlags(&g_helper_side[device], hipStreamNonBlocking);
+    if (e == hipSuccess && !g_helper_early[device])
+        e = hipStreamCreateWithFlags(&g_helper_early[device], hipStreamNonBlocking);
+    side = g_helper_side[device];
+    early = g_helper_early[device];
+    return e;
 }
 
 physgpu_status read_counters(physgpu_context* ctx)
@@ -445,11 +468,10 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     };
     A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->owns_stream = true;
-    A(hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
+    A(helper_streams(device, c->side_stream, c->early_stream));
     A(hipEventCreateWithFlags(&c->sort_event, hipEventDisableTiming));
     A(hipEventCreateWithFlags(&c->fork_event, hipEventDisableTiming));
     A(hipEventCreateWithFlags(&c->join_event, hipEventDisableTiming));
-    A(hipStreamCreateWithFlags(&c->early_stream, hipStreamNonBlocking));
     A(hipEventCreateWithFlags(&c->early_join_event, hipEventDisableTiming));
     A(alloc(c->pose, 7 * N));
     A(alloc(c->pose4, 2 * N));

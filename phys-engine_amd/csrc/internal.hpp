@@ -132,7 +132,8 @@ struct physgpu_context {
     bool owns_stream = false;
     // Second stream of the narrow phase: the SS / SB / BB kernels run on it
     // while the main stream runs EPA (fork and join by events, so the step
-    // stays one capturable sequence on the main stream).
+    // stays one capturable sequence on the main stream).  This and the third
+    // stream are the device's, shared by all its contexts.
     hipStream_t side_stream = nullptr;
     hipEvent_t sort_event = nullptr;  // pairs emitted: the canonical sort may start
     hipEvent_t fork_event = nullptr;
