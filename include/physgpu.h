@@ -82,7 +82,15 @@ typedef struct physgpu_proxy_pair {
 
 /* phys::BroadPhaseStats (broadphase.hpp:21-42), field for field.  Timings
  * are device time in ms measured with HIP events when timing is enabled on
- * the context, else 0. */
+ * the context, else 0.  After a uniform-grid call the grid_* fields are the
+ * reference grid's own counters over the same AABBs (findGridPairs,
+ * src/collision/broadphase.cpp:90-172: cell = upper median of the largest
+ * sides, entries = cells of the AABBs within 64 cells, comparisons = entry
+ * pairs sharing a cell plus the overflow pass), computed on demand when the
+ * stats are asked for (a few launches and host waits on the context's
+ * stream), not the work of the GPU's own grid.  In a batch of worlds each
+ * world is its own grid: the counts are summed and grid_cell_size is the
+ * largest world cell. */
 typedef struct physgpu_broadphase_stats {
     double record_build_ms;
     double record_sort_ms;

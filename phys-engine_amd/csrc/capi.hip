@@ -115,7 +115,8 @@ void free_all(physgpu_context* c)
                     c->adjoint_skipped, c->partials, c->tree_tight, c->tree_fat, c->tree_rank,
                     c->tree_rank_user, c->tree_stage_ids, c->tree_stage_bounds,
                     c->tree_stage_flags, c->tree_stage_pairs, c->epa_records,
-                    c->epa_unresolved_pairs, c->epa_flags, c->epa_epoch};
+                    c->epa_unresolved_pairs, c->epa_flags, c->epa_epoch, c->gs_scalars,
+                    c->gs_ranges, c->gs_tags, c->gs_owner, c->gs_counts, c->gs_list};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -280,7 +281,9 @@ void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool windo
     ctx->last_algorithm = algorithm;
 }
 
-void fill_broad_stats(physgpu_context* ctx, physgpu_broadphase_stats* s)
+// The grid fields are the reference grid's own counters over the resident
+// AABBs (reference_grid_stats), not the work of the GPU's grid.
+physgpu_status fill_broad_stats(physgpu_context* ctx, physgpu_broadphase_stats* s)
 {
     std::memset(s, 0, sizeof(*s));
     const Counters& c = *ctx->host_counters;
@@ -288,12 +291,7 @@ void fill_broad_stats(physgpu_context* ctx, physgpu_broadphase_stats* s)
     if (ctx->last_algorithm == PHYSGPU_SWEEP_AND_PRUNE) {
         s->x_window_comparisons = c.window_tests;
     } else if (ctx->last_algorithm == PHYSGPU_UNIFORM_GRID) {
-        s->grid_comparisons = c.grid_tests + c.grid_overflow_tests;
-        s->grid_entries = c.grid_entries;
-        s->grid_overflow_aabbs = c.overflow_count;
-        float cell;
-        std::memcpy(&cell, &c.cell_bits, 4);
-        s->grid_cell_size = cell;
+        return reference_grid_stats(ctx, s);
     } else if (ctx->last_algorithm == PHYSGPU_DYNAMIC_TREE) {
         s->tree_insertions = ctx->n;
         s->tree_node_pair_visits = c.tree_visits;
@@ -301,6 +299,7 @@ void fill_broad_stats(physgpu_context* ctx, physgpu_broadphase_stats* s)
         s->tree_height = c.tree_height;
         s->tree_proxy_count = ctx->n;
     }
+    return PHYSGPU_OK;
 }
 
 void fill_np_stats(const Counters& c, physgpu_narrowphase_stats* s)
@@ -735,8 +734,11 @@ physgpu_status physgpu_find_candidate_pairs(physgpu_context* ctx, const physgpu_
         return st;
     const Counters& c = *ctx->host_counters;
     *count = c.pair_count;
-    if (stats)
-        fill_broad_stats(ctx, stats);
+    if (stats) {
+        st = fill_broad_stats(ctx, stats);
+        if (st != PHYSGPU_OK)
+            return st;
+    }
     if (c.tree_overflow)
         return fail(PHYSGPU_DEVICE_ERROR, "LBVH traversal stack overflow");
     if (c.pair_count > ctx->cap_p)
@@ -1242,7 +1244,9 @@ physgpu_status physgpu_step_stats_get(physgpu_context* ctx, physgpu_step_stats* 
     collect_times(ctx);
     std::memset(s, 0, sizeof(*s));
     const Counters& c = *ctx->host_counters;
-    fill_broad_stats(ctx, &s->broad_phase);
+    st = fill_broad_stats(ctx, &s->broad_phase);
+    if (st != PHYSGPU_OK)
+        return st;
     fill_np_stats(c, &s->narrow_phase);
     if (ctx->timing) {
         s->broad_phase_ms = ctx->last_broadphase_ms;
@@ -1251,8 +1255,10 @@ physgpu_status physgpu_step_stats_get(physgpu_context* ctx, physgpu_step_stats* 
     // possiblePairs (physics_world.cpp:571-577): each collider pairs with the
     // earlier ones of its world that sit on other bodies.
     const uint32_t n = ctx->n;
-    std::vector<uint32_t> off = ctx->host_world_offsets;
-    if (off.empty())
+    std::vector<uint32_t> off;
+    if (ctx->n_worlds)
+        off = ctx->host_world_offsets;
+    else
         off = {0u, n};
     std::unordered_map<uint32_t, uint64_t> per_body;
     uint64_t possible = 0;

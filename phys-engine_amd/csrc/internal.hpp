@@ -281,6 +281,18 @@ struct physgpu_context {
     uint64_t* world_pair_off = nullptr;
     uint64_t* world_manifold_off = nullptr;
 
+    // The reference grid's statistics (gridstats.hip), allocated on demand and
+    // grown only: two scalar blocks (bytes), per-record cell ranges, the cell
+    // table (tags, owners, counts) and two collision lists.
+    unsigned char* gs_scalars = nullptr;
+    int4* gs_ranges = nullptr;
+    uint64_t* gs_tags = nullptr;
+    uint64_t* gs_owner = nullptr;
+    uint32_t* gs_counts = nullptr;
+    uint64_t* gs_list = nullptr;
+    size_t gs_scalars_cap = 0, gs_ranges_cap = 0, gs_tags_cap = 0, gs_owner_cap = 0,
+           gs_counts_cap = 0, gs_list_cap = 0;
+
     // Adjoint staging for host-memory calls (floats) and its skip counter.
     float* adjoint_scratch = nullptr;
     size_t adjoint_capacity = 0;
@@ -384,6 +396,10 @@ void launch_sap_sweep(physgpu_context* ctx);
 void launch_sap_window_tests(physgpu_context* ctx);
 // grid.hip
 void launch_grid(physgpu_context* ctx, double cell_size);
+// gridstats.hip: the reference grid's counters (cell size, entries,
+// comparisons, overflow AABBs) of the resident AABBs, per world of a batch,
+// summed; runs on the main stream and waits for it.
+physgpu_status reference_grid_stats(physgpu_context* ctx, physgpu_broadphase_stats* s);
 // lbvh.hip
 void launch_lbvh(physgpu_context* ctx);
 void launch_lbvh_refit(physgpu_context* ctx);

@@ -104,6 +104,19 @@ class EpaUnresolvedWarning(RuntimeWarning):
     CollisionPipeline.unresolved_pairs() lists them."""
 
 
+class AdjointSkippedWarning(RuntimeWarning):
+    """Some manifolds got zero gradients from physgpu_contact_adjoint:
+    sphere-hull manifolds (their EPA polytope's vertices depend on earlier
+    faces' directions, which the adjoint does not follow) and hull manifolds
+    whose EPA ran past the LDS polytope bounds."""
+
+
+def _warn_skipped(skipped, m):
+    if skipped:
+        warnings.warn(f"{skipped} of {m} manifolds have zero gradients (sphere-hull, or EPA "
+                      f"past the LDS bounds)", AdjointSkippedWarning, stacklevel=3)
+
+
 def _check(status, allow=()):
     if status == abi.OK or status in allow:
         return status
@@ -358,11 +371,13 @@ class CollisionPipeline:
             _check(library().physgpu_contact_adjoint(self._ctx, m, ptr(gn), ptr(gd), ptr(gp), None,
                                                      out.ctypes.data, abi.HOST,
                                                      C.byref(skipped)))
+            _warn_skipped(skipped.value, m)
             return out[:, :m], skipped.value
         out = np.zeros((7, max(self.count, 1)), np.float32)
         _check(library().physgpu_contact_adjoint(self._ctx, m, ptr(gn), ptr(gd), ptr(gp),
                                                  out.ctypes.data, None, abi.HOST,
                                                  C.byref(skipped)))
+        _warn_skipped(skipped.value, m)
         return out[:, :self.count], skipped.value
 
     # Device-pointer forms (torch tensors' data_ptr()); the caller orders its
@@ -381,6 +396,7 @@ class CollisionPipeline:
             self._ctx, m, C.c_void_p(g_normal_ptr), C.c_void_p(g_depth_ptr),
             C.c_void_p(g_point_ptr), C.c_void_p(grad_pose_ptr), None, abi.DEVICE,
             C.byref(skipped)))
+        _warn_skipped(skipped.value, m)
         return skipped.value
 
     def step_counts(self):
