@@ -57,18 +57,21 @@ def log(*a):
 def kernel_bytes(name, n, pairs, hull_bodies, hull_points, table_entries):
     """Algorithmic (compulsory) HBM bytes of one launch, for the roofline.
 
-    Per body: pose 28 B + type 1 B + shape 12 B read, AABB 24 B written;
+    Per body: pose 28 B + type 1 B + shape 12 B read, AABB 24 B written
+    (with the grid's keys: an 8-byte sort item and a 4-byte flag more);
     sorted records are 24 B of bounds + 4 B of index; sort items are 8 B;
-    a pair record is 8 B."""
+    a pair record is 8 B; the grid's bucket pass reads an item and a 32-byte
+    record and writes the record per body, and writes every 8-byte entry of
+    the cell table."""
     return {
         "aabb_from_pose": n * 65 + hull_bodies * 8 + hull_points * 12,
+        "aabb_grid_keys": n * (65 + 8 + 4) + hull_bodies * 8 + hull_points * 12,
         "aabb_reduce": n * 24,
         "sap_keys": n * (4 + 8),
         "gather_sorted": n * (8 + 24 + 28),
         "sap_sweep": n * 28 + pairs * 8,
         "grid_keys": n * (24 + 8 + 4),
-        "grid_table_clear": table_entries * 8,
-        "grid_cell_ranges": n * 8,
+        "grid_buckets": n * (8 + 32 + 32) + table_entries * 8,
         "grid_query": n * 28 + pairs * 8,
         "lbvh_morton": n * (24 + 8),
         "lbvh_query": n * 28 + pairs * 8,
@@ -508,11 +511,11 @@ def main():
     assert hip.hipMalloc(C.byref(pose_dev), C.c_size_t(scene.pose.nbytes)) == 0
     assert hip.hipMemcpy(pose_dev, scene.pose.ctypes.data_as(C.c_void_p),
                          C.c_size_t(scene.pose.nbytes), 1) == 0  # host to device
-    table_entries = 0
-    k = 2
-    while k < 9 and (1 << (3 * k)) < 2 * n:
-        k += 1
-    table_entries = 1 << (3 * k)
+    # Grid cell table: 2^t entries, the smallest t >= 6 with 2^t >= 2 n.
+    t = 6
+    while t < 27 and (1 << t) < 2 * n:
+        t += 1
+    table_entries = 1 << t
 
     per = {}
     for name in args.algorithms.split(","):

@@ -15,6 +15,7 @@
 #include <stddef.h>
 
 #include "devmath.hpp"
+#include "gridkey.hpp"
 #include "internal.hpp"
 #include "wave.hpp"
 
@@ -358,6 +359,149 @@ __global__ __launch_bounds__(kBlock) void k_aabbs_step(
     st.flush(partials);
 }
 
+// The collision step's AABB refresh fused with the uniform grid's keys: one
+// workgroup of kKeyThreads per radix-sort tile of kSortThreads * kItems
+// colliders.  Spheres and boxes take one thread each; the tile's hulls are
+// listed in LDS and take one 16-lane row each (64 rows per round).  Every
+// collider's record is written, then its cell key (grid_cell_key with the
+// cell from the shapes, shape_cell), its sort item (key | index << 32), its
+// in-grid flag and, for an overflow body, a slot of the overflow list; the
+// tile's histogram of the bucket digit (key bits low .. low + 7, in-grid
+// bodies) goes to hist[digit * tiles + tile], the layout the radix pass
+// reads.  Workgroup 0 also resets the step counters (no other workgroup
+// touches them: the overflow slots come from over_ctr).
+constexpr uint32_t kKeyThreads = 1024;
+
+template <uint32_t kItems>
+__global__ __launch_bounds__(kKeyThreads) void k_aabbs_grid_keys(
+    uint32_t n, const float4* __restrict__ pose, const float4* __restrict__ shape,
+    const float* __restrict__ hpts, uint32_t nh, const uint32_t* __restrict__ world_of,
+    float4* __restrict__ aabb, Counters* counters, const uint32_t* __restrict__ shape_cell,
+    float requested, uint32_t bits_per_axis, uint32_t table_bits, uint32_t low,
+    uint64_t* __restrict__ items, uint32_t* __restrict__ overflow,
+    uint32_t* __restrict__ over_ctr, uint32_t* __restrict__ flag, uint32_t* __restrict__ hist)
+{
+    constexpr uint32_t kTile = kSortThreads * kItems;
+    static_assert(kTile % kKeyThreads == 0, "whole rounds per tile");
+    __shared__ uint32_t counts[256];
+    __shared__ uint32_t hull_ids[kTile];
+    __shared__ uint32_t hull_count;
+    if (blockIdx.x == 0)
+        reset_counters(counters);
+    const float cell = requested > 0.0f ? requested : __uint_as_float(*shape_cell);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        counters->cell_bits = __float_as_uint(cell);
+    if (threadIdx.x < 256)
+        counts[threadIdx.x] = 0;
+    if (threadIdx.x == 0)
+        hull_count = 0;
+    __syncthreads();
+    const double inv = 1.0 / double(cell);
+    auto emit = [&](uint32_t i, const float b[6], uint32_t world) {
+        const uint32_t key = grid_cell_key(b, world, cell, inv, bits_per_axis, table_bits);
+        items[i] = uint64_t(key) | (uint64_t(i) << 32);
+        flag[i] = key != kOverflowKey ? 1u : 0u;
+        if (key == kOverflowKey)
+            overflow[atomicAdd(over_ctr, 1u)] = i;
+        else
+            atomicAdd(&counts[(key >> low) & 0xFFu], 1u);
+    };
+    const uint32_t base = blockIdx.x * kTile;
+#pragma unroll
+    for (uint32_t r = 0; r < kTile / kKeyThreads; ++r) {
+        const uint32_t i = base + r * kKeyThreads + threadIdx.x;
+        if (i < n) {
+            float b[6];
+            if (body_aabb(i, pose, shape, b)) {
+                const uint32_t w = world_of ? world_of[i] : 0u;
+                store_rec(aabb, i, b, i, w);
+                emit(i, b, w);
+            } else {
+                hull_ids[atomicAdd(&hull_count, 1u)] = i;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t hulls = hull_count;
+    const uint32_t lane = lane_id(), rl = lane % kHullRowLanes, row_base = lane - rl;
+    constexpr uint32_t kRows = kKeyThreads / kHullRowLanes;
+    const uint32_t row = threadIdx.x / kHullRowLanes;
+    const uint32_t rounds = (hulls + kRows - 1) / kRows;
+    for (uint32_t k = 0; k < rounds; ++k) {
+        const uint32_t h = k * kRows + row;
+        if (h >= hulls)
+            continue;  // whole rows (h is row-uniform)
+        const uint32_t i = hull_ids[h];
+        float b[6];
+        hull_row_aabb(i, pose, shape, hpts, nh, rl, row_base, b);
+        if (rl == 0) {
+            const uint32_t w = world_of ? world_of[i] : 0u;
+            store_rec(aabb, i, b, i, w);
+            emit(i, b, w);
+        }
+    }
+    __syncthreads();
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    if (threadIdx.x < 256)
+        hist[threadIdx.x * tiles + blockIdx.x] = counts[threadIdx.x];
+}
+
+// Largest AABB side any orientation can give each collider (sphere 2 r, box
+// 2 |h|, hull 2 max |p_k|): their maximum (float key, out[0]) and sum
+// (2^-16 m fixed point, out[1..2]).  One atomic per workgroup; run once per
+// upload.
+__global__ __launch_bounds__(kBlock) void k_shape_bound(uint32_t n, const float4* __restrict__ shape,
+                                                        const float* __restrict__ hpts, uint32_t nh,
+                                                        uint32_t* __restrict__ out)
+{
+    uint32_t mx = 0;
+    unsigned long long sum = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 s0 = shape[2 * size_t(i)];
+        const uint32_t ty = __float_as_uint(s0.w);
+        float d;
+        if (ty == PHYSGPU_SPHERE) {
+            d = 2.0f * s0.x;
+        } else if (ty == PHYSGPU_BOX) {
+            d = 2.0f * sqrtf(s0.x * s0.x + s0.y * s0.y + s0.z * s0.z);
+        } else {
+            uint32_t first, count;
+            hull_range(shape, i, first, count);
+            float r2 = 0.0f;
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t p = first + k;
+                const float x = hpts[p], y = hpts[nh + p], z = hpts[2 * nh + p];
+                r2 = fmaxf(r2, x * x + y * y + z * z);
+            }
+            d = 2.0f * sqrtf(r2);
+        }
+        if (!(d >= 0.0f))
+            d = 0.0f;  // NaN data: such a body takes the overflow path anyway
+        mx = max(mx, float_key(d));
+        sum += (unsigned long long)(fminf(d, 65536.0f) * 65536.0f);
+    }
+    mx = wave_max_u32(mx);
+    sum = wave_sum_u64(sum);
+    if (lane_id() == 0) {
+        atomicMax(&out[0], mx);
+        atomicAdd(reinterpret_cast<unsigned long long*>(out + 2), sum);
+    }
+}
+
+// Grid cell of the collision step from the shape bounds: the largest bound,
+// capped at 8x the mean bound (larger bodies take the overflow path), widened
+// by 2^-8 (in-grid bodies must be at most 1 - 2^-9 of the cell); out[1].
+__global__ void k_shape_cell(uint32_t n, uint32_t* __restrict__ out)
+{
+    const float dmax = key_float(out[0]);
+    const unsigned long long sum = *reinterpret_cast<const unsigned long long*>(out + 2);
+    const double mean = n ? double(sum) / 65536.0 / double(n) : 1.0;
+    float cell = fminf(dmax, float(8.0 * mean));
+    if (!(cell > 0.0f))
+        cell = 1.0f;
+    out[1] = __float_as_uint(cell * (1.0f + 1.0f / 256.0f));
+}
+
 __global__ __launch_bounds__(kBlock) void k_rec_to_aabb(uint32_t n, const float4* __restrict__ rec,
                                                         physgpu_aabb* __restrict__ out)
 {
@@ -507,6 +651,40 @@ void launch_compute_aabbs(physgpu_context* ctx, bool with_stats)
         hipLaunchKernelGGL(k_hull_aabbs, dim3(stride_grid(16ull * ctx->n)), dim3(kBlock), 0,
                            ctx->stream, ctx->hull_list, ctx->hull_list_count, ctx->pose4,
                            ctx->shape4, ctx->hull_points, ctx->n_hull_points, world_of, ctx->aabb);
+    time_end(ctx, t);
+}
+
+void launch_shape_cell(physgpu_context* ctx)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->shape_cell, 4u, 0u);
+    if (ctx->n)
+        hipLaunchKernelGGL(k_shape_bound, dim3(stride_grid(ctx->n)), dim3(kBlock), 0, ctx->stream,
+                           ctx->n, ctx->shape4, ctx->hull_points, ctx->n_hull_points,
+                           ctx->shape_cell);
+    hipLaunchKernelGGL(k_shape_cell, dim3(1), dim3(1), 0, ctx->stream, ctx->n, ctx->shape_cell);
+}
+
+void launch_aabbs_grid_keys(physgpu_context* ctx, float cell, const GridBits& g)
+{
+    ctx->resident_aabbs_valid = true;
+    Timer t;
+    time_begin(ctx, "aabb_grid_keys", t);
+    const uint32_t* world_of = ctx->n_worlds ? ctx->world_of : nullptr;
+    const uint32_t tile = kSortThreads * ctx->tuning.sort_items;
+    const dim3 grid(div_up(ctx->n, tile)), block(kKeyThreads);
+#define PG_AABB_KEYS(K)                                                                        \
+    hipLaunchKernelGGL(k_aabbs_grid_keys<K>, grid, block, 0, ctx->stream, ctx->n, ctx->pose4,   \
+                       ctx->shape4, ctx->hull_points, ctx->n_hull_points, world_of, ctx->aabb, \
+                       ctx->counters, ctx->shape_cell + 1, cell, g.axis, g.table, g.low,       \
+                       ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr, ctx->grid_flag,   \
+                       ctx->histogram)
+    if (ctx->tuning.sort_items == 4)
+        PG_AABB_KEYS(4);
+    else if (ctx->tuning.sort_items == 16)
+        PG_AABB_KEYS(16);
+    else
+        PG_AABB_KEYS(8);
+#undef PG_AABB_KEYS
     time_end(ctx, t);
 }
 

@@ -102,9 +102,10 @@ hipError_t alloc(T*& p, size_t count)
 void free_all(physgpu_context* c)
 {
     void* ptrs[] = {c->pose, c->pose4, c->shape4, c->shape_type, c->shape_data, c->hull_range, c->hull_points,
-                    c->aabb, c->sorted_aabb, c->items_a, c->items_b, c->hull_list,
+                    c->aabb, c->sorted_aabb, c->items_a, c->items_b, c->hull_list, c->shape_cell,
                     c->hull_list_count,
-                    c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_flag, c->node_left,
+                    c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_over_ctr,
+                    c->grid_flag, c->node_left,
                     c->node_right, c->node_parent, c->node_first, c->node_last, c->node_box, c->level_box,
                     c->pairs_raw, c->pairs_tmp, c->seg_off, c->seg_tiles, c->pair_rank,
                     c->pair_perm, c->sort_scratch, c->sort_scratch_perm,
@@ -249,12 +250,15 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
 // Broad phase on the resident AABB planes (ctx->n bodies).  With defer_sort
 // the emitted pairs are left unordered (pairs_raw) for a narrow phase that
 // sorts them on its side stream.
+// With grid_fused (uniform grid, n >= 2) the AABB refresh of the resident
+// colliders is part of the grid's key pass and nothing else needs the width
+// statistics.
 void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool window_stats,
-                    bool defer_sort = false, bool stats_done = false)
+                    bool defer_sort = false, bool stats_done = false, bool grid_fused = false)
 {
     ctx->tree_build_current = false;
     ctx->tree_pairs_loaded = false;
-    if (!stats_done)
+    if (!stats_done && !grid_fused)
         launch_aabb_reductions(ctx);
     // The broad phase's closing fold (if any) also finishes the pair count.
     ctx->finish_pending = ctx->n >= 2;
@@ -268,7 +272,7 @@ void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool windo
             if (window_stats)
                 launch_sap_window_tests(ctx);
         } else if (algorithm == PHYSGPU_UNIFORM_GRID) {
-            launch_grid(ctx, cell);
+            launch_grid(ctx, cell, grid_fused);
         } else {
             launch_lbvh(ctx);
         }
@@ -364,12 +368,16 @@ void enqueue_step(physgpu_context* ctx, int algorithm, double cell)
     // (The AABB launch resets the step counters.)
     if (ctx->has_world)
         launch_compose_world(ctx);
-    launch_compute_aabbs(ctx, true);
+    // The uniform grid computes the AABBs in its key pass (one launch).
+    const bool fused = algorithm == PHYSGPU_UNIFORM_GRID && ctx->n >= 2;
+    if (!fused)
+        launch_compute_aabbs(ctx, true);
     // Stage brackets as StepStats times them (physics_world.cpp:554-636,
-    // 640-738): the broad phase after the AABB refresh, then the narrow phase.
+    // 640-738): the broad phase after the AABB refresh (for the grid with
+    // it), then the narrow phase.
     Timer bp, np;
     time_begin(ctx, kStageBroadphase, bp);
-    run_broadphase(ctx, algorithm, cell, false, true, true);
+    run_broadphase(ctx, algorithm, cell, false, true, true, fused);
     time_end(ctx, bp);
     ctx->np_pairs = ctx->pairs_raw;
     ctx->np_perm = ctx->pair_perm;
@@ -453,10 +461,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     c->cap_n = max_colliders;
     c->cap_p = max_pairs;
     c->cap_h = max_hull_points;
-    uint32_t k = 2;
-    while (k < 9 && (1ull << (3 * k)) < 2ull * max_colliders)
-        ++k;
-    c->table_bits = 3 * k;
+    c->table_bits = grid_table_bits(max_colliders);
     const size_t N = max_colliders, P = max_pairs;
     const size_t tiles = std::max<size_t>(div_up(N, kSortMinTile), div_up(P, kSortMinTile));
     c->hist_capacity = uint32_t(256 * tiles);
@@ -481,6 +486,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->hull_range, 2 * N));
     A(alloc(c->hull_points, 3 * size_t(std::max<uint32_t>(max_hull_points, 1))));
     A(alloc(c->hull_list, N));
+    A(alloc(c->shape_cell, 4));
     A(alloc(c->hull_list_count, 1));
     A(alloc(c->aabb, 2 * N));
     A(alloc(c->sorted_aabb, 2 * N));
@@ -490,6 +496,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->digit_total, 256));
     A(alloc(c->cell_table, 2 * (size_t(1) << c->table_bits)));
     A(alloc(c->overflow_list, N));
+    A(alloc(c->grid_over_ctr, 1));
     A(alloc(c->grid_flag, N));
     A(alloc(c->node_left, N));
     A(alloc(c->node_right, N));
@@ -531,6 +538,8 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
         A(hipMemsetAsync(c->epa_flags, 0, (2 * size_t(c->epa_record_cap) + 2) * 4, c->stream));
     if (e == hipSuccess)
         A(hipMemsetAsync(c->epa_epoch, 0, 4, c->stream));
+    if (e == hipSuccess)
+        A(hipMemsetAsync(c->grid_over_ctr, 0, 4, c->stream));
     A(alloc(c->counters, 1));
     A(hipHostMalloc(reinterpret_cast<void**>(&c->host_counters), sizeof(Counters)));
     if (e != hipSuccess) {
@@ -629,13 +638,14 @@ physgpu_status physgpu_set_colliders(physgpu_context* ctx, const physgpu_collide
                                       3ull * col->hull_point_count * 4, kind, ctx->stream));
     }
     ctx->n = n;
+    ctx->n_hull_points = col->hull_point_count;
     launch_pose_aos(ctx);
     launch_shape_aos(ctx);
     if (hulls)
         launch_hull_list(ctx);
+    launch_shape_cell(ctx);
     PG_TRY_HIP(hipGetLastError());
     PG_TRY_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->n_hull_points = col->hull_point_count;
     ctx->has_hulls = hulls;
     ctx->colliders_valid = true;
     ctx->has_world = false;
@@ -1706,10 +1716,7 @@ constexpr size_t kScratchAlign = 256;
 
 uint32_t noalloc_table_bits()
 {
-    uint32_t k = 2;
-    while (k < 9 && (1ull << (3 * k)) < 2ull * kNoAllocBodies)
-        ++k;
-    return 3 * k;
+    return grid_table_bits(kNoAllocBodies);
 }
 
 // Assigns the broad-phase buffers of `c` (sized for the maximum) from `base`
@@ -1734,6 +1741,7 @@ size_t carve_noalloc(physgpu_context& c, char* base)
     take(c.digit_total, 256);
     take(c.cell_table, 2 * (size_t(1) << noalloc_table_bits()));
     take(c.overflow_list, N);
+    take(c.grid_over_ctr, 1);
     take(c.grid_flag, N);
     take(c.node_left, N);
     take(c.node_right, N);
@@ -1762,11 +1770,8 @@ constexpr size_t noalloc_bytes()
 {
     const size_t N = kNoAllocBodies, P = kNoAllocPairs, a = kScratchAlign;
     const size_t tiles = (std::max(N, size_t(P)) + kSortMinTile - 1) / kSortMinTile;
-    size_t k = 2;
-    while (k < 9 && (size_t(1) << (3 * k)) < 2 * N)
-        ++k;
-    return 27 * a + 2 * N * 16 * 2 + N * 8 * 2 + 256 * tiles * 4 + 256 * 4 +
-           2 * (size_t(1) << (3 * k)) * 4 + N * 4 * 2 + N * 4 * 2 + 2 * N * 4 + N * 4 +
+    return 28 * a + 2 * N * 16 * 2 + N * 8 * 2 + 256 * tiles * 4 + 256 * 4 + 4 +
+           2 * (size_t(1) << grid_table_bits(N)) * 4 + N * 4 * 2 + N * 4 * 2 + 2 * N * 4 + N * 4 +
            2 * N * 16 + N * 4 + 2 * lbvh_level_boxes(N) * 16 + P * 8 * 2 +
            (N + 1) * 4 + ((N + 1 + 2047) / 2048) * 4 + P * 4 * 2 + P * 8 + P * 4 + 2 * N * 4 +
            size_t(kPartialSlots) * kMaxPartialBlocks * 8 + sizeof(Counters);

@@ -10,17 +10,24 @@
 //   * bodies larger than the cell (or with cell coordinates out of range) go
 //     to an overflow list and are tested against every body, as the
 //     reference's overflow path does (broadphase.cpp:162-166);
-//   * home-cell keys are 3-D Morton codes of the cell coordinates wrapped to
-//     k bits per axis (table of 2^3k entries, 2^3k >= 2N): spatially close
+//   * home-cell keys are 3-D Morton codes of the cell coordinates (k bits
+//     per axis) cut to their low t bits (table of 2^t entries, 2^t >= 2N,
+//     t >= 6 so that every axis keeps at least two bits): spatially close
 //     cells are close in memory; far cells that wrap onto the same key only
 //     add AABB tests, never change the result;
 //   * of two overlapping bodies with different home cells, the one whose cell
 //     comes first in (z, y, x) order finds the pair (the other cell is at an
 //     offset after (0, 0, 0) in that order, one of the 13 queried); in a
 //     shared home cell the lower original index does; so each unordered pair
-//     appears once (the 27 neighbour keys are distinct for k >= 2, so a
+//     appears once (the 27 neighbour keys are distinct: three consecutive
+//     coordinates differ in their low two bits, which every key keeps, so a
 //     wrapped key never brings back the other half, and every body sits in
-//     exactly one cell).
+//     exactly one cell);
+//   * the records are put in key order by buckets: one stable radix pass
+//     (two past 2^23 table entries) over the key bits [low, t) groups the
+//     keys by bucket, then one workgroup per bucket counts its 2^low cells
+//     in LDS, writes their table entries (start, end) -- every entry of the
+//     table, so no clearing pass -- and places each record.
 // Cell coordinates use floor(min / c) in double precision, a monotone map.
 // Batches of worlds: every world has its own cells.  A world's keys are its
 // Morton keys XOR a per-world hash (a bijection, so one world's 27 neighbour
@@ -34,33 +41,12 @@
 #include <algorithm>
 
 #include "devmath.hpp"
+#include "gridkey.hpp"
 #include "internal.hpp"
 #include "wave.hpp"
 
 namespace pg {
 namespace {
-
-__device__ __forceinline__ uint32_t spread3(uint32_t v)
-{
-    v &= 0x3FFu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
-__device__ __forceinline__ uint32_t cell_key(int32_t x, int32_t y, int32_t z, uint32_t mask)
-{
-    return spread3(uint32_t(x) & mask) | (spread3(uint32_t(y) & mask) << 1) |
-           (spread3(uint32_t(z) & mask) << 2);
-}
-
-// Per-world key offset (0 for world 0): multiplicative hash to table bits.
-__device__ __forceinline__ uint32_t world_key(uint32_t world, uint32_t bits_per_axis)
-{
-    return world ? (world * 0x9E3779B1u) >> (32u - 3u * bits_per_axis) : 0u;
-}
 
 // Cell side for this step: explicit (> 0) or automatic from the reductions.
 __device__ __forceinline__ float step_cell(const Counters* c, float requested, uint32_t n)
@@ -75,54 +61,34 @@ __device__ __forceinline__ float step_cell(const Counters* c, float requested, u
     return cell * (1.0f + 1.0f / 256.0f);
 }
 
-// Home cell of a body; false when it must use the overflow path.
-__device__ __forceinline__ bool home_cell(float x, float y, float z, double inv, int32_t cell[3])
-{
-    const double v[3] = {floor(double(x) * inv), floor(double(y) * inv), floor(double(z) * inv)};
-    for (int a = 0; a < 3; ++a) {
-        if (!(v[a] > -1073741824.0 && v[a] < 1073741824.0))
-            return false;
-        cell[a] = int32_t(v[a]);
-    }
-    return true;
-}
-
-// Key of body i (i < n): its home cell's key, or 1 << 3k for the overflow
-// list; writes the sort item, the in-grid flag and, for overflow bodies,
-// their list entry.  Every lane of the wave calls it.
+// Key of body i (i < n): its home cell's key, or kOverflowKey for the
+// overflow list; writes the sort item, the in-grid flag and, for overflow
+// bodies, their list entry.  Every lane of the wave calls it.
 __device__ __forceinline__ uint32_t grid_key(uint32_t n, const float4* __restrict__ aabb,
                                              Counters* counters, float cell,
-                                             uint32_t bits_per_axis, uint64_t* __restrict__ items,
+                                             uint32_t bits_per_axis, uint32_t table_bits,
+                                             uint64_t* __restrict__ items,
                                              uint32_t* __restrict__ overflow,
+                                             uint32_t* __restrict__ over_ctr,
                                              uint32_t* __restrict__ flag, uint32_t i)
 {
-    bool in_grid = false;
-    uint32_t key = 1u << (3 * bits_per_axis);  // sorts after every cell key
+    uint32_t key = kOverflowKey;
     if (i < n) {
         const Rec r = load_rec(aabb, i);
-        const float limit = cell * (1.0f - 1.0f / 512.0f);
-        const bool fits = (r.b[3] - r.b[0]) <= limit && (r.b[4] - r.b[1]) <= limit &&
-                          (r.b[5] - r.b[2]) <= limit;
-        int32_t c[3], top[3];
-        const double inv = 1.0 / double(cell);
-        if (fits && home_cell(r.b[0], r.b[1], r.b[2], inv, c) &&
-            home_cell(r.b[3], r.b[4], r.b[5], inv, top)) {
-            in_grid = true;
-            key = cell_key(c[0], c[1], c[2], (1u << bits_per_axis) - 1u) ^
-                  world_key(r.world, bits_per_axis);
-        }
+        key = grid_cell_key(r.b, r.world, cell, 1.0 / double(cell), bits_per_axis, table_bits);
+        const bool in_grid = key != kOverflowKey;
         items[i] = (uint64_t)key | ((uint64_t)i << 32);
         flag[i] = in_grid ? 1u : 0u;
     }
     // Overflow list (rare: one atomic per wave that has an overflow body).
-    const bool over = i < n && !in_grid;
+    const bool over = i < n && key == kOverflowKey;
     const uint64_t om = __ballot(over);
     const uint32_t lane = lane_id();
     if (om) {
         const uint32_t leader = (uint32_t)__ffsll((unsigned long long)om) - 1u;
         uint32_t base = 0;
         if (lane == leader)
-            base = atomicAdd(&counters->overflow_count, (uint32_t)__popcll(om));
+            base = atomicAdd(over_ctr, (uint32_t)__popcll(om));
         base = __shfl(base, (int)leader);
         if (over)
             overflow[base + __popcll(om & lanes_below(lane))] = i;
@@ -131,13 +97,16 @@ __device__ __forceinline__ uint32_t grid_key(uint32_t n, const float4* __restric
 }
 
 // One workgroup per radix-sort tile (tile_items keys): the keys, and the
-// tile's histogram of the sort's first digit (key bits 0..7) in the sort's
-// layout (hist[digit * tiles + tile]), so the sort skips its first upsweep.
+// tile's histogram of the bucket pass's first digit (key bits low .. low+7,
+// in-grid bodies only) in the sort's layout (hist[digit * tiles + tile]), so
+// the sort skips its first upsweep.
 __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* __restrict__ aabb,
                                                       Counters* counters, float requested,
-                                                      uint32_t bits_per_axis,
+                                                      uint32_t bits_per_axis, uint32_t table_bits,
+                                                      uint32_t low,
                                                       uint64_t* __restrict__ items,
                                                       uint32_t* __restrict__ overflow,
+                                                      uint32_t* __restrict__ over_ctr,
                                                       uint32_t* __restrict__ flag,
                                                       uint32_t tile_items,
                                                       uint32_t* __restrict__ hist)
@@ -151,43 +120,224 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* 
     __syncthreads();
     for (uint32_t r = 0; r < tile_items; r += kBlock) {
         const uint32_t i = blockIdx.x * tile_items + r + threadIdx.x;
-        const uint32_t key = grid_key(n, aabb, counters, cell, bits_per_axis, items, overflow,
-                                      flag, i);
-        if (i < n)
-            atomicAdd(&counts[key & 0xFFu], 1u);
+        const uint32_t key = grid_key(n, aabb, counters, cell, bits_per_axis, table_bits, items,
+                                      overflow, over_ctr, flag, i);
+        if (i < n && key != kOverflowKey)
+            atomicAdd(&counts[(key >> low) & 0xFFu], 1u);
     }
     __syncthreads();
     const uint32_t tiles = (n + tile_items - 1) / tile_items;
     hist[threadIdx.x * tiles + blockIdx.x] = counts[threadIdx.x];
 }
 
-// Bodies in cells = all bodies minus the overflow ones.
-__global__ __launch_bounds__(kBlock) void k_fill(uint32_t* __restrict__ p, uint64_t count,
-                                                 uint32_t value)
+// Bucket kernel: workgroups of 1024 threads, 4 items per thread in flight.
+constexpr uint32_t kBucketThreads = 1024;
+constexpr uint32_t kBucketItems = 4;
+
+// Sum of v over the workgroup (every thread calls it; v from each thread).
+__device__ __forceinline__ uint32_t bucket_block_sum(uint32_t v, uint32_t* lds_wave)
 {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = value;
+    for (uint32_t off = 32; off > 0; off >>= 1)
+        v += (uint32_t)__shfl_xor((int)v, off);
+    const uint32_t wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63u) == 0)
+        lds_wave[wave] = v;
+    __syncthreads();
+    uint32_t all = 0;
+    for (uint32_t w = 0; w < kBucketThreads / 64; ++w)
+        all += lds_wave[w];
+    return all;
 }
 
-// Cell ranges over the sorted in-grid prefix [0, entries): table[2 key] =
-// first position, table[2 key + 1] = one past the last (0, 0 = empty).
-// In-grid bodies sort before the overflow ones (their keys carry the top bit).
-__global__ __launch_bounds__(kBlock) void k_cell_ranges(uint32_t n, const uint64_t* __restrict__ items,
-                                                        Counters* counters,
-                                                        uint32_t* __restrict__ table)
+// Exclusive prefix of v over the workgroup's threads in thread order.
+__device__ __forceinline__ uint32_t bucket_block_scan(uint32_t v, uint32_t* lds_wave)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t entries = n - counters->overflow_count;
-    if (p == 0)
-        counters->grid_entries = entries;
-    if (p >= entries)
-        return;
-    const uint32_t key = uint32_t(items[p]);
-    if (p == 0 || uint32_t(items[p - 1]) != key)
-        table[2 * size_t(key)] = p;
-    if (p + 1 == entries || uint32_t(items[p + 1]) != key)
-        table[2 * size_t(key) + 1] = p + 1;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= off)
+            incl += t;
+    }
+    __syncthreads();
+    if (lane == 63)
+        lds_wave[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w)
+        before += lds_wave[w];
+    return before + incl - v;
+}
+
+// First position p in [lo, hi) whose digit (item bits shift .. shift+7) is
+// >= target, hi when none; positions are sorted by that digit.  The whole
+// workgroup searches: each round tests up to kBucketThreads evenly spaced
+// positions and keeps the span between the last failing and the first
+// passing one, so a range of up to 2^20 positions takes two rounds.
+__device__ __forceinline__ uint32_t bucket_lower_bound(const uint64_t* __restrict__ items,
+                                                       uint32_t lo, uint32_t hi, uint32_t shift,
+                                                       uint32_t target, uint32_t* slot)
+{
+    while (hi > lo) {
+        const uint32_t span = hi - lo;
+        const uint32_t samples = min(span, kBucketThreads);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            *slot = samples;
+        __syncthreads();
+        // Sample j stands for positions (pos(j - 1), pos(j)], pos(j) =
+        // lo + floor((j + 1) span / samples) - 1 (pos(samples - 1) = hi - 1).
+        if (threadIdx.x < samples) {
+            const uint32_t pos =
+                lo + uint32_t((uint64_t(threadIdx.x + 1) * span) / samples) - 1u;
+            if (((uint32_t(items[pos]) >> shift) & 0xFFu) >= target)
+                atomicMin(slot, threadIdx.x);
+        }
+        __syncthreads();
+        const uint32_t j = *slot;
+        if (j == samples)
+            return hi;
+        const uint32_t first = lo + uint32_t((uint64_t(j) * span) / samples);   // pos(j-1) + 1
+        const uint32_t last = lo + uint32_t((uint64_t(j + 1) * span) / samples) - 1u;  // pos(j)
+        // The answer lies in [first, last] and `last` passes.
+        lo = first;
+        hi = last;
+    }
+    return lo;
+}
+
+// Records of the in-grid bodies in cell-key order, and every entry of the
+// cell table; also the overflow list's length.  Workgroup `bucket` owns the keys whose bits [low, t) equal it,
+// i.e. the 2^low cells bucket * 2^low + c, and the contiguous range [s, e) of
+// the items sorted by those bits (single pass: s = the items of all smaller
+// buckets, from the pass's digit totals; two passes: the range of the high
+// digit from the totals, then the low digit's sub-range by search).  Counting
+// sort in LDS: counts per cell, exclusive scan, the cells' table entries
+// (start, end; start = end for an empty cell), then each record is copied to
+// start + an LDS cursor (records of one cell in no particular order: the
+// query's results do not depend on it).
+template <uint32_t kLowMax>
+__global__ __launch_bounds__(kBucketThreads) void k_grid_buckets(
+    const uint64_t* __restrict__ items, uint32_t low, uint32_t two_pass,
+    const uint32_t* __restrict__ digit_total, const float4* __restrict__ aabb,
+    float4* __restrict__ sorted, uint2* __restrict__ table, Counters* counters,
+    uint32_t* __restrict__ over_ctr)
+{
+    __shared__ __align__(16) uint32_t cnt[1u << kLowMax];
+    __shared__ uint32_t lds_wave[kBucketThreads / 64];
+    __shared__ uint32_t slot;
+    const uint32_t bucket = blockIdx.x;
+    if (bucket == 0 && threadIdx.x == 0) {
+        // The key kernel's overflow list is complete: publish its length for
+        // the query and the overflow pass, and zero the slot counter for the
+        // next step.
+        counters->overflow_count = *over_ctr;
+        *over_ctr = 0;
+    }
+    const uint32_t cells = 1u << low;
+    const uint32_t hi_digit = two_pass ? bucket >> 8 : bucket;
+    // Range of the (high) digit: all items of smaller digits before it.
+    const uint32_t d = threadIdx.x < 256 ? digit_total[threadIdx.x] : 0u;
+    uint32_t s = bucket_block_sum(threadIdx.x < hi_digit ? d : 0u, lds_wave);
+    uint32_t e = s + digit_total[hi_digit];
+    if (two_pass) {
+        const uint32_t lo_digit = bucket & 0xFFu;
+        const uint32_t hs = s, he = e;
+        s = bucket_lower_bound(items, hs, he, low, lo_digit, &slot);
+        e = bucket_lower_bound(items, s, he, low, lo_digit + 1u, &slot);
+    }
+    for (uint32_t c = threadIdx.x; c < cells; c += kBucketThreads)
+        cnt[c] = 0;
+    __syncthreads();
+    const uint32_t cmask = cells - 1u;
+    // kBucketItems items per thread and chunk, loads issued before their uses.
+    for (uint32_t p0 = s; p0 < e; p0 += kBucketItems * kBucketThreads) {
+        uint32_t c[kBucketItems];
+#pragma unroll
+        for (uint32_t r = 0; r < kBucketItems; ++r) {
+            const uint32_t p = p0 + r * kBucketThreads + threadIdx.x;
+            c[r] = uint32_t(items[p < e ? p : s]) & cmask;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kBucketItems; ++r)
+            if (p0 + r * kBucketThreads + threadIdx.x < e)
+                atomicAdd(&cnt[c[r]], 1u);
+    }
+    __syncthreads();
+    // Exclusive scan of the counts: thread t takes cells [t * per, (t + 1) * per),
+    // read and written four at a time.
+    if (cells >= 4 * kBucketThreads) {
+        const uint32_t per4 = cells / (4 * kBucketThreads);
+        uint4* cnt4 = reinterpret_cast<uint4*>(cnt) + threadIdx.x * per4;
+        uint32_t sum = 0;
+        for (uint32_t k = 0; k < per4; ++k) {
+            const uint4 v = cnt4[k];
+            sum += v.x + v.y + v.z + v.w;
+        }
+        uint32_t run = bucket_block_scan(sum, lds_wave);
+        __syncthreads();
+        for (uint32_t k = 0; k < per4; ++k) {
+            const uint4 v = cnt4[k];
+            uint4 o;
+            o.x = run;
+            o.y = o.x + v.x;
+            o.z = o.y + v.y;
+            o.w = o.z + v.z;
+            run = o.w + v.w;
+            cnt4[k] = o;
+        }
+    } else {
+        const uint32_t per = cells > kBucketThreads ? cells / kBucketThreads : 1u;
+        const uint32_t c0 = threadIdx.x * per;
+        uint32_t sum = 0;
+        if (c0 < cells)
+            for (uint32_t k = 0; k < per; ++k)
+                sum += cnt[c0 + k];
+        uint32_t run = bucket_block_scan(sum, lds_wave);
+        __syncthreads();
+        if (c0 < cells)
+            for (uint32_t k = 0; k < per; ++k) {
+                const uint32_t v = cnt[c0 + k];
+                cnt[c0 + k] = run;
+                run += v;
+            }
+    }
+    __syncthreads();
+    const uint32_t key0 = bucket << low;
+    for (uint32_t c = threadIdx.x; c < cells; c += kBucketThreads) {
+        const uint32_t end = c + 1 < cells ? cnt[c + 1] : e - s;
+        table[key0 + c] = make_uint2(s + cnt[c], s + end);
+    }
+    __syncthreads();
+    for (uint32_t p0 = s; p0 < e; p0 += kBucketItems * kBucketThreads) {
+        // Unconditional loads (an empty slot reads item s and record 0, and
+        // stores nothing), so the arrays stay in registers.
+        uint64_t it[kBucketItems];
+#pragma unroll
+        for (uint32_t r = 0; r < kBucketItems; ++r) {
+            const uint32_t p = p0 + r * kBucketThreads + threadIdx.x;
+            it[r] = items[p < e ? p : s];
+        }
+        uint32_t pos[kBucketItems];
+        float4 lo[kBucketItems], hi[kBucketItems];
+#pragma unroll
+        for (uint32_t r = 0; r < kBucketItems; ++r) {
+            const uint32_t p = p0 + r * kBucketThreads + threadIdx.x;
+            const size_t idx = p < e ? size_t(it[r] >> 32) : 0;
+            lo[r] = aabb[2 * idx];
+            hi[r] = aabb[2 * idx + 1];
+            pos[r] = p < e ? s + atomicAdd(&cnt[uint32_t(it[r]) & cmask], 1u) : 0u;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kBucketItems; ++r) {
+            const uint32_t p = p0 + r * kBucketThreads + threadIdx.x;
+            if (p < e) {
+                sorted[2 * size_t(pos[r])] = lo[r];
+                sorted[2 * size_t(pos[r]) + 1] = hi[r];
+            }
+        }
+    }
 }
 
 // Candidates of a body: bodies whose home cell lies, on every axis, in
@@ -196,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_ranges(uint32_t n, const uint64
 __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4* __restrict__ s,
                                                        const uint32_t* __restrict__ table,
                                                        uint32_t bits_per_axis,
-                                                       Counters* counters,
+                                                       uint32_t table_bits, Counters* counters,
                                                        uint64_t* __restrict__ out, uint64_t cap,
                                                        uint64_t* __restrict__ tests_part)
 {
@@ -206,6 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
     const float cell = __uint_as_float(counters->cell_bits);
     const double inv = 1.0 / double(cell);
     const uint32_t mask = (1u << bits_per_axis) - 1u;
+    const uint32_t tmask = (1u << table_bits) - 1u;
     const uint2* range = reinterpret_cast<const uint2*>(table);
     unsigned long long tests = 0;
     // Grid-stride over sorted positions; all lanes stay in the loops together
@@ -223,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
             me = load_rec(s, p);
             home_cell(me.b[0], me.b[1], me.b[2], inv, c);
             home_cell(me.b[3], me.b[4], me.b[5], inv, top);
-            wk = world_key(me.world, bits_per_axis);
+            wk = world_key(me.world, table_bits);
         }
         // Half neighbourhood: of two overlapping bodies in different cells,
         // the one whose home cell comes first in (z, y, x) order finds the
@@ -244,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                 r[j] = make_uint2(0u, 0u);
                 if (active && (dz > 0 || j >= 4) && c[0] + dx <= top[0] && c[1] + dy <= top[1] &&
                     c[2] + dz <= top[2])
-                    r[j] = range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask) ^ wk];
+                    r[j] = range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask, tmask) ^ wk];
             }
             uint32_t q = 0, qe = 0, next = 0;
             for (;;) {
@@ -322,33 +473,61 @@ __global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const floa
 
 }  // namespace
 
-void launch_grid(physgpu_context* ctx, double cell_size)
+GridBits grid_bits(uint32_t n, uint32_t table_bits)
+{
+    GridBits g;
+    g.table = 6;
+    while (g.table < table_bits && (uint64_t(1) << g.table) < 2ull * n)
+        ++g.table;
+    g.axis = (g.table + 2) / 3;
+    g.bucket = g.table <= 23 ? std::min<uint32_t>(g.table, 8) : g.table - 15;
+    g.low = g.table - g.bucket;
+    return g;
+}
+
+void launch_grid(physgpu_context* ctx, double cell_size, bool fused)
 {
     const uint32_t n = ctx->n;
-    const uint32_t bpa = ctx->table_bits / 3;
-    const uint32_t table_size = 1u << ctx->table_bits;
+    const GridBits g = grid_bits(n, ctx->table_bits);
     Timer t;
-    time_begin(ctx, "grid_keys", t);
-    const uint32_t tile_items = kSortThreads * ctx->tuning.sort_items;
-    hipLaunchKernelGGL(k_grid_keys, dim3(div_up(n, tile_items)), dim3(kBlock), 0, ctx->stream, n,
-                       ctx->aabb, ctx->counters, float(cell_size), bpa, ctx->items_a,
-                       ctx->overflow_list, ctx->grid_flag, tile_items, ctx->histogram);
-    time_end(ctx, t);
-    uint64_t* sorted = nullptr;
-    radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, 0, 3 * bpa + 1, 0, 0, &sorted,
-               ctx->aabb, ctx->sorted_aabb, true);
-    time_begin(ctx, "grid_table_clear", t);
-    hipLaunchKernelGGL(k_fill, dim3(stride_grid(2ull * table_size)), dim3(kBlock), 0, ctx->stream,
-                       ctx->cell_table, 2ull * table_size, 0u);
-    time_end(ctx, t);
-    time_begin(ctx, "grid_cell_ranges", t);
-    hipLaunchKernelGGL(k_cell_ranges, dim3(div_up(n, kBlock)), dim3(kBlock), 0, ctx->stream, n,
-                       sorted, ctx->counters, ctx->cell_table);
+    if (fused) {
+        launch_aabbs_grid_keys(ctx, float(cell_size), g);
+    } else {
+        time_begin(ctx, "grid_keys", t);
+        const uint32_t tile_items = kSortThreads * ctx->tuning.sort_items;
+        hipLaunchKernelGGL(k_grid_keys, dim3(div_up(n, tile_items)), dim3(kBlock), 0,
+                           ctx->stream, n, ctx->aabb, ctx->counters, float(cell_size), g.axis,
+                           g.table, g.low, ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr,
+                           ctx->grid_flag, tile_items, ctx->histogram);
+        time_end(ctx, t);
+    }
+    // Items grouped by bucket (key bits [low, t)), overflow bodies dropped;
+    // the kept count goes to grid_entries.
+    uint64_t* grouped = nullptr;
+    radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, g.low, g.bucket, 0, 0, &grouped,
+               nullptr, nullptr, true, kOverflowKey, &ctx->counters->grid_entries);
+    time_begin(ctx, "grid_buckets", t);
+    const dim3 bg(1u << g.bucket), bb(kBucketThreads);
+    const uint32_t two = g.bucket > 8 ? 1u : 0u;
+    uint2* table = reinterpret_cast<uint2*>(ctx->cell_table);
+    if (g.low <= 10)
+        hipLaunchKernelGGL(k_grid_buckets<10>, bg, bb, 0, ctx->stream, grouped, g.low, two,
+                           ctx->digit_total, ctx->aabb, ctx->sorted_aabb, table, ctx->counters,
+                           ctx->grid_over_ctr);
+    else if (g.low <= 13)
+        hipLaunchKernelGGL(k_grid_buckets<13>, bg, bb, 0, ctx->stream, grouped, g.low, two,
+                           ctx->digit_total, ctx->aabb, ctx->sorted_aabb, table, ctx->counters,
+                           ctx->grid_over_ctr);
+    else
+        hipLaunchKernelGGL(k_grid_buckets<15>, bg, bb, 0, ctx->stream, grouped, g.low, two,
+                           ctx->digit_total, ctx->aabb, ctx->sorted_aabb, table, ctx->counters,
+                           ctx->grid_over_ctr);
     time_end(ctx, t);
     time_begin(ctx, "grid_query", t);
     hipLaunchKernelGGL(k_grid_query, dim3(ctx->tuning.query_groups),
                        dim3(ctx->tuning.query_block), 0, ctx->stream, n,
-                       ctx->sorted_aabb, ctx->cell_table, bpa, ctx->counters, ctx->pairs_raw,
+                       ctx->sorted_aabb, ctx->cell_table, g.axis, g.table, ctx->counters,
+                       ctx->pairs_raw,
                        ctx->cap_p, ctx->partials + size_t(kSlotGridTests) * kMaxPartialBlocks);
     time_end(ctx, t);
     time_begin(ctx, "grid_overflow", t);

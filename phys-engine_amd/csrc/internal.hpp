@@ -160,6 +160,9 @@ struct physgpu_context {
     float4* shape4 = nullptr;        // 2 x cap_n
     float* hull_points = nullptr;    // 3 x cap_h
     bool has_hulls = false;
+    // Largest AABB side bound of the shapes (max key, sum) and the collision
+    // step's grid cell from it (float bits in word 1), per upload.
+    uint32_t* shape_cell = nullptr;
     uint32_t* hull_list = nullptr;        // hull collider indices (cap_n), built per upload
     uint32_t* hull_list_count = nullptr;  // their count (device)
     bool colliders_valid = false;
@@ -180,6 +183,9 @@ struct physgpu_context {
     uint32_t* cell_table = nullptr;  // 2 x table_size (start, end)
     uint32_t table_bits = 0;
     uint32_t* overflow_list = nullptr;
+    // Slots taken in overflow_list by the key kernel (device; zero between
+    // steps: the bucket kernel publishes it as the overflow count and zeroes it).
+    uint32_t* grid_over_ctr = nullptr;
     uint32_t* grid_flag = nullptr;   // 1 when the body sits in a cell
 
     // LBVH: internal nodes [0, n-1), leaves by sorted position.
@@ -373,6 +379,9 @@ void launch_compute_aabbs(physgpu_context* ctx, bool with_stats = false);
 void launch_pose_aos(physgpu_context* ctx, const float* src = nullptr);
 void launch_shape_aos(physgpu_context* ctx);  // shape planes -> ctx->shape4
 void launch_hull_list(physgpu_context* ctx);
+// Grid cell of the collision step from the resident shapes (ctx->shape_cell),
+// once per upload.
+void launch_shape_cell(physgpu_context* ctx);
 void launch_aabb_reductions(physgpu_context* ctx);
 void launch_convert_aabbs_out(physgpu_context* ctx, physgpu_aabb* dev_out);
 void launch_convert_aabbs_in(physgpu_context* ctx, const physgpu_aabb* dev_in);
@@ -385,17 +394,42 @@ void launch_compose_world(physgpu_context* ctx);
 // With first_hist_done the caller has written the first pass's per-tile
 // histogram (ctx->histogram[digit * tiles + tile], tiles of
 // kSortThreads x ctx->tuning.sort_items items, n items on the host).
+// Items with a bit of `drop` set are left out by the first pass (and must
+// not be counted in a histogram the caller wrote): the result holds the
+// others, in order, at the front; their count is written to *kept (device),
+// which the later passes read as their item count.
 void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                 const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
                 uint32_t bits1, uint64_t** result, const float4* gather_src = nullptr,
-                float4* gather_dst = nullptr, bool first_hist_done = false);
+                float4* gather_dst = nullptr, bool first_hist_done = false,
+                uint64_t drop = 0, uint32_t* kept = nullptr);
 // sap.hip
 void launch_sap_keys(physgpu_context* ctx);
 void launch_gather_sorted(physgpu_context* ctx, const uint64_t* sorted_items);
 void launch_sap_sweep(physgpu_context* ctx);
 void launch_sap_window_tests(physgpu_context* ctx);
 // grid.hip
-void launch_grid(physgpu_context* ctx, double cell_size);
+// Key bits of the grid for n bodies in a context whose table has 2^table_bits
+// entries: t table bits (2^t >= 2n, 6 <= t <= table_bits), Morton bits per
+// axis, bucket bits (radix-sorted) and the low bits counted per bucket.
+struct GridBits {
+    uint32_t table, axis, bucket, low;
+};
+// Table bits of a context for up to `bodies` colliders: 2^t >= 2 bodies,
+// 6 <= t <= 27.
+constexpr uint32_t grid_table_bits(uint64_t bodies)
+{
+    uint32_t t = 6;
+    while (t < 27 && (uint64_t(1) << t) < 2 * bodies)
+        ++t;
+    return t;
+}
+GridBits grid_bits(uint32_t n, uint32_t table_bits);
+// With fused, the collision step's AABB refresh is part of the grid's key
+// pass (launch_aabbs_grid_keys, cell from ctx->shape_cell unless cell_size > 0).
+void launch_grid(physgpu_context* ctx, double cell_size, bool fused = false);
+// aabb.hip: AABB records of the resident colliders with the grid's keys.
+void launch_aabbs_grid_keys(physgpu_context* ctx, float cell, const GridBits& g);
 // gridstats.hip: the reference grid's counters (cell size, entries,
 // comparisons, overflow AABBs) of the resident AABBs, per world of a batch,
 // summed; runs on the main stream and waits for it.

@@ -715,7 +715,6 @@ __device__ int epa_wave(EpaShared& S, const Bodies& B, const Shape& sa, const po
 // multiplicity past 2^32 - 1 (-1); every other run gives the reference's
 // result.
 constexpr uint32_t kContFaces = PHYSGPU_EPA_MAX_DISTINCT_FACES;
-constexpr uint32_t kContChunks = kContFaces / 64;
 
 // Hand-over record (32-bit words): header, vertices (SoA), faces (SoA).
 enum : uint32_t {

@@ -33,7 +33,7 @@ __device__ __forceinline__ uint32_t item_count(uint32_t n, const uint32_t* dev_c
 template <uint32_t kItems>
 __global__ __launch_bounds__(kSortThreads) void k_upsweep(const uint64_t* __restrict__ items,
                                                           uint32_t n, const uint32_t* dev_count,
-                                                          uint32_t shift,
+                                                          uint32_t shift, uint64_t drop,
                                                           uint32_t* __restrict__ hist)
 {
     constexpr uint32_t kSortTile = kSortThreads * kItems;
@@ -49,8 +49,11 @@ __global__ __launch_bounds__(kSortThreads) void k_upsweep(const uint64_t* __rest
 #pragma unroll
     for (uint32_t r = 0; r < kItems; ++r) {
         uint32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < count)
-            atomicAdd(&counts[digit_of(items[i], shift)], 1u);
+        if (i < count) {
+            const uint64_t it = items[i];
+            if (!(it & drop))
+                atomicAdd(&counts[digit_of(it, shift)], 1u);
+        }
     }
     __syncthreads();
     hist[threadIdx.x * tiles + tile] = counts[threadIdx.x];
@@ -114,11 +117,12 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
                                                             uint64_t* __restrict__ out,
                                                             uint32_t n,
                                                             const uint32_t* dev_count,
-                                                            uint32_t shift,
+                                                            uint32_t shift, uint64_t drop,
                                                             const uint32_t* __restrict__ hist,
                                                             const uint32_t* __restrict__ digit_total,
                                                             const float4* __restrict__ gsrc,
-                                                            float4* __restrict__ gdst)
+                                                            float4* __restrict__ gdst,
+                                                            uint32_t* __restrict__ kept_out)
 {
     constexpr uint32_t kSortTile = kSortThreads * kItems;
     constexpr uint32_t kWaveItems = kSortTile / kWaves;  // contiguous sub-tile per wave
@@ -126,6 +130,7 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
     __shared__ uint32_t wave_count[kWaves][256];
     __shared__ uint32_t tile_start[256];
     __shared__ uint32_t global_start[256];
+    __shared__ uint32_t kept;
 
     const uint32_t count = item_count(n, dev_count);
     const uint32_t tiles = (count + kSortTile - 1) / kSortTile;
@@ -145,6 +150,8 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
         uint32_t all;
         const uint32_t digit_base = block_exclusive_scan(digit_total[threadIdx.x], &all);
         global_start[threadIdx.x] = digit_base + hist[threadIdx.x * tiles + tile];
+        if (kept_out && tile == 0 && threadIdx.x == 0)
+            *kept_out = all;  // items of the pass that were not dropped
     }
     __syncthreads();
 
@@ -156,8 +163,8 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
 #pragma unroll
     for (uint32_t r = 0; r < kItems; ++r) {
         const uint32_t local = wave * kWaveItems + r * 64u + lane;
-        const bool ok = local < valid;
-        item[r] = ok ? in[base + local] : 0ull;
+        item[r] = local < valid ? in[base + local] : 0ull;
+        const bool ok = local < valid && !(item[r] & drop);
         const uint32_t d = digit_of(item[r], shift);
         dig[r] = d;
         uint64_t peers = __ballot(ok);
@@ -209,13 +216,15 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
             tile_start[lane * 4 + k] = run;
             run += v[k];
         }
+        if (lane == 63)
+            kept = run;  // items of the tile that are not dropped
     }
     __syncthreads();
 
 #pragma unroll
     for (uint32_t r = 0; r < kItems; ++r) {
         const uint32_t local = wave * kWaveItems + r * 64u + lane;
-        if (local < valid) {
+        if (local < valid && !(item[r] & drop)) {
             const uint32_t d = dig[r];
             tile_items[tile_start[d] + wave_count[wave][d] + rank[r]] = item[r];
         }
@@ -227,7 +236,7 @@ __global__ __launch_bounds__(kSortThreads) void k_downsweep(const uint64_t* __re
 #pragma unroll
     for (uint32_t r = 0; r < kItems; ++r) {
         const uint32_t s = r * kSortThreads + threadIdx.x;
-        if (s < valid) {
+        if (s < kept) {
             const uint64_t it = tile_items[s];
             const uint32_t d = digit_of(it, shift);
             const uint32_t pos = global_start[d] + (s - tile_start[d]);
@@ -246,7 +255,8 @@ template <uint32_t kItems>
 void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                       const uint32_t* device_count, uint32_t shift0, uint32_t bits0,
                       uint32_t shift1, uint32_t bits1, uint64_t** result,
-                      const float4* gather_src, float4* gather_dst, bool first_hist_done)
+                      const float4* gather_src, float4* gather_dst, bool first_hist_done,
+                      uint64_t drop, uint32_t* kept)
 {
     const uint32_t tiles = div_up(n, kSortThreads * kItems);
     const uint32_t passes = (bits0 + 7) / 8 + (bits1 + 7) / 8;
@@ -256,21 +266,27 @@ void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n
     auto pass = [&](uint32_t shift) {
         const bool first = done == 0;
         const bool last = ++done == passes;
+        const uint64_t pass_drop = first ? drop : 0ull;
+        // After a pass that dropped items, the rest count only the kept ones.
+        const uint32_t* count_in = first || !kept ? device_count : kept;
         Timer t;
         if (!(first && first_hist_done)) {
             time_begin(ctx, "radix_upsweep", t);
             hipLaunchKernelGGL(k_upsweep<kItems>, dim3(tiles), dim3(kSortThreads), 0,
-                               ctx->stream, src, n, device_count, shift, ctx->histogram);
+                               ctx->stream, src, n, count_in, shift, pass_drop,
+                               ctx->histogram);
             time_end(ctx, t);
         }
         time_begin(ctx, "radix_scan", t);
         hipLaunchKernelGGL(k_scan_rows<kItems>, dim3(256), dim3(kSortThreads), 0, ctx->stream,
-                           ctx->histogram, n, device_count, ctx->digit_total);
+                           ctx->histogram, n, count_in, ctx->digit_total);
         time_end(ctx, t);
         time_begin(ctx, "radix_downsweep", t);
         hipLaunchKernelGGL(k_downsweep<kItems>, dim3(tiles), dim3(kSortThreads), 0, ctx->stream,
-                           src, dst, n, device_count, shift, ctx->histogram, ctx->digit_total,
-                           last ? gather_src : nullptr, last ? gather_dst : nullptr);
+                           src, dst, n, count_in, shift, pass_drop, ctx->histogram,
+                           ctx->digit_total,
+                           last ? gather_src : nullptr, last ? gather_dst : nullptr,
+                           first ? kept : nullptr);
         time_end(ctx, t);
         uint64_t* tmp = src;
         src = dst;
@@ -290,22 +306,22 @@ void radix_sort_tiled(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n
 void radix_sort(physgpu_context* ctx, uint64_t* a, uint64_t* b, uint32_t n,
                 const uint32_t* device_count, uint32_t shift0, uint32_t bits0, uint32_t shift1,
                 uint32_t bits1, uint64_t** result, const float4* gather_src, float4* gather_dst,
-                bool first_hist_done)
+                bool first_hist_done, uint64_t drop, uint32_t* kept)
 {
     // Tile = 256 threads x sort_items (4, 8 or 16); the histogram is sized for
     // the smallest tile.
     switch (ctx->tuning.sort_items) {
     case 4:
         radix_sort_tiled<4>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
-                                gather_src, gather_dst, first_hist_done);
+                                gather_src, gather_dst, first_hist_done, drop, kept);
         break;
     case 16:
         radix_sort_tiled<16>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
-                                gather_src, gather_dst, first_hist_done);
+                                gather_src, gather_dst, first_hist_done, drop, kept);
         break;
     default:
         radix_sort_tiled<8>(ctx, a, b, n, device_count, shift0, bits0, shift1, bits1, result,
-                                gather_src, gather_dst, first_hist_done);
+                                gather_src, gather_dst, first_hist_done, drop, kept);
         break;
     }
 }
