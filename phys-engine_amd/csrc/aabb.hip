@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kKeyThreads) void k_aabbs_grid_keys(
     float4* __restrict__ aabb, Counters* counters, const uint32_t* __restrict__ shape_cell,
     float requested, uint32_t bits_per_axis, uint32_t table_bits, uint32_t low,
     uint64_t* __restrict__ items, uint32_t* __restrict__ overflow,
-    uint32_t* __restrict__ over_ctr, uint32_t* __restrict__ flag, uint32_t* __restrict__ hist)
+    uint32_t* __restrict__ over_ctr, uint32_t* __restrict__ hist)
 {
     constexpr uint32_t kTile = kSortThreads * kItems;
     static_assert(kTile % kKeyThreads == 0, "whole rounds per tile");
@@ -400,7 +400,6 @@ __global__ __launch_bounds__(kKeyThreads) void k_aabbs_grid_keys(
     auto emit = [&](uint32_t i, const float b[6], uint32_t world) {
         const uint32_t key = grid_cell_key(b, world, cell, inv, bits_per_axis, table_bits);
         items[i] = uint64_t(key) | (uint64_t(i) << 32);
-        flag[i] = key != kOverflowKey ? 1u : 0u;
         if (key == kOverflowKey)
             overflow[atomicAdd(over_ctr, 1u)] = i;
         else
@@ -676,8 +675,7 @@ void launch_aabbs_grid_keys(physgpu_context* ctx, float cell, const GridBits& g)
     hipLaunchKernelGGL(k_aabbs_grid_keys<K>, grid, block, 0, ctx->stream, ctx->n, ctx->pose4,   \
                        ctx->shape4, ctx->hull_points, ctx->n_hull_points, world_of, ctx->aabb, \
                        ctx->counters, ctx->shape_cell + 1, cell, g.axis, g.table, g.low,       \
-                       ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr, ctx->grid_flag,   \
-                       ctx->histogram)
+                       ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr, ctx->histogram)
     if (ctx->tuning.sort_items == 4)
         PG_AABB_KEYS(4);
     else if (ctx->tuning.sort_items == 16)

@@ -105,7 +105,7 @@ void free_all(physgpu_context* c)
                     c->aabb, c->sorted_aabb, c->items_a, c->items_b, c->hull_list, c->shape_cell,
                     c->hull_list_count,
                     c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_over_ctr,
-                    c->grid_flag, c->node_left,
+                    c->node_left,
                     c->node_right, c->node_parent, c->node_first, c->node_last, c->node_box, c->level_box,
                     c->pairs_raw, c->pairs_tmp, c->seg_off, c->seg_tiles, c->pair_rank,
                     c->pair_perm, c->sort_scratch, c->sort_scratch_perm,
@@ -253,8 +253,11 @@ physgpu_status resolve_tuning(const physgpu_tuning& in, physgpu_tuning& out)
 // With grid_fused (uniform grid, n >= 2) the AABB refresh of the resident
 // colliders is part of the grid's key pass and nothing else needs the width
 // statistics.
+// With finish_later the pair count is clamped to the capacity by the narrow
+// phase that follows (its classification), not by a launch of its own here.
 void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool window_stats,
-                    bool defer_sort = false, bool stats_done = false, bool grid_fused = false)
+                    bool defer_sort = false, bool stats_done = false, bool grid_fused = false,
+                    bool finish_later = false)
 {
     ctx->tree_build_current = false;
     ctx->tree_pairs_loaded = false;
@@ -277,7 +280,7 @@ void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool windo
             launch_lbvh(ctx);
         }
     }
-    if (ctx->n < 2 || ctx->finish_pending)
+    if ((ctx->n < 2 || ctx->finish_pending) && !finish_later)
         launch_finish_pairs(ctx);
     ctx->finish_pending = false;
     if (!defer_sort)
@@ -377,7 +380,7 @@ void enqueue_step(physgpu_context* ctx, int algorithm, double cell)
     // it), then the narrow phase.
     Timer bp, np;
     time_begin(ctx, kStageBroadphase, bp);
-    run_broadphase(ctx, algorithm, cell, false, true, true, fused);
+    run_broadphase(ctx, algorithm, cell, false, true, true, fused, true);
     time_end(ctx, bp);
     ctx->np_pairs = ctx->pairs_raw;
     ctx->np_perm = ctx->pair_perm;
@@ -497,7 +500,6 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->cell_table, 2 * (size_t(1) << c->table_bits)));
     A(alloc(c->overflow_list, N));
     A(alloc(c->grid_over_ctr, 1));
-    A(alloc(c->grid_flag, N));
     A(alloc(c->node_left, N));
     A(alloc(c->node_right, N));
     A(alloc(c->node_parent, 2 * N));
@@ -540,6 +542,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
         A(hipMemsetAsync(c->epa_epoch, 0, 4, c->stream));
     if (e == hipSuccess)
         A(hipMemsetAsync(c->grid_over_ctr, 0, 4, c->stream));
+
     A(alloc(c->counters, 1));
     A(hipHostMalloc(reinterpret_cast<void**>(&c->host_counters), sizeof(Counters)));
     if (e != hipSuccess) {
@@ -1742,7 +1745,6 @@ size_t carve_noalloc(physgpu_context& c, char* base)
     take(c.cell_table, 2 * (size_t(1) << noalloc_table_bits()));
     take(c.overflow_list, N);
     take(c.grid_over_ctr, 1);
-    take(c.grid_flag, N);
     take(c.node_left, N);
     take(c.node_right, N);
     take(c.node_parent, 2 * N);
@@ -1770,7 +1772,7 @@ constexpr size_t noalloc_bytes()
 {
     const size_t N = kNoAllocBodies, P = kNoAllocPairs, a = kScratchAlign;
     const size_t tiles = (std::max(N, size_t(P)) + kSortMinTile - 1) / kSortMinTile;
-    return 28 * a + 2 * N * 16 * 2 + N * 8 * 2 + 256 * tiles * 4 + 256 * 4 + 4 +
+    return 27 * a + 2 * N * 16 * 2 + N * 8 * 2 + 256 * tiles * 4 + 256 * 4 + 4 - N * 4 +
            2 * (size_t(1) << grid_table_bits(N)) * 4 + N * 4 * 2 + N * 4 * 2 + 2 * N * 4 + N * 4 +
            2 * N * 16 + N * 4 + 2 * lbvh_level_boxes(N) * 16 + P * 8 * 2 +
            (N + 1) * 4 + ((N + 1 + 2047) / 2048) * 4 + P * 4 * 2 + P * 8 + P * 4 + 2 * N * 4 +

@@ -62,23 +62,20 @@ __device__ __forceinline__ float step_cell(const Counters* c, float requested, u
 }
 
 // Key of body i (i < n): its home cell's key, or kOverflowKey for the
-// overflow list; writes the sort item, the in-grid flag and, for overflow
-// bodies, their list entry.  Every lane of the wave calls it.
+// overflow list; writes the sort item and, for overflow bodies, their list
+// entry.  Every lane of the wave calls it.
 __device__ __forceinline__ uint32_t grid_key(uint32_t n, const float4* __restrict__ aabb,
                                              Counters* counters, float cell,
                                              uint32_t bits_per_axis, uint32_t table_bits,
                                              uint64_t* __restrict__ items,
                                              uint32_t* __restrict__ overflow,
-                                             uint32_t* __restrict__ over_ctr,
-                                             uint32_t* __restrict__ flag, uint32_t i)
+                                             uint32_t* __restrict__ over_ctr, uint32_t i)
 {
     uint32_t key = kOverflowKey;
     if (i < n) {
         const Rec r = load_rec(aabb, i);
         key = grid_cell_key(r.b, r.world, cell, 1.0 / double(cell), bits_per_axis, table_bits);
-        const bool in_grid = key != kOverflowKey;
         items[i] = (uint64_t)key | ((uint64_t)i << 32);
-        flag[i] = in_grid ? 1u : 0u;
     }
     // Overflow list (rare: one atomic per wave that has an overflow body).
     const bool over = i < n && key == kOverflowKey;
@@ -107,7 +104,6 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* 
                                                       uint64_t* __restrict__ items,
                                                       uint32_t* __restrict__ overflow,
                                                       uint32_t* __restrict__ over_ctr,
-                                                      uint32_t* __restrict__ flag,
                                                       uint32_t tile_items,
                                                       uint32_t* __restrict__ hist)
 {
@@ -121,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* 
     for (uint32_t r = 0; r < tile_items; r += kBlock) {
         const uint32_t i = blockIdx.x * tile_items + r + threadIdx.x;
         const uint32_t key = grid_key(n, aabb, counters, cell, bits_per_axis, table_bits, items,
-                                      overflow, over_ctr, flag, i);
+                                      overflow, over_ctr, i);
         if (i < n && key != kOverflowKey)
             atomicAdd(&counts[(key >> low) & 0xFFu], 1u);
     }
@@ -343,132 +339,191 @@ __global__ __launch_bounds__(kBucketThreads) void k_grid_buckets(
 // Candidates of a body: bodies whose home cell lies, on every axis, in
 // [home - 1, floor(max / cell)] (a partner's min is below this body's max and
 // less than one cell below its min), i.e. 2 or 3 cells per axis.
+//
+// One wavefront takes 64 bodies (consecutive sorted positions) at a time and
+// walks their candidates together: per z-slab each lane loads its up to nine
+// cell ranges, a wave scan numbers all (body, candidate) tasks of the slab,
+// and the tasks are tested 64 per round, one per lane: each round the owners
+// write their tasks of the round (candidate position, owner lane, home-cell
+// flag) into LDS and every lane reads one, loads that candidate's record and
+// tests it against its owner's record (LDS).  The wave waits for
+// ceil(tasks / 64) dependent record loads per slab instead of the largest
+// candidate count of its lanes.  The neighbour keys are assembled from the
+// spread coordinates of the three x, three y and two z cells.
+//
+// Bodies on the overflow path (rare) are tested here too, as the reference's
+// overflow pass does (broadphase.cpp:162-166): every in-grid body against
+// each of them, and each pair of them once (lower index first).
+// Overflow bodies, run by the workgroups past the query's own (the query
+// launch carries kOverflowGroups more): (o, j) for every in-grid body j, and
+// (o, o') for two overflow bodies with o < o'.  Workgroup b of them takes
+// overflow bodies b, b + kOverflowGroups, ...
+constexpr uint32_t kOverflowGroups = 64;
+
+__device__ __forceinline__ void grid_overflow_role(uint32_t entries, uint32_t over, uint32_t b,
+                                                const float4* __restrict__ s,
+                                                const float4* __restrict__ aabb,
+                                                const uint32_t* __restrict__ overflow,
+                                                Counters* counters, WaveQueue& queue,
+                                                uint64_t* __restrict__ out, uint64_t cap)
+{
+    for (uint32_t k = b; k < over; k += kOverflowGroups) {
+        const uint32_t o = overflow[k];
+        const Rec ro = load_rec(aabb, o);
+        for (uint32_t p0 = 0; p0 < entries; p0 += blockDim.x) {
+            const uint32_t p = p0 + threadIdx.x;
+            bool hit = false;
+            uint32_t j = 0;
+            if (p < entries) {
+                const Rec rj = load_rec(s, p);
+                j = rj.index;
+                hit = rj.world == ro.world && rec_overlap(ro.b, rj.b);
+            }
+            wq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap);
+        }
+        for (uint32_t k0 = 0; k0 < over; k0 += blockDim.x) {
+            const uint32_t k2 = k0 + threadIdx.x;
+            bool hit = false;
+            uint32_t j = 0;
+            if (k2 < over) {
+                j = overflow[k2];
+                if (o < j) {
+                    const Rec rj = load_rec(aabb, j);
+                    hit = rj.world == ro.world && rec_overlap(ro.b, rj.b);
+                }
+            }
+            wq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4* __restrict__ s,
                                                        const uint32_t* __restrict__ table,
                                                        uint32_t bits_per_axis,
                                                        uint32_t table_bits, Counters* counters,
                                                        uint64_t* __restrict__ out, uint64_t cap,
-                                                       uint64_t* __restrict__ tests_part)
+                                                       const float4* __restrict__ aabb,
+                                                       const uint32_t* __restrict__ overflow)
 {
     __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
-    WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
+    __shared__ float4 own_lds[kBlock / 64][2][64];
+    __shared__ uint32_t task_lds[kBlock / 64][64];
+    __shared__ uint8_t owner_lds[kBlock / 64][64];
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    WaveQueue queue{queue_lds[wave], 0};
     const uint32_t entries = n - counters->overflow_count;
+    const uint32_t groups = gridDim.x - kOverflowGroups;  // the query's own
+    if (blockIdx.x >= groups) {
+        grid_overflow_role(entries, counters->overflow_count, blockIdx.x - groups, s, aabb,
+                           overflow, counters, queue, out, cap);
+        wq_flush_block(queue, out, &counters->pair_count, cap);
+        return;
+    }
+    float4* own = &own_lds[wave][0][0];
+    uint32_t* task = task_lds[wave];
+    uint8_t* task_owner = owner_lds[wave];
     const float cell = __uint_as_float(counters->cell_bits);
     const double inv = 1.0 / double(cell);
     const uint32_t mask = (1u << bits_per_axis) - 1u;
     const uint32_t tmask = (1u << table_bits) - 1u;
     const uint2* range = reinterpret_cast<const uint2*>(table);
-    unsigned long long tests = 0;
     // Grid-stride over sorted positions; all lanes stay in the loops together
-    // so that the wave-wide append is executed convergently.
-    const uint32_t stride = gridDim.x * blockDim.x;
+    // so that the wave-wide operations are executed convergently.
+    const uint32_t stride = groups * blockDim.x;
     const uint32_t rounds = (entries + stride - 1) / stride;
-    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t block = xcd_block(blockIdx.x, groups);
     for (uint32_t round = 0; round < rounds; ++round) {
         const uint32_t p = round * stride + block * blockDim.x + threadIdx.x;
         const bool active = p < entries;
-        Rec me{{0, 0, 0, 0, 0, 0}, 0, 0};
         int32_t c[3] = {0, 0, 0}, top[3] = {0, 0, 0};  // both in range for in-grid bodies
         uint32_t wk = 0;
         if (active) {
-            me = load_rec(s, p);
-            home_cell(me.b[0], me.b[1], me.b[2], inv, c);
-            home_cell(me.b[3], me.b[4], me.b[5], inv, top);
-            wk = world_key(me.world, table_bits);
+            const float4 r0 = s[2 * size_t(p)], r1 = s[2 * size_t(p) + 1];
+            own[lane] = r0;
+            own[64 + lane] = r1;
+            home_cell(r0.x, r0.y, r0.z, inv, c);
+            home_cell(r0.w, r1.x, r1.y, inv, top);
+            wk = world_key(__float_as_uint(r1.w), table_bits);
         }
+        // Spread coordinates of the neighbour cells: x and y at -1, 0, +1,
+        // z at 0, +1 (a cell's key is (sx | sy | sz) & tmask ^ wk).
+        const uint32_t sx[3] = {spread3(uint32_t(c[0] - 1) & mask), spread3(uint32_t(c[0]) & mask),
+                                spread3(uint32_t(c[0] + 1) & mask)};
+        const uint32_t sy[3] = {spread3(uint32_t(c[1] - 1) & mask) << 1,
+                                spread3(uint32_t(c[1]) & mask) << 1,
+                                spread3(uint32_t(c[1] + 1) & mask) << 1};
+        const uint32_t sz[2] = {spread3(uint32_t(c[2]) & mask) << 2,
+                                spread3(uint32_t(c[2] + 1) & mask) << 2};
         // Half neighbourhood: of two overlapping bodies in different cells,
         // the one whose home cell comes first in (z, y, x) order finds the
         // pair (the other's cell is at offset (dx, dy, dz) > (0, 0, 0) in
         // that order, and within its [home - 1, top] range); in the home
         // cell itself the lower index does.  So the slab dz = -1 and the
         // first four cells of the slab dz = 0 are never read.
-        // One z-slab of 3 x 3 neighbour cells at a time: the range loads are
-        // issued together, then each lane walks the candidates of the slab's
-        // cells in one loop, so the wave waits for one table round trip and
-        // max(candidates) record loads per slab instead of one round trip and
-        // one loop per cell.
         for (int dz = 0; dz <= 1; ++dz) {
+            const bool zok = active && c[2] + dz <= top[2];
             uint2 r[9];
+            uint32_t cnt = 0;
 #pragma unroll
             for (int j = 0; j < 9; ++j) {
                 const int dx = j % 3 - 1, dy = j / 3 - 1;
                 r[j] = make_uint2(0u, 0u);
-                if (active && (dz > 0 || j >= 4) && c[0] + dx <= top[0] && c[1] + dy <= top[1] &&
-                    c[2] + dz <= top[2])
-                    r[j] = range[cell_key(c[0] + dx, c[1] + dy, c[2] + dz, mask, tmask) ^ wk];
+                if (zok && (dz > 0 || j >= 4) && c[0] + dx <= top[0] && c[1] + dy <= top[1])
+                    r[j] = range[((sx[dx + 1] | sy[dy + 1] | sz[dz]) & tmask) ^ wk];
             }
-            uint32_t q = 0, qe = 0, next = 0;
-            for (;;) {
-                if (q >= qe && next < 9) {
-                    // Next non-empty cell of the slab (static indices keep r in registers).
-                    uint32_t nq = 0, nqe = 0, nn = 9;
 #pragma unroll
-                    for (int j = 8; j >= 0; --j)
-                        if (uint32_t(j) >= next && r[j].y > r[j].x) {
-                            nq = r[j].x;
-                            nqe = r[j].y;
-                            nn = j + 1;
-                        }
-                    q = nq;
-                    qe = nqe;
-                    next = nn;
-                }
-                const bool more = q < qe;
-                if (__ballot(more) == 0)
-                    break;
-                bool hit = false;
-                uint32_t b = 0;
-                if (more) {
-                    const Rec o = load_rec(s, q);
-                    b = o.index;
-                    const bool home = dz == 0 && next == 5;  // the walk is in cell j = 4
-                    if ((!home || me.index < b) && me.world == o.world) {
-                        ++tests;
-                        hit = rec_overlap(me.b, o.b);
+            for (int j = 0; j < 9; ++j) {
+                r[j].y = r[j].y > r[j].x ? r[j].y - r[j].x : 0u;  // now (start, length)
+                cnt += r[j].y;
+            }
+            // Exclusive prefix of the lanes' task counts, and their total.
+            uint32_t incl = cnt;
+#pragma unroll
+            for (uint32_t off = 1; off < 64; off <<= 1) {
+                const uint32_t t = (uint32_t)__shfl_up((int)incl, off);
+                if (lane >= off)
+                    incl += t;
+            }
+            const uint32_t first = incl - cnt;
+            const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+#pragma unroll 1
+            for (uint32_t base = 0; base < total; base += 64) {
+                // Owners write their tasks in [base, base + 64).
+                uint32_t so = first;
+#pragma unroll
+                for (int j = 0; j < 9; ++j) {
+                    const uint32_t lo = max(so, base), hi = min(so + r[j].y, base + 64u);
+                    const uint8_t tag = uint8_t(lane | ((dz == 0 && j == 4) ? 64u : 0u));
+#pragma unroll 1
+                    for (uint32_t t = lo; t < hi; ++t) {
+                        task[t - base] = r[j].x + (t - so);
+                        task_owner[t - base] = tag;
                     }
-                    ++q;
+                    so += r[j].y;
                 }
-                wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count, cap);
+                wave_sync();
+                bool hit = false;
+                uint32_t a = 0, b = 0;
+                if (base + lane < total) {
+                    const uint32_t q = task[lane], w = task_owner[lane];
+                    const uint32_t owner = w & 63u;
+                    const bool home = (w >> 6) & 1u;
+                    const float4 m0 = own[owner], m1 = own[64 + owner];
+                    const Rec o = load_rec(s, q);
+                    a = __float_as_uint(m1.z);
+                    b = o.index;
+                    if ((!home || a < b) && __float_as_uint(m1.w) == o.world) {
+                        const float mb[6] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y};
+                        hit = rec_overlap(mb, o.b);
+                    }
+                }
+                wave_sync();  // the next round rewrites the tasks
+                wq_push(queue, hit, pair_value(a, b), out, &counters->pair_count, cap);
             }
         }
+        wave_sync();  // the next bodies rewrite the owner records
     }
     wq_flush_block(queue, out, &counters->pair_count, cap);
-    block_partial(tests, tests_part);
-}
-
-// Overflow bodies against every body (broadphase.cpp:163-166): pair (o, j)
-// is tested when j != o and (j is in the grid or o < j).
-__global__ __launch_bounds__(kBlock) void k_grid_overflow(uint32_t n, const float4* __restrict__ aabb,
-                                                          const uint32_t* __restrict__ overflow,
-                                                          const uint32_t* __restrict__ flag,
-                                                          Counters* counters,
-                                                          uint64_t* __restrict__ out, uint64_t cap,
-                                                          uint64_t* __restrict__ tests_part)
-{
-    __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
-    WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
-    const uint32_t count = counters->overflow_count;
-    unsigned long long tests = 0;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t rounds = (n + stride - 1) / stride;
-    for (uint32_t k = blockIdx.y; k < count; k += gridDim.y) {
-        const uint32_t o = overflow[k];
-        const Rec ro = load_rec(aabb, o);
-        for (uint32_t round = 0; round < rounds; ++round) {
-            const uint32_t j = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
-            bool hit = false;
-            if (j < n && j != o && (flag[j] || o < j)) {
-                const Rec rj = load_rec(aabb, j);
-                if (rj.world == ro.world) {
-                    ++tests;
-                    hit = rec_overlap(ro.b, rj.b);
-                }
-            }
-            wq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap);
-        }
-    }
-    wq_flush_block(queue, out, &counters->pair_count, cap);
-    block_partial(tests, tests_part + blockIdx.y * gridDim.x);
 }
 
 }  // namespace
@@ -498,7 +553,7 @@ void launch_grid(physgpu_context* ctx, double cell_size, bool fused)
         hipLaunchKernelGGL(k_grid_keys, dim3(div_up(n, tile_items)), dim3(kBlock), 0,
                            ctx->stream, n, ctx->aabb, ctx->counters, float(cell_size), g.axis,
                            g.table, g.low, ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr,
-                           ctx->grid_flag, tile_items, ctx->histogram);
+                           tile_items, ctx->histogram);
         time_end(ctx, t);
     }
     // Items grouped by bucket (key bits [low, t)), overflow bodies dropped;
@@ -524,26 +579,11 @@ void launch_grid(physgpu_context* ctx, double cell_size, bool fused)
                            ctx->grid_over_ctr);
     time_end(ctx, t);
     time_begin(ctx, "grid_query", t);
-    hipLaunchKernelGGL(k_grid_query, dim3(ctx->tuning.query_groups),
+    hipLaunchKernelGGL(k_grid_query, dim3(ctx->tuning.query_groups + kOverflowGroups),
                        dim3(ctx->tuning.query_block), 0, ctx->stream, n,
                        ctx->sorted_aabb, ctx->cell_table, g.axis, g.table, ctx->counters,
-                       ctx->pairs_raw,
-                       ctx->cap_p, ctx->partials + size_t(kSlotGridTests) * kMaxPartialBlocks);
+                       ctx->pairs_raw, ctx->cap_p, ctx->aabb, ctx->overflow_list);
     time_end(ctx, t);
-    time_begin(ctx, "grid_overflow", t);
-    // Overflow bodies are rare; a small grid keeps the idle launch cheap.
-    const dim3 og(std::min<uint32_t>(stride_grid(n), 512), 8);
-    hipLaunchKernelGGL(k_grid_overflow, og, dim3(kBlock), 0, ctx->stream, n, ctx->aabb,
-                       ctx->overflow_list, ctx->grid_flag, ctx->counters, ctx->pairs_raw,
-                       ctx->cap_p, ctx->partials + size_t(kSlotGridOverflowTests) * kMaxPartialBlocks);
-    time_end(ctx, t);
-    FoldList fold{};
-    fold.e[0] = {kSlotGridTests, ctx->tuning.query_groups, kFoldSum64,
-                 uint32_t(offsetof(Counters, grid_tests))};
-    fold.e[1] = {kSlotGridOverflowTests, og.x * og.y, kFoldSum64,
-                 uint32_t(offsetof(Counters, grid_overflow_tests))};
-    fold.count = 2;
-    launch_fold(ctx, fold);
 }
 
 }  // namespace pg

@@ -186,7 +186,6 @@ struct physgpu_context {
     // Slots taken in overflow_list by the key kernel (device; zero between
     // steps: the bucket kernel publishes it as the overflow count and zeroes it).
     uint32_t* grid_over_ctr = nullptr;
-    uint32_t* grid_flag = nullptr;   // 1 when the body sits in a cell
 
     // LBVH: internal nodes [0, n-1), leaves by sorted position.
     int32_t* node_left = nullptr;

@@ -1079,7 +1079,13 @@ __global__ __launch_bounds__(kClassifyThreads) void k_classify(Bodies B, World W
     constexpr uint32_t kWavesPerBlock = kClassifyThreads / 64;
     __shared__ uint32_t wave_count[kWavesPerBlock][kClasses];
     __shared__ uint32_t block_base[kClasses];
-    const uint32_t count = c->sort_count;
+    // Pairs held: the broad phase's count clamped to the capacity (written
+    // here for the kernels after the narrow phase; the canonical sort on the
+    // side stream clamps it itself).
+    const unsigned long long emitted = c->pair_count;
+    const uint32_t count = uint32_t(emitted < cap ? emitted : cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        c->sort_count = count;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (count + stride - 1) / stride;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;

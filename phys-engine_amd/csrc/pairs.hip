@@ -50,12 +50,21 @@ __global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t* __restrict__ p, u
         p[i] = 0;
 }
 
+// Pairs held: the emitted count clamped to the capacity (sort_count may be
+// written by a kernel that runs concurrently on the main stream).
+__device__ __forceinline__ uint32_t pairs_held(const Counters* c, uint64_t cap)
+{
+    const unsigned long long p = c->pair_count;
+    return uint32_t(p < cap ? p : cap);
+}
+
 __global__ __launch_bounds__(kBlock) void k_pair_rank(const uint64_t* __restrict__ pairs,
                                                       const Counters* __restrict__ c,
+                                                      uint64_t cap,
                                                       uint32_t* __restrict__ seg,
                                                       uint32_t* __restrict__ rank)
 {
-    const uint32_t count = c->sort_count;
+    const uint32_t count = pairs_held(c, cap);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < count;
          p += gridDim.x * blockDim.x)
         rank[p] = atomicAdd(&seg[uint32_t(pairs[p] >> 32)], 1u);
@@ -160,12 +169,13 @@ __global__ __launch_bounds__(256) void k_tile_apply(uint32_t* __restrict__ v, ui
 
 __global__ __launch_bounds__(kBlock) void k_pair_scatter(const uint64_t* __restrict__ pairs,
                                                          const Counters* __restrict__ c,
+                                                         uint64_t cap,
                                                          const uint32_t* __restrict__ seg,
                                                          const uint32_t* __restrict__ rank,
                                                          uint64_t* __restrict__ out,
                                                          uint32_t* __restrict__ perm)
 {
-    const uint32_t count = c->sort_count;
+    const uint32_t count = pairs_held(c, cap);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < count;
          p += gridDim.x * blockDim.x) {
         const uint64_t v = pairs[p];
@@ -459,14 +469,14 @@ void launch_sort_pairs(physgpu_context* ctx, hipStream_t stream)
     hipLaunchKernelGGL(k_zero_u32, dim3(stride_grid(segs)), dim3(kBlock), 0, stream,
                        ctx->seg_off, uint64_t(segs));
     hipLaunchKernelGGL(k_pair_rank, dim3(pgrid), dim3(kBlock), 0, stream, ctx->pairs_raw,
-                       ctx->counters, ctx->seg_off, ctx->pair_rank);
+                       ctx->counters, ctx->cap_p, ctx->seg_off, ctx->pair_rank);
     time_end(ctx, t);
     time_begin(ctx, "pair_scan", t, stream);
     scan_u32(ctx, ctx->seg_off, segs, stream);
     time_end(ctx, t);
     time_begin(ctx, "pair_scatter", t, stream);
     hipLaunchKernelGGL(k_pair_scatter, dim3(pgrid), dim3(kBlock), 0, stream, ctx->pairs_raw,
-                       ctx->counters, ctx->seg_off, ctx->pair_rank, ctx->pairs_tmp,
+                       ctx->counters, ctx->cap_p, ctx->seg_off, ctx->pair_rank, ctx->pairs_tmp,
                        ctx->pair_perm);
     time_end(ctx, t);
     time_begin(ctx, "pair_segment_sort", t, stream);
