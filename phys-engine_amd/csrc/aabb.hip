@@ -379,9 +379,14 @@ __global__ __launch_bounds__(kKeyThreads) void k_aabbs_grid_keys(
     float4* __restrict__ aabb, Counters* counters, const uint32_t* __restrict__ shape_cell,
     float requested, uint32_t bits_per_axis, uint32_t table_bits, uint32_t low,
     uint64_t* __restrict__ items, uint32_t* __restrict__ overflow,
-    uint32_t* __restrict__ over_ctr, uint32_t* __restrict__ hist)
+    uint32_t* __restrict__ over_ctr, uint32_t* __restrict__ hist, uint32_t* __restrict__ bounds,
+    uint32_t bound_words)
 {
     constexpr uint32_t kTile = kSortThreads * kItems;
+    // Bucket ranges of a two-pass sort start empty.
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < bound_words;
+         i += gridDim.x * blockDim.x)
+        bounds[i] = 0;
     static_assert(kTile % kKeyThreads == 0, "whole rounds per tile");
     __shared__ uint32_t counts[256];
     __shared__ uint32_t hull_ids[kTile];
@@ -675,7 +680,8 @@ void launch_aabbs_grid_keys(physgpu_context* ctx, float cell, const GridBits& g)
     hipLaunchKernelGGL(k_aabbs_grid_keys<K>, grid, block, 0, ctx->stream, ctx->n, ctx->pose4,   \
                        ctx->shape4, ctx->hull_points, ctx->n_hull_points, world_of, ctx->aabb, \
                        ctx->counters, ctx->shape_cell + 1, cell, g.axis, g.table, g.low,       \
-                       ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr, ctx->histogram)
+                       ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr, ctx->histogram,   \
+                       ctx->bucket_bounds, g.bucket > 8 ? 2u << g.bucket : 0u)
     if (ctx->tuning.sort_items == 4)
         PG_AABB_KEYS(4);
     else if (ctx->tuning.sort_items == 16)

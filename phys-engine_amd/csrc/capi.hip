@@ -105,7 +105,7 @@ void free_all(physgpu_context* c)
                     c->aabb, c->sorted_aabb, c->items_a, c->items_b, c->hull_list, c->shape_cell,
                     c->hull_list_count,
                     c->histogram, c->digit_total, c->cell_table, c->overflow_list, c->grid_over_ctr,
-                    c->node_left,
+                    c->bucket_bounds, c->node_left,
                     c->node_right, c->node_parent, c->node_first, c->node_last, c->node_box, c->level_box,
                     c->pairs_raw, c->pairs_tmp, c->seg_off, c->seg_tiles, c->pair_rank,
                     c->pair_perm, c->sort_scratch, c->sort_scratch_perm,
@@ -500,6 +500,7 @@ physgpu_status physgpu_create(physgpu_context** out, int device, uint32_t max_co
     A(alloc(c->cell_table, 2 * (size_t(1) << c->table_bits)));
     A(alloc(c->overflow_list, N));
     A(alloc(c->grid_over_ctr, 1));
+    A(alloc(c->bucket_bounds, 2 * kMaxGridBuckets));
     A(alloc(c->node_left, N));
     A(alloc(c->node_right, N));
     A(alloc(c->node_parent, 2 * N));
@@ -1745,6 +1746,7 @@ size_t carve_noalloc(physgpu_context& c, char* base)
     take(c.cell_table, 2 * (size_t(1) << noalloc_table_bits()));
     take(c.overflow_list, N);
     take(c.grid_over_ctr, 1);
+    take(c.bucket_bounds, 2 * kMaxGridBuckets);
     take(c.node_left, N);
     take(c.node_right, N);
     take(c.node_parent, 2 * N);
@@ -1772,7 +1774,8 @@ constexpr size_t noalloc_bytes()
 {
     const size_t N = kNoAllocBodies, P = kNoAllocPairs, a = kScratchAlign;
     const size_t tiles = (std::max(N, size_t(P)) + kSortMinTile - 1) / kSortMinTile;
-    return 27 * a + 2 * N * 16 * 2 + N * 8 * 2 + 256 * tiles * 4 + 256 * 4 + 4 - N * 4 +
+    return 28 * a + 2 * N * 16 * 2 + N * 8 * 2 + 256 * tiles * 4 + 256 * 4 + 4 - N * 4 +
+           2 * kMaxGridBuckets * 4 +
            2 * (size_t(1) << grid_table_bits(N)) * 4 + N * 4 * 2 + N * 4 * 2 + 2 * N * 4 + N * 4 +
            2 * N * 16 + N * 4 + 2 * lbvh_level_boxes(N) * 16 + P * 8 * 2 +
            (N + 1) * 4 + ((N + 1 + 2047) / 2048) * 4 + P * 4 * 2 + P * 8 + P * 4 + 2 * N * 4 +

@@ -10,24 +10,23 @@
 //   * bodies larger than the cell (or with cell coordinates out of range) go
 //     to an overflow list and are tested against every body, as the
 //     reference's overflow path does (broadphase.cpp:162-166);
-//   * home-cell keys are 3-D Morton codes of the cell coordinates (k bits
-//     per axis) cut to their low t bits (table of 2^t entries, 2^t >= 2N,
-//     t >= 6 so that every axis keeps at least two bits): spatially close
-//     cells are close in memory; far cells that wrap onto the same key only
-//     add AABB tests, never change the result;
+//   * home-cell keys are 3-D Morton codes of the cell coordinates wrapped to
+//     k bits per axis (table of 2^3k entries, 2^3k >= 2N, k >= 2): spatially
+//     close cells are close in memory; far cells that wrap onto the same key
+//     only add AABB tests, never change the result;
 //   * of two overlapping bodies with different home cells, the one whose cell
 //     comes first in (z, y, x) order finds the pair (the other cell is at an
 //     offset after (0, 0, 0) in that order, one of the 13 queried); in a
 //     shared home cell the lower original index does; so each unordered pair
-//     appears once (the 27 neighbour keys are distinct: three consecutive
-//     coordinates differ in their low two bits, which every key keeps, so a
-//     wrapped key never brings back the other half, and every body sits in
-//     exactly one cell);
+//     appears once (the 27 neighbour keys are distinct for k >= 2: three
+//     consecutive coordinates differ in their low two bits, so a wrapped key
+//     never brings back the other half, and every body sits in exactly one
+//     cell);
 //   * the records are put in key order by buckets: one stable radix pass
-//     (two past 2^23 table entries) over the key bits [low, t) groups the
+//     (two past 2^21 table entries) over the key bits [low, 3k) groups the
 //     keys by bucket, then one workgroup per bucket counts its 2^low cells
-//     in LDS, writes their table entries (start, end) -- every entry of the
-//     table, so no clearing pass -- and places each record.
+//     (low <= 13) in LDS, writes their table entries (start, end) -- every
+//     entry of the table, so no clearing pass -- and places each record.
 // Cell coordinates use floor(min / c) in double precision, a monotone map.
 // Batches of worlds: every world has its own cells.  A world's keys are its
 // Morton keys XOR a per-world hash (a bijection, so one world's 27 neighbour
@@ -105,8 +104,14 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(uint32_t n, const float4* 
                                                       uint32_t* __restrict__ overflow,
                                                       uint32_t* __restrict__ over_ctr,
                                                       uint32_t tile_items,
-                                                      uint32_t* __restrict__ hist)
+                                                      uint32_t* __restrict__ hist,
+                                                      uint32_t* __restrict__ bounds,
+                                                      uint32_t bound_words)
 {
+    // Bucket ranges of a two-pass sort start empty.
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < bound_words;
+         i += gridDim.x * blockDim.x)
+        bounds[i] = 0;
     __shared__ uint32_t counts[256];
     static_assert(kBlock == 256, "one histogram bin per thread");
     const float cell = step_cell(counters, requested, n);
@@ -166,41 +171,24 @@ __device__ __forceinline__ uint32_t bucket_block_scan(uint32_t v, uint32_t* lds_
     return before + incl - v;
 }
 
-// First position p in [lo, hi) whose digit (item bits shift .. shift+7) is
-// >= target, hi when none; positions are sorted by that digit.  The whole
-// workgroup searches: each round tests up to kBucketThreads evenly spaced
-// positions and keeps the span between the last failing and the first
-// passing one, so a range of up to 2^20 positions takes two rounds.
-__device__ __forceinline__ uint32_t bucket_lower_bound(const uint64_t* __restrict__ items,
-                                                       uint32_t lo, uint32_t hi, uint32_t shift,
-                                                       uint32_t target, uint32_t* slot)
+// Bucket ranges after two radix passes (buckets of more than 8 bits): the
+// items are sorted by bucket, so bucket b occupies [first p with bucket b,
+// one past the last); each position that starts or ends a run writes it.
+// Buckets without items keep the zeros written by the key pass.
+__global__ __launch_bounds__(kBlock) void k_bucket_bounds(const uint64_t* __restrict__ items,
+                                                          const Counters* __restrict__ counters,
+                                                          uint32_t low,
+                                                          uint32_t* __restrict__ bounds)
 {
-    while (hi > lo) {
-        const uint32_t span = hi - lo;
-        const uint32_t samples = min(span, kBucketThreads);
-        __syncthreads();
-        if (threadIdx.x == 0)
-            *slot = samples;
-        __syncthreads();
-        // Sample j stands for positions (pos(j - 1), pos(j)], pos(j) =
-        // lo + floor((j + 1) span / samples) - 1 (pos(samples - 1) = hi - 1).
-        if (threadIdx.x < samples) {
-            const uint32_t pos =
-                lo + uint32_t((uint64_t(threadIdx.x + 1) * span) / samples) - 1u;
-            if (((uint32_t(items[pos]) >> shift) & 0xFFu) >= target)
-                atomicMin(slot, threadIdx.x);
-        }
-        __syncthreads();
-        const uint32_t j = *slot;
-        if (j == samples)
-            return hi;
-        const uint32_t first = lo + uint32_t((uint64_t(j) * span) / samples);   // pos(j-1) + 1
-        const uint32_t last = lo + uint32_t((uint64_t(j + 1) * span) / samples) - 1u;  // pos(j)
-        // The answer lies in [first, last] and `last` passes.
-        lo = first;
-        hi = last;
+    const uint32_t entries = counters->grid_entries;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < entries;
+         p += gridDim.x * blockDim.x) {
+        const uint32_t b = uint32_t(items[p]) >> low;
+        if (p == 0 || (uint32_t(items[p - 1]) >> low) != b)
+            bounds[2 * b] = p;
+        if (p + 1 == entries || (uint32_t(items[p + 1]) >> low) != b)
+            bounds[2 * b + 1] = p + 1;
     }
-    return lo;
 }
 
 // Records of the in-grid bodies in cell-key order, and every entry of the
@@ -215,14 +203,13 @@ __device__ __forceinline__ uint32_t bucket_lower_bound(const uint64_t* __restric
 // query's results do not depend on it).
 template <uint32_t kLowMax>
 __global__ __launch_bounds__(kBucketThreads) void k_grid_buckets(
-    const uint64_t* __restrict__ items, uint32_t low, uint32_t two_pass,
+    const uint64_t* __restrict__ items, uint32_t low, const uint32_t* __restrict__ bounds,
     const uint32_t* __restrict__ digit_total, const float4* __restrict__ aabb,
     float4* __restrict__ sorted, uint2* __restrict__ table, Counters* counters,
     uint32_t* __restrict__ over_ctr)
 {
     __shared__ __align__(16) uint32_t cnt[1u << kLowMax];
     __shared__ uint32_t lds_wave[kBucketThreads / 64];
-    __shared__ uint32_t slot;
     const uint32_t bucket = blockIdx.x;
     if (bucket == 0 && threadIdx.x == 0) {
         // The key kernel's overflow list is complete: publish its length for
@@ -232,16 +219,16 @@ __global__ __launch_bounds__(kBucketThreads) void k_grid_buckets(
         *over_ctr = 0;
     }
     const uint32_t cells = 1u << low;
-    const uint32_t hi_digit = two_pass ? bucket >> 8 : bucket;
-    // Range of the (high) digit: all items of smaller digits before it.
-    const uint32_t d = threadIdx.x < 256 ? digit_total[threadIdx.x] : 0u;
-    uint32_t s = bucket_block_sum(threadIdx.x < hi_digit ? d : 0u, lds_wave);
-    uint32_t e = s + digit_total[hi_digit];
-    if (two_pass) {
-        const uint32_t lo_digit = bucket & 0xFFu;
-        const uint32_t hs = s, he = e;
-        s = bucket_lower_bound(items, hs, he, low, lo_digit, &slot);
-        e = bucket_lower_bound(items, s, he, low, lo_digit + 1u, &slot);
+    uint32_t s, e;
+    if (bounds) {
+        // Two passes: the range found by k_bucket_bounds (empty: 0, 0).
+        s = bounds[2 * bucket];
+        e = bounds[2 * bucket + 1];
+    } else {
+        // One pass: all items of smaller digits come before the bucket.
+        const uint32_t d = threadIdx.x < 256 ? digit_total[threadIdx.x] : 0u;
+        s = bucket_block_sum(threadIdx.x < bucket ? d : 0u, lds_wave);
+        e = s + digit_total[bucket];
     }
     for (uint32_t c = threadIdx.x; c < cells; c += kBucketThreads)
         cnt[c] = 0;
@@ -306,32 +293,37 @@ __global__ __launch_bounds__(kBucketThreads) void k_grid_buckets(
         table[key0 + c] = make_uint2(s + cnt[c], s + end);
     }
     __syncthreads();
-    for (uint32_t p0 = s; p0 < e; p0 += kBucketItems * kBucketThreads) {
+    // Records are moved by lane pairs: both lanes of a pair take the same
+    // item, the even one claims its position, and each moves one 16-byte half
+    // of the record (one 32-byte access per record and instruction pair).
+    const uint32_t q = threadIdx.x >> 1, half = threadIdx.x & 1u;
+    constexpr uint32_t kPairs = kBucketThreads / 2;
+    for (uint32_t p0 = s; p0 < e; p0 += 2 * kBucketItems * kPairs) {
         // Unconditional loads (an empty slot reads item s and record 0, and
         // stores nothing), so the arrays stay in registers.
-        uint64_t it[kBucketItems];
+        uint64_t it[2 * kBucketItems];
 #pragma unroll
-        for (uint32_t r = 0; r < kBucketItems; ++r) {
-            const uint32_t p = p0 + r * kBucketThreads + threadIdx.x;
+        for (uint32_t r = 0; r < 2 * kBucketItems; ++r) {
+            const uint32_t p = p0 + r * kPairs + q;
             it[r] = items[p < e ? p : s];
         }
-        uint32_t pos[kBucketItems];
-        float4 lo[kBucketItems], hi[kBucketItems];
+        uint32_t pos[2 * kBucketItems];
+        float4 rec[2 * kBucketItems];
 #pragma unroll
-        for (uint32_t r = 0; r < kBucketItems; ++r) {
-            const uint32_t p = p0 + r * kBucketThreads + threadIdx.x;
+        for (uint32_t r = 0; r < 2 * kBucketItems; ++r) {
+            const uint32_t p = p0 + r * kPairs + q;
             const size_t idx = p < e ? size_t(it[r] >> 32) : 0;
-            lo[r] = aabb[2 * idx];
-            hi[r] = aabb[2 * idx + 1];
-            pos[r] = p < e ? s + atomicAdd(&cnt[uint32_t(it[r]) & cmask], 1u) : 0u;
+            rec[r] = aabb[2 * idx + half];
+            uint32_t mine = 0;
+            if (p < e && half == 0)
+                mine = s + atomicAdd(&cnt[uint32_t(it[r]) & cmask], 1u);
+            pos[r] = mine + (uint32_t)__shfl_xor((int)mine, 1);
         }
 #pragma unroll
-        for (uint32_t r = 0; r < kBucketItems; ++r) {
-            const uint32_t p = p0 + r * kBucketThreads + threadIdx.x;
-            if (p < e) {
-                sorted[2 * size_t(pos[r])] = lo[r];
-                sorted[2 * size_t(pos[r]) + 1] = hi[r];
-            }
+        for (uint32_t r = 0; r < 2 * kBucketItems; ++r) {
+            const uint32_t p = p0 + r * kPairs + q;
+            if (p < e)
+                sorted[2 * size_t(pos[r]) + half] = rec[r];
         }
     }
 }
@@ -531,14 +523,16 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
 GridBits grid_bits(uint32_t n, uint32_t table_bits)
 {
     GridBits g;
-    g.table = 6;
-    while (g.table < table_bits && (uint64_t(1) << g.table) < 2ull * n)
-        ++g.table;
-    g.axis = (g.table + 2) / 3;
-    g.bucket = g.table <= 23 ? std::min<uint32_t>(g.table, 8) : g.table - 15;
-    g.low = g.table - g.bucket;
+    g.table = std::min(grid_table_bits(n), table_bits);
+    g.axis = g.table / 3;
+    // A few thousand cells per bucket: up to 2^21 entries 8 bucket bits (one
+    // pass), beyond one more cell bit per axis step (2^24: 14 + 10, 2^27:
+    // 15 + 12, two passes).
+    g.low = g.table <= 21 ? (g.table > 8 ? g.table - 8 : 0u) : std::min(13 + (g.table - 21) / 3, 15u);
+    g.bucket = g.table - g.low;
     return g;
 }
+static_assert(grid_table_bits(uint64_t(1) << 30) - 15 <= 12, "bucket bits within kMaxGridBuckets");
 
 void launch_grid(physgpu_context* ctx, double cell_size, bool fused)
 {
@@ -553,7 +547,8 @@ void launch_grid(physgpu_context* ctx, double cell_size, bool fused)
         hipLaunchKernelGGL(k_grid_keys, dim3(div_up(n, tile_items)), dim3(kBlock), 0,
                            ctx->stream, n, ctx->aabb, ctx->counters, float(cell_size), g.axis,
                            g.table, g.low, ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr,
-                           tile_items, ctx->histogram);
+                           tile_items, ctx->histogram, ctx->bucket_bounds,
+                           g.bucket > 8 ? 2u << g.bucket : 0u);
         time_end(ctx, t);
     }
     // Items grouped by bucket (key bits [low, t)), overflow bodies dropped;
@@ -562,21 +557,27 @@ void launch_grid(physgpu_context* ctx, double cell_size, bool fused)
     radix_sort(ctx, ctx->items_a, ctx->items_b, n, nullptr, g.low, g.bucket, 0, 0, &grouped,
                nullptr, nullptr, true, kOverflowKey, &ctx->counters->grid_entries);
     time_begin(ctx, "grid_buckets", t);
+    const uint32_t* bounds = nullptr;
+    if (g.bucket > 8) {
+        hipLaunchKernelGGL(k_bucket_bounds, dim3(stride_grid(n)), dim3(kBlock), 0, ctx->stream,
+                           grouped, ctx->counters, g.low, ctx->bucket_bounds);
+        bounds = ctx->bucket_bounds;
+    }
     const dim3 bg(1u << g.bucket), bb(kBucketThreads);
-    const uint32_t two = g.bucket > 8 ? 1u : 0u;
     uint2* table = reinterpret_cast<uint2*>(ctx->cell_table);
+#define PG_BUCKETS(L)                                                                         \
+    hipLaunchKernelGGL(k_grid_buckets<L>, bg, bb, 0, ctx->stream, grouped, g.low, bounds,      \
+                       ctx->digit_total, ctx->aabb, ctx->sorted_aabb, table, ctx->counters,   \
+                       ctx->grid_over_ctr)
     if (g.low <= 10)
-        hipLaunchKernelGGL(k_grid_buckets<10>, bg, bb, 0, ctx->stream, grouped, g.low, two,
-                           ctx->digit_total, ctx->aabb, ctx->sorted_aabb, table, ctx->counters,
-                           ctx->grid_over_ctr);
+        PG_BUCKETS(10);
     else if (g.low <= 13)
-        hipLaunchKernelGGL(k_grid_buckets<13>, bg, bb, 0, ctx->stream, grouped, g.low, two,
-                           ctx->digit_total, ctx->aabb, ctx->sorted_aabb, table, ctx->counters,
-                           ctx->grid_over_ctr);
+        PG_BUCKETS(13);
+    else if (g.low <= 14)
+        PG_BUCKETS(14);
     else
-        hipLaunchKernelGGL(k_grid_buckets<15>, bg, bb, 0, ctx->stream, grouped, g.low, two,
-                           ctx->digit_total, ctx->aabb, ctx->sorted_aabb, table, ctx->counters,
-                           ctx->grid_over_ctr);
+        PG_BUCKETS(15);
+#undef PG_BUCKETS
     time_end(ctx, t);
     time_begin(ctx, "grid_query", t);
     hipLaunchKernelGGL(k_grid_query, dim3(ctx->tuning.query_groups + kOverflowGroups),

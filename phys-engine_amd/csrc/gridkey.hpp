@@ -18,7 +18,7 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v)
 }
 
 // Key of a cell: its Morton code over the low k bits of each coordinate
-// (mask = 2^k - 1), cut to the table's t bits (tmask = 2^t - 1).
+// (mask = 2^k - 1), within the table's t bits (tmask = 2^t - 1).
 __device__ __forceinline__ uint32_t cell_key(int32_t x, int32_t y, int32_t z, uint32_t mask,
                                              uint32_t tmask)
 {

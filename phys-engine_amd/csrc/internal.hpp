@@ -186,6 +186,9 @@ struct physgpu_context {
     // Slots taken in overflow_list by the key kernel (device; zero between
     // steps: the bucket kernel publishes it as the overflow count and zeroes it).
     uint32_t* grid_over_ctr = nullptr;
+    // Per-bucket (start, end) after two radix passes (2 x kMaxGridBuckets),
+    // zeroed by the key pass of each grid step.
+    uint32_t* bucket_bounds = nullptr;
 
     // LBVH: internal nodes [0, n-1), leaves by sorted position.
     int32_t* node_left = nullptr;
@@ -409,23 +412,23 @@ void launch_sap_sweep(physgpu_context* ctx);
 void launch_sap_window_tests(physgpu_context* ctx);
 // grid.hip
 // Key bits of the grid for n bodies in a context whose table has 2^table_bits
-// entries: t table bits (2^t >= 2n, 6 <= t <= table_bits), Morton bits per
-// axis, bucket bits (radix-sorted) and the low bits counted per bucket.
+// entries: t table bits (3k with 2^3k >= 2n, k >= 2, t <= table_bits), Morton
+// bits per axis k, bucket bits (radix-sorted, one pass up to 8, two up to
+// 16) and the low bits counted per bucket (at most 13).
 struct GridBits {
     uint32_t table, axis, bucket, low;
 };
-// Table bits of a context for up to `bodies` colliders: 2^t >= 2 bodies,
-// 6 <= t <= 27.
+// Table bits of a context for up to `bodies` colliders: 3k with 2^3k >= 2
+// bodies, 2 <= k <= 9.
 constexpr uint32_t grid_table_bits(uint64_t bodies)
 {
-    uint32_t t = 6;
-    while (t < 27 && (uint64_t(1) << t) < 2 * bodies)
-        ++t;
-    return t;
+    uint32_t k = 2;
+    while (k < 9 && (uint64_t(1) << (3 * k)) < 2 * bodies)
+        ++k;
+    return 3 * k;
 }
 GridBits grid_bits(uint32_t n, uint32_t table_bits);
-// With fused, the collision step's AABB refresh is part of the grid's key
-// pass (launch_aabbs_grid_keys, cell from ctx->shape_cell unless cell_size > 0).
+constexpr uint32_t kMaxGridBuckets = 1u << 12;  // 27 table bits, 15 counted per bucket
 void launch_grid(physgpu_context* ctx, double cell_size, bool fused = false);
 // aabb.hip: AABB records of the resident colliders with the grid's keys.
 void launch_aabbs_grid_keys(physgpu_context* ctx, float cell, const GridBits& g);

@@ -2169,21 +2169,21 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
     const uint32_t grid = kStrideGrid;
     const physgpu_tuning& tu = ctx->tuning;
     Timer t;
-    // The canonical pair order is built on the side stream meanwhile; the
-    // compaction reads the results through its permutation.
-    if (sort_on_side) {
+    // The canonical pair order is built on the side stream meanwhile (from
+    // the broad phase's pairs); the compaction reads the results through its
+    // permutation.  The classification is enqueued first, so that a captured
+    // step keeps it on the broad phase's queue, and the sort after the SS /
+    // SB / BB kernels, which it would otherwise delay.
+    if (sort_on_side)
         hipEventRecord(ctx->sort_event, ctx->stream);
-        hipStreamWaitEvent(ctx->side_stream, ctx->sort_event, 0);
-        launch_sort_pairs(ctx, ctx->side_stream);
-    }
     time_begin(ctx, "np_classify", t);
     hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0, ctx->stream, B,
                        W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit,
                        ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks,
                        ctx->epa_epoch);
     time_end(ctx, t);
-    // With hulls, GJK + EPA run here while the SS / SB / BB kernels run on the
-    // side stream.
+    // With hulls, GJK + EPA run here while the SS / SB / BB kernels and then
+    // the canonical sort run on the side stream.
     const bool fork = ctx->has_hulls;
     hipStream_t cs = ctx->stream;
     if (fork) {
@@ -2234,6 +2234,10 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
                        W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, opt, ctx->slots,
                        ctx->hit, ctx->partials + size_t(kSlotContacts + 2) * kMaxPartialBlocks);
     time_end(ctx, t);
+    if (sort_on_side) {
+        hipStreamWaitEvent(ctx->side_stream, ctx->sort_event, 0);
+        launch_sort_pairs(ctx, ctx->side_stream);
+    }
 
     if (fork || sort_on_side) {
         hipEventRecord(ctx->join_event, ctx->side_stream);
