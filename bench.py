@@ -37,8 +37,9 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 # with its commands is profiles/grid_step_mixed1M_hbm_traffic.md.  Valid for
 # the 1,048,576-body mixed scene only.
 MEASURED_TRAFFIC_1M = {
-    "grid_query": 58.75e6,
-    "aabb_from_pose": 152.53e6,
+    "grid_query": 58.40e6,
+    "grid_buckets": 185.53e6,
+    "aabb_grid_keys": 160.42e6,
 }
 
 
@@ -206,10 +207,6 @@ def fp64_error(scene, pairs, man):
     return out
 
 
-# Narrow-phase kernels on the step's main stream (the hand-over consumer,
-# np_epa_early, runs beside np_gjk_epa on a stream of its own).
-NP_KERNELS = ("np_classify", "np_gjk_epa", "np_epa_continue", "np_sphere_sphere",
-              "np_sphere_box", "np_box_box", "np_compact")
 CLASS_NAMES = ("sphere_sphere", "sphere_box", "box_box", "sphere_hull", "box_hull", "hull_hull")
 
 
@@ -251,12 +248,13 @@ def roofline_narrowphase(per_class, kernels):
     """FP32 VALU roofline of the narrow phase: the dominant kernel (GJK + EPA
     of the hull pairs; its flops include the iterations continued past the LDS
     bounds by the consumer kernels) and the whole narrow phase (every class
-    over the summed time of its main-stream kernels), against MI355X's FP32
-    vector peak."""
+    over the narrow-phase stage of the step, classification to compaction,
+    whose SS / SB / BB kernels and canonical sort overlap GJK + EPA on other
+    streams), against MI355X's FP32 vector peak."""
     hull = sum(v["flops"] for k, v in per_class.items() if k.endswith("hull"))
     allf = sum(v["flops"] for v in per_class.values())
     us_gjk = kernels.get("np_gjk_epa")
-    us_np = sum(kernels.get(k, 0.0) for k in NP_KERNELS)
+    us_np = kernels.get("stage_narrowphase", 0.0)
     peak = VALU_FP32_PEAK_TFLOPS
 
     def rate(fl, us):
@@ -511,11 +509,11 @@ def main():
     assert hip.hipMalloc(C.byref(pose_dev), C.c_size_t(scene.pose.nbytes)) == 0
     assert hip.hipMemcpy(pose_dev, scene.pose.ctypes.data_as(C.c_void_p),
                          C.c_size_t(scene.pose.nbytes), 1) == 0  # host to device
-    # Grid cell table: 2^t entries, the smallest t >= 6 with 2^t >= 2 n.
-    t = 6
-    while t < 27 and (1 << t) < 2 * n:
-        t += 1
-    table_entries = 1 << t
+    # Grid cell table: 2^3k entries, the smallest k >= 2 with 2^3k >= 2 n.
+    k = 2
+    while k < 9 and (1 << (3 * k)) < 2 * n:
+        k += 1
+    table_entries = 1 << (3 * k)
 
     per = {}
     for name in args.algorithms.split(","):

@@ -299,7 +299,7 @@ physgpu_status physgpu_compute_aabbs(physgpu_context* ctx, physgpu_aabb* out);
  * `stats` (may be NULL) is overwritten.  Unlike the reference, the stateless
  * call accepts PHYSGPU_DYNAMIC_TREE (an LBVH is built for the call).
  * `grid_cell_size` <= 0 selects the automatic cell (the largest AABB side,
- * widened by 1/1024). */
+ * capped at 8x the mean side, widened by 1/256). */
 physgpu_status physgpu_find_candidate_pairs(physgpu_context* ctx, const physgpu_aabb* bounds,
                                             uint32_t n, int algorithm, double grid_cell_size,
                                             physgpu_pair* out, uint64_t capacity, uint64_t* count,
@@ -317,7 +317,10 @@ physgpu_status physgpu_generate_contacts(physgpu_context* ctx, const physgpu_pai
 
 /* The collision section of PhysicsWorld::step, device resident: AABBs,
  * broad phase, narrow phase.  Results stay on the device (read them with the
- * download calls); the two counts are returned. */
+ * download calls); the two counts are returned.  `grid_cell_size` <= 0
+ * selects the automatic cell, here from the shapes (the largest side any
+ * collider's AABB can have at any orientation, capped at 8x the mean,
+ * widened by 1/256); the pair set does not depend on the cell. */
 physgpu_status physgpu_collide(physgpu_context* ctx, int algorithm, double grid_cell_size,
                                const physgpu_narrowphase_options* options,
                                uint64_t* pair_count, uint64_t* manifold_count);
