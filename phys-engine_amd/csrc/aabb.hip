@@ -380,9 +380,17 @@ __global__ __launch_bounds__(kKeyThreads) void k_aabbs_grid_keys(
     float requested, uint32_t bits_per_axis, uint32_t table_bits, uint32_t low,
     uint64_t* __restrict__ items, uint32_t* __restrict__ overflow,
     uint32_t* __restrict__ over_ctr, uint32_t* __restrict__ hist, uint32_t* __restrict__ bounds,
-    uint32_t bound_words)
+    uint32_t bound_words, uint64_t* __restrict__ epa_partials, uint32_t* __restrict__ epa_epoch)
 {
     constexpr uint32_t kTile = kSortThreads * kItems;
+    // The narrow phase of this step: a new epoch for its EPA hand-over flags,
+    // and its EPA partial slots zeroed (the work of its classification, which
+    // the grid query does).
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *epa_epoch += 1u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kEpaPartialSlots * kEpaPartials;
+         i += gridDim.x * blockDim.x)
+        epa_partials[(i / kEpaPartials) * kMaxPartialBlocks + i % kEpaPartials] = 0;
     // Bucket ranges of a two-pass sort start empty.
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < bound_words;
          i += gridDim.x * blockDim.x)
@@ -681,7 +689,8 @@ void launch_aabbs_grid_keys(physgpu_context* ctx, float cell, const GridBits& g)
                        ctx->shape4, ctx->hull_points, ctx->n_hull_points, world_of, ctx->aabb, \
                        ctx->counters, ctx->shape_cell + 1, cell, g.axis, g.table, g.low,       \
                        ctx->items_a, ctx->overflow_list, ctx->grid_over_ctr, ctx->histogram,   \
-                       ctx->bucket_bounds, g.bucket > 8 ? 2u << g.bucket : 0u)
+                       ctx->bucket_bounds, g.bucket > 8 ? 2u << g.bucket : 0u,                  \
+                       ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks, ctx->epa_epoch)
     if (ctx->tuning.sort_items == 4)
         PG_AABB_KEYS(4);
     else if (ctx->tuning.sort_items == 16)

@@ -261,6 +261,7 @@ void run_broadphase(physgpu_context* ctx, int algorithm, double cell, bool windo
 {
     ctx->tree_build_current = false;
     ctx->tree_pairs_loaded = false;
+    ctx->np_classified = false;
     if (!stats_done && !grid_fused)
         launch_aabb_reductions(ctx);
     // The broad phase's closing fold (if any) also finishes the pair count.

@@ -328,6 +328,186 @@ __global__ __launch_bounds__(kBucketThreads) void k_grid_buckets(
     }
 }
 
+// ---- Pair emission with the narrow phase's classification (collision step)
+// The pairs a workgroup emits get their slots (emission indices) in
+// pairs_raw as before, and, when `Classify::buckets` is set, also the work
+// the narrow phase's classification does (physics_world.cpp:630-633, 670):
+// hit[slot] = 0, and slot appended to the bucket of its shape-pair class
+// unless both colliders belong to one body.  Counts are combined per
+// wavefront flush (one atomic per class) and per workgroup at the end.
+struct Classify {
+    uint32_t* buckets;            // kClasses x cap candidate indices (null: plain emission)
+    uint32_t* hit;
+    uint32_t* class_count;        // Counters::class_count
+    const uint8_t* type;          // collider shape types
+    const uint32_t* body;         // collider bodies (null: one body per collider)
+};
+
+__device__ __forceinline__ uint32_t slot_class(const Classify& k, uint64_t v)
+{
+    const uint32_t a = uint32_t(v >> 32), b = uint32_t(v);
+    if (k.body && k.body[a] == k.body[b])
+        return kClasses;  // same body: erased before the narrow phase
+    // shapePair (physics_world.cpp:670-671): SS, SB, BB, then the hull classes.
+    const uint32_t ta = k.type[a], tb = k.type[b];
+    const uint32_t lo = min(ta, tb), hi = max(ta, tb);
+    return hi < 2 ? lo + hi : 3 + lo;
+}
+
+// Writes entries [0, count) of a wavefront's queue at slots base + i (those
+// below cap), classifying them when k.buckets is set.  `cbase` (per class,
+// wave-uniform) is where this wavefront's class entries start in the
+// buckets; the per-class counts are what the caller reserved.  All lanes call
+// it.
+__device__ __forceinline__ void emit_classified(const uint64_t* buf, uint32_t count,
+                                                unsigned long long base, uint64_t* __restrict__ out,
+                                                uint64_t cap, const Classify& k,
+                                                const uint32_t cbase[kClasses])
+{
+    const uint32_t lane = lane_id();
+    uint32_t run[kClasses] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t i0 = 0; i0 < count; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const unsigned long long slot = base + i;
+        const bool valid = i < count && slot < cap;
+        uint32_t cls = kClasses;
+        if (valid) {
+            const uint64_t v = buf[i];
+            out[slot] = v;
+            if (k.buckets) {
+                k.hit[slot] = 0;
+                cls = slot_class(k, v);
+            }
+        }
+        if (!k.buckets)
+            continue;
+#pragma unroll
+        for (uint32_t q = 0; q < kClasses; ++q) {
+            const uint64_t m = __ballot(cls == q);
+            if (cls == q)
+                k.buckets[q * cap + cbase[q] + run[q] + (uint32_t)__popcll(m & lanes_below(lane))] =
+                    uint32_t(slot);
+            run[q] += (uint32_t)__popcll(m);
+        }
+    }
+}
+
+// Per-class counts of entries [0, count) of a wavefront's queue whose slots
+// (base + i) are below cap.  All lanes call it; the counts are wave-uniform.
+__device__ __forceinline__ void count_classes(const uint64_t* buf, uint32_t count,
+                                              unsigned long long base, uint64_t cap,
+                                              const Classify& k, uint32_t n[kClasses])
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t q = 0; q < kClasses; ++q)
+        n[q] = 0;
+    for (uint32_t i0 = 0; i0 < count; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const bool valid = i < count && base + i < cap;
+        const uint32_t cls = valid ? slot_class(k, buf[i]) : kClasses;
+#pragma unroll
+        for (uint32_t q = 0; q < kClasses; ++q)
+            n[q] += (uint32_t)__popcll(__ballot(cls == q));
+    }
+}
+
+__device__ __forceinline__ void cq_flush(WaveQueue& q, uint64_t* __restrict__ out,
+                                         unsigned long long* counter, uint64_t cap,
+                                         const Classify& k)
+{
+    if (q.count == 0)
+        return;
+    wave_sync();
+    const uint32_t lane = lane_id();
+    unsigned long long base = 0;
+    if (lane == 0)
+        base = atomicAdd(counter, (unsigned long long)q.count);
+    base = __shfl(base, 0);
+    uint32_t cbase[kClasses] = {0, 0, 0, 0, 0, 0};
+    if (k.buckets) {
+        uint32_t n[kClasses];
+        count_classes(q.buf, q.count, base, cap, k, n);
+        uint32_t mine = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kClasses; ++c)
+            if (lane == c && n[c])
+                mine = atomicAdd(&k.class_count[c], n[c]);
+#pragma unroll
+        for (uint32_t c = 0; c < kClasses; ++c)
+            cbase[c] = (uint32_t)__shfl((int)mine, c);
+    }
+    emit_classified(q.buf, q.count, base, out, cap, k, cbase);
+    wave_sync();
+    q.count = 0;
+}
+
+__device__ __forceinline__ void cq_push(WaveQueue& q, bool pred, uint64_t value,
+                                        uint64_t* __restrict__ out, unsigned long long* counter,
+                                        uint64_t cap, const Classify& k)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0)
+        return;
+    const uint32_t add = (uint32_t)__popcll(mask);
+    if (q.count + add > kWaveQueue)
+        cq_flush(q, out, counter, cap, k);
+    if (pred)
+        q.buf[q.count + (uint32_t)__popcll(mask & lanes_below(lane_id()))] = value;
+    q.count += add;
+}
+
+// Final flush of the workgroup's queues: one atomic for the pairs and one per
+// class for the whole workgroup.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void cq_flush_block(WaveQueue& q, uint64_t* __restrict__ out,
+                                               unsigned long long* counter, uint64_t cap,
+                                               const Classify& k)
+{
+    __shared__ uint32_t offset[kMaxBlockWaves];
+    __shared__ uint32_t class_off[kMaxBlockWaves][kClasses];
+    __shared__ unsigned long long base_all;
+    __shared__ uint32_t class_base[kClasses];
+    const uint32_t wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6, lane = lane_id();
+    __syncthreads();
+    if (lane == 0)
+        offset[wave] = q.count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            const uint32_t c = offset[w];
+            offset[w] = run;
+            run += c;
+        }
+        base_all = run ? atomicAdd(counter, (unsigned long long)run) : 0ull;
+    }
+    __syncthreads();
+    const unsigned long long base = base_all + offset[wave];
+    uint32_t cbase[kClasses] = {0, 0, 0, 0, 0, 0};
+    if (k.buckets) {
+        uint32_t n[kClasses];
+        count_classes(q.buf, q.count, base, cap, k, n);
+        if (lane == 0)
+            for (uint32_t c = 0; c < kClasses; ++c)
+                class_off[wave][c] = n[c];
+        __syncthreads();
+        if (threadIdx.x < kClasses) {
+            const uint32_t c = threadIdx.x;
+            uint32_t run = 0;
+            for (uint32_t w = 0; w < waves; ++w) {
+                const uint32_t v = class_off[w][c];
+                class_off[w][c] = run;
+                run += v;
+            }
+            class_base[c] = run ? atomicAdd(&k.class_count[c], run) : 0u;
+        }
+        __syncthreads();
+        for (uint32_t c = 0; c < kClasses; ++c)
+            cbase[c] = class_base[c] + class_off[wave][c];
+    }
+    emit_classified(q.buf, q.count, base, out, cap, k, cbase);
+    q.count = 0;
+}
+
 // Candidates of a body: bodies whose home cell lies, on every axis, in
 // [home - 1, floor(max / cell)] (a partner's min is below this body's max and
 // less than one cell below its min), i.e. 2 or 3 cells per axis.
@@ -357,7 +537,8 @@ __device__ __forceinline__ void grid_overflow_role(uint32_t entries, uint32_t ov
                                                 const float4* __restrict__ aabb,
                                                 const uint32_t* __restrict__ overflow,
                                                 Counters* counters, WaveQueue& queue,
-                                                uint64_t* __restrict__ out, uint64_t cap)
+                                                uint64_t* __restrict__ out, uint64_t cap,
+                                                const Classify& cls)
 {
     for (uint32_t k = b; k < over; k += kOverflowGroups) {
         const uint32_t o = overflow[k];
@@ -371,7 +552,7 @@ __device__ __forceinline__ void grid_overflow_role(uint32_t entries, uint32_t ov
                 j = rj.index;
                 hit = rj.world == ro.world && rec_overlap(ro.b, rj.b);
             }
-            wq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap);
+            cq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap, cls);
         }
         for (uint32_t k0 = 0; k0 < over; k0 += blockDim.x) {
             const uint32_t k2 = k0 + threadIdx.x;
@@ -384,7 +565,7 @@ __device__ __forceinline__ void grid_overflow_role(uint32_t entries, uint32_t ov
                     hit = rj.world == ro.world && rec_overlap(ro.b, rj.b);
                 }
             }
-            wq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap);
+            cq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap, cls);
         }
     }
 }
@@ -395,7 +576,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                                                        uint32_t table_bits, Counters* counters,
                                                        uint64_t* __restrict__ out, uint64_t cap,
                                                        const float4* __restrict__ aabb,
-                                                       const uint32_t* __restrict__ overflow)
+                                                       const uint32_t* __restrict__ overflow,
+                                                       Classify cls)
 {
     __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
     __shared__ float4 own_lds[kBlock / 64][2][64];
@@ -407,8 +589,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
     const uint32_t groups = gridDim.x - kOverflowGroups;  // the query's own
     if (blockIdx.x >= groups) {
         grid_overflow_role(entries, counters->overflow_count, blockIdx.x - groups, s, aabb,
-                           overflow, counters, queue, out, cap);
-        wq_flush_block(queue, out, &counters->pair_count, cap);
+                           overflow, counters, queue, out, cap, cls);
+        cq_flush_block(queue, out, &counters->pair_count, cap, cls);
         return;
     }
     float4* own = &own_lds[wave][0][0];
@@ -510,12 +692,12 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                     }
                 }
                 wave_sync();  // the next round rewrites the tasks
-                wq_push(queue, hit, pair_value(a, b), out, &counters->pair_count, cap);
+                cq_push(queue, hit, pair_value(a, b), out, &counters->pair_count, cap, cls);
             }
         }
         wave_sync();  // the next bodies rewrite the owner records
     }
-    wq_flush_block(queue, out, &counters->pair_count, cap);
+    cq_flush_block(queue, out, &counters->pair_count, cap, cls);
 }
 
 }  // namespace
@@ -579,11 +761,19 @@ void launch_grid(physgpu_context* ctx, double cell_size, bool fused)
         PG_BUCKETS(15);
 #undef PG_BUCKETS
     time_end(ctx, t);
+    // The collision step's query also classifies the pairs for the narrow
+    // phase (launch_narrowphase then skips its classification).
+    Classify cls{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (fused) {
+        cls = Classify{ctx->buckets, ctx->hit, ctx->counters->class_count, ctx->shape_type,
+                       ctx->has_world && ctx->world_has_body ? ctx->body : nullptr};
+        ctx->np_classified = true;
+    }
     time_begin(ctx, "grid_query", t);
     hipLaunchKernelGGL(k_grid_query, dim3(ctx->tuning.query_groups + kOverflowGroups),
                        dim3(ctx->tuning.query_block), 0, ctx->stream, n,
                        ctx->sorted_aabb, ctx->cell_table, g.axis, g.table, ctx->counters,
-                       ctx->pairs_raw, ctx->cap_p, ctx->aabb, ctx->overflow_list);
+                       ctx->pairs_raw, ctx->cap_p, ctx->aabb, ctx->overflow_list, cls);
     time_end(ctx, t);
 }
 

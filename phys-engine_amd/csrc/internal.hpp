@@ -65,6 +65,13 @@ enum PartialSlot : uint32_t {
     kSlotEpaItemsDone,  // GJK + EPA work items finished (the hand-over consumer's stop test)
     kPartialSlots
 };
+// EPA's partial slots: contacts of classes 3..5, failures, overflows, items
+// done, written by atomic adds spread over kEpaPartials addresses each and
+// zeroed before each narrow phase (by the classification, or by a grid
+// step's key pass).
+constexpr uint32_t kEpaPartialFirst = kSlotContacts + 3;
+constexpr uint32_t kEpaPartialSlots = 6;
+constexpr uint32_t kEpaPartials = 1024;
 enum FoldOp : uint32_t { kFoldSum64 = 0, kFoldSum32, kFoldMax32, kFoldMin32 };
 struct FoldEntry {
     uint32_t slot;    // PartialSlot
@@ -243,6 +250,7 @@ struct physgpu_context {
     bool resident_aabbs_valid = true; // ctx->aabb holds the resident colliders' bounds
     bool finish_pending = false;      // the next fold also finishes the broad phase's pairs
     bool tree_pairs_loaded = false;   // the candidate pairs are a tree query's (rank space)
+    bool np_classified = false;       // the broad phase has classified the pairs (grid step)
 
     // Narrow phase.  Its input pairs and, when they are not in canonical
     // order, the emission index of each canonical position (np_perm; null

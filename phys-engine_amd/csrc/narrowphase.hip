@@ -32,11 +32,6 @@ constexpr uint32_t kEpaMaxIterations = PHYSGPU_EPA_MAX_ITERATIONS;
 constexpr uint32_t kMaxVertices = 4 + kEpaMaxIterations;
 constexpr uint32_t kMaxFaces = PHYSGPU_EPA_MAX_FACES;
 constexpr uint32_t kMaxEdges = PHYSGPU_EPA_MAX_HORIZON;
-// EPA's partial slots: contacts of classes 3..5, failures, overflows, written
-// by atomic adds spread over kEpaPartials addresses each.
-constexpr uint32_t kEpaPartialFirst = kSlotContacts + 3;
-constexpr uint32_t kEpaPartialSlots = 6;
-constexpr uint32_t kEpaPartials = 1024;
 // Work items finished by GJK + EPA (kSlotEpaItemsDone), counted on
 // kEpaDoneSlots addresses one cache line apart.
 constexpr uint32_t kEpaDoneSlots = 64;
@@ -2003,11 +1998,16 @@ static_assert(kScanTile == kBlock, "one compaction tile per workgroup pass");
 
 __global__ __launch_bounds__(kBlock) void k_tile_hits(const uint32_t* __restrict__ hit,
                                                       const uint32_t* __restrict__ perm,
-                                                      const Counters* c,
+                                                      Counters* c, uint64_t cap,
                                                       uint32_t* __restrict__ tile_sums)
 {
     __shared__ uint32_t part[kBlock / 64];
-    const uint32_t count = c->sort_count;
+    // Pairs held (the broad phase's count clamped to the capacity), for this
+    // and the kernels after it.
+    const unsigned long long emitted = c->pair_count;
+    const uint32_t count = uint32_t(emitted < cap ? emitted : cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        c->sort_count = count;
     const uint32_t tiles = (count + kScanTile - 1) / kScanTile;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint32_t k = tile * kScanTile + threadIdx.x;
@@ -2176,12 +2176,17 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
     // SB / BB kernels, which it would otherwise delay.
     if (sort_on_side)
         hipEventRecord(ctx->sort_event, ctx->stream);
-    time_begin(ctx, "np_classify", t);
-    hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0, ctx->stream, B,
-                       W, pairs, ctx->counters, ctx->buckets, ctx->cap_p, ctx->hit,
-                       ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks,
-                       ctx->epa_epoch);
-    time_end(ctx, t);
+    // A grid collision step has classified its pairs in the broad phase's
+    // query (and started the EPA epoch with its key pass).
+    if (!ctx->np_classified) {
+        time_begin(ctx, "np_classify", t);
+        hipLaunchKernelGGL(k_classify, dim3(kClassifyGrid), dim3(kClassifyThreads), 0,
+                           ctx->stream, B, W, pairs, ctx->counters, ctx->buckets, ctx->cap_p,
+                           ctx->hit, ctx->partials + size_t(kEpaPartialFirst) * kMaxPartialBlocks,
+                           ctx->epa_epoch);
+        time_end(ctx, t);
+    }
+    ctx->np_classified = false;
     // With hulls, GJK + EPA run here while the SS / SB / BB kernels and then
     // the canonical sort run on the side stream.
     const bool fork = ctx->has_hulls;
@@ -2264,7 +2269,7 @@ void launch_narrowphase(physgpu_context* ctx, bool world, bool sort_on_side)
     }
     time_begin(ctx, "np_compact", t);
     hipLaunchKernelGGL(k_tile_hits, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->hit,
-                       ctx->np_perm, ctx->counters, ctx->tile_sums);
+                       ctx->np_perm, ctx->counters, ctx->cap_p, ctx->tile_sums);
     hipLaunchKernelGGL(k_tile_scan, dim3(1 + fold.count), dim3(kFoldThreads), 0, ctx->stream,
                        ctx->tile_sums, ctx->counters, ctx->partials, fold);
     hipLaunchKernelGGL(k_compact, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->hit,
