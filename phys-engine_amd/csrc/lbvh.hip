@@ -13,16 +13,17 @@
 //      leaves, so its box is a range union built from precomputed boxes of
 //      aligned 32 / 1024 / 32768-leaf groups; no atomics or fences (a
 //      bottom-up refit would need device-scope fences between XCDs).
-// Query: one thread per leaf, a stackless depth-first walk by ropes.  Node
-// k's children are the split's two sides, so the subtree after one whose
-// leaves end at l (l < n - 1) in depth-first order is the right child of the
-// node that splits l | l + 1: the build records that child per split
-// position, and the walk goes there after a subtree it skips or finishes
-// (the last leaf's rope ends the walk).  Depth-first order is leaf order, so
-// leaf p's walk starts at its own rope: it covers exactly the leaves after p
-// (each pair is found once, by its lower sorted position), descending into a
-// subtree only when its box meets p's.  Children and ropes are encoded with
-// bit 31 set for leaves; kEnd ends the walk.
+// Query: a stackless depth-first walk by ropes.  Node k's children are the
+// split's two sides, so the subtree after one whose leaves end at l
+// (l < n - 1) in depth-first order is the right child of the node that splits
+// l | l + 1: the build records that child per split position, and the walk
+// goes there after a subtree it skips or finishes (the last leaf's rope ends
+// the walk).  Depth-first order is leaf order, so the walk of a run of
+// leaves starts at the rope of its last leaf: it covers exactly the leaves
+// after the run (each pair is found once, by its lower sorted position),
+// descending into a subtree only when its box meets a box of the run.  Runs
+// of four leaves share one walk (k_query).  Children and ropes are encoded
+// with bit 31 set for leaves; kEnd ends the walk.
 #include <stddef.h>
 
 #include <algorithm>
@@ -224,62 +225,98 @@ __global__ __launch_bounds__(kBlock) void k_node_boxes(uint32_t n, const float4*
     store_rec(nodes, node, u, uint32_t(left[node]), end + 1 < n ? rope_of_split[end] : kEnd);
 }
 
-__global__ __launch_bounds__(kBlock) void k_query(uint32_t n, const float4* __restrict__ s,
+// Query by packets: a wavefront owns 64 consecutive sorted leaves, one per
+// lane, in packets of kPacket lanes.  Pairs inside a packet are tested lane
+// against lane.  The leaves after a packet are reached by one walk that the
+// packet's lanes share, from the rope of the packet's last leaf, descending
+// into a subtree when its box meets the box of any lane of the packet (every
+// node above a leaf that meets a lane's box meets that box too); every leaf
+// the walk reaches is tested by each lane of the packet.  A packet's walk is
+// the union of its lanes' walks: one node read serves the packet's lanes, and
+// the wavefront's loop runs as long as its longest packet walk.  (A test
+// against the union of the packet's boxes instead of each lane's would let a
+// packet that straddles a jump of the Morton curve walk a large part of the
+// scene.)
+// Four lanes to a packet: per packet about 28 dependent node reads at 1M
+// mixed bodies (16 per lane alone, 120 for a whole wavefront), while a
+// wavefront issues 16 reads per step instead of 64.
+constexpr uint32_t kPacket = 4;
+static_assert(kPacket >= 1 && kPacket <= 64 && (kPacket & (kPacket - 1)) == 0,
+              "packet: a power of two up to a wavefront");
+constexpr uint32_t kQueryBlock = kBlock;
+
+__global__ __launch_bounds__(kQueryBlock) void k_query(uint32_t n, const float4* __restrict__ s,
                                                   const uint32_t* __restrict__ rope_of_split,
                                                   const float4* __restrict__ nodes,
                                                   Counters* counters, uint64_t* __restrict__ out,
                                                   uint64_t cap, uint64_t* __restrict__ visits_part,
                                                   uint64_t* __restrict__ leaves_part)
 {
-    __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
+    __shared__ uint64_t queue_lds[kQueryBlock / 64][kWaveQueue];
     WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
     unsigned long long visits = 0, leaves = 0;
     const unsigned long long max_steps = 2ull * n + 64;
-    // Grid-stride over leaves; every lane stays in the loops so the
-    // wave-wide appends run converged.
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t rounds = (n + stride - 1) / stride;
-    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
-    for (uint32_t round = 0; round < rounds; ++round) {
-        const uint32_t p = round * stride + block * blockDim.x + threadIdx.x;
-        const bool active = p < n && n >= 2;
-        Rec me{{0, 0, 0, 0, 0, 0}, 0, 0};
-        uint32_t node = kEnd;
-        if (active) {
-            me = load_rec(s, p);
-            node = p + 1 < n ? rope_of_split[p] : kEnd;
+    const uint32_t lane = lane_id();
+    const uint32_t rank = lane & (kPacket - 1);  // lane within its packet
+    const uint32_t head = lane - rank;           // first lane of the packet
+    const uint64_t packet_lanes = (kPacket == 64 ? ~0ull : ((1ull << kPacket) - 1)) << head;
+    const uint32_t waves = gridDim.x * (kQueryBlock / 64);
+    const uint32_t wave0 =
+        xcd_block(blockIdx.x, gridDim.x) * (kQueryBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t chunks = (n + 63) / 64;
+    for (uint32_t ck = wave0; ck < chunks; ck += waves) {
+        const uint32_t p = ck * 64 + lane;
+        const bool active = p < n;
+        const Rec me = load_rec(s, active ? p : n - 1);
+        // Pairs inside the packet: each lane against the later lanes of its
+        // packet.
+        for (uint32_t j = 1; j < kPacket; ++j) {
+            const int src = int(lane + j < 64 ? lane + j : lane);
+            const float ob[6] = {__shfl(me.b[0], src), __shfl(me.b[1], src),
+                                 __shfl(me.b[2], src), __shfl(me.b[3], src),
+                                 __shfl(me.b[4], src), __shfl(me.b[5], src)};
+            const uint32_t oi = __shfl(me.index, src);
+            const bool valid = rank + j < kPacket && p + j < n;
+            leaves += valid ? 1u : 0u;
+            wq_push(queue, valid && rec_overlap(me.b, ob), pair_value(me.index, oi), out,
+                    &counters->pair_count, cap);
         }
-        // Converged loop: every lane steps until all walks have ended.  A walk
-        // enters each node at most once; the cap only guards against a
-        // malformed tree.
+        // The leaves after the packet.  A walk enters each node at most once;
+        // the cap only guards against a malformed tree.
+        const uint32_t first = ck * 64 + head;
+        const uint32_t last = min(first + kPacket, n) - 1;
+        uint32_t node = first < n && last + 1 < n ? rope_of_split[last] : kEnd;
         unsigned long long steps = 0;
         for (;;) {
             if (node != kEnd && steps > max_steps) {
-                atomicAdd(&counters->tree_overflow, 1u);
+                if (rank == 0)
+                    atomicAdd(&counters->tree_overflow, 1u);
                 node = kEnd;
             }
             const bool more = node != kEnd;
             if (__ballot(more) == 0)
                 break;
             bool hit = false;
-            uint32_t b = 0;
+            uint32_t other = 0;
             if (more) {
                 ++steps;
                 if (node & kLeaf) {
                     const uint32_t q = node & ~kLeaf;
                     const Rec o = load_rec(s, q);
-                    const uint32_t rope = q + 1 < n ? rope_of_split[q] : kEnd;
-                    ++leaves;
-                    hit = rec_overlap(me.b, o.b);
-                    b = o.index;
-                    node = rope;
+                    node = q + 1 < n ? rope_of_split[q] : kEnd;
+                    leaves += active ? 1u : 0u;
+                    hit = active && rec_overlap(me.b, o.b);
+                    other = o.index;
                 } else {
-                    ++visits;
                     const Rec nd = load_rec(nodes, node);
-                    node = rec_overlap(me.b, nd.b) ? nd.index : nd.world;
+                    visits += rank == 0 ? 1u : 0u;
+                    // Every lane of a packet takes this branch together.
+                    const bool meets = (__ballot(active && rec_overlap(me.b, nd.b)) &
+                                        packet_lanes) != 0;
+                    node = meets ? nd.index : nd.world;
                 }
             }
-            wq_push(queue, hit, pair_value(me.index, b), out, &counters->pair_count, cap);
+            wq_push(queue, hit, pair_value(me.index, other), out, &counters->pair_count, cap);
         }
     }
     wq_flush_block(queue, out, &counters->pair_count, cap);
@@ -350,8 +387,9 @@ void launch_lbvh_boxes_and_query(physgpu_context* ctx)
                        ctx->node_box);
     time_end(ctx, t);
     time_begin(ctx, "lbvh_query", t);
-    const uint32_t qgrid = std::min<uint32_t>(div_up(n, kBlock), 4096);
-    hipLaunchKernelGGL(k_query, dim3(qgrid), dim3(kBlock), 0, ctx->stream, n,
+    const uint32_t qgrid =
+        std::min<uint32_t>(div_up(div_up(n, 64), kQueryBlock / 64), kMaxPartialBlocks);
+    hipLaunchKernelGGL(k_query, dim3(qgrid), dim3(kQueryBlock), 0, ctx->stream, n,
                        ctx->sorted_aabb, reinterpret_cast<const uint32_t*>(ctx->node_right),
                        ctx->node_box, ctx->counters, ctx->pairs_raw, ctx->cap_p,
                        ctx->partials + size_t(kSlotTreeVisits) * kMaxPartialBlocks,
