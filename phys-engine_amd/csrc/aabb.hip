@@ -426,7 +426,8 @@ __global__ __launch_bounds__(kKeyThreads) void k_aabbs_grid_keys(
             float b[6];
             if (body_aabb(i, pose, shape, b)) {
                 const uint32_t w = world_of ? world_of[i] : 0u;
-                store_rec(aabb, i, b, i, w);
+                const uint32_t ty = __float_as_uint(shape[2 * size_t(i)].w);
+                store_rec(aabb, i, b, i, w | (ty << kRecTypeShift));
                 emit(i, b, w);
             } else {
                 hull_ids[atomicAdd(&hull_count, 1u)] = i;
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(kKeyThreads) void k_aabbs_grid_keys(
         hull_row_aabb(i, pose, shape, hpts, nh, rl, row_base, b);
         if (rl == 0) {
             const uint32_t w = world_of ? world_of[i] : 0u;
-            store_rec(aabb, i, b, i, w);
+            store_rec(aabb, i, b, i, w | (uint32_t(PHYSGPU_HULL) << kRecTypeShift));
             emit(i, b, w);
         }
     }

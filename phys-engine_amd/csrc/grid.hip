@@ -73,7 +73,8 @@ __device__ __forceinline__ uint32_t grid_key(uint32_t n, const float4* __restric
     uint32_t key = kOverflowKey;
     if (i < n) {
         const Rec r = load_rec(aabb, i);
-        key = grid_cell_key(r.b, r.world, cell, 1.0 / double(cell), bits_per_axis, table_bits);
+        key = grid_cell_key(r.b, rec_world(r.world), cell, 1.0 / double(cell), bits_per_axis,
+                            table_bits);
         items[i] = (uint64_t)key | ((uint64_t)i << 32);
     }
     // Overflow list (rare: one atomic per wave that has an overflow body).
@@ -335,21 +336,38 @@ __global__ __launch_bounds__(kBucketThreads) void k_grid_buckets(
 // hit[slot] = 0, and slot appended to the bucket of its shape-pair class
 // unless both colliders belong to one body.  Counts are combined per
 // wavefront flush (one atomic per class) and per workgroup at the end.
+// Entries of a wavefront's pair queue in the grid query: 512 (4 KiB of LDS).
+// Every mid-query flush takes a pair-counter and six class-counter atomics,
+// all on one counter line; at 16M bodies (about 450 pairs per wavefront)
+// 256 entries make those flushes cost the query a quarter of its time.
+constexpr uint32_t kGridQueue = 512;
+
 struct Classify {
     uint32_t* buckets;            // kClasses x cap candidate indices (null: plain emission)
     uint32_t* hit;
     uint32_t* class_count;        // Counters::class_count
-    const uint8_t* type;          // collider shape types
     const uint32_t* body;         // collider bodies (null: one body per collider)
 };
 
+// Queue entries are pair records whose halves carry the colliders' shape
+// types in their top two bits (the records' world-word bits); the emitted
+// records have them cleared.
+constexpr uint64_t kPairIndexBits = (uint64_t(kRecWorldMask) << 32) | kRecWorldMask;
+
+__device__ __forceinline__ uint64_t typed_pair(uint32_t a, uint32_t world_a, uint32_t b,
+                                               uint32_t world_b)
+{
+    const uint32_t ta = a | (world_a & ~kRecWorldMask), tb = b | (world_b & ~kRecWorldMask);
+    return a < b ? (uint64_t(ta) << 32) | tb : (uint64_t(tb) << 32) | ta;
+}
+
 __device__ __forceinline__ uint32_t slot_class(const Classify& k, uint64_t v)
 {
-    const uint32_t a = uint32_t(v >> 32), b = uint32_t(v);
-    if (k.body && k.body[a] == k.body[b])
+    const uint32_t hi32 = uint32_t(v >> 32), lo32 = uint32_t(v);
+    if (k.body && k.body[hi32 & kRecWorldMask] == k.body[lo32 & kRecWorldMask])
         return kClasses;  // same body: erased before the narrow phase
     // shapePair (physics_world.cpp:670-671): SS, SB, BB, then the hull classes.
-    const uint32_t ta = k.type[a], tb = k.type[b];
+    const uint32_t ta = hi32 >> kRecTypeShift, tb = lo32 >> kRecTypeShift;
     const uint32_t lo = min(ta, tb), hi = max(ta, tb);
     return hi < 2 ? lo + hi : 3 + lo;
 }
@@ -373,7 +391,7 @@ __device__ __forceinline__ void emit_classified(const uint64_t* buf, uint32_t co
         uint32_t cls = kClasses;
         if (valid) {
             const uint64_t v = buf[i];
-            out[slot] = v;
+            out[slot] = v & kPairIndexBits;
             if (k.buckets) {
                 k.hit[slot] = 0;
                 cls = slot_class(k, v);
@@ -449,7 +467,7 @@ __device__ __forceinline__ void cq_push(WaveQueue& q, bool pred, uint64_t value,
     if (mask == 0)
         return;
     const uint32_t add = (uint32_t)__popcll(mask);
-    if (q.count + add > kWaveQueue)
+    if (q.count + add > kGridQueue)
         cq_flush(q, out, counter, cap, k);
     if (pred)
         q.buf[q.count + (uint32_t)__popcll(mask & lanes_below(lane_id()))] = value;
@@ -546,26 +564,30 @@ __device__ __forceinline__ void grid_overflow_role(uint32_t entries, uint32_t ov
         for (uint32_t p0 = 0; p0 < entries; p0 += blockDim.x) {
             const uint32_t p = p0 + threadIdx.x;
             bool hit = false;
-            uint32_t j = 0;
+            uint32_t j = 0, wj = 0;
             if (p < entries) {
                 const Rec rj = load_rec(s, p);
                 j = rj.index;
-                hit = rj.world == ro.world && rec_overlap(ro.b, rj.b);
+                wj = rj.world;
+                hit = rec_world(rj.world) == rec_world(ro.world) && rec_overlap(ro.b, rj.b);
             }
-            cq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap, cls);
+            cq_push(queue, hit, typed_pair(o, ro.world, j, wj), out, &counters->pair_count, cap,
+                    cls);
         }
         for (uint32_t k0 = 0; k0 < over; k0 += blockDim.x) {
             const uint32_t k2 = k0 + threadIdx.x;
             bool hit = false;
-            uint32_t j = 0;
+            uint32_t j = 0, wj = 0;
             if (k2 < over) {
                 j = overflow[k2];
                 if (o < j) {
                     const Rec rj = load_rec(aabb, j);
-                    hit = rj.world == ro.world && rec_overlap(ro.b, rj.b);
+                    wj = rj.world;
+                    hit = rec_world(rj.world) == rec_world(ro.world) && rec_overlap(ro.b, rj.b);
                 }
             }
-            cq_push(queue, hit, pair_value(o, j), out, &counters->pair_count, cap, cls);
+            cq_push(queue, hit, typed_pair(o, ro.world, j, wj), out, &counters->pair_count, cap,
+                    cls);
         }
     }
 }
@@ -579,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                                                        const uint32_t* __restrict__ overflow,
                                                        Classify cls)
 {
-    __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
+    __shared__ uint64_t queue_lds[kBlock / 64][kGridQueue];
     __shared__ float4 own_lds[kBlock / 64][2][64];
     __shared__ uint32_t task_lds[kBlock / 64][64];
     __shared__ uint8_t owner_lds[kBlock / 64][64];
@@ -617,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
             own[64 + lane] = r1;
             home_cell(r0.x, r0.y, r0.z, inv, c);
             home_cell(r0.w, r1.x, r1.y, inv, top);
-            wk = world_key(__float_as_uint(r1.w), table_bits);
+            wk = world_key(rec_world(__float_as_uint(r1.w)), table_bits);
         }
         // Spread coordinates of the neighbour cells: x and y at -1, 0, +1,
         // z at 0, +1 (a cell's key is (sx | sy | sz) & tmask ^ wk).
@@ -677,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                 }
                 wave_sync();
                 bool hit = false;
-                uint32_t a = 0, b = 0;
+                uint32_t a = 0, b = 0, wa = 0, wb = 0;
                 if (base + lane < total) {
                     const uint32_t q = task[lane], w = task_owner[lane];
                     const uint32_t owner = w & 63u;
@@ -686,13 +708,16 @@ __global__ __launch_bounds__(kBlock) void k_grid_query(uint32_t n, const float4*
                     const Rec o = load_rec(s, q);
                     a = __float_as_uint(m1.z);
                     b = o.index;
-                    if ((!home || a < b) && __float_as_uint(m1.w) == o.world) {
+                    wa = __float_as_uint(m1.w);
+                    wb = o.world;
+                    if ((!home || a < b) && rec_world(wa) == rec_world(wb)) {
                         const float mb[6] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y};
                         hit = rec_overlap(mb, o.b);
                     }
                 }
                 wave_sync();  // the next round rewrites the tasks
-                cq_push(queue, hit, pair_value(a, b), out, &counters->pair_count, cap, cls);
+                cq_push(queue, hit, typed_pair(a, wa, b, wb), out, &counters->pair_count, cap,
+                        cls);
             }
         }
         wave_sync();  // the next bodies rewrite the owner records
@@ -763,9 +788,9 @@ void launch_grid(physgpu_context* ctx, double cell_size, bool fused)
     time_end(ctx, t);
     // The collision step's query also classifies the pairs for the narrow
     // phase (launch_narrowphase then skips its classification).
-    Classify cls{nullptr, nullptr, nullptr, nullptr, nullptr};
+    Classify cls{nullptr, nullptr, nullptr, nullptr};
     if (fused) {
-        cls = Classify{ctx->buckets, ctx->hit, ctx->counters->class_count, ctx->shape_type,
+        cls = Classify{ctx->buckets, ctx->hit, ctx->counters->class_count,
                        ctx->has_world && ctx->world_has_body ? ctx->body : nullptr};
         ctx->np_classified = true;
     }

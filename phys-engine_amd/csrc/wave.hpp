@@ -77,7 +77,18 @@ namespace pg {
 // {max.y, max.z, index bits, world bits}; 32 bytes, one sector, so a
 // scattered candidate costs one 32-byte access instead of six 4-byte planes.
 // `world` is the collider's world in a batch of worlds (0 for one world):
-// only records of the same world can pair.
+// only records of the same world can pair.  Records written by the uniform
+// grid's key pass also carry the collider's shape type in the world word's
+// top two bits (colliders and worlds number at most 2^30), so that the grid
+// query classifies the pairs it emits without reading the shape types.
+constexpr uint32_t kRecTypeShift = 30;
+constexpr uint32_t kRecWorldMask = (1u << kRecTypeShift) - 1u;
+
+__device__ __forceinline__ uint32_t rec_world(uint32_t world_word)
+{
+    return world_word & kRecWorldMask;
+}
+
 struct Rec {
     float b[6];  // min x y z, max x y z
     uint32_t index;
