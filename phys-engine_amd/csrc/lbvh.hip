@@ -253,6 +253,15 @@ static_assert(kPacket >= 1 && kPacket <= 64 && (kPacket & (kPacket - 1)) == 0,
               "packet: a power of two up to a wavefront");
 constexpr uint32_t kQueryBlock = kBlock;
 
+// A wavefront's 16 packet slots take the 32 packets (128 leaves) of its
+// range in turn: a slot whose walk has ended takes the range's next packet at
+// once, so the wavefront runs about as long as its slots' summed walks rather
+// than its longest packet in each batch of 16.  At 1M bodies the ranges make
+// 8192 wavefronts, one per wave slot of the GPU (16 packets per range: two
+// rounds of wavefronts, 64: half the slots idle).
+constexpr uint32_t kSlots = 64 / kPacket;
+constexpr uint32_t kRangePackets = 32;
+
 __global__ __launch_bounds__(kQueryBlock) void k_query(uint32_t n, const float4* __restrict__ s,
                                                   const uint32_t* __restrict__ rope_of_split,
                                                   const float4* __restrict__ nodes,
@@ -266,47 +275,60 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(uint32_t n, const float4*
     const unsigned long long max_steps = 2ull * n + 64;
     const uint32_t lane = lane_id();
     const uint32_t rank = lane & (kPacket - 1);  // lane within its packet
-    const uint32_t head = lane - rank;           // first lane of the packet
+    const uint32_t head = lane - rank;           // first lane of the packet slot
+    const uint32_t slot = lane / kPacket;
     const uint64_t packet_lanes = (kPacket == 64 ? ~0ull : ((1ull << kPacket) - 1)) << head;
+    const uint64_t heads_below = __ballot(rank == 0) & lanes_below(head);
     const uint32_t waves = gridDim.x * (kQueryBlock / 64);
     const uint32_t wave0 =
         xcd_block(blockIdx.x, gridDim.x) * (kQueryBlock / 64) + (threadIdx.x >> 6);
-    const uint32_t chunks = (n + 63) / 64;
-    for (uint32_t ck = wave0; ck < chunks; ck += waves) {
-        const uint32_t p = ck * 64 + lane;
-        const bool active = p < n;
-        const Rec me = load_rec(s, active ? p : n - 1);
-        // Pairs inside the packet: each lane against the later lanes of its
-        // packet.
-        for (uint32_t j = 1; j < kPacket; ++j) {
-            const int src = int(lane + j < 64 ? lane + j : lane);
-            const float ob[6] = {__shfl(me.b[0], src), __shfl(me.b[1], src),
-                                 __shfl(me.b[2], src), __shfl(me.b[3], src),
-                                 __shfl(me.b[4], src), __shfl(me.b[5], src)};
-            const uint32_t oi = __shfl(me.index, src);
-            const bool valid = rank + j < kPacket && p + j < n;
-            leaves += valid ? 1u : 0u;
-            wq_push(queue, valid && rec_overlap(me.b, ob), pair_value(me.index, oi), out,
-                    &counters->pair_count, cap);
-        }
-        // The leaves after the packet.  A walk enters each node at most once;
-        // the cap only guards against a malformed tree.
-        const uint32_t first = ck * 64 + head;
-        const uint32_t last = min(first + kPacket, n) - 1;
-        uint32_t node = first < n && last + 1 < n ? rope_of_split[last] : kEnd;
+    const uint32_t packets = (n + kPacket - 1) / kPacket;
+    for (uint32_t range = wave0 * kRangePackets; range < packets; range += waves * kRangePackets) {
+        const uint32_t range_end = min(range + kRangePackets, packets);
+        uint32_t next = range + kSlots;  // the range's next unassigned packet (uniform)
+        uint32_t pk = range + slot;
+        bool live = pk < range_end;  // slot holds a packet (slot-uniform)
+        bool fresh = live;           // packet just taken: pairs inside it not yet tested
+        uint32_t node = kEnd;
         unsigned long long steps = 0;
+        Rec me{{0, 0, 0, 0, 0, 0}, 0, 0};
+        uint32_t p = 0;
+        bool active = false;
         for (;;) {
+            // Slots with a fresh packet: its records, the pairs inside it and
+            // the start of its walk (the rope of its last leaf).
+            if (__ballot(fresh)) {
+                if (fresh) {
+                    p = pk * kPacket + rank;
+                    active = p < n;
+                    me = load_rec(s, active ? p : n - 1);
+                    const uint32_t last = min(pk * kPacket + kPacket, n) - 1;
+                    node = last + 1 < n ? rope_of_split[last] : kEnd;
+                    steps = 0;
+                }
+                for (uint32_t j = 1; j < kPacket; ++j) {
+                    const int src = int(lane + j < 64 ? lane + j : lane);
+                    const float ob[6] = {__shfl(me.b[0], src), __shfl(me.b[1], src),
+                                         __shfl(me.b[2], src), __shfl(me.b[3], src),
+                                         __shfl(me.b[4], src), __shfl(me.b[5], src)};
+                    const uint32_t oi = __shfl(me.index, src);
+                    const bool valid = fresh && rank + j < kPacket && p + j < n;
+                    leaves += valid ? 1u : 0u;
+                    wq_push(queue, valid && rec_overlap(me.b, ob), pair_value(me.index, oi), out,
+                            &counters->pair_count, cap);
+                }
+                fresh = false;
+            }
+            // One step of every live walk.  A walk enters each node at most
+            // once; the cap only guards against a malformed tree.
             if (node != kEnd && steps > max_steps) {
                 if (rank == 0)
                     atomicAdd(&counters->tree_overflow, 1u);
                 node = kEnd;
             }
-            const bool more = node != kEnd;
-            if (__ballot(more) == 0)
-                break;
             bool hit = false;
             uint32_t other = 0;
-            if (more) {
+            if (node != kEnd) {
                 ++steps;
                 if (node & kLeaf) {
                     const uint32_t q = node & ~kLeaf;
@@ -325,6 +347,21 @@ __global__ __launch_bounds__(kQueryBlock) void k_query(uint32_t n, const float4*
                 }
             }
             wq_push(queue, hit, pair_value(me.index, other), out, &counters->pair_count, cap);
+            // Slots whose walk ended take the range's next packets, in slot
+            // order; with none left they fall idle.
+            const bool done = live && node == kEnd;
+            const uint64_t done_heads = __ballot(done && rank == 0);
+            if (done_heads) {
+                const uint32_t k = uint32_t(__popcll(done_heads & heads_below));
+                if (done) {
+                    pk = next + k;
+                    live = pk < range_end;
+                    fresh = live;
+                }
+                next += uint32_t(__popcll(done_heads));
+            }
+            if (__ballot(live) == 0)
+                break;
         }
     }
     wq_flush_block(queue, out, &counters->pair_count, cap);
@@ -396,8 +433,8 @@ void launch_lbvh_boxes_and_query(physgpu_context* ctx)
                        ctx->node_box);
     time_end(ctx, t);
     time_begin(ctx, "lbvh_query", t);
-    const uint32_t qgrid =
-        std::min<uint32_t>(div_up(div_up(n, 64), kQueryBlock / 64), kMaxPartialBlocks);
+    const uint32_t qgrid = std::min<uint32_t>(
+        div_up(div_up(div_up(n, kPacket), kRangePackets), kQueryBlock / 64), kMaxPartialBlocks);
     hipLaunchKernelGGL(k_query, dim3(qgrid), dim3(kQueryBlock), 0, ctx->stream, n,
                        ctx->sorted_aabb, reinterpret_cast<const uint32_t*>(ctx->node_right),
                        ctx->node_box, ctx->counters, ctx->pairs_raw, ctx->cap_p,
