@@ -3,14 +3,17 @@
 // Records are sorted by (min.x, index) with the radix sort, then gathered into
 // a sorted copy of the 32-byte AABB records.  The reference scans, for each row i, the records j > i
 // until min_j.x > max_i.x and emits the pair when Y and Z overlap inclusively.
-// On gfx950 the sweep is turned around: a workgroup owns 1024 consecutive
+// On gfx950 the sweep is turned around: a workgroup owns 2048 consecutive
 // sorted records as *columns* (4 per lane, held in registers) and streams the
 // rows that can reach them through LDS.  A row i can reach column j only when
 // min_i.x >= min_j.x - max side, so the candidate rows of a workgroup form a
-// contiguous range found by one binary search.  Each row is a broadcast LDS
-// read; a row that ends before the wave's first column is skipped with a
-// wave-uniform branch; otherwise all 64 lanes test their 4 columns.  Hits are
-// compacted with a wavefront ballot and one atomic per wave.
+// contiguous range found by one binary search.  The workgroup's columns are
+// ordered by min.y in LDS and dealt to its wavefronts in that order, so each
+// wavefront holds a band of y.  Each row is a broadcast LDS read; a row that
+// ends before the wave's first column or whose y interval misses the wave's
+// band is skipped with a wave-uniform branch; otherwise all 64 lanes test
+// their 4 columns.  Hits are compacted with a wavefront ballot and one atomic
+// per wave.
 #include <stddef.h>
 
 #include "devmath.hpp"
@@ -22,8 +25,11 @@ namespace {
 
 constexpr uint32_t kColsPerLane = 4;
 constexpr uint32_t kColsPerWave = 64 * kColsPerLane;
-constexpr uint32_t kColsPerBlock = kColsPerWave * (kBlock / 64);  // 1024
-constexpr uint32_t kRowChunk = kBlock;
+// 512 threads: 2048 columns and eight y-bands per workgroup (at 1M bodies
+// 256 threads sweep in 1.26 ms, 512 in 1.18, 1024 in 1.29).
+constexpr uint32_t kSweepThreads = 512;
+constexpr uint32_t kColsPerBlock = kColsPerWave * (kSweepThreads / 64);
+constexpr uint32_t kRowChunk = kSweepThreads;
 
 __global__ __launch_bounds__(kBlock) void k_sap_keys(uint32_t n, const float4* __restrict__ aabb,
                                                      uint64_t* __restrict__ items)
@@ -48,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_sorted(uint32_t n,
     sorted[2 * size_t(i) + 1] = aabb[2 * idx + 1];
 }
 
-__global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float4* __restrict__ s,
+__global__ __launch_bounds__(kSweepThreads) void k_sap_sweep(uint32_t n, const float4* __restrict__ s,
                                                       const Counters* counters_ro,
                                                       Counters* counters,
                                                       uint64_t* __restrict__ out, uint64_t cap)
@@ -57,21 +63,50 @@ __global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float4* 
     __shared__ float4 row_a[kRowChunk];
     __shared__ float2 row_b[kRowChunk];
     __shared__ uint32_t s_lo;
-    __shared__ uint64_t queue_lds[kBlock / 64][kWaveQueue];
+    __shared__ uint64_t queue_lds[kSweepThreads / 64][kWaveQueue];
     WaveQueue queue{queue_lds[threadIdx.x >> 6], 0};
 
+    __shared__ uint64_t order[kColsPerBlock];
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x >> 6;
     const uint32_t j0 = blockIdx.x * kColsPerBlock;
-    const uint32_t wave_first = j0 + wave * kColsPerWave;
 
-    // Columns in registers.
+    // The workgroup's columns ordered by min.y (bitonic sort of (min.y key,
+    // column) in LDS), so that each wavefront holds a band of y: a row whose
+    // y interval misses a wavefront's band skips it with a uniform branch.
+    for (uint32_t c = threadIdx.x; c < kColsPerBlock; c += kSweepThreads) {
+        const uint32_t j = j0 + c;
+        const uint32_t key = j < n ? float_key(s[2 * size_t(j)].y) : 0xFFFFFFFFu;
+        order[c] = (uint64_t(key) << 32) | c;
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= kColsPerBlock; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = threadIdx.x; t < kColsPerBlock / 2; t += kSweepThreads) {
+                const uint32_t lo = 2 * t - (t & (stride - 1));
+                const uint32_t hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const uint64_t a = order[lo], b = order[hi];
+                if ((a > b) == up) {
+                    order[lo] = b;
+                    order[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // Columns in registers: wavefront w takes sorted positions
+    // [256 w, 256 w + 256).
     float c_minx[kColsPerLane], c_miny[kColsPerLane], c_maxy[kColsPerLane];
     float c_minz[kColsPerLane], c_maxz[kColsPerLane];
     uint32_t c_idx[kColsPerLane], c_j[kColsPerLane];
+    const uint32_t wave = threadIdx.x >> 6;
+    float wave_min_x = __int_as_float(0x7f800000), ylo = __int_as_float(0x7f800000);
+    float yhi = __int_as_float(0xff800000);
+    uint32_t wave_last = 0;  // rows must be below this wavefront's last column
 #pragma unroll
     for (uint32_t k = 0; k < kColsPerLane; ++k) {
-        const uint32_t j = wave_first + k * 64u + lane;
+        const uint32_t j = j0 + uint32_t(order[wave * kColsPerWave + k * 64u + lane] & 0xFFFFFFFFu);
         c_j[k] = j;
         if (j < n) {
             const Rec r = load_rec(s, j);
@@ -81,15 +116,23 @@ __global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float4* 
             c_maxy[k] = r.b[4];
             c_maxz[k] = r.b[5];
             c_idx[k] = r.index;
+            wave_min_x = fminf(wave_min_x, r.b[0]);
+            ylo = fminf(ylo, r.b[1]);
+            yhi = fmaxf(yhi, r.b[4]);
+            wave_last = max(wave_last, j + 1);
         } else {
             c_minx[k] = __int_as_float(0x7f800000);  // +inf: never <= max_i.x
             c_miny[k] = c_maxy[k] = c_minz[k] = c_maxz[k] = 0.0f;
             c_idx[k] = 0;
         }
     }
-    const float wave_min_x =
-        wave_first < n ? s[2 * size_t(wave_first)].x : __int_as_float(0x7f800000);
-    const uint32_t wave_last = min(wave_first + kColsPerWave, n);  // rows must be < this
+    for (uint32_t off = 32; off > 0; off >>= 1) {
+        wave_min_x = fminf(wave_min_x, __shfl_xor(wave_min_x, off));
+        ylo = fminf(ylo, __shfl_xor(ylo, off));
+        yhi = fmaxf(yhi, __shfl_xor(yhi, off));
+        wave_last = max(wave_last, (uint32_t)__shfl_xor((int)wave_last, off));
+    }
+    // (fminf / fmaxf pass over a NaN bound: such a column meets no row.)
 
     if (threadIdx.x == 0) {
         // First row that can reach column j0: min_i.x >= min_j0.x - max side.
@@ -137,9 +180,10 @@ __global__ __launch_bounds__(kBlock) void k_sap_sweep(uint32_t n, const float4* 
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) {
                 const uint32_t i = chunk + r0 + u;
-                // Past the chunk's rows, or the row's window ends before this
-                // wave's columns (wave-uniform).
-                if (r0 + u >= rows || ra[u].x < wave_min_x)
+                // Past the chunk's rows, the row's window ends before this
+                // wave's columns, or its y interval misses their band
+                // (wave-uniform).
+                if (r0 + u >= rows || ra[u].x < wave_min_x || ra[u].z < ylo || ra[u].y > yhi)
                     continue;
                 const float rx = ra[u].x, ry0 = ra[u].y, ry1 = ra[u].z, rz0 = ra[u].w;
                 const float rz1 = rb[u].x;
@@ -211,7 +255,7 @@ void launch_sap_sweep(physgpu_context* ctx)
 {
     Timer t;
     time_begin(ctx, "sap_sweep", t);
-    hipLaunchKernelGGL(k_sap_sweep, dim3(div_up(ctx->n, kColsPerBlock)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_sap_sweep, dim3(div_up(ctx->n, kColsPerBlock)), dim3(kSweepThreads), 0,
                        ctx->stream, ctx->n, ctx->sorted_aabb, ctx->counters, ctx->counters,
                        ctx->pairs_raw, ctx->cap_p);
     time_end(ctx, t);
