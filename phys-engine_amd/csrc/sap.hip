@@ -9,11 +9,11 @@
 // min_i.x >= min_j.x - max side, so the candidate rows of a workgroup form a
 // contiguous range found by one binary search.  The workgroup's columns are
 // ordered by min.y in LDS and dealt to its wavefronts in that order, so each
-// wavefront holds a band of y.  Each row is a broadcast LDS read; a row that
-// ends before the wave's first column or whose y interval misses the wave's
-// band is skipped with a wave-uniform branch; otherwise all 64 lanes test
-// their 4 columns.  Hits are compacted with a wavefront ballot and one atomic
-// per wave.
+// wavefront holds a band of y.  The thread that stages a row in LDS lists it
+// for every wavefront whose columns its window and y interval can reach; each
+// wavefront reads only its listed rows (broadcast LDS reads), and all 64
+// lanes test their 4 columns.  Hits are compacted with a wavefront ballot and
+// one atomic per wave.
 #include <stddef.h>
 
 #include "devmath.hpp"
@@ -134,6 +134,22 @@ __global__ __launch_bounds__(kSweepThreads) void k_sap_sweep(uint32_t n, const f
     }
     // (fminf / fmaxf pass over a NaN bound: such a column meets no row.)
 
+    // Each wavefront's row filter, shared: a row of a chunk goes onto the
+    // list of every wavefront whose window and y band it reaches.
+    constexpr uint32_t kWaves = kSweepThreads / 64;
+    __shared__ float band_lds[kWaves][3];
+    __shared__ uint32_t last_lds[kWaves];
+    __shared__ uint32_t list_count[2][kWaves];  // by chunk parity
+    __shared__ uint16_t list_rows[kWaves][kRowChunk];
+    if (threadIdx.x < 2 * kWaves)
+        list_count[threadIdx.x / kWaves][threadIdx.x % kWaves] = 0;
+    if (lane == 0) {
+        band_lds[wave][0] = wave_min_x;
+        band_lds[wave][1] = ylo;
+        band_lds[wave][2] = yhi;
+        last_lds[wave] = wave_last;
+    }
+
     if (threadIdx.x == 0) {
         // First row that can reach column j0: min_i.x >= min_j0.x - max side.
         const float wmax = key_float(counters_ro->wmax_x_key);
@@ -153,7 +169,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sap_sweep(uint32_t n, const f
     const uint32_t row_begin = s_lo;
     const uint32_t row_end = min(j0 + kColsPerBlock, n) - 1;  // rows < last column
 
-    for (uint32_t chunk = row_begin; chunk < row_end; chunk += kRowChunk) {
+    uint32_t parity = 0;
+    for (uint32_t chunk = row_begin; chunk < row_end; chunk += kRowChunk, parity ^= 1u) {
+        // Every wavefront has read the previous chunk's lists and counts.
         __syncthreads();
         {
             const uint32_t i = chunk + threadIdx.x;
@@ -161,30 +179,40 @@ __global__ __launch_bounds__(kSweepThreads) void k_sap_sweep(uint32_t n, const f
                 const Rec r = load_rec(s, i);
                 row_a[threadIdx.x] = make_float4(r.b[3], r.b[1], r.b[4], r.b[2]);
                 row_b[threadIdx.x] = make_float2(r.b[5], __uint_as_float(r.index));
+                // The wavefronts this row can meet: its window reaches their
+                // columns (max.x >= their least min.x, a column after it) and
+                // its y interval meets their band.
+#pragma unroll
+                for (uint32_t w = 0; w < kWaves; ++w) {
+                    if (r.b[3] >= band_lds[w][0] && r.b[4] >= band_lds[w][1] &&
+                        r.b[1] <= band_lds[w][2] && i + 1 < last_lds[w])
+                        list_rows[w][atomicAdd(&list_count[parity][w], 1u)] =
+                            uint16_t(threadIdx.x);
+                }
             }
         }
         __syncthreads();
-        // Rows of this chunk that precede at least one column of this wave.
-        const uint32_t limit = wave_last > chunk + 1 ? wave_last - 1 - chunk : 0u;
-        const uint32_t rows = min(min(kRowChunk, row_end - chunk), limit);
+        const uint32_t rows = list_count[parity][wave];
+        // The other parity's counts were read before this chunk's first
+        // barrier; they start the next chunk at zero.
+        if (threadIdx.x < kWaves)
+            list_count[parity ^ 1u][threadIdx.x] = 0;
         // Four rows per step: their LDS reads are issued together.
         for (uint32_t r0 = 0; r0 < rows; r0 += 4) {
             float4 ra[4];
             float2 rb[4];
+            uint32_t ro[4];
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) {
-                const uint32_t r = min(r0 + u, rows - 1);
-                ra[u] = row_a[r];
-                rb[u] = row_b[r];
+                ro[u] = list_rows[wave][min(r0 + u, rows - 1)];
+                ra[u] = row_a[ro[u]];
+                rb[u] = row_b[ro[u]];
             }
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) {
-                const uint32_t i = chunk + r0 + u;
-                // Past the chunk's rows, the row's window ends before this
-                // wave's columns, or its y interval misses their band
-                // (wave-uniform).
-                if (r0 + u >= rows || ra[u].x < wave_min_x || ra[u].z < ylo || ra[u].y > yhi)
+                if (r0 + u >= rows)
                     continue;
+                const uint32_t i = chunk + ro[u];
                 const float rx = ra[u].x, ry0 = ra[u].y, ry1 = ra[u].z, rz0 = ra[u].w;
                 const float rz1 = rb[u].x;
                 const uint32_t ri = __float_as_uint(rb[u].y);
