@@ -37,9 +37,9 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 # with its commands is profiles/grid_step_mixed1M_hbm_traffic.md.  Valid for
 # the 1,048,576-body mixed scene only.
 MEASURED_TRAFFIC_1M = {
-    "grid_query": 60.75e6,
-    "grid_buckets": 185.60e6,
-    "aabb_grid_keys": 160.60e6,
+    "grid_query": 60.76e6,
+    "grid_buckets": 185.61e6,
+    "aabb_grid_keys": 160.54e6,
 }
 
 
